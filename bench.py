@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Throughput of the time-stepping core on one MI355X.
+
+Workload: the plain-method configuration at one GPU's real shape,
+1200 x 600 x 400 nodes (reference axes ni=600 fault-normal, nj=1200 strike,
+nk=400 depth) at 50 m, curvilinear with topography, layered medium, one
+40 x 18 km slip-weakening fault, CFS-PML on five faces and a free surface.
+A step is one full RK4 time step (4 stages) of the whole core: fault plane,
+volume stencil with PML and free surface, split-node columns, accumulators.
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below). ``--gpus N`` with N > 1 exits:
+this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "gpu-cgfd3d-dynamic_amd"))
+sys.path.insert(0, str(REPO))
+
+METRIC = "Gcell-updates/s + %HBM peak; batched vs plain; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per point of one volume-stencil launch (FP32, RK4 with
+# each array touched once per stage): stage 0 reads cur(=pre) 36 + metric 36 +
+# media 12 and writes tmp 36 + end 36; stages 1-2 also read pre 36 + end 36;
+# stage 3 reads cur 36 + 48 + end 36 and writes end 36.
+STAGE_BYTES = (156, 228, 228, 156)
+BYTES_PER_CELL_UPDATE = sum(STAGE_BYTES)  # 768
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(profile_dir: Path):
+    """HBM bytes per volume launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE in KiB, scaled by the copy-kernel calibration)."""
+    f = profile_dir / "k_vol_config2_pmc.txt"
+    if not f.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(threads: int, steps: int = 3):
+    """The oracle (FP32 CPU restatement, OpenMP) on a bounded sample: the same
+    configuration generator at 1/4 linear scale (150 x 300 x 100 nodes)."""
+    from cgfd import configs, model
+    import oracle
+    sc = configs.config2(scale=0.25, steps=steps)
+    st = model.Setup(sc)
+    n = oracle.Oracle.threads(threads)
+    o = oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults))
+    o.run(1)
+    t = time.perf_counter()
+    o.run(steps)
+    el = time.perf_counter() - t
+    cells = sc.ni * sc.nj * sc.nk * steps
+    o.close()
+    st.close()
+    return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": n, "kind": "port",
+            "sample": f"oracle FP32 on the same generator at 1/4 scale ({sc.ni}x{sc.nj}x{sc.nk}), "
+                      f"{steps} steps after 1 warm-up, {n} OpenMP threads, {el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the workload (1.0 = 1200x600x400)")
+    ap.add_argument("--variant", type=int, default=0, help="volume kernel variant (0 = default)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the whole domain on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding is design-only)", file=sys.stderr)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    from cgfd import configs, model
+    t0 = time.perf_counter()
+    sc = configs.config2(scale=args.scale, steps=args.steps + args.warmup)
+    log(f"[bench] generated {sc.name} {sc.ni}x{sc.nj}x{sc.nk} in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    st = model.Setup(sc)
+    log(f"[bench] host setup in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    core = model.Core(st, variant=args.variant)
+    core.sync()
+    log(f"[bench] device upload in {time.perf_counter() - t0:.1f} s")
+    # free host copies (the core holds everything in HBM)
+    del sc.x, sc.y, sc.z, sc.lam, sc.mu, sc.rho
+    if args.warmup:
+        core.run(args.warmup)
+        core.sync()
+    log(f"[bench] warm-up {args.warmup} steps done")
+    ms_total, ms_vol, n_vol = core.run_timed(args.steps)
+    core.sync()
+    log(f"[bench] timed {args.steps} steps: {ms_total:.1f} ms total, volume {ms_vol:.1f} ms in {n_vol} launches")
+    w = core.read_fault(0)
+    import numpy as np
+    slip_max = float(np.nanmax(w[6]))
+
+    ni, nj, nk = sc.ni, sc.nj, sc.nk
+    cells = ni * nj * nk * args.steps
+    sec = ms_total / 1e3
+    value = cells / sec / 1e9
+    hbm_frac = value * 1e9 * BYTES_PER_CELL_UPDATE / (HBM_PEAK_GBS * 1e9)
+    nvol_planes = nk - 4  # the top four layers go to the free-surface kernel
+    vol_pts = ni * nj * nvol_planes
+    launches_per_step = 4
+    bytes_vol = sum(STAGE_BYTES) * vol_pts * (n_vol // launches_per_step)
+    achieved = bytes_vol / (ms_vol / 1e3) / 1e9 if ms_vol > 0 else 0.0
+    traffic = read_traffic(REPO / "profiles")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_total / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": f"{sc.name}: plain-method rupture, {nj}x{ni}x{nk} nodes (strike x normal x depth) "
+                               f"at 50 m, curvilinear, PML 20, free surface, 1 fault, FP32",
+                   "grid": [ni, nj, nk], "parallelism": "single GPU", "hbm_frac_768B_model": round(hbm_frac, 4),
+                   "max_slip_m": round(slip_max, 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_vol (volume stencil + PML + RK update)",
+                     "avg_launch_ms": round(ms_vol / max(n_vol, 1), 4)},
+    }
+    core.close()
+    st.close()
+    if rank == 0 and not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
+    if rank == 0:
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

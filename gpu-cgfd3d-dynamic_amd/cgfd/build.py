@@ -1,0 +1,81 @@
+"""Build recipes for the native libraries (in-tree, no build system).
+
+``build_core()``   hipcc -> gpu-cgfd3d-dynamic_amd/_build/libcgfd.so
+                   (HIP kernels for gfx950 + the C ABI + the host setup)
+``build_oracle()`` g++   -> oracle/_build/libcgfd_oracle.so (test infrastructure)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG_DIR = Path(__file__).resolve().parent.parent          # gpu-cgfd3d-dynamic_amd/
+REPO = PKG_DIR.parent
+CSRC = PKG_DIR / "csrc"
+HOST = PKG_DIR / "host"
+INCLUDE = REPO / "include"
+BUILD_DIR = PKG_DIR / "_build"
+LIB_PATH = BUILD_DIR / "libcgfd.so"
+ORACLE_DIR = REPO / "oracle"
+ORACLE_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle.so"
+
+HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_vol_lds.hip"]
+HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp"]
+
+# Packed FP32 instructions stay disabled: hipcc otherwise forms v_pk_*_f32
+# from scalar float code.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wno-unused-result", "-fopenmp",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm toolchain required to build libcgfd.so)")
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> None:
+    proc = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if proc.returncode != 0:
+        sys.stderr.write(proc.stdout)
+        raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
+
+
+def build_core(verbose: bool = False) -> Path:
+    """Compile the HIP core + host setup into one shared library."""
+    BUILD_DIR.mkdir(parents=True, exist_ok=True)
+    srcs = [str(CSRC / s) for s in HIP_SOURCES if (CSRC / s).exists()]
+    srcs += [str(HOST / s) for s in HOST_SOURCES]
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    cmd = [_hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", f"-I{HOST}", *srcs, "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def build_oracle(verbose: bool = False) -> Path:
+    """Compile the CPU restatement used as the checker (OpenMP, -O2)."""
+    out = ORACLE_LIB
+    out.parent.mkdir(parents=True, exist_ok=True)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-march=x86-64-v2",
+           "-ffp-contract=off", f"-I{INCLUDE}", str(ORACLE_DIR / "cgfd_oracle.cpp"), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, out)
+    return out
+
+
+if __name__ == "__main__":
+    print(build_core(verbose=True))
+    print(build_oracle(verbose=True))

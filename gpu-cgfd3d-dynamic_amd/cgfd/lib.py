@@ -1,0 +1,135 @@
+"""ctypes bindings of include/cgfd_core.h and include/cgfd_setup.h.
+
+The product path is the HIP library ``_build/libcgfd.so``; there is no CPU
+fallback: loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .build import LIB_PATH
+
+MAX_FAULTS = 5
+NCMP = 9
+NFAULT_OUT = 11
+NPG = 15
+COMPAT_FIX_F2W_SWAP = 1
+
+ERR = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE", -5: "ERR_NAN", -6: "ERR_NODEV"}
+
+VP = C.c_void_p
+I32 = C.c_int
+F32 = C.c_float
+
+
+class FaultDesc(C.Structure):
+    _fields_ = [
+        ("fault_x_index", I32),
+        ("T0x", VP), ("T0y", VP), ("T0z", VP), ("mu_s", VP), ("mu_d", VP), ("Dc", VP), ("C0", VP),
+        ("united", VP), ("faultgrid", VP), ("rup_index_y", VP), ("rup_index_z", VP),
+        ("Tn0", VP), ("Ts10", VP), ("Ts20", VP),
+        ("rho_f", VP), ("vec_s1", VP), ("vec_s2", VP),
+        ("D", VP * 12), ("matPlus2Min", VP * 5), ("matMin2Plus", VP * 5), ("matM2", VP * 6), ("matF", VP * 14),
+    ]
+
+
+class Desc(C.Structure):
+    _fields_ = [
+        ("ni", I32), ("nj", I32), ("nk", I32),
+        ("metric", VP), ("lambda_", VP), ("mu", VP), ("rho", VP),
+        ("free_top", I32),
+        ("pml_is_sides", (I32 * 2) * 3),
+        ("pml_ni1", (I32 * 2) * 3), ("pml_ni2", (I32 * 2) * 3),
+        ("pml_nj1", (I32 * 2) * 3), ("pml_nj2", (I32 * 2) * 3),
+        ("pml_nk1", (I32 * 2) * 3), ("pml_nk2", (I32 * 2) * 3),
+        ("pml_A", (VP * 2) * 3), ("pml_B", (VP * 2) * 3), ("pml_D", (VP * 2) * 3),
+        ("imethod", I32), ("number_fault", I32),
+        ("fault", FaultDesc * MAX_FAULTS),
+        ("dt", F32), ("t0", F32), ("nt_total", I32), ("device_id", I32),
+        ("qc_check_nan_every", I32), ("compat_flags", I32),
+    ]
+
+
+class SetupParams(C.Structure):
+    _fields_ = [
+        ("ni", I32), ("nj", I32), ("nk", I32),
+        ("x", VP), ("y", VP), ("z", VP),
+        ("lambda_", VP), ("mu", VP), ("rho", VP),
+        ("free_top", I32),
+        ("pml_layers", (I32 * 2) * 3),
+        ("pml_alpha_max", (F32 * 2) * 3), ("pml_beta_max", (F32 * 2) * 3), ("pml_velocity", (F32 * 2) * 3),
+        ("imethod", I32), ("number_fault", I32),
+        ("fault_x_index", I32 * MAX_FAULTS), ("fault_grid", I32 * (4 * MAX_FAULTS)),
+        ("init_stress", (VP * 7) * MAX_FAULTS),
+        ("united_margin", I32),
+        ("dt", F32), ("t0", F32), ("nt_total", I32), ("device_id", I32),
+    ]
+
+
+_LIB = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libcgfd.so (built by cgfd.build.build_core)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with __graft_entry__.build() "
+                           "(the HIP extension is required; there is no CPU fallback)")
+    lib = C.CDLL(str(p))
+    sig = {
+        "cgfd_last_error": (C.c_char_p, []),
+        "cgfd_version": (C.c_char_p, []),
+        "cgfd_core_create": (I32, [C.POINTER(Desc), C.POINTER(VP)]),
+        "cgfd_core_destroy": (I32, [VP]),
+        "cgfd_core_run": (I32, [VP, I32]),
+        "cgfd_core_step_index": (I32, [VP]),
+        "cgfd_core_sync": (I32, [VP]),
+        "cgfd_core_read_wavefield": (I32, [VP, VP]),
+        "cgfd_core_write_wavefield": (I32, [VP, VP]),
+        "cgfd_core_read_fault": (I32, [VP, I32, VP]),
+        "cgfd_core_read_fault_state": (I32, [VP, I32, VP]),
+        "cgfd_core_read_pg": (I32, [VP, VP]),
+        "cgfd_core_set_stations": (I32, [VP, I32, VP, I32]),
+        "cgfd_core_read_stations": (I32, [VP, VP, C.POINTER(I32)]),
+        "cgfd_core_run_timed": (I32, [VP, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32)]),
+        "cgfd_core_set_variant": (I32, [VP, I32]),
+        "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),
+        "cgfd_setup_create": (I32, [C.POINTER(SetupParams), C.POINTER(VP)]),
+        "cgfd_setup_destroy": (I32, [VP]),
+        "cgfd_setup_get_desc": (I32, [VP, C.POINTER(Desc)]),
+        "cgfd_setup_coords": (C.POINTER(F32), [VP]),
+        "cgfd_setup_vec_n": (C.POINTER(F32), [VP, I32]),
+        "cgfd_setup_metric": (I32, [I32, I32, I32, VP, VP, VP, VP]),
+        "cgfd_setup_dt_max": (I32, [VP, C.POINTER(F32)]),
+        "cgfd_pml_cal_R": (F32, [F32]),
+        "cgfd_pml_cal_dmax": (F32, [F32, F32, F32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+class CgfdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERR.get(code, code)}: {msg}")
+        self.code = code
+
+
+def check(lib: C.CDLL, rc: int) -> None:
+    if rc != 0:
+        raise CgfdError(rc, lib.cgfd_last_error().decode())
+
+
+def ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(VP)

@@ -1,0 +1,215 @@
+"""Scenario -> host setup -> boundary descriptor -> GPU core.
+
+A :class:`Scenario` holds what the reference reads from its par file and
+input NetCDF files (grid, media, boundaries, faults and their initial
+stress/friction, time stepping). :class:`Setup` runs the C++ host setup of
+the driver on it (include/cgfd_setup.h) and exposes the boundary descriptor;
+:class:`Core` drives the HIP time-stepping core through the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import lib as L
+
+FAULT_FIELDS = ("Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0")  # init_stress_<id>.nc variables
+
+
+@dataclass
+class FaultSpec:
+    x_index: int                      # 0-based physical x index of the plane
+    grid: tuple                       # 0-based (j1, j2, k1, k2) = fault_grid
+    fields: dict                      # FAULT_FIELDS -> float32 (nk, nj)
+
+
+@dataclass
+class Scenario:
+    ni: int
+    nj: int
+    nk: int
+    x: np.ndarray                     # (nk, nj, ni) float32 physical coordinates
+    y: np.ndarray
+    z: np.ndarray
+    lam: np.ndarray                   # (nk, nj, ni) float32
+    mu: np.ndarray
+    rho: np.ndarray                   # density
+    free_top: bool = True
+    pml_layers: list = field(default_factory=lambda: [[0, 0], [0, 0], [0, 0]])
+    pml_alpha_max: float = 3.14
+    pml_beta_max: float = 2.0
+    pml_velocity: float = 7000.0
+    imethod: int = 2
+    faults: list = field(default_factory=list)
+    united_margin: int = 40
+    dt: float = 0.008
+    t0: float = 0.0
+    nt_total: int = 1
+    stations: np.ndarray | None = None   # (n, 3) physical (i, j, k)
+    name: str = "scenario"
+
+    @property
+    def shape_ext(self):
+        return (self.nk + 6, self.nj + 6, self.ni + 6)
+
+
+class Setup:
+    """Host setup (metric, PML profiles, fault coefficients and masks)."""
+
+    def __init__(self, sc: Scenario, device_id: int = 0):
+        self.lib = L.load_library()
+        self.sc = sc
+        keep = []
+
+        def arr(a, dtype=np.float32):
+            a = np.ascontiguousarray(a, dtype=dtype)
+            keep.append(a)
+            return L.ptr(a)
+
+        p = L.SetupParams()
+        p.ni, p.nj, p.nk = sc.ni, sc.nj, sc.nk
+        p.x, p.y, p.z = arr(sc.x), arr(sc.y), arr(sc.z)
+        p.lambda_, p.mu, p.rho = arr(sc.lam), arr(sc.mu), arr(sc.rho)
+        p.free_top = int(sc.free_top)
+        for d in range(3):
+            for s in range(2):
+                p.pml_layers[d][s] = int(sc.pml_layers[d][s])
+                p.pml_alpha_max[d][s] = sc.pml_alpha_max
+                p.pml_beta_max[d][s] = sc.pml_beta_max
+                p.pml_velocity[d][s] = sc.pml_velocity
+        p.imethod = sc.imethod
+        p.number_fault = len(sc.faults)
+        for fid, f in enumerate(sc.faults):
+            p.fault_x_index[fid] = f.x_index
+            for q in range(4):
+                p.fault_grid[4 * fid + q] = int(f.grid[q])
+            for q, name in enumerate(FAULT_FIELDS):
+                p.init_stress[fid][q] = arr(f.fields[name])
+        p.united_margin = sc.united_margin
+        p.dt, p.t0, p.nt_total, p.device_id = sc.dt, sc.t0, sc.nt_total, device_id
+        h = C.c_void_p()
+        L.check(self.lib, self.lib.cgfd_setup_create(C.byref(p), C.byref(h)))
+        del keep  # the setup copied everything it needs
+        self.handle = h
+        self.desc = L.Desc()
+        L.check(self.lib, self.lib.cgfd_setup_get_desc(h, C.byref(self.desc)))
+
+    def dt_max(self) -> float:
+        v = C.c_float()
+        L.check(self.lib, self.lib.cgfd_setup_dt_max(self.handle, C.byref(v)))
+        return float(v.value)
+
+    def coords(self) -> np.ndarray:
+        nz, ny, nx = self.sc.shape_ext
+        n = 3 * nz * ny * nx
+        pp = self.lib.cgfd_setup_coords(self.handle)
+        return np.ctypeslib.as_array(pp, shape=(n,)).reshape(3, nz, ny, nx).copy()
+
+    def metric(self) -> np.ndarray:
+        nz, ny, nx = self.sc.shape_ext
+        pp = C.cast(self.desc.metric, C.POINTER(C.c_float))
+        return np.ctypeslib.as_array(pp, shape=(10 * nz * ny * nx,)).reshape(10, nz, ny, nx).copy()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cgfd_setup_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Core:
+    """The GPU time-stepping core (one HIP stream, one device)."""
+
+    def __init__(self, setup: Setup, compat_flags: int = 0, nan_every: int = 0, variant: int = 0):
+        self.lib = L.load_library()
+        self.setup = setup
+        d = L.Desc.from_buffer_copy(setup.desc)
+        d.compat_flags = compat_flags
+        d.qc_check_nan_every = nan_every
+        self.desc = d
+        h = C.c_void_p()
+        L.check(self.lib, self.lib.cgfd_core_create(C.byref(d), C.byref(h)))
+        self.handle = h
+        if variant:
+            L.check(self.lib, self.lib.cgfd_core_set_variant(h, variant))
+        sc = setup.sc
+        self.shape_ext = sc.shape_ext
+        self.nfault = len(sc.faults)
+
+    def set_variant(self, v: int):
+        L.check(self.lib, self.lib.cgfd_core_set_variant(self.handle, v))
+
+    def run(self, nsteps: int):
+        L.check(self.lib, self.lib.cgfd_core_run(self.handle, nsteps))
+
+    def run_timed(self, nsteps: int):
+        tot, vol, nv = C.c_float(), C.c_float(), C.c_int()
+        L.check(self.lib, self.lib.cgfd_core_run_timed(self.handle, nsteps, C.byref(tot), C.byref(vol), C.byref(nv)))
+        return tot.value, vol.value, nv.value
+
+    def sync(self):
+        L.check(self.lib, self.lib.cgfd_core_sync(self.handle))
+
+    @property
+    def step_index(self) -> int:
+        return self.lib.cgfd_core_step_index(self.handle)
+
+    def read_wavefield(self) -> np.ndarray:
+        nz, ny, nx = self.shape_ext
+        out = np.empty((9, nz, ny, nx), np.float32)
+        L.check(self.lib, self.lib.cgfd_core_read_wavefield(self.handle, L.ptr(out)))
+        return out
+
+    def write_wavefield(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        L.check(self.lib, self.lib.cgfd_core_write_wavefield(self.handle, L.ptr(w)))
+
+    def read_fault(self, fid: int = 0) -> np.ndarray:
+        nz, ny, _ = self.shape_ext
+        out = np.empty((11, nz, ny), np.float32)
+        L.check(self.lib, self.lib.cgfd_core_read_fault(self.handle, fid, L.ptr(out)))
+        return out
+
+    def read_fault_state(self, fid: int = 0) -> np.ndarray:
+        nz, ny, _ = self.shape_ext
+        out = np.empty((9, 2, nz, ny), np.float32)
+        L.check(self.lib, self.lib.cgfd_core_read_fault_state(self.handle, fid, L.ptr(out)))
+        return out
+
+    def read_pg(self) -> np.ndarray:
+        nz, ny, nx = self.shape_ext
+        out = np.empty((15, ny, nx), np.float32)
+        L.check(self.lib, self.lib.cgfd_core_read_pg(self.handle, L.ptr(out)))
+        return out
+
+    def set_stations(self, ijk_ext: np.ndarray, nt_max: int):
+        a = np.ascontiguousarray(ijk_ext, np.int32).reshape(-1, 3)
+        L.check(self.lib, self.lib.cgfd_core_set_stations(self.handle, a.shape[0], L.ptr(a), nt_max))
+        self._nst = a.shape[0]
+        self._ntmax = nt_max
+
+    def read_stations(self) -> np.ndarray:
+        buf = np.empty((self._nst, self._ntmax, 9), np.float32)
+        nt = C.c_int()
+        L.check(self.lib, self.lib.cgfd_core_read_stations(self.handle, L.ptr(buf), C.byref(nt)))
+        n = nt.value
+        flat = buf.reshape(-1)[: self._nst * n * 9]
+        return flat.reshape(self._nst, n, 9).copy()
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cgfd_core_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

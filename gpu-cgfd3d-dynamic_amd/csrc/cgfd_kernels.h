@@ -1,0 +1,59 @@
+/*
+ * cgfd_kernels.h -- host-side launchers of the gfx950 kernels.
+ */
+#ifndef CGFD_KERNELS_H
+#define CGFD_KERNELS_H
+
+#include "cgfd_dev.h"
+
+namespace cgfd {
+
+/* one fault in device memory (fault_one_t + fault_coef_one_t + the T1
+   traction columns of fault_wav_t; fault_info.h:11-119, fault_wav_t.h:18-63) */
+struct FaultDev {
+  int i0;                        /* ghost-inclusive x index of the fault plane */
+  const float *T0;               /* [3][sslice] T0x,T0y,T0z */
+  const float *fric;             /* [4][sslice] mu_s, mu_d, Dc, C0 */
+  const int *united, *faultgrid, *rupy, *rupz;
+  int *flag_rup, *init_t0_flag;
+  float *out;                    /* [11][sslice] Tn..Init_t0 */
+  const float *rho_f;            /* [2][sslice] */
+  const float *vs;               /* [6][sslice] s1x,s1y,s1z,s2x,s2y,s2z */
+  const float *D;                /* [12][9][sslice] D21_1..D33_2 */
+  const float *P2M, *M2P;        /* [5][9][sslice] */
+  const float *M2;               /* [6][9][sslice] T1toVx_Min,VytoVx_Min,VztoVx_Min,.._Plus */
+  const float *Fm;               /* [14][9][ny] free-surface row */
+  float *T1;                     /* [3][7][sslice] */
+  float *hT1, *mT1;              /* [3][sslice] */
+};
+
+struct FaultArgs {
+  Grid g;
+  const float *met, *lam, *mu, *slw, *matVx2Vz, *matVy2Vz;
+  float *w_cur, *w_pre, *w_tnew, *w_end;
+  float *f_cur, *f_pre, *f_tnew, *f_end;   /* level bases; fault id at +id*18*sslice */
+  int nfault;
+  FaultDev f[5];
+  int istage, idir, jdir, kdir;
+  int isfree, imethod, fix_q1;
+  float dt, ca, cb, rkb;
+  const int *it_dev;                        /* completed steps (device counter) */
+};
+
+void launch_volume(int variant, const VolArgs &a, int idir, int jdir, int kdir, int top, hipStream_t st);
+void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
+                    hipStream_t st);
+void launch_pg(const Grid &g, const float *wend, const float *wpre, float *PG, float *Dis, float dt, hipStream_t st);
+
+void launch_wave2fault(const FaultArgs &a, hipStream_t st);
+void launch_trial(const FaultArgs &a, hipStream_t st);
+void launch_fault_update(const FaultArgs &a, hipStream_t st);
+void launch_step_end(int *it_dev, hipStream_t st);
+void launch_zero(float *p, long n, hipStream_t st);
+void launch_nan_check(const float *p, long n, int *flag, hipStream_t st);
+void launch_record_stations(const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max,
+                            const int *it_dev, int it0, hipStream_t st);
+
+}  // namespace cgfd
+
+#endif

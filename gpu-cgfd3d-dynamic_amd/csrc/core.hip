@@ -1,0 +1,629 @@
+/*
+ * core.hip -- the C ABI of include/cgfd_core.h: device state, uploads, and
+ * the time loop (drv_rk_curv_col_allstep, src/forward/drv_rk_curv_col.cu:28-667)
+ * on one GPU without the MPI halo exchange.
+ *
+ * Device layout: the reference's SoA layout (nx = ni+6 fastest, 3 ghosts per
+ * side). The wavefield has four levels used as pre / stage-input ping-pong A /
+ * ping-pong B / end: stage 0 reads pre and writes A, stage 1 reads A and
+ * writes B, stage 2 reads B and writes A, stage 3 reads A and writes end only.
+ * pre and end swap after every step, as the reference's pointers do
+ * (drv_rk_curv_col.cu:622-635). The same scheme holds for the split-node
+ * fault state; the PML auxiliaries are point-local and keep the reference's
+ * pre / tmp / end levels.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cgfd_core.h"
+#include "../host/cgfd_errors.h"
+#include "cgfd_kernels.h"
+
+using namespace cgfd;
+
+#define HIPCHK(call)                                                                              \
+  do {                                                                                            \
+    hipError_t e_ = (call);                                                                       \
+    if (e_ != hipSuccess)                                                                         \
+      return cgfd_set_error(CGFD_ERR_HIP, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                            __FILE__, __LINE__);                                                  \
+  } while (0)
+
+namespace {
+
+/* RK4 (fd_t.cu:38-45) and operator pairs (fd_t.cu:149-159) */
+const float kRkA[3] = {0.5f, 0.5f, 1.0f};
+const float kRkB[4] = {(float)(1.0 / 6.0), (float)(1.0 / 3.0), (float)(1.0 / 3.0), (float)(1.0 / 6.0)};
+const int kPairs[8][3] = {{0, 0, 0}, {1, 1, 0}, {1, 1, 1}, {0, 0, 1}, {0, 1, 0}, {1, 0, 0}, {1, 0, 1}, {0, 1, 1}};
+
+struct DevBuf {
+  std::vector<void *> ptrs;
+  ~DevBuf() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+struct cgfd_core {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  Grid g{};
+  int ni = 0, nj = 0, nk = 0;
+  float dt = 0, t0 = 0;
+  int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
+  int variant = 1;
+  DevBuf mem;
+
+  float *wave = nullptr;   /* 4 levels x 9 x sicmp */
+  int lv_pre = 0, lv_a = 1, lv_b = 2, lv_end = 3;
+  float *met = nullptr, *lam = nullptr, *mu = nullptr, *slw = nullptr;
+  float *matVx2Vz = nullptr, *matVy2Vz = nullptr;
+
+  PmlFace pml[3][2]{};
+  int pml_on = 0;
+
+  float *fwav = nullptr;   /* 4 levels x nfault x 18 x sslice */
+  int fl_pre = 0, fl_a = 1, fl_b = 2, fl_end = 3;
+  FaultDev flt[5]{};
+
+  int *it_dev = nullptr;
+  int *nan_flag = nullptr;
+  int it_host = 0;
+  float *PG = nullptr, *Dis = nullptr;
+
+  long *st_idx = nullptr;
+  float *st_buf = nullptr;
+  int st_n = 0, st_nt = 0, st_it0 = 0;
+
+  float *wlev(int l) { return wave + (long)l * 9 * g.sicmp; }
+  float *flev(int l) { return fwav + (long)l * nfault * 18 * g.sslice; }
+
+  template <class T>
+  int alloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return CGFD_OK;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(T));
+    if (e != hipSuccess)
+      return cgfd_set_error(CGFD_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    mem.ptrs.push_back(q);
+    *p = (T *)q;
+    return CGFD_OK;
+  }
+};
+
+static int check_desc(const cgfd_desc *d) {
+  if (!d) return cgfd_set_error(CGFD_ERR_ARG, "null descriptor");
+  if (d->ni < 8 || d->nj < 8 || d->nk < 8)
+    return cgfd_set_error(CGFD_ERR_ARG, "grid too small: ni=%d nj=%d nk=%d (each must be >= 8)", d->ni, d->nj, d->nk);
+  if (!d->metric || !d->lambda || !d->mu || !d->rho) return cgfd_set_error(CGFD_ERR_ARG, "metric/media pointers missing");
+  if (!(d->dt > 0.0f)) return cgfd_set_error(CGFD_ERR_ARG, "dt must be > 0 (got %g)", (double)d->dt);
+  if (d->number_fault < 0 || d->number_fault > CGFD_MAX_FAULTS)
+    return cgfd_set_error(CGFD_ERR_ARG, "number_fault=%d out of range", d->number_fault);
+  if (d->number_fault > 0 && d->imethod != 1 && d->imethod != 2)
+    return cgfd_set_error(CGFD_ERR_ARG, "dynamic_method must be 1 or 2 (got %d)", d->imethod);
+  for (int id = 0; id < d->number_fault; id++) {
+    const cgfd_fault_desc &f = d->fault[id];
+    if (f.fault_x_index < 4 || f.fault_x_index > d->ni - 5)
+      return cgfd_set_error(CGFD_ERR_ARG, "fault %d: x index %d outside [4, ni-5]", id, f.fault_x_index);
+    if (!f.T0x || !f.T0y || !f.T0z || !f.mu_s || !f.mu_d || !f.Dc || !f.C0 || !f.united || !f.faultgrid ||
+        !f.rup_index_y || !f.rup_index_z || !f.rho_f || !f.vec_s1 || !f.vec_s2)
+      return cgfd_set_error(CGFD_ERR_ARG, "fault %d: required array missing", id);
+    for (int q = 0; q < 12; q++)
+      if (!f.D[q]) return cgfd_set_error(CGFD_ERR_ARG, "fault %d: D matrix %d missing", id, q);
+    for (int q = 0; q < 5; q++)
+      if ((d->imethod == 1) && (!f.matPlus2Min[q] || !f.matMin2Plus[q]))
+        return cgfd_set_error(CGFD_ERR_ARG, "fault %d: method-1 matrix %d missing", id, q);
+    for (int q = 0; q < 6; q++)
+      if (d->imethod == 2 && !f.matM2[q]) return cgfd_set_error(CGFD_ERR_ARG, "fault %d: method-2 matrix %d missing", id, q);
+    if (d->free_top)
+      for (int q = 0; q < 14; q++)
+        if (!f.matF[q]) return cgfd_set_error(CGFD_ERR_ARG, "fault %d: free-surface matrix %d missing", id, q);
+  }
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++) {
+      if (!d->pml_is_sides[dim][side]) continue;
+      int lo[3] = {d->pml_ni1[dim][side], d->pml_nj1[dim][side], d->pml_nk1[dim][side]};
+      int hi[3] = {d->pml_ni2[dim][side], d->pml_nj2[dim][side], d->pml_nk2[dim][side]};
+      int n[3] = {d->ni, d->nj, d->nk};
+      for (int a = 0; a < 3; a++)
+        if (lo[a] < 3 || hi[a] > n[a] + 2 || hi[a] < lo[a])
+          return cgfd_set_error(CGFD_ERR_ARG, "PML face (%d,%d): index range outside the physical grid", dim, side);
+      if (!d->pml_A[dim][side] || !d->pml_B[dim][side] || !d->pml_D[dim][side])
+        return cgfd_set_error(CGFD_ERR_ARG, "PML face (%d,%d): coefficient arrays missing", dim, side);
+      if (d->free_top && dim == 2 && side == 1)
+        return cgfd_set_error(CGFD_ERR_ARG, "PML and free surface both requested on the top face");
+    }
+  return CGFD_OK;
+}
+
+/* host [node][9] -> device SoA [9][node] for a list of matrices */
+static void soa9(std::vector<float> &dst, size_t off, const float *src, size_t nodes) {
+  for (size_t p = 0; p < nodes; p++)
+    for (int e = 0; e < 9; e++) dst[off + e * nodes + p] = src[p * 9 + e];
+}
+
+static int upload_faults(cgfd_core *c, const cgfd_desc *d) {
+  const size_t ss = c->g.sslice, ny = c->g.ny;
+  for (int id = 0; id < c->nfault; id++) {
+    const cgfd_fault_desc &f = d->fault[id];
+    FaultDev &F = c->flt[id];
+    F.i0 = f.fault_x_index + 3;
+    std::vector<float> h;
+    float *dp;
+    int *ip;
+    int rc;
+    /* T0 [3][ss] */
+    h.assign(3 * ss, 0.f);
+    for (size_t p = 0; p < ss; p++) { h[p] = f.T0x[p]; h[ss + p] = f.T0y[p]; h[2 * ss + p] = f.T0z[p]; }
+    if ((rc = c->alloc(&dp, h.size()))) return rc;
+    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    F.T0 = dp;
+    /* friction [4][ss] */
+    h.assign(4 * ss, 0.f);
+    for (size_t p = 0; p < ss; p++) { h[p] = f.mu_s[p]; h[ss + p] = f.mu_d[p]; h[2 * ss + p] = f.Dc[p]; h[3 * ss + p] = f.C0[p]; }
+    if ((rc = c->alloc(&dp, h.size()))) return rc;
+    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    F.fric = dp;
+    /* masks */
+    const int *msk[4] = {f.united, f.faultgrid, f.rup_index_y, f.rup_index_z};
+    const int *dmsk[4];
+    for (int q = 0; q < 4; q++) {
+      if ((rc = c->alloc(&ip, ss))) return rc;
+      HIPCHK(hipMemcpy(ip, msk[q], ss * 4, hipMemcpyHostToDevice));
+      dmsk[q] = ip;
+    }
+    F.united = dmsk[0]; F.faultgrid = dmsk[1]; F.rupy = dmsk[2]; F.rupz = dmsk[3];
+    if ((rc = c->alloc(&ip, 2 * ss))) return rc;
+    HIPCHK(hipMemset(ip, 0, 2 * ss * 4));  /* flag_rup = 0 (fault_info.cu:1069), init_t0_flag = 0 (:960) */
+    F.flag_rup = ip;
+    F.init_t0_flag = ip + ss;
+    /* outputs [11][ss] */
+    h.assign(11 * ss, 0.f);
+    for (size_t p = 0; p < ss; p++) {
+      h[p] = f.Tn0 ? f.Tn0[p] : 0.f;
+      h[ss + p] = f.Ts10 ? f.Ts10[p] : 0.f;
+      h[2 * ss + p] = f.Ts20 ? f.Ts20[p] : 0.f;
+      h[10 * ss + p] = -9999.9f;  /* Init_t0 (fault_info.cu:1068) */
+    }
+    if ((rc = c->alloc(&dp, h.size()))) return rc;
+    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    F.out = dp;
+    /* rho_f [2][ss] */
+    if ((rc = c->alloc(&dp, 2 * ss))) return rc;
+    HIPCHK(hipMemcpy(dp, f.rho_f, 2 * ss * 4, hipMemcpyHostToDevice));
+    F.rho_f = dp;
+    /* strike / dip vectors [6][ss] */
+    h.assign(6 * ss, 0.f);
+    for (size_t p = 0; p < ss; p++)
+      for (int e = 0; e < 3; e++) { h[e * ss + p] = f.vec_s1[p * 3 + e]; h[(3 + e) * ss + p] = f.vec_s2[p * 3 + e]; }
+    if ((rc = c->alloc(&dp, h.size()))) return rc;
+    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    F.vs = dp;
+    /* D [12][9][ss] */
+    h.assign(12 * 9 * ss, 0.f);
+    for (int q = 0; q < 12; q++) soa9(h, q * 9 * ss, f.D[q], ss);
+    if ((rc = c->alloc(&dp, h.size()))) return rc;
+    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    F.D = dp;
+    /* method-1 and method-2 maps (only the ones the method reads are required) */
+    if (d->imethod == 1) {
+      h.assign(5 * 9 * ss, 0.f);
+      for (int q = 0; q < 5; q++) soa9(h, q * 9 * ss, f.matPlus2Min[q], ss);
+      if ((rc = c->alloc(&dp, h.size()))) return rc;
+      HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+      F.P2M = dp;
+      for (int q = 0; q < 5; q++) soa9(h, q * 9 * ss, f.matMin2Plus[q], ss);
+      if ((rc = c->alloc(&dp, h.size()))) return rc;
+      HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+      F.M2P = dp;
+    } else {
+      h.assign(6 * 9 * ss, 0.f);
+      for (int q = 0; q < 6; q++) soa9(h, q * 9 * ss, f.matM2[q], ss);
+      if ((rc = c->alloc(&dp, h.size()))) return rc;
+      HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+      F.M2 = dp;
+    }
+    /* free-surface row [14][9][ny] */
+    h.assign(14 * 9 * ny, 0.f);
+    if (c->isfree)
+      for (int q = 0; q < 14; q++) soa9(h, q * 9 * ny, f.matF[q], ny);
+    if ((rc = c->alloc(&dp, h.size()))) return rc;
+    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    F.Fm = dp;
+    /* T1 [3][7][ss], hT1/mT1 [3][ss]: zero (fault_wav_t.cu:59-78) */
+    if ((rc = c->alloc(&dp, 3 * 7 * ss + 6 * ss))) return rc;
+    HIPCHK(hipMemset(dp, 0, (3 * 7 * ss + 6 * ss) * 4));
+    F.T1 = dp;
+    F.hT1 = dp + 21 * ss;
+    F.mT1 = dp + 24 * ss;
+  }
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
+  if (!out) return cgfd_set_error(CGFD_ERR_ARG, "null output pointer");
+  *out = nullptr;
+  int rc = check_desc(d);
+  if (rc) return rc;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) return cgfd_set_error(CGFD_ERR_NODEV, "no HIP device available");
+  if (d->device_id < 0 || d->device_id >= ndev)
+    return cgfd_set_error(CGFD_ERR_ARG, "device_id %d out of range (%d devices)", d->device_id, ndev);
+  cgfd_core *c = new (std::nothrow) cgfd_core();
+  if (!c) return cgfd_set_error(CGFD_ERR_NOMEM, "host allocation failed");
+  auto fail = [&](int code) { cgfd_core_destroy(c); return code; };
+  c->dev = d->device_id;
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "hipSetDevice failed"));
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+    return fail(cgfd_set_error(CGFD_ERR_HIP, "hipStreamCreate failed"));
+  Grid &g = c->g;
+  c->ni = d->ni; c->nj = d->nj; c->nk = d->nk;
+  g.ni = d->ni; g.nj = d->nj; g.nk = d->nk;
+  g.nx = g.ni + 6; g.ny = g.nj + 6; g.nz = g.nk + 6;
+  g.nk2 = 3 + g.nk - 1;
+  g.siy = g.nx; g.siz = (long)g.nx * g.ny; g.sicmp = g.siz * g.nz; g.sslice = (long)g.ny * g.nz;
+  c->dt = d->dt; c->t0 = d->t0; c->nt_total = d->nt_total;
+  c->isfree = d->free_top ? 1 : 0;
+  c->imethod = d->imethod;
+  c->nfault = d->number_fault;
+  c->fix_q1 = (d->compat_flags & CGFD_COMPAT_FIX_F2W_SWAP) ? 1 : 0;
+  c->nan_every = d->qc_check_nan_every;
+  const long sc = g.sicmp;
+
+  /* wavefield: 4 levels, zeroed (alloc.cu:448-457) */
+  if ((rc = c->alloc(&c->wave, (size_t)4 * 9 * sc))) return fail(rc);
+  if (hipMemset(c->wave, 0, (size_t)4 * 9 * sc * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+  /* metric and media */
+  if ((rc = c->alloc(&c->met, (size_t)10 * sc))) return fail(rc);
+  if ((rc = c->alloc(&c->lam, (size_t)3 * sc))) return fail(rc);
+  c->mu = c->lam + sc;
+  c->slw = c->lam + 2 * sc;
+  if (hipMemcpy(c->met, d->metric, (size_t)10 * sc * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->lam, d->lambda, (size_t)sc * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->mu, d->mu, (size_t)sc * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->slw, d->rho, (size_t)sc * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(cgfd_set_error(CGFD_ERR_HIP, "upload of metric/media failed"));
+  /* free-surface matrices (bdry_free_set calloc, bdry_t.cu:60-71) */
+  if ((rc = c->alloc(&c->matVx2Vz, (size_t)2 * 9 * g.siz))) return fail(rc);
+  c->matVy2Vz = c->matVx2Vz + 9 * g.siz;
+  if (hipMemset(c->matVx2Vz, 0, (size_t)2 * 9 * g.siz * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+  /* PML */
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++) {
+      PmlFace &f = c->pml[dim][side];
+      f.on = d->pml_is_sides[dim][side] ? 1 : 0;
+      if (!f.on) continue;
+      c->pml_on = 1;
+      f.i1 = d->pml_ni1[dim][side]; f.i2 = d->pml_ni2[dim][side];
+      f.j1 = d->pml_nj1[dim][side]; f.j2 = d->pml_nj2[dim][side];
+      f.k1 = d->pml_nk1[dim][side]; f.k2 = d->pml_nk2[dim][side];
+      f.an = f.i2 - f.i1 + 1; f.bn = f.j2 - f.j1 + 1;
+      int cn = f.k2 - f.k1 + 1;
+      f.n = (long)f.an * f.bn * cn;
+      int np = dim == 0 ? f.an : (dim == 1 ? f.bn : cn);
+      float *cf;
+      if ((rc = c->alloc(&cf, (size_t)3 * np))) return fail(rc);
+      if (hipMemcpy(cf, d->pml_A[dim][side], np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(cf + np, d->pml_B[dim][side], np * 4, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(cf + 2 * np, d->pml_D[dim][side], np * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return fail(cgfd_set_error(CGFD_ERR_HIP, "PML coefficient upload failed"));
+      f.A = cf; f.B = cf + np; f.D = cf + 2 * np;
+      float *aux;
+      if ((rc = c->alloc(&aux, (size_t)3 * 9 * f.n))) return fail(rc);
+      if (hipMemset(aux, 0, (size_t)3 * 9 * f.n * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+      f.pre = aux; f.tmp = aux + 9 * f.n; f.end = aux + 18 * f.n;
+    }
+  /* faults */
+  if (c->nfault > 0) {
+    size_t n = (size_t)4 * c->nfault * 18 * g.sslice;
+    if ((rc = c->alloc(&c->fwav, n))) return fail(rc);
+    if (hipMemset(c->fwav, 0, n * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+    if ((rc = upload_faults(c, d))) return fail(rc);
+  }
+  if ((rc = c->alloc(&c->it_dev, 2))) return fail(rc);
+  if (hipMemset(c->it_dev, 0, 8) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+  c->nan_flag = c->it_dev + 1;
+  if (c->isfree) {
+    if ((rc = c->alloc(&c->PG, (size_t)18 * g.siz))) return fail(rc);
+    if (hipMemset(c->PG, 0, (size_t)18 * g.siz * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+    c->Dis = c->PG + 15 * g.siz;
+    launch_dvh2dvz(g, c->met, c->lam, c->mu, c->matVx2Vz, c->matVy2Vz, c->st);
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess || hipGetLastError() != hipSuccess)
+    return fail(cgfd_set_error(CGFD_ERR_HIP, "setup kernels failed"));
+  *out = c;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_destroy(cgfd_core *c) {
+  if (!c) return CGFD_OK;
+  (void)hipSetDevice(c->dev);
+  if (c->st) {
+    (void)hipStreamSynchronize(c->st);
+    (void)hipStreamDestroy(c->st);
+  }
+  delete c;
+  return CGFD_OK;
+}
+
+namespace {
+
+/* arguments of the volume kernels for one stage */
+void fill_vol(cgfd_core *c, VolArgs &v, int istage, const float *cur, float *tnew) {
+  v.g = c->g;
+  v.cur = cur;
+  v.pre = c->wlev(c->lv_pre);
+  v.tnew = tnew;
+  v.end = c->wlev(c->lv_end);
+  v.met = c->met; v.lam = c->lam; v.mu = c->mu; v.slw = c->slw;
+  v.matVx2Vz = c->matVx2Vz; v.matVy2Vz = c->matVy2Vz;
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++) v.pml[dim][side] = c->pml[dim][side];
+  v.pml_on = c->pml_on;
+  v.isfree = c->isfree;
+  v.istage = istage;
+  v.ca = istage < 3 ? kRkA[istage] * c->dt : 0.0f;
+  v.cb = kRkB[istage] * c->dt;
+  v.fs.nfault = c->nfault;
+  for (int q = 0; q < c->nfault; q++) { v.fs.i0[q] = c->flt[q].i0; v.fs.united[q] = c->flt[q].united; }
+  v.zchunk = 0;
+}
+
+void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int kdir, float *wcur, float *wnew,
+                float *fcur, float *fnew) {
+  f.g = c->g;
+  f.met = c->met; f.lam = c->lam; f.mu = c->mu; f.slw = c->slw;
+  f.matVx2Vz = c->matVx2Vz; f.matVy2Vz = c->matVy2Vz;
+  f.w_cur = wcur; f.w_pre = c->wlev(c->lv_pre); f.w_tnew = wnew; f.w_end = c->wlev(c->lv_end);
+  f.f_cur = fcur; f.f_pre = c->flev(c->fl_pre); f.f_tnew = fnew; f.f_end = c->flev(c->fl_end);
+  f.nfault = c->nfault;
+  for (int q = 0; q < c->nfault; q++) f.f[q] = c->flt[q];
+  f.istage = istage; f.idir = idir; f.jdir = jdir; f.kdir = kdir;
+  f.isfree = c->isfree; f.imethod = c->imethod; f.fix_q1 = c->fix_q1;
+  f.dt = c->dt;
+  f.ca = istage < 3 ? kRkA[istage] * c->dt : 0.0f;
+  f.cb = kRkB[istage] * c->dt;
+  f.rkb = kRkB[istage];
+  f.it_dev = c->it_dev;
+}
+
+/* one time step; if ev != nullptr, event pairs bracket the volume launches */
+int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
+  const int ipair = c->it_host % 8;
+  const int nk = c->g.nk;
+  const int ntop = c->isfree ? (nk < 4 ? nk : 4) : 0;
+  const int nvol = nk - ntop;
+  if (c->st_n > 0)
+    launch_record_stations(c->wlev(c->lv_pre), c->g.sicmp, c->st_idx, c->st_n, c->st_buf, c->st_nt, c->it_dev,
+                           c->st_it0, c->st);
+  for (int s = 0; s < 4; s++) {
+    const int idir = (kPairs[ipair][0] + s) % 2;
+    const int jdir = (kPairs[ipair][1] + s) % 2;
+    const int kdir = (kPairs[ipair][2] + s) % 2;
+    const int lcur = s == 0 ? c->lv_pre : (s == 2 ? c->lv_b : c->lv_a);
+    const int lnew = s == 0 ? c->lv_a : (s == 1 ? c->lv_b : c->lv_a);
+    float *wcur = c->wlev(lcur);
+    float *wnew = s < 3 ? c->wlev(lnew) : nullptr;
+    float *fcur = nullptr, *fnew = nullptr;
+    if (c->nfault > 0) {
+      const int fcurl = s == 0 ? c->fl_pre : (s == 2 ? c->fl_b : c->fl_a);
+      const int fnewl = s == 0 ? c->fl_a : (s == 1 ? c->fl_b : c->fl_a);
+      fcur = c->flev(fcurl);
+      fnew = s < 3 ? c->flev(fnewl) : nullptr;
+      FaultArgs fa;
+      fill_fault(c, fa, s, idir, jdir, kdir, wcur, wnew, fcur, fnew);
+      launch_wave2fault(fa, c->st);
+      launch_trial(fa, c->st);
+    }
+    VolArgs va;
+    fill_vol(c, va, s, wcur, wnew);
+    if (nvol > 0) {
+      va.k_lo = 0;
+      va.nk_run = nvol;
+      if (ev) HIPCHK(hipEventRecord(ev[2 * (*nev)], c->st));
+      launch_volume(c->variant, va, idir, jdir, kdir, 0, c->st);
+      if (ev) {
+        HIPCHK(hipEventRecord(ev[2 * (*nev) + 1], c->st));
+        (*nev)++;
+      }
+    }
+    if (ntop > 0) {
+      va.k_lo = nk - ntop;
+      va.nk_run = ntop;
+      launch_volume(c->variant, va, idir, jdir, kdir, 1, c->st);
+    }
+    if (c->nfault > 0) {
+      FaultArgs fa;
+      fill_fault(c, fa, s, idir, jdir, kdir, wcur, wnew, fcur, fnew);
+      launch_fault_update(fa, c->st);
+    }
+  }
+  if (c->isfree) launch_pg(c->g, c->wlev(c->lv_end), c->wlev(c->lv_pre), c->PG, c->Dis, c->dt, c->st);
+  launch_step_end(c->it_dev, c->st);
+  HIPCHK(hipGetLastError());
+  /* swap pre <-> end (drv_rk_curv_col.cu:622-635) */
+  std::swap(c->lv_pre, c->lv_end);
+  std::swap(c->fl_pre, c->fl_end);
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++) std::swap(c->pml[dim][side].pre, c->pml[dim][side].end);
+  c->it_host++;
+  if (c->nan_every > 0 && (c->it_host % c->nan_every) == 0) {
+    launch_nan_check(c->wlev(c->lv_pre), 9 * c->g.sicmp, c->nan_flag, c->st);
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, c->nan_flag, 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (flag) return cgfd_set_error(CGFD_ERR_NAN, "non-finite wavefield value after step %d", c->it_host);
+  }
+  return CGFD_OK;
+}
+
+}  // namespace
+
+extern "C" int cgfd_core_run(cgfd_core *c, int nsteps) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (nsteps < 0) return cgfd_set_error(CGFD_ERR_ARG, "nsteps must be >= 0");
+  HIPCHK(hipSetDevice(c->dev));
+  for (int s = 0; s < nsteps; s++) {
+    int rc = do_step(c, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_run_timed(cgfd_core *c, int nsteps, float *ms_total, float *ms_volume, int *n_volume) {
+  if (!c || nsteps < 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  HIPCHK(hipSetDevice(c->dev));
+  int nmax = 4 * nsteps;
+  std::vector<hipEvent_t> ev(2 * nmax + 2);
+  for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+  int nev = 0, rc = CGFD_OK;
+  HIPCHK(hipEventRecord(ev[2 * nmax], c->st));
+  for (int s = 0; s < nsteps && rc == CGFD_OK; s++) rc = do_step(c, ev.data(), &nev);
+  HIPCHK(hipEventRecord(ev[2 * nmax + 1], c->st));
+  HIPCHK(hipEventSynchronize(ev[2 * nmax + 1]));
+  float tot = 0, vol = 0;
+  HIPCHK(hipEventElapsedTime(&tot, ev[2 * nmax], ev[2 * nmax + 1]));
+  for (int q = 0; q < nev; q++) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, ev[2 * q], ev[2 * q + 1]));
+    vol += t;
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (ms_total) *ms_total = tot;
+  if (ms_volume) *ms_volume = vol;
+  if (n_volume) *n_volume = nev;
+  return rc;
+}
+
+extern "C" int cgfd_core_step_index(const cgfd_core *c) { return c ? c->it_host : -1; }
+
+extern "C" int cgfd_core_sync(cgfd_core *c) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipGetLastError());
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_wavefield(cgfd_core *c, float *dst) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(dst, c->wlev(c->lv_pre), (size_t)9 * c->g.sicmp * 4, hipMemcpyDeviceToHost));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_write_wavefield(cgfd_core *c, const float *src) {
+  if (!c || !src) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const Grid &g = c->g;
+  std::vector<float> h((size_t)9 * g.sicmp, 0.0f);
+  for (int cmp = 0; cmp < 9; cmp++)
+    for (int k = 3; k < 3 + g.nk; k++)
+      for (int j = 3; j < 3 + g.nj; j++) {
+        size_t o = (size_t)cmp * g.sicmp + (size_t)k * g.siz + (size_t)j * g.siy;
+        std::memcpy(&h[o + 3], &src[o + 3], (size_t)g.ni * 4);
+      }
+  HIPCHK(hipMemcpy(c->wlev(c->lv_pre), h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_fault(cgfd_core *c, int id, float *dst) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  if (id < 0 || id >= c->nfault) return cgfd_set_error(CGFD_ERR_ARG, "fault id %d out of range", id);
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(dst, c->flt[id].out, (size_t)11 * c->g.sslice * 4, hipMemcpyDeviceToHost));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_fault_state(cgfd_core *c, int id, float *dst) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  if (id < 0 || id >= c->nfault) return cgfd_set_error(CGFD_ERR_ARG, "fault id %d out of range", id);
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const float *src = c->flev(c->fl_pre) + (long)id * 18 * c->g.sslice;
+  HIPCHK(hipMemcpy(dst, src, (size_t)18 * c->g.sslice * 4, hipMemcpyDeviceToHost));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_pg(cgfd_core *c, float *dst) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  size_t n = (size_t)15 * c->g.siz;
+  if (!c->isfree) {
+    std::memset(dst, 0, n * 4);
+    return CGFD_OK;
+  }
+  HIPCHK(hipMemcpy(dst, c->PG, n * 4, hipMemcpyDeviceToHost));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int nt_max) {
+  if (!c || n < 0 || (n > 0 && (!ijk || nt_max <= 0))) return cgfd_set_error(CGFD_ERR_ARG, "bad station arguments");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  std::vector<long> idx(n);
+  for (int s = 0; s < n; s++) {
+    int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
+    if (i < 3 || i > c->g.ni + 2 || j < 3 || j > c->g.nj + 2 || k < 3 || k > c->g.nk + 2)
+      return cgfd_set_error(CGFD_ERR_ARG, "station %d at (%d,%d,%d) is outside the physical grid", s, i, j, k);
+    idx[s] = i + (long)j * c->g.siy + (long)k * c->g.siz;
+  }
+  int rc;
+  if (n > 0) {
+    if ((rc = c->alloc(&c->st_idx, (size_t)n))) return rc;
+    HIPCHK(hipMemcpy(c->st_idx, idx.data(), (size_t)n * sizeof(long), hipMemcpyHostToDevice));
+    if ((rc = c->alloc(&c->st_buf, (size_t)n * nt_max * 9))) return rc;
+    HIPCHK(hipMemset(c->st_buf, 0, (size_t)n * nt_max * 9 * 4));
+  }
+  c->st_n = n;
+  c->st_nt = nt_max;
+  c->st_it0 = c->it_host;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int rec = c->it_host - c->st_it0;
+  if (rec > c->st_nt) rec = c->st_nt;
+  if (nt) *nt = rec;
+  if (c->st_n == 0) return CGFD_OK;
+  std::vector<float> h((size_t)c->st_n * c->st_nt * 9);
+  HIPCHK(hipMemcpy(h.data(), c->st_buf, h.size() * 4, hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->st_n; s++)
+    std::memcpy(dst + (size_t)s * rec * 9, &h[(size_t)s * c->st_nt * 9], (size_t)rec * 9 * 4);
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (variant == 0) variant = 1;
+  if (variant != 1) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  c->variant = variant;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_drv_rk_curv_col_allstep(const cgfd_desc *d, float *wave_out, float *fault_out, float *pg_out) {
+  cgfd_core *c = nullptr;
+  int rc = cgfd_core_create(d, &c);
+  if (rc) return rc;
+  rc = cgfd_core_run(c, d->nt_total);
+  if (!rc && wave_out) rc = cgfd_core_read_wavefield(c, wave_out);
+  for (int id = 0; !rc && fault_out && id < c->nfault; id++)
+    rc = cgfd_core_read_fault(c, id, fault_out + (size_t)id * 11 * c->g.sslice);
+  if (!rc && pg_out) rc = cgfd_core_read_pg(c, pg_out);
+  cgfd_core_destroy(c);
+  return rc;
+}

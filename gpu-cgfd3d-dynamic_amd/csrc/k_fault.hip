@@ -1,0 +1,650 @@
+/*
+ * k_fault.hip -- fault-plane kernels of one RK stage (one thread per fault
+ * node; grid z = fault id).
+ *
+ *  k_wave2fault : welded nodes copy the continuum state onto both split nodes
+ *                 (wave2fault_gpu, transform.cu:100-212).
+ *  k_trial      : trial traction + linear slip-weakening friction
+ *                 (trial_slipweakening_gpu, trial_slipweakening.cu:91-388),
+ *                 the stage accumulation of Tn/Ts1/Ts2
+ *                 (fault_var_stage_update_gpu, fault_wav_t.cu:290-316), and
+ *                 the split-node average written back onto the stage input
+ *                 (fault2wave_gpu, transform.cu:292-376).
+ *  k_fault_update<IDIR> : split-node velocity rhs (fault_velo, :191-429) and
+ *                 stress rhs (fault_stress_F/B, :431-1879) on the six columns
+ *                 around the plane, fused with the RK update of those columns,
+ *                 of the 18 split-node fields (fault_wav_update, fault_wav_t.cu:
+ *                 318-366), fault2wave onto the stage output, and at the last
+ *                 stage the per-step accumulators (fault_var_update_gpu,
+ *                 fault_wav_t.cu:197-264).
+ * The F and B stress variants are one template: with s the split side whose
+ * x-derivative is the one-sided difference into the domain (plus for a
+ * forward x-operator) and o the other side, the formulas of the two
+ * reference kernels coincide.
+ */
+#include "cgfd_dev.h"
+#include "cgfd_kernels.h"
+
+namespace cgfd {
+
+enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+
+/* split-node state: component c (Vx,Vy,Vz,T2x,T2y,T2z,T3x,T3y,T3z), side m */
+DEVI long fidx(int c, int m, long ss) { return (long)(2 * c + m) * ss; }
+
+/* metric matrix [xi eta zeta] as columns (transform.cu:147-149) */
+DEVI void metric_cols(const float *met, long sc, long p, float M[3][3]) {
+  M[0][0] = met[sc + p];     M[0][1] = met[4 * sc + p]; M[0][2] = met[7 * sc + p];
+  M[1][0] = met[2 * sc + p]; M[1][1] = met[5 * sc + p]; M[1][2] = met[8 * sc + p];
+  M[2][0] = met[3 * sc + p]; M[2][1] = met[6 * sc + p]; M[2][2] = met[9 * sc + p];
+}
+
+/* ---------------------------------------------------------------------- */
+__global__ void __launch_bounds__(256) k_wave2fault(FaultArgs a) {
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iz = blockIdx.y * blockDim.y + threadIdx.y;
+  const int id = blockIdx.z;
+  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  const FaultDev &F = a.f[id];
+  const long ss = a.g.sslice, sc = a.g.sicmp;
+  const long pf = (iy + 3) + (long)(iz + 3) * a.g.ny;
+  if (F.united[pf] != 1) return;
+  const long p = F.i0 + (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
+  const float *W = a.w_cur;
+  float M[3][3], S[3][3], Tr[3][3];
+  metric_cols(a.met, sc, p, M);
+  float jac = a.met[p];
+  float txx = W[TXX * sc + p], tyy = W[TYY * sc + p], tzz = W[TZZ * sc + p];
+  float tyz = W[TYZ * sc + p], txz = W[TXZ * sc + p], txy = W[TXY * sc + p];
+  S[0][0] = txx; S[0][1] = txy; S[0][2] = txz;
+  S[1][0] = txy; S[1][1] = tyy; S[1][2] = tyz;
+  S[2][0] = txz; S[2][1] = tyz; S[2][2] = tzz;
+  mm33(S, M, Tr);
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) Tr[r][c] *= jac;
+#pragma unroll
+  for (int c = 0; c < 3; c++) F.T1[(long)c * 7 * ss + 3 * ss + pf] = Tr[c][0];
+  float *fc = a.f_cur + (long)id * 18 * ss;
+  float v[3] = {W[VX * sc + p], W[VY * sc + p], W[VZ * sc + p]};
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      fc[fidx(c, m, ss) + pf] = v[c];
+      fc[fidx(3 + c, m, ss) + pf] = Tr[c][1];
+      fc[fidx(6 + c, m, ss) + pf] = Tr[c][2];
+    }
+  }
+}
+
+/* split-node average back onto the wavefield column i0 (transform.cu:331-372),
+   writing sigma_xz/sigma_yz into swapped arrays unless fix_q1 (the call at
+   transform.cu:274-276 passes Tyz, Txz to parameters named Txz, Tyz) */
+DEVI void fault2wave_node(const FaultArgs &a, const FaultDev &F, long pf, long p, float *w, const float *fl) {
+  const long sc = a.g.sicmp, ss = a.g.sslice;
+  float M[3][3], Tr[3][3], S[3][3];
+  metric_cols(a.met, sc, p, M);
+  float rj = (float)(1.0 / a.met[p]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    Tr[c][0] = F.T1[(long)c * 7 * ss + 3 * ss + pf];
+    Tr[c][1] = (float)((fl[fidx(3 + c, 0, ss) + pf] + fl[fidx(3 + c, 1, ss) + pf]) * 0.5);
+    Tr[c][2] = (float)((fl[fidx(6 + c, 0, ss) + pf] + fl[fidx(6 + c, 1, ss) + pf]) * 0.5);
+  }
+  inv3(M);
+  mm33(Tr, M, S);
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) S[r][c] *= rj;
+#pragma unroll
+  for (int c = 0; c < 3; c++) w[c * sc + p] = (float)((fl[fidx(c, 0, ss) + pf] + fl[fidx(c, 1, ss) + pf]) * 0.5);
+  w[TXX * sc + p] = S[0][0];
+  w[TYY * sc + p] = S[1][1];
+  w[TZZ * sc + p] = S[2][2];
+  w[TXY * sc + p] = (float)((S[0][1] + S[1][0]) * 0.5);
+  float sxz = (float)((S[0][2] + S[2][0]) * 0.5);
+  float syz = (float)((S[1][2] + S[2][1]) * 0.5);
+  w[(a.fix_q1 ? TXZ : TYZ) * sc + p] = sxz;
+  w[(a.fix_q1 ? TYZ : TXZ) * sc + p] = syz;
+}
+
+/* z-derivative of a split-node traction with the odd image across the free
+   surface (trial_slipweakening.cu:202-227) */
+DEVI float dz_free_image(const float *col, long ny, int n_free, int kdir) {
+  float v[7];
+#pragma unroll
+  for (int l = -3; l <= 3; l++) v[l + 3] = col[l * ny];
+  v[n_free] = 0.0f;
+  for (int l = n_free + 1; l < 7; l++) v[l] = -v[2 * n_free - l];
+  return vecc(v + 3, kdir);
+}
+
+/* ---------------------------------------------------------------------- */
+__global__ void __launch_bounds__(128) k_trial(FaultArgs a) {
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iz = blockIdx.y * blockDim.y + threadIdx.y;
+  const int id = blockIdx.z;
+  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  const FaultDev &F = a.f[id];
+  const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny;
+  const long pf = (iy + 3) + (long)(iz + 3) * ny;
+  if (F.united[pf] != 0) return;
+  const int jdir = a.jdir, kdir = a.kdir;
+  const float *W = a.w_cur;
+  const float *met = a.met;
+  const long row = (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
+  const long p0 = F.i0 + row;
+  const float *fc = a.f_cur + (long)id * 18 * ss;
+  const float *fp = a.f_pre + (long)id * 18 * ss;
+  const int km = a.g.nk - (iz + 1);
+  const int n_free = km + 3;
+  const int ry = F.rupy[pf], rz = F.rupz[pf];
+
+  float Mrho[2], R[2][3];
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    float jac0 = met[p0];
+    float rho = F.rho_f[m * ss + pf];
+    Mrho[m] = (float)(0.5 * jac0 * rho);
+    float t1[3][3];  /* [comp][l-1] */
+#pragma unroll
+    for (int l = 1; l <= 3; l++) {
+      long q = p0 + (2 * m - 1) * l;
+      float x1 = met[sc + q], x2 = met[2 * sc + q], x3 = met[3 * sc + q], jl = met[q];
+      float txx = W[TXX * sc + q], tyy = W[TYY * sc + q], tzz = W[TZZ * sc + q];
+      float tyz = W[TYZ * sc + q], txz = W[TXZ * sc + q], txy = W[TXY * sc + q];
+      t1[0][l - 1] = jl * (x1 * txx + x2 * txy + x3 * txz);
+      t1[1][l - 1] = jl * (x1 * txy + x2 * tyy + x3 * tyz);
+      t1[2][l - 1] = jl * (x1 * txz + x2 * tyz + x3 * tzz);
+      long slot = 3 + (2 * m - 1) * l;
+#pragma unroll
+      for (int c = 0; c < 3; c++) F.T1[(long)c * 7 * ss + slot * ss + pf] = t1[c][l - 1];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float *p2 = fc + fidx(3 + c, m, ss) + pf;
+      const float *p3 = fc + fidx(6 + c, m, ss) + pf;
+      float dy = 0.0f, dz = 0.0f;
+      if (ry == 1) dy = mac22(p2, 1, jdir);
+      if (ry == 0) dy = macdrp(p2, 1, jdir);
+      if (rz == 1) dz = mac22(p3, ny, kdir);
+      if (rz == 0) dz = macdrp(p3, ny, kdir);
+      if (a.isfree == 1 && km <= 3) dz = dz_free_image(p3, ny, n_free, kdir);
+      /* image sums (trial_slipweakening.cu:230-256): slot 2,1,0 minus / 4,5,6 plus */
+      float rr = (float)(kA1 * t1[c][0] + kA2 * t1[c][1] + kA3 * t1[c][2]);
+      R[m][c] = (float)(0.5 * ((2 * m - 1) * rr + (dy + dz)));
+    }
+  }
+  float Trial[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float dV = fp[fidx(c, 1, ss) + pf] - fp[fidx(c, 0, ss) + pf];
+    Trial[c] = (float)(2.0 * (Mrho[0] * Mrho[1] * dV / a.dt + Mrho[0] * R[1][c] - Mrho[1] * R[0][c]) /
+                       (kA0 * (Mrho[0] + Mrho[1])));
+  }
+  float vs1[3] = {F.vs[pf], F.vs[ss + pf], F.vs[2 * ss + pf]};
+  float vs2[3] = {F.vs[3 * ss + pf], F.vs[4 * ss + pf], F.vs[5 * ss + pf]};
+  float vn[3] = {met[sc + p0], met[2 * sc + p0], met[3 * sc + p0]};
+  float vn0 = norm3(vn);
+  float jacvec = met[p0] * vn0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) vn[c] /= vn0;
+  float T0[3] = {F.T0[pf], F.T0[ss + pf], F.T0[2 * ss + pf]};
+  float TL[3], Ts[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) TL[c] = Trial[c] / jacvec + T0[c];
+  float Tn0 = dot3(TL, vn);
+#pragma unroll
+  for (int c = 0; c < 3; c++) Ts[c] = TL[c] - vn[c] * Tn0;
+  float Ts0 = norm3(Ts);
+  int ifchange = 0;
+  float Tau_n, Tau_s;
+  if (Tn0 >= 0.0f) { Tau_n = 0.0f; ifchange = 1; }  /* no opening */
+  else Tau_n = Tn0;
+  float mus = F.fric[pf], mud = F.fric[ss + pf], Dc = F.fric[2 * ss + pf], C0 = F.fric[3 * ss + pf];
+  float Slip = F.out[6 * ss + pf];
+  float fr = (Slip <= Dc) ? mus - (mus - mud) * Slip / Dc : mud;
+  float Tau_c = -fr * Tau_n + C0;
+  if (Ts0 >= Tau_c) { Tau_s = Tau_c; F.flag_rup[pf] = 1; ifchange = 1; }
+  else { Tau_s = Ts0; F.flag_rup[pf] = 0; }
+  float Tau[3];
+  if (ifchange == 1) {
+    if (fabsf(Ts0) < 1e-1) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) Tau[c] = Tau_n * vn[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; c++) Tau[c] = Tau_n * vn[c] + (Tau_s / Ts0) * Ts[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; c++) Tau[c] = TL[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    F.T1[(long)c * 7 * ss + 3 * ss + pf] = (ifchange == 1) ? (Tau[c] - T0[c]) * jacvec : Trial[c];
+  float tTs1 = dot3(Tau, vs1), tTs2 = dot3(Tau, vs2), tTn = Tau_n;
+  if (a.istage == 0) {
+    F.out[pf] = a.rkb * tTn; F.out[ss + pf] = a.rkb * tTs1; F.out[2 * ss + pf] = a.rkb * tTs2;
+  } else {
+    F.out[pf] += a.rkb * tTn; F.out[ss + pf] += a.rkb * tTs1; F.out[2 * ss + pf] += a.rkb * tTs2;
+  }
+  /* split-node average onto the stage input (transform: fault2wave_onestage(w_cur)) */
+  fault2wave_node(a, F, pf, p0, a.w_cur, fc);
+}
+
+/* Hooke's law on one column point (sv_curv_col_el_iso_fault_gpu.cu:1126-1154) */
+DEVI void hooke(const float *met, long sc, long p, float l, float m, const float *Dx, const float *Dy, const float *Dz,
+                float *h) {
+  float l2m = (float)(l + 2.0 * m);
+  float xx = met[sc + p], xy = met[2 * sc + p], xz = met[3 * sc + p];
+  float ex = met[4 * sc + p], ey = met[5 * sc + p], ez = met[6 * sc + p];
+  float zx = met[7 * sc + p], zy = met[8 * sc + p], zz = met[9 * sc + p];
+  h[0] = l2m * (xx * Dx[0] + ex * Dy[0] + zx * Dz[0]) + l * (xy * Dx[1] + ey * Dy[1] + zy * Dz[1] + xz * Dx[2] + ez * Dy[2] + zz * Dz[2]);
+  h[1] = l2m * (xy * Dx[1] + ey * Dy[1] + zy * Dz[1]) + l * (xx * Dx[0] + ex * Dy[0] + zx * Dz[0] + xz * Dx[2] + ez * Dy[2] + zz * Dz[2]);
+  h[2] = l2m * (xz * Dx[2] + ez * Dy[2] + zz * Dz[2]) + l * (xx * Dx[0] + ex * Dy[0] + zx * Dz[0] + xy * Dx[1] + ey * Dy[1] + zy * Dz[1]);
+  h[5] = m * (xy * Dx[0] + xx * Dx[1] + ey * Dy[0] + ex * Dy[1] + zy * Dz[0] + zx * Dz[1]); /* Txy */
+  h[4] = m * (xz * Dx[0] + xx * Dx[2] + ez * Dy[0] + ex * Dy[2] + zz * Dz[0] + zx * Dz[2]); /* Txz */
+  h[3] = m * (xz * Dx[1] + xy * Dx[2] + ez * Dy[1] + ey * Dy[2] + zz * Dz[1] + zy * Dz[2]); /* Tyz */
+}
+
+/* RK update of wavefield components [c0, c0+nc) at p with rhs h[] */
+DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
+  const long sc = a.g.sicmp;
+  for (int q = 0; q < nc; q++) {
+    long o = (long)(c0 + q) * sc + p;
+    if (a.istage == 0) {
+      float pv = a.w_pre[o];
+      a.w_tnew[o] = pv + a.ca * h[q];
+      a.w_end[o] = pv + a.cb * h[q];
+    } else if (a.istage < 3) {
+      a.w_tnew[o] = a.w_pre[o] + a.ca * h[q];
+      a.w_end[o] += a.cb * h[q];
+    } else {
+      a.w_end[o] += a.cb * h[q];
+    }
+  }
+}
+
+DEVI void mv_soa(const float *M, long st, const float *x, float *y) { mv3(M, st, x, y); }
+
+template <int IDIR>
+__global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
+  const int iy = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iz = blockIdx.y * blockDim.y + threadIdx.y;
+  const int id = blockIdx.z;
+  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  const FaultDev &F = a.f[id];
+  const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
+  const long pf = (iy + 3) + (long)(iz + 3) * ny;
+  if (F.united[pf] != 0) return;
+  const int jdir = a.jdir, kdir = a.kdir;
+  const int i0 = F.i0;
+  const long row = (long)(iy + 3) * sy + (long)(iz + 3) * sz;
+  const long p0 = i0 + row;
+  const int km = a.g.nk - (iz + 1);
+  const int n_free = km + 3;
+  const bool surf = (a.isfree == 1);
+  const float *met = a.met;
+  const float *W = a.w_cur;
+  const float *fc = a.f_cur + (long)id * 18 * ss;
+  const int ry = F.rupy[pf], rz = F.rupz[pf];
+  const float *T1 = F.T1;
+
+  /* ================= velocity rhs on the six columns ================= */
+  const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
+  const float *Tyz = W + TYZ * sc, *Txz = W + TXZ * sc, *Txy = W + TXY * sc;
+#pragma unroll 1
+  for (int i = i0 - 3; i <= i0 + 3; i++) {
+    const int n = i0 - i;
+    if (n == 0) continue;
+    float v1[3][7], v2[3][7], v3[3][7];
+#pragma unroll
+    for (int l = -3; l <= 3; l++) {
+      long q = (i + l) + row;
+      float jq = met[q], a1 = met[sc + q], a2 = met[2 * sc + q], a3 = met[3 * sc + q];
+      v1[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+      v1[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+      v1[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+      q = i + row + l * sy;
+      jq = met[q]; a1 = met[4 * sc + q]; a2 = met[5 * sc + q]; a3 = met[6 * sc + q];
+      v2[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+      v2[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+      v2[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+      q = i + row + l * sz;
+      jq = met[q]; a1 = met[7 * sc + q]; a2 = met[8 * sc + q]; a3 = met[9 * sc + q];
+      v3[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+      v3[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+      v3[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    }
+    float hv[3];
+    long p = i + row;
+    float rrhojac = a.slw[p] / met[p];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float *v = v1[c];
+      v[n + 3] = T1[(long)c * 7 * ss + 3 * ss + pf];
+      /* traction image across the fault (fault_gpu.cu:266-291) */
+      if (n == 2) { v[6] = (float)(2.0 * v[5] - v[4]); }
+      if (n == 1) { v[5] = (float)(2.0 * v[4] - v[3]); v[6] = (float)(2.0 * v[4] - v[2]); }
+      if (n == -1) { v[0] = (float)(2.0 * v[2] - v[4]); v[1] = (float)(2.0 * v[2] - v[3]); }
+      if (n == -2) { v[0] = (float)(2.0 * v[1] - v[2]); }
+      if (surf && km <= 3) {
+        float *u = v3[c];
+        u[n_free] = 0.0f;
+        for (int l = n_free + 1; l < 7; l++) u[l] = -u[2 * n_free - l];
+      }
+      float dx = vecc(v1[c] + 3, IDIR), dy = vecc(v2[c] + 3, jdir), dz = vecc(v3[c] + 3, kdir);
+      hv[c] = (dx + dy + dz) * rrhojac;
+    }
+    rk_cols(a, p, 0, 3, hv);
+  }
+
+  /* split-node velocity rhs (fault_gpu.cu:326-425) */
+  float fhV[2][3];
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    float rrhojac = (float)(1.0 / (F.rho_f[m * ss + pf] * met[p0]));
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const float *t = T1 + (long)c * 7 * ss + pf;
+      float dx;
+      if (m == 0) dx = (float)(kA0 * t[3 * ss] - kA1 * t[2 * ss] - kA2 * t[1 * ss] - kA3 * t[0]);
+      else dx = (float)(-kA0 * t[3 * ss] + kA1 * t[4 * ss] + kA2 * t[5 * ss] + kA3 * t[6 * ss]);
+      const float *p2 = fc + fidx(3 + c, m, ss) + pf;
+      const float *p3 = fc + fidx(6 + c, m, ss) + pf;
+      float dy = 0.0f, dz = 0.0f;
+      if (ry == 1) dy = mac22(p2, 1, jdir);
+      if (ry == 0) dy = macdrp(p2, 1, jdir);
+      if (rz == 1) dz = mac22(p3, ny, kdir);
+      if (rz == 0) dz = macdrp(p3, ny, kdir);
+      if (surf && km <= 3) dz = dz_free_image(p3, ny, n_free, kdir);
+      fhV[m][c] = (dx + dy + dz) * rrhojac;
+    }
+  }
+
+  /* ================= stress rhs ================= */
+  const float *V[3] = {W, W + sc, W + 2 * sc};
+  const float *fV[3] = {fc + fidx(0, 0, ss), fc + fidx(1, 0, ss), fc + fidx(2, 0, ss)};
+  const long pm = pf, pp = pf + ss;
+  float DxV[8][3], DyV[8][3], DzV[8][3];
+  constexpr int S_ = IDIR == 1 ? 4 : 3;  /* side with the one-sided x difference */
+  constexpr int O_ = 7 - S_;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (IDIR == 1) {  /* A, B, C (fault_gpu.cu:518-543) */
+      DxV[7][c] = macdrp_f(V[c] + (i0 + 3) + row, 1);
+      DxV[6][c] = mac24(V[c] + (i0 + 2) + row, 1, 1);
+      DxV[5][c] = mac22(V[c] + (i0 + 1) + row, 1, 1);
+    } else {  /* G, F, E (:1248-1274) */
+      DxV[0][c] = macdrp_b(V[c] + (i0 - 3) + row, 1);
+      DxV[1][c] = mac24(V[c] + (i0 - 2) + row, 1, 0);
+      DxV[2][c] = mac22(V[c] + (i0 - 1) + row, 1, 0);
+    }
+    if (ry == 1) { DyV[4][c] = mac22(fV[c] + pp, 1, jdir); DyV[3][c] = mac22(fV[c] + pm, 1, jdir); }
+    else { DyV[4][c] = macdrp(fV[c] + pp, 1, jdir); DyV[3][c] = macdrp(fV[c] + pm, 1, jdir); }
+    if (rz == 1) { DzV[4][c] = mac22(fV[c] + pp, ny, kdir); DzV[3][c] = mac22(fV[c] + pm, ny, kdir); }
+    else { DzV[4][c] = macdrp(fV[c] + pp, ny, kdir); DzV[3][c] = macdrp(fV[c] + pm, ny, kdir); }
+  }
+  const float *Ms = (IDIR == 1 ? F.P2M : F.M2P) + pf; /* 5 matrices, stride 9*ss between them */
+  const float *M2 = F.M2 + pf;                        /* T1toVx,VytoVx,VztoVx (minus), then plus */
+  const long ms = ss, mq = 9 * ss;
+  float dtT1[3] = {0.0f, 0.0f, 0.0f};
+  if (F.faultgrid[pf] == 1) {
+    dtT1[0] = F.hT1[pf]; dtT1[1] = F.hT1[ss + pf]; dtT1[2] = F.hT1[2 * ss + pf];
+  }
+  float o1[3], o2[3], o3[3], o4[3], o5[3];
+  /* method 2 on both sides with the full matrices (fault_gpu.cu:662-707) */
+  auto method2 = [&]() {
+    mv_soa(M2 + 3 * mq, ms, dtT1, o1); mv_soa(M2 + 4 * mq, ms, DyV[4], o2); mv_soa(M2 + 5 * mq, ms, DzV[4], o3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) DxV[4][c] = o1[c] - o2[c] - o3[c];
+    mv_soa(M2 + 0 * mq, ms, dtT1, o1); mv_soa(M2 + 1 * mq, ms, DyV[3], o2); mv_soa(M2 + 2 * mq, ms, DzV[3], o3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) DxV[3][c] = o1[c] - o2[c] - o3[c];
+  };
+  /* method 1: one-sided difference on side s, continuity maps it to side o */
+  auto method1 = [&]() {
+    mv_soa(Ms + 0 * mq, ms, DxV[S_], o1); mv_soa(Ms + 1 * mq, ms, DyV[S_], o2); mv_soa(Ms + 2 * mq, ms, DzV[S_], o3);
+    mv_soa(Ms + 3 * mq, ms, DyV[O_], o4); mv_soa(Ms + 4 * mq, ms, DzV[O_], o5);
+#pragma unroll
+    for (int c = 0; c < 3; c++) DxV[O_][c] = o1[c] + o2[c] + o3[c] - o4[c] - o5[c];
+  };
+  if (a.imethod == 1) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (IDIR == 1) DxV[4][c] = V[c][(i0 + 1) + row] - fV[c][pp];
+      else DxV[3][c] = fV[c][pm] - V[c][(i0 - 1) + row];
+    }
+    method1();
+  } else {
+    method2();
+  }
+  if (surf && km == 0) {
+    /* split nodes on the free surface (fault_gpu.cu:709-813 / :1427-1517) */
+    const long fs_ = ny;  /* SoA stride of the free-surface row */
+    const float *Fm = F.Fm + (iy + 3);
+    const long fq = 9 * ny;
+    if (a.imethod == 1) {
+      const float *Mf = Fm + (IDIR == 1 ? 4 : 7) * fq;
+      mv_soa(Mf, fs_, DxV[S_], o1); mv_soa(Mf + fq, fs_, DyV[S_], o2); mv_soa(Mf + 2 * fq, fs_, DyV[O_], o3);
+#pragma unroll
+      for (int c = 0; c < 3; c++) DxV[O_][c] = o1[c] + o2[c] - o3[c];
+    } else {
+      mv_soa(Fm + 12 * fq, fs_, dtT1, o1); mv_soa(Fm + 13 * fq, fs_, DyV[4], o2);
+#pragma unroll
+      for (int c = 0; c < 3; c++) DxV[4][c] = o1[c] - o2[c];
+      mv_soa(Fm + 10 * fq, fs_, dtT1, o1); mv_soa(Fm + 11 * fq, fs_, DyV[3], o2);
+#pragma unroll
+      for (int c = 0; c < 3; c++) DxV[3][c] = o1[c] - o2[c];
+    }
+    mv_soa(Fm + 2 * fq, fs_, DxV[4], o1); mv_soa(Fm + 3 * fq, fs_, DyV[4], o2);
+#pragma unroll
+    for (int c = 0; c < 3; c++) DzV[4][c] = o1[c] + o2[c];
+    mv_soa(Fm + 0 * fq, fs_, DxV[3], o1); mv_soa(Fm + 1 * fq, fs_, DyV[3], o2);
+#pragma unroll
+    for (int c = 0; c < 3; c++) DzV[3][c] = o1[c] + o2[c];
+  }
+  if (surf && (km == 1 || km == 2)) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if (km == 1) { DzV[4][c] = mac22(fV[c] + pp, ny, kdir); DzV[3][c] = mac22(fV[c] + pm, ny, kdir); }
+      else { DzV[4][c] = mac24(fV[c] + pp, ny, kdir); DzV[3][c] = mac24(fV[c] + pm, ny, kdir); }
+    }
+    if (a.imethod == 1) method1();
+    else method2();
+  }
+  /* traction rates of the split nodes (fault_gpu.cu:936-1018) */
+  float fhT2[2][3], fhT3[2][3];
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+    const int q = m == 1 ? 4 : 3;
+    const float *Db = F.D + (long)(m * 6) * mq + pf;
+    float g1[3], g2[3], g3[3];
+    mv_soa(Db + 0 * mq, ms, DxV[q], g1); mv_soa(Db + 1 * mq, ms, DyV[q], g2); mv_soa(Db + 2 * mq, ms, DzV[q], g3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) fhT2[m][c] = g1[c] + g2[c] + g3[c];
+    mv_soa(Db + 3 * mq, ms, DxV[q], g1); mv_soa(Db + 4 * mq, ms, DyV[q], g2); mv_soa(Db + 5 * mq, ms, DzV[q], g3);
+#pragma unroll
+    for (int c = 0; c < 3; c++) fhT3[m][c] = g1[c] + g2[c] + g3[c];
+  }
+  /* x differences next to the fault on side o, using that side's velocity */
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (IDIR == 1) {  /* E, F, G (fault_gpu.cu:1020-1063) */
+      float vm = fV[c][pm];
+      DxV[2][c] = vm - V[c][(i0 - 1) + row];
+      float t3[3] = {V[c][(i0 - 2) + row], V[c][(i0 - 1) + row], vm};
+      DxV[1][c] = vec24_f(t3);
+      float t5[5] = {V[c][(i0 - 4) + row], V[c][(i0 - 3) + row], V[c][(i0 - 2) + row], V[c][(i0 - 1) + row], vm};
+      DxV[0][c] = vecdrp_f(t5);
+    } else {  /* C, B, A (:1694-1728) */
+      float vp = fV[c][pp];
+      DxV[5][c] = V[c][(i0 + 1) + row] - vp;
+      float t3[3] = {vp, V[c][(i0 + 1) + row], V[c][(i0 + 2) + row]};
+      DxV[6][c] = vec24_b(t3);
+      float t5[5] = {vp, V[c][(i0 + 1) + row], V[c][(i0 + 2) + row], V[c][(i0 + 3) + row], V[c][(i0 + 4) + row]};
+      DxV[7][c] = vecdrp_b(t5);
+    }
+  }
+  /* Hooke on the six columns (:1065-1155) with the RK update */
+#pragma unroll 1
+  for (int n = 0; n < 8; n++) {
+    const int mm = n < 4 ? n - 3 : n - 4;
+    if (mm == 0) continue;
+    const long p = (i0 + mm) + row;
+    float dx[3], dy[3], dz[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      dx[c] = DxV[n][c];
+      dy[c] = macdrp(V[c] + p, sy, jdir);
+      dz[c] = macdrp(V[c] + p, sz, kdir);
+    }
+    if (surf && km == 0) {
+      long ij = ((long)(i0 + mm) + (long)(iy + 3) * sy) * 9;
+      const float *X = a.matVx2Vz + ij, *Y = a.matVy2Vz + ij;
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+        dz[r] = X[3 * r] * dx[0] + X[3 * r + 1] * dx[1] + X[3 * r + 2] * dx[2] + Y[3 * r] * dy[0] + Y[3 * r + 1] * dy[1] +
+                Y[3 * r + 2] * dy[2];
+    }
+    if (surf && km == 1) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) dz[c] = mac22(V[c] + p, sz, kdir);
+    }
+    if (surf && km == 2) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) dz[c] = mac24(V[c] + p, sz, kdir);
+    }
+    float ht[6];
+    hooke(met, sc, p, a.lam[p], a.mu[p], dx, dy, dz, ht);
+    rk_cols(a, p, TXX, 6, ht);
+  }
+
+  /* ================= split-node state RK update ================= */
+  float *fpre = a.f_pre + (long)id * 18 * ss;
+  float *fnew = a.istage < 3 ? a.f_tnew + (long)id * 18 * ss : nullptr;
+  float *fend = a.f_end + (long)id * 18 * ss;
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      float r = c < 3 ? fhV[m][c] : (c < 6 ? fhT2[m][c - 3] : fhT3[m][c - 6]);
+      long o = fidx(c, m, ss) + pf;
+      if (a.istage == 0) {
+        float pv = fpre[o];
+        fnew[o] = pv + a.ca * r;
+        fend[o] = pv + a.cb * r;
+      } else if (a.istage < 3) {
+        fnew[o] = fpre[o] + a.ca * r;
+        fend[o] += a.cb * r;
+      } else {
+        fend[o] += a.cb * r;
+      }
+    }
+  /* split-node average onto the stage output (drv_rk_curv_col.cu:356-359, 541-544) */
+  if (a.istage < 3) fault2wave_node(a, F, pf, p0, a.w_tnew, fnew);
+  else fault2wave_node(a, F, pf, p0, a.w_end, fend);
+
+  if (a.istage == 3) {
+    /* per-step accumulators (fault_var_update_gpu, fault_wav_t.cu:219-262) */
+    float dV[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) dV[c] = fend[fidx(c, 1, ss) + pf] - fend[fidx(c, 0, ss) + pf];
+    float Vs1 = dV[0] * F.vs[pf] + dV[1] * F.vs[ss + pf] + dV[2] * F.vs[2 * ss + pf];
+    float Vs2 = dV[0] * F.vs[3 * ss + pf] + dV[1] * F.vs[4 * ss + pf] + dV[2] * F.vs[5 * ss + pf];
+    float Vs = sqrtf(Vs2 * Vs2 + Vs1 * Vs1);
+    float dt = a.dt;
+    F.out[3 * ss + pf] = Vs;
+    F.out[4 * ss + pf] = Vs1;
+    F.out[5 * ss + pf] = Vs2;
+    F.out[6 * ss + pf] += Vs * dt;
+    F.out[7 * ss + pf] += Vs1 * dt;
+    F.out[8 * ss + pf] += Vs2 * dt;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float t = T1[(long)c * 7 * ss + 3 * ss + pf];
+      F.hT1[c * ss + pf] = (t - F.mT1[c * ss + pf]) / dt;
+      F.mT1[c * ss + pf] = t;
+    }
+    if (Vs > F.out[9 * ss + pf]) F.out[9 * ss + pf] = Vs;
+    if (F.init_t0_flag[pf] == 0) {
+      if (Vs > 1e-3) {
+        F.out[10 * ss + pf] = (*a.it_dev + 1) * dt;
+        F.init_t0_flag[pf] = 1;
+        F.flag_rup[pf] = 1;
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+__global__ void k_step_end(int *it_dev) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *it_dev += 1;
+}
+
+__global__ void k_zero(float *p, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0.0f;
+}
+
+__global__ void k_nan_check(const float *p, long n, int *flag) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  int bad = 0;
+  for (; i < n; i += stride) bad |= !isfinite(p[i]);
+  if (bad) atomicOr(flag, 1);
+}
+
+/* receivers (io_recv_keep, drv_rk_curv_col.cu:215): 9 components of level pre */
+__global__ void k_record(const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max, const int *it_dev,
+                         int it0) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 9) return;
+  int s = t / 9, c = t % 9;
+  int it = *it_dev - it0;
+  if (it < 0 || it >= nt_max) return;
+  buf[((long)s * nt_max + it) * 9 + c] = w[c * sicmp + idx[s]];
+}
+
+/* ---------------------------------------------------------------------- */
+void launch_wave2fault(const FaultArgs &a, hipStream_t st) {
+  if (a.nfault <= 0) return;
+  dim3 blk(64, 4, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault);
+  hipLaunchKernelGGL(k_wave2fault, grd, blk, 0, st, a);
+}
+void launch_trial(const FaultArgs &a, hipStream_t st) {
+  if (a.nfault <= 0) return;
+  dim3 blk(64, 2, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 1) / 2, a.nfault);
+  hipLaunchKernelGGL(k_trial, grd, blk, 0, st, a);
+}
+void launch_fault_update(const FaultArgs &a, hipStream_t st) {
+  if (a.nfault <= 0) return;
+  dim3 blk(64, 1, 1), grd((a.g.nj + 63) / 64, a.g.nk, a.nfault);
+  if (a.idir == 1) hipLaunchKernelGGL(k_fault_update<1>, grd, blk, 0, st, a);
+  else hipLaunchKernelGGL(k_fault_update<0>, grd, blk, 0, st, a);
+}
+void launch_step_end(int *it_dev, hipStream_t st) { hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, it_dev); }
+void launch_zero(float *p, long n, hipStream_t st) {
+  if (n <= 0) return;
+  long blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+}
+void launch_nan_check(const float *p, long n, int *flag, hipStream_t st) {
+  long blocks = (n + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_nan_check, dim3((unsigned)blocks), dim3(256), 0, st, p, n, flag);
+}
+void launch_record_stations(const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max,
+                            const int *it_dev, int it0, hipStream_t st) {
+  if (n <= 0) return;
+  int t = n * 9;
+  hipLaunchKernelGGL(k_record, dim3((t + 255) / 256), dim3(256), 0, st, w, sicmp, idx, n, buf, nt_max, it_dev, it0);
+}
+
+}  // namespace cgfd

@@ -1,0 +1,423 @@
+/*
+ * k_volume.hip -- fused volume kernels of one RK stage.
+ *
+ * One launch computes, for every point it owns, what the reference computes
+ * in four passes per stage:
+ *   1. the interior velocity-stress rhs (sv_curv_col_el_iso_rhs_inner_gpu,
+ *      sv_curv_col_el_iso_gpu.cu:231-388);
+ *   2. on the top four layers with a free surface, the traction-image hV
+ *      (timg, :398-600) and the reduced-order hT (vlow, :606-747);
+ *   3. the CFS-PML correction and auxiliary rhs of every face containing the
+ *      point (:824-1290), in the reference's face order;
+ *   4. the RK4 stage update of the wavefield and PML auxiliaries
+ *      (wav_update/_end, wav_t.cu:218-234, driven by drv_rk_curv_col.cu:329-568)
+ * so the rhs never goes to HBM. Stage input (cur) and stage output (tnew) are
+ * different buffers, which makes the in-place stencil hazard of a fused
+ * update impossible. Points on the seven fault columns of non-welded fault
+ * nodes are left to the fault kernel, which overwrites their rhs in the
+ * reference (sv_curv_col_el_iso_fault_gpu.cu:11-189).
+ */
+#include "cgfd_dev.h"
+#include "cgfd_kernels.h"
+
+namespace cgfd {
+
+/* component order Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy (wav_t.cu:56-106) */
+enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
+
+/* interior rhs at a point (sv_curv_col_el_iso_gpu.cu:349-386) */
+DEVI void rhs_interior(const float *Dx, const float *Dy, const float *Dz, float xx, float xy, float xz, float ex,
+                       float ey, float ez, float zx, float zy, float zz, float l, float m, float s, float *h) {
+  float l2m = l + 2.0f * m;
+  h[VX] = s * (xx * Dx[TXX] + xy * Dx[TXY] + xz * Dx[TXZ] + ex * Dy[TXX] + ey * Dy[TXY] + ez * Dy[TXZ] + zx * Dz[TXX] + zy * Dz[TXY] + zz * Dz[TXZ]);
+  h[VY] = s * (xx * Dx[TXY] + xy * Dx[TYY] + xz * Dx[TYZ] + ex * Dy[TXY] + ey * Dy[TYY] + ez * Dy[TYZ] + zx * Dz[TXY] + zy * Dz[TYY] + zz * Dz[TYZ]);
+  h[VZ] = s * (xx * Dx[TXZ] + xy * Dx[TYZ] + xz * Dx[TZZ] + ex * Dy[TXZ] + ey * Dy[TYZ] + ez * Dy[TZZ] + zx * Dz[TXZ] + zy * Dz[TYZ] + zz * Dz[TZZ]);
+  h[TXX] = l2m * (xx * Dx[VX] + ex * Dy[VX] + zx * Dz[VX]) + l * (xy * Dx[VY] + ey * Dy[VY] + zy * Dz[VY] + xz * Dx[VZ] + ez * Dy[VZ] + zz * Dz[VZ]);
+  h[TYY] = l2m * (xy * Dx[VY] + ey * Dy[VY] + zy * Dz[VY]) + l * (xx * Dx[VX] + ex * Dy[VX] + zx * Dz[VX] + xz * Dx[VZ] + ez * Dy[VZ] + zz * Dz[VZ]);
+  h[TZZ] = l2m * (xz * Dx[VZ] + ez * Dy[VZ] + zz * Dz[VZ]) + l * (xx * Dx[VX] + ex * Dy[VX] + zx * Dz[VX] + xy * Dx[VY] + ey * Dy[VY] + zy * Dz[VY]);
+  h[TXY] = m * (xy * Dx[VX] + xx * Dx[VY] + ey * Dy[VX] + ex * Dy[VY] + zy * Dz[VX] + zx * Dz[VY]);
+  h[TXZ] = m * (xz * Dx[VX] + xx * Dx[VZ] + ez * Dy[VX] + ex * Dy[VZ] + zz * Dz[VX] + zx * Dz[VZ]);
+  h[TYZ] = m * (xz * Dx[VY] + xy * Dx[VZ] + ez * Dy[VY] + ey * Dy[VZ] + zz * Dz[VY] + zy * Dz[VZ]);
+}
+
+/* CFS-PML correction of every face containing (i,j,k) plus the auxiliary
+   RK update (sv_curv_col_el_iso_gpu.cu:916-1288, drv_rk_curv_col.cu:368-567) */
+DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *Dx, const float *Dy, const float *Dz,
+                    float l, float m, float s, float *h) {
+  const long sc = a.g.sicmp;
+#pragma unroll 1
+  for (int dim = 0; dim < 3; dim++) {
+#pragma unroll 1
+    for (int side = 0; side < 2; side++) {
+      const PmlFace &f = a.pml[dim][side];
+      if (!f.on) continue;
+      if (i < f.i1 || i > f.i2 || j < f.j1 || j > f.j2 || k < f.k1 || k > f.k2) continue;
+      long ia = (long)(k - f.k1) * ((long)f.bn * f.an) + (long)(j - f.j1) * f.an + (i - f.i1);
+      int ic = dim == 0 ? i - f.i1 : (dim == 1 ? j - f.j1 : k - f.k1);
+      float cD = f.D[ic], cA = f.A[ic], cB = f.B[ic];
+      float cB1 = cB - 1.0f;
+      const float *D = dim == 0 ? Dx : (dim == 1 ? Dy : Dz);
+      const float *mr = a.met + (long)(1 + 3 * dim) * sc;
+      float e1 = mr[p], e2 = mr[p + sc], e3 = mr[p + 2 * sc];
+      float l2m = l + 2.0f * m;
+      float r[9];
+      r[VX] = s * (e1 * D[TXX] + e2 * D[TXY] + e3 * D[TXZ]);
+      r[VY] = s * (e1 * D[TXY] + e2 * D[TYY] + e3 * D[TYZ]);
+      r[VZ] = s * (e1 * D[TXZ] + e2 * D[TYZ] + e3 * D[TZZ]);
+      r[TXX] = l2m * e1 * D[VX] + l * e2 * D[VY] + l * e3 * D[VZ];
+      r[TYY] = l * e1 * D[VX] + l2m * e2 * D[VY] + l * e3 * D[VZ];
+      r[TZZ] = l * e1 * D[VX] + l * e2 * D[VY] + l2m * e3 * D[VZ];
+      r[TXY] = m * (e2 * D[VX] + e1 * D[VY]);
+      r[TXZ] = m * (e3 * D[VX] + e1 * D[VZ]);
+      r[TYZ] = m * (e3 * D[VY] + e2 * D[VZ]);
+      float *auxc = (a.istage == 0 ? f.pre : f.tmp);
+      float av[9], ar[9];
+#pragma unroll
+      for (int c = 0; c < 9; c++) av[c] = auxc[c * f.n + ia];
+#pragma unroll
+      for (int c = 0; c < 9; c++) {
+        h[c] += cB1 * r[c] - cB * av[c];
+        ar[c] = cD * r[c] - cA * av[c];
+      }
+      if (dim < 2 && a.isfree && k == a.g.nk2) {
+        /* free-surface contribution on x/y faces (:1000-1060, :1146-1205) */
+        long ij = ((long)i + (long)j * a.g.siy) * 9;
+        const float *X = (dim == 0 ? a.matVx2Vz : a.matVy2Vz) + ij;
+        float q0 = X[0] * D[VX] + X[1] * D[VY] + X[2] * D[VZ];
+        float q1 = X[3] * D[VX] + X[4] * D[VY] + X[5] * D[VZ];
+        float q2 = X[6] * D[VX] + X[7] * D[VY] + X[8] * D[VZ];
+        float zx = a.met[7 * sc + p], zy = a.met[8 * sc + p], zz = a.met[9 * sc + p];
+        float qT[9];
+        qT[TXX] = l2m * (zx * q0) + l * (zy * q1 + zz * q2);
+        qT[TYY] = l2m * (zy * q1) + l * (zx * q0 + zz * q2);
+        qT[TZZ] = l2m * (zz * q2) + l * (zx * q0 + zy * q1);
+        qT[TXY] = m * (zy * q0 + zx * q1);
+        qT[TXZ] = m * (zz * q0 + zx * q2);
+        qT[TYZ] = m * (zz * q1 + zy * q2);
+#pragma unroll
+        for (int c = 3; c < 9; c++) {
+          h[c] = (float)(h[c] + (cB - 1.0) * qT[c]);
+          ar[c] += cD * qT[c];
+        }
+      }
+      /* auxiliary RK update; the slab point is owned by this thread only */
+      if (a.istage == 0) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          float pv = f.pre[c * f.n + ia];
+          f.tmp[c * f.n + ia] = pv + a.ca * ar[c];
+          f.end[c * f.n + ia] = pv + a.cb * ar[c];
+        }
+      } else if (a.istage < 3) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          f.tmp[c * f.n + ia] = f.pre[c * f.n + ia] + a.ca * ar[c];
+          f.end[c * f.n + ia] += a.cb * ar[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) f.end[c * f.n + ia] += a.cb * ar[c];
+      }
+    }
+  }
+}
+
+DEVI bool fault_column(const FaultSkip &fs, int i, long pf) {
+#pragma unroll 1
+  for (int q = 0; q < fs.nfault; q++) {
+    int d = i - fs.i0[q];
+    if (d >= -3 && d <= 3 && fs.united[q][pf] == 0) return true;
+  }
+  return false;
+}
+
+/* RK stage update of the nine wavefield components at p */
+DEVI void rk_point(const VolArgs &a, long p, const float *h) {
+  const long sc = a.g.sicmp;
+  if (a.istage == 0) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      float pv = a.pre[c * sc + p];
+      a.tnew[c * sc + p] = pv + a.ca * h[c];
+      a.end[c * sc + p] = pv + a.cb * h[c];
+    }
+  } else if (a.istage < 3) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      a.tnew[c * sc + p] = a.pre[c * sc + p] + a.ca * h[c];
+      a.end[c * sc + p] += a.cb * h[c];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 9; c++) a.end[c * sc + p] += a.cb * h[c];
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * Direct-load volume kernel: one thread per point, neighbours from the
+ * cache hierarchy. Handles physical planes k_lo .. k_lo+nk_run-1.
+ * ---------------------------------------------------------------------- */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_direct(VolArgs a) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ix >= a.g.ni || iy >= a.g.nj) return;
+  const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  float Dx[9], Dy[9], Dz[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float *q = a.cur + c * sc + p;
+    Dx[c] = dfd<ID>(q, 1);
+    Dy[c] = dfd<JD>(q, sy);
+    Dz[c] = dfd<KD>(q, sz);
+  }
+  const float *M = a.met;
+  float xx = M[sc + p], xy = M[2 * sc + p], xz = M[3 * sc + p];
+  float ex = M[4 * sc + p], ey = M[5 * sc + p], ez = M[6 * sc + p];
+  float zx = M[7 * sc + p], zy = M[8 * sc + p], zz = M[9 * sc + p];
+  float l = a.lam[p], m = a.mu[p], s = a.slw[p];
+  float h[9];
+  rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
+  if (a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h);
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  rk_point(a, p, h);
+}
+
+/* ------------------------------------------------------------------------
+ * Top-layer kernel (free surface): physical planes nk-4 .. nk-1, reproducing
+ * the overwrite order inner -> timg -> vlow -> cfspml of the reference.
+ * ---------------------------------------------------------------------- */
+__device__ const int kOpIdx[2][5] = {{-3, -2, -1, 0, 1}, {-1, 0, 1, 2, 3}};
+
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_top(VolArgs a) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ix >= a.g.ni || iy >= a.g.nj) return;
+  const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
+  const int nk2 = a.g.nk2;
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  float Dx[9], Dy[9], Dz[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float *q = a.cur + c * sc + p;
+    Dx[c] = dfd<ID>(q, 1);
+    Dy[c] = dfd<JD>(q, sy);
+    Dz[c] = dfd<KD>(q, sz);
+  }
+  const float *M = a.met;
+  const float *jac = M, *xi_x = M + sc, *xi_y = M + 2 * sc, *xi_z = M + 3 * sc;
+  const float *et_x = M + 4 * sc, *et_y = M + 5 * sc, *et_z = M + 6 * sc;
+  const float *zt_x = M + 7 * sc, *zt_y = M + 8 * sc, *zt_z = M + 9 * sc;
+  float xx = xi_x[p], xy = xi_y[p], xz = xi_z[p];
+  float ex = et_x[p], ey = et_y[p], ez = et_z[p];
+  float zx = zt_x[p], zy = zt_y[p], zz = zt_z[p];
+  float l = a.lam[p], m = a.mu[p], s = a.slw[p];
+  float h[9];
+  rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
+  if (a.isfree) {
+    const int *fx = kOpIdx[ID], *fy = kOpIdx[JD], *fz = kOpIdx[KD];
+    const int k_min = nk2 - fz[4];
+    if (k >= k_min) {
+      /* timg: conservative tractions with an odd image above the surface */
+      const int n_free = nk2 - k - fz[0];
+      const float slwjac = a.slw[p] / jac[p];
+      const float *T = a.cur;
+      const float *Ta[3][3] = {{T + TXX * sc, T + TXY * sc, T + TXZ * sc},
+                               {T + TXY * sc, T + TYY * sc, T + TYZ * sc},
+                               {T + TXZ * sc, T + TYZ * sc, T + TZZ * sc}};
+#pragma unroll 1
+      for (int c = 0; c < 3; c++) {
+        const float *A = Ta[c][0], *B = Ta[c][1], *C = Ta[c][2];
+        float vx[5], ve[5], vz[5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          long q = p + fx[n];
+          vx[n] = jac[q] * (xi_x[q] * A[q] + xi_y[q] * B[q] + xi_z[q] * C[q]);
+          q = p + fy[n] * sy;
+          ve[n] = jac[q] * (et_x[q] * A[q] + et_y[q] * B[q] + et_z[q] * C[q]);
+        }
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          if (n < n_free) {
+            long q = p + fz[n] * sz;
+            vz[n] = jac[q] * (zt_x[q] * A[q] + zt_y[q] * B[q] + zt_z[q] * C[q]);
+          } else if (n == n_free) {
+            vz[n] = 0.0f;
+          } else {
+            long q = p + (long)(fz[n] - 2 * (n - n_free)) * sz;
+            vz[n] = -jac[q] * (zt_x[q] * A[q] + zt_y[q] * B[q] + zt_z[q] * C[q]);
+          }
+        }
+        h[c] = (vecdrp(vx, ID) + vecdrp(ve, JD) + vecdrp(vz, KD)) * slwjac;
+      }
+    }
+    if (k >= nk2 - 2) {
+      /* vlow: reduced-order z derivative of V near the surface */
+      const float *Vx = a.cur + p, *Vy = a.cur + sc + p, *Vz = a.cur + 2 * sc + p;
+      float dv[3][3];  /* [axis][comp] */
+      dv[0][0] = macdrp(Vx, 1, ID); dv[1][0] = macdrp(Vx, sy, JD);
+      dv[0][1] = macdrp(Vy, 1, ID); dv[1][1] = macdrp(Vy, sy, JD);
+      dv[0][2] = macdrp(Vz, 1, ID); dv[1][2] = macdrp(Vz, sy, JD);
+      if (k == nk2) {
+        long ij = ((long)i + (long)j * sy) * 9;
+        const float *X = a.matVx2Vz + ij, *Y = a.matVy2Vz + ij;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+          dv[2][r] = X[3 * r] * dv[0][0] + X[3 * r + 1] * dv[0][1] + X[3 * r + 2] * dv[0][2] + Y[3 * r] * dv[1][0] +
+                     Y[3 * r + 1] * dv[1][1] + Y[3 * r + 2] * dv[1][2];
+      } else if (k == nk2 - 1) {
+        dv[2][0] = mac22(Vx, sz, KD); dv[2][1] = mac22(Vy, sz, KD); dv[2][2] = mac22(Vz, sz, KD);
+      } else {
+        dv[2][0] = mac24(Vx, sz, KD); dv[2][1] = mac24(Vy, sz, KD); dv[2][2] = mac24(Vz, sz, KD);
+      }
+      float l2m = (float)(l + 2.0 * m);
+      float DxVx = dv[0][0], DyVx = dv[1][0], DzVx = dv[2][0];
+      float DxVy = dv[0][1], DyVy = dv[1][1], DzVy = dv[2][1];
+      float DxVz = dv[0][2], DyVz = dv[1][2], DzVz = dv[2][2];
+      h[TXX] = l2m * (xx * DxVx + ex * DyVx + zx * DzVx) + l * (xy * DxVy + ey * DyVy + zy * DzVy + xz * DxVz + ez * DyVz + zz * DzVz);
+      h[TYY] = l2m * (xy * DxVy + ey * DyVy + zy * DzVy) + l * (xx * DxVx + ex * DyVx + zx * DzVx + xz * DxVz + ez * DyVz + zz * DzVz);
+      h[TZZ] = l2m * (xz * DxVz + ez * DyVz + zz * DzVz) + l * (xx * DxVx + ex * DyVx + zx * DzVx + xy * DxVy + ey * DyVy + zy * DzVy);
+      h[TXY] = m * (xy * DxVx + xx * DxVy + ey * DyVx + ex * DyVy + zy * DzVx + zx * DzVy);
+      h[TXZ] = m * (xz * DxVx + xx * DxVz + ez * DyVx + ex * DyVz + zz * DzVx + zx * DzVz);
+      h[TYZ] = m * (xz * DxVy + xy * DxVz + ez * DyVy + ey * DyVz + zz * DzVy + zy * DzVz);
+    }
+  }
+  if (a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h);
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  rk_point(a, p, h);
+}
+
+/* ------------------------------------------------------------------------
+ * free-surface conversion matrices (sv_curv_col_el_iso_dvh2dvz_gpu,
+ * sv_curv_col_el_iso_gpu.cu:1296-1405)
+ * ---------------------------------------------------------------------- */
+__global__ void k_dvh2dvz(Grid g, const float *__restrict__ met, const float *__restrict__ lam,
+                          const float *__restrict__ mu, float *matVx2Vz, float *matVy2Vz) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ix >= g.ni || iy >= g.nj) return;
+  const long sc = g.sicmp;
+  long p = (ix + 3) + (long)(iy + 3) * g.siy + (long)g.nk2 * g.siz;
+  float e11 = met[sc + p], e12 = met[2 * sc + p], e13 = met[3 * sc + p];
+  float e21 = met[4 * sc + p], e22 = met[5 * sc + p], e23 = met[6 * sc + p];
+  float e31 = met[7 * sc + p], e32 = met[8 * sc + p], e33 = met[9 * sc + p];
+  float l = lam[p], m = mu[p];
+  float l2m = l + 2.0f * m;
+  float A[3][3], B[3][3], C[3][3], AB[3][3], AC[3][3];
+  A[0][0] = l2m * e31 * e31 + m * (e32 * e32 + e33 * e33);
+  A[0][1] = l * e31 * e32 + m * e32 * e31;
+  A[0][2] = l * e31 * e33 + m * e33 * e31;
+  A[1][0] = l * e32 * e31 + m * e31 * e32;
+  A[1][1] = l2m * e32 * e32 + m * (e31 * e31 + e33 * e33);
+  A[1][2] = l * e32 * e33 + m * e33 * e32;
+  A[2][0] = l * e33 * e31 + m * e31 * e33;
+  A[2][1] = l * e33 * e32 + m * e32 * e33;
+  A[2][2] = l2m * e33 * e33 + m * (e31 * e31 + e32 * e32);
+  inv3(A);
+  B[0][0] = -l2m * e31 * e11 - m * (e32 * e12 + e33 * e13);
+  B[0][1] = -l * e31 * e12 - m * e32 * e11;
+  B[0][2] = -l * e31 * e13 - m * e33 * e11;
+  B[1][0] = -l * e32 * e11 - m * e31 * e12;
+  B[1][1] = -l2m * e32 * e12 - m * (e31 * e11 + e33 * e13);
+  B[1][2] = -l * e32 * e13 - m * e33 * e12;
+  B[2][0] = -l * e33 * e11 - m * e31 * e13;
+  B[2][1] = -l * e33 * e12 - m * e32 * e13;
+  B[2][2] = -l2m * e33 * e13 - m * (e31 * e11 + e32 * e12);
+  C[0][0] = -l2m * e31 * e21 - m * (e32 * e22 + e33 * e23);
+  C[0][1] = -l * e31 * e22 - m * e32 * e21;
+  C[0][2] = -l * e31 * e23 - m * e33 * e21;
+  C[1][0] = -l * e32 * e21 - m * e31 * e22;
+  C[1][1] = -l2m * e32 * e22 - m * (e31 * e21 + e33 * e23);
+  C[1][2] = -l * e32 * e23 - m * e33 * e22;
+  C[2][0] = -l * e33 * e21 - m * e31 * e23;
+  C[2][1] = -l * e33 * e22 - m * e32 * e23;
+  C[2][2] = -l2m * e33 * e23 - m * (e31 * e21 + e32 * e22);
+  mm33(A, B, AB);
+  mm33(A, C, AC);
+  long ij = ((long)(iy + 3) * g.siy + (ix + 3)) * 9;
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      matVx2Vz[ij + 3 * r + c] = AB[r][c];
+      matVy2Vz[ij + 3 * r + c] = AC[r][c];
+    }
+}
+
+/* surface PGV/PGA/PGD (PG_calcu_gpu, wav_t.cu:137-216) */
+__global__ void k_pg(Grid g, const float *__restrict__ wend, const float *__restrict__ wpre, float *PG, float *Dis,
+                     float dt) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ix >= g.ni || iy >= g.nj) return;
+  const long sc = g.sicmp, siz = g.siz;
+  long p = (ix + 3) + (long)(iy + 3) * g.siy + (long)g.nk2 * siz;
+  long q = (ix + 3) + (long)(iy + 3) * g.siy;
+  float vx1 = wend[p], vy1 = wend[sc + p], vz1 = wend[2 * sc + p];
+  float vx0 = wpre[p], vy0 = wpre[sc + p], vz0 = wpre[2 * sc + p];
+  float Ax = fabsf((vx1 - vx0) / dt), Ay = fabsf((vy1 - vy0) / dt), Az = fabsf((vz1 - vz0) / dt);
+  float dx = Dis[q], dy = Dis[siz + q], dz = Dis[2 * siz + q];
+  dx = (float)(dx + 0.5 * (vx1 + vx0) * dt);
+  dy = (float)(dy + 0.5 * (vy1 + vy0) * dt);
+  dz = (float)(dz + 0.5 * (vz1 + vz0) * dt);
+  Dis[q] = dx; Dis[siz + q] = dy; Dis[2 * siz + q] = dz;
+  float V = sqrtf(vx1 * vx1 + vy1 * vy1 + vz1 * vz1);
+  float Vh = sqrtf(vx1 * vx1 + vy1 * vy1);
+  float A = sqrtf(Ax * Ax + Ay * Ay + Az * Az);
+  float Ah = sqrtf(Ax * Ax + Ay * Ay);
+  float D = sqrtf(dx * dx + dy * dy + dz * dz);
+  float Dh = sqrtf(dx * dx + dy * dy);
+  float vals[15] = {V, Vh, fabsf(vx1), fabsf(vy1), fabsf(vz1), A, Ah, Ax, Ay, Az, D, Dh, fabsf(dx), fabsf(dy), fabsf(dz)};
+#pragma unroll
+  for (int c = 0; c < 15; c++) {
+    float o = PG[c * siz + q];
+    if (o < vals[c]) PG[c * siz + q] = vals[c];
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* host launchers */
+template <int ID, int JD, int KD>
+static void launch_dir(int variant, const VolArgs &a, int top, hipStream_t st) {
+  (void)variant;
+  dim3 blk(64, 4, 1);
+  dim3 grd((a.g.ni + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);
+  if (top) {
+    hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), grd, blk, 0, st, a);
+    return;
+  }
+  hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), grd, blk, 0, st, a);
+}
+
+void launch_volume(int variant, const VolArgs &a, int idir, int jdir, int kdir, int top, hipStream_t st) {
+  int key = idir * 4 + jdir * 2 + kdir;
+  switch (key) {
+    case 0: launch_dir<0, 0, 0>(variant, a, top, st); break;
+    case 1: launch_dir<0, 0, 1>(variant, a, top, st); break;
+    case 2: launch_dir<0, 1, 0>(variant, a, top, st); break;
+    case 3: launch_dir<0, 1, 1>(variant, a, top, st); break;
+    case 4: launch_dir<1, 0, 0>(variant, a, top, st); break;
+    case 5: launch_dir<1, 0, 1>(variant, a, top, st); break;
+    case 6: launch_dir<1, 1, 0>(variant, a, top, st); break;
+    default: launch_dir<1, 1, 1>(variant, a, top, st); break;
+  }
+}
+
+void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
+                    hipStream_t st) {
+  dim3 blk(16, 16, 1);
+  dim3 grd((g.ni + 15) / 16, (g.nj + 15) / 16, 1);
+  hipLaunchKernelGGL(k_dvh2dvz, grd, blk, 0, st, g, met, lam, mu, mx, my);
+}
+
+void launch_pg(const Grid &g, const float *wend, const float *wpre, float *PG, float *Dis, float dt, hipStream_t st) {
+  dim3 blk(64, 4, 1);
+  dim3 grd((g.ni + 63) / 64, (g.nj + 3) / 4, 1);
+  hipLaunchKernelGGL(k_pg, grd, blk, 0, st, g, wend, wpre, PG, Dis, dt);
+}
+
+}  // namespace cgfd

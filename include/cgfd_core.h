@@ -1,0 +1,199 @@
+/*
+ * cgfd_core.h -- C ABI of the MI355X-native time-stepping core.
+ *
+ * This is the drop-in boundary for the reference's
+ *   int drv_rk_curv_col_allstep(fd_t*, gd_t*, gd_metric_t*, md_t*, par_t*,
+ *                               bdryfree_t*, bdrypml_t*, bdryexp_t*, wav_t*,
+ *                               mympi_t*, fault_coef_t*, fault_t*, fault_wav_t*,
+ *                               iorecv_t*, ioline_t*, iofault_t*, ioslice_t*,
+ *                               iosnap_t*, io_fault_recv_t*, float dt,
+ *                               int nt_total, float t0, char*, char*)
+ *   (src/forward/drv_rk_curv_col.h:18-42, body drv_rk_curv_col.cu:28-667),
+ * called once from main (src/forward/main_curv_col_el_3d.cu:711-718).
+ *
+ * Everything crosses the boundary as plain host pointers in the reference's
+ * host layouts:
+ *   - volume arrays: nx*ny*nz floats, index i + j*nx + k*nx*ny, nx = ni+6
+ *     (3 ghost points per side, gd_t.cu:2151-2153);
+ *   - fault-plane arrays: ny*nz values, index j + k*ny (fault_info.cu:166);
+ *   - 3x3 matrices: 9 consecutive floats per node, row-major
+ *     (fault_info.cu:220-228).
+ * The library copies what it needs to the GPU in create() and retains no host
+ * pointer afterwards. All functions return CGFD_OK (0) or a negative status;
+ * cgfd_last_error() gives the message of the last failure on the calling
+ * thread. Nothing in the library calls exit().
+ *
+ * One core object per host thread; a core owns its device memory and its HIP
+ * stream. Objects are not internally thread-safe.
+ */
+#ifndef CGFD_CORE_H
+#define CGFD_CORE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGFD_MAX_FAULTS 5   /* fault_t.fault_one[5], fault_info.h:129 */
+#define CGFD_NCMP 9         /* Vx,Vy,Vz,Txx,Tyy,Tzz,Tyz,Txz,Txy (wav_t.cu:56-106) */
+#define CGFD_NGHOST 3       /* fd_t.cu:63-65 */
+#define CGFD_NFAULT_OUT 11  /* Tn,Ts1,Ts2,Vs,Vs1,Vs2,Slip,Slip1,Slip2,Peak_vs,Init_t0 */
+#define CGFD_NPG 15         /* PGV,PGVh,PGVx,PGVy,PGVz,PGA..,PGD.. (wav_t.cu:160-174) */
+
+/* wavefield component order (wav_t.cu:56-106) */
+enum { CGFD_VX = 0, CGFD_VY, CGFD_VZ, CGFD_TXX, CGFD_TYY, CGFD_TZZ,
+       CGFD_TYZ, CGFD_TXZ, CGFD_TXY };
+
+/* status codes */
+#define CGFD_OK          0
+#define CGFD_ERR_ARG    -1   /* invalid argument / inconsistent descriptor */
+#define CGFD_ERR_HIP    -2   /* HIP runtime error (message has the HIP string) */
+#define CGFD_ERR_NOMEM  -3   /* device or host allocation failed */
+#define CGFD_ERR_STATE  -4   /* call not valid in the object's state */
+#define CGFD_ERR_NAN    -5   /* non-finite value detected by the NaN check */
+#define CGFD_ERR_NODEV  -6   /* no usable GPU */
+
+/*
+ * One split-node fault: fault_t.fault_one[id] + fault_coef_t.fault_coef_one[id]
+ * as fault_set/fault_coef_cal leave them (fault_info.cu:97-1074). Plane arrays
+ * have ny*nz entries (ghosts included; only physical nodes are read).
+ */
+typedef struct cgfd_fault_desc {
+  int fault_x_index;        /* 0-based physical x index (par_t.cu:323) */
+  /* fault_one_t inputs (fault_info.h:89-95) */
+  const float *T0x, *T0y, *T0z, *mu_s, *mu_d, *Dc, *C0;
+  /* masks (fault_info.h:113-116) */
+  const int *united, *faultgrid, *rup_index_y, *rup_index_z;
+  /* initial values of Tn, Ts1, Ts2 (fault_info.cu:1015-1023); may be NULL -> 0 */
+  const float *Tn0, *Ts10, *Ts20;
+  /* fault_coef_one_t (fault_info.h:11-76) */
+  const float *rho_f;             /* [2][ny*nz], side 0 minus, 1 plus (density) */
+  const float *vec_s1, *vec_s2;   /* [ny*nz][3] strike / dip unit vectors */
+  /* D21_1,D22_1,D23_1,D31_1,D32_1,D33_1,D21_2,D22_2,D23_2,D31_2,D32_2,D33_2 */
+  const float *D[12];             /* each [ny*nz][9] */
+  const float *matPlus2Min[5];    /* matPlus2Min1..5, [ny*nz][9] */
+  const float *matMin2Plus[5];    /* matMin2Plus1..5, [ny*nz][9] */
+  /* matT1toVx_Min, matVytoVx_Min, matVztoVx_Min,
+     matT1toVx_Plus, matVytoVx_Plus, matVztoVx_Plus */
+  const float *matM2[6];          /* each [ny*nz][9] */
+  /* free-surface row (k = nk2), each [ny][9]:
+     matVx2Vz1, matVy2Vz1, matVx2Vz2, matVy2Vz2,
+     matPlus2Min1f, matPlus2Min2f, matPlus2Min3f,
+     matMin2Plus1f, matMin2Plus2f, matMin2Plus3f,
+     matT1toVxf_Min, matVytoVxf_Min, matT1toVxf_Plus, matVytoVxf_Plus */
+  const float *matF[14];
+} cgfd_fault_desc;
+
+/*
+ * Everything drv_rk_curv_col_allstep reads from its struct arguments on the
+ * single-process path.
+ */
+typedef struct cgfd_desc {
+  int ni, nj, nk;           /* gd_t ni,nj,nk (physical points) */
+  /* gd_metric_t v4d: 10 x nx*ny*nz in the order jac, xi_x, xi_y, xi_z,
+     eta_x, eta_y, eta_z, zeta_x, zeta_y, zeta_z (gd_t.cu:150-190) */
+  const float *metric;
+  /* md_t: lambda, mu and rho, where rho already holds 1/rho
+     (md_rho_to_slow, main_curv_col_el_3d.cu:705) */
+  const float *lambda, *mu, *rho;
+  /* bdryfree_t is_sides_free[2][1] (only the top is implemented upstream) */
+  int free_top;
+  /* bdrypml_t (bdry_t.h:97-123): per face [dim][side] */
+  int pml_is_sides[3][2];
+  int pml_ni1[3][2], pml_ni2[3][2], pml_nj1[3][2], pml_nj2[3][2];
+  int pml_nk1[3][2], pml_nk2[3][2];   /* ghost-inclusive index ranges */
+  const float *pml_A[3][2], *pml_B[3][2], *pml_D[3][2];  /* nlay+1 each */
+  /* par_t */
+  int imethod;              /* dynamic_method, 1 or 2 (par_t.cu:313-315) */
+  int number_fault;
+  cgfd_fault_desc fault[CGFD_MAX_FAULTS];
+  float dt, t0;
+  int nt_total;             /* number_of_time_steps + 1 (main:479) */
+  int device_id;            /* HIP device ordinal */
+  int qc_check_nan_every;   /* >0: check the wavefield for NaN every n steps
+                               (the reference only prints, drv_rk_curv_col.cu:594-596) */
+  int compat_flags;         /* 0 = reproduce the reference exactly; see below */
+} cgfd_desc;
+
+/* compat_flags bits */
+#define CGFD_COMPAT_FIX_F2W_SWAP 1  /* write sigma_xz into Txz and sigma_yz into Tyz
+                                       in fault2wave; the reference swaps them
+                                       (transform.cu:274-276 vs :295) */
+
+typedef struct cgfd_core cgfd_core;
+
+/* last error message of the calling thread ("" if none) */
+const char *cgfd_last_error(void);
+
+/* library version string */
+const char *cgfd_version(void);
+
+/* Allocates device memory, uploads the descriptor's arrays, computes the
+   free-surface conversion matrices (drv_rk_curv_col.cu:184-201) and zeroes
+   the state exactly as alloc.cu does. */
+int cgfd_core_create(const cgfd_desc *desc, cgfd_core **out);
+
+int cgfd_core_destroy(cgfd_core *core);
+
+/* Advances nsteps time steps from the current step counter (the body of the
+   time loop, drv_rk_curv_col.cu:209-636, without the I/O hooks). */
+int cgfd_core_run(cgfd_core *core, int nsteps);
+
+/* Current step counter (number of completed steps). */
+int cgfd_core_step_index(const cgfd_core *core);
+
+/* Blocks until all queued work of the core has finished. */
+int cgfd_core_sync(cgfd_core *core);
+
+/* Copies the current wavefield (level "pre" = state at t0 + it*dt) into
+   host_dst[9][nx*ny*nz], reference layout. */
+int cgfd_core_read_wavefield(cgfd_core *core, float *host_dst);
+
+/* Overwrites the current wavefield (level "pre") from host_src[9][nx*ny*nz].
+   Ghost points are forced to zero (the reference keeps them at zero on one
+   process, alloc.cu:448-457). */
+int cgfd_core_write_wavefield(cgfd_core *core, const float *host_src);
+
+/* Fault outputs of fault `id`: host_dst[11][ny*nz] in the order
+   Tn,Ts1,Ts2,Vs,Vs1,Vs2,Slip,Slip1,Slip2,Peak_vs,Init_t0 (fault_info.cu:865-942). */
+int cgfd_core_read_fault(cgfd_core *core, int id, float *host_dst);
+
+/* Fault split-node state of fault `id` at level "pre": host_dst[9][2][ny*nz]
+   (Vx,Vy,Vz,T2x,T2y,T2z,T3x,T3y,T3z; side 0 minus, 1 plus). */
+int cgfd_core_read_fault_state(cgfd_core *core, int id, float *host_dst);
+
+/* Surface peak values host_dst[15][nx*ny] (wav_t.cu:160-174); zero if the
+   top is not free. */
+int cgfd_core_read_pg(cgfd_core *core, float *host_dst);
+
+/* Receivers: n stations at grid indices (ghost-inclusive i,j,k). The core
+   records Vx..Txy of level "pre" at every step before the step is taken
+   (io_recv_keep, drv_rk_curv_col.cu:215) into a device buffer of nt_max
+   samples. */
+int cgfd_core_set_stations(cgfd_core *core, int n, const int *ijk, int nt_max);
+/* host_dst[n][nt_recorded][9]; returns the number of recorded samples in *nt */
+int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
+
+/* Timing support: runs nsteps and returns the total elapsed device time of
+   the stream (HIP events) in *ms_total and the summed duration of the volume
+   stencil kernel launches in *ms_volume (events bracket each launch), with the
+   number of volume launches in *n_volume. */
+int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
+                        float *ms_volume, int *n_volume);
+
+/* Kernel variant selection for the volume stencil (0 = default/tuned,
+   1 = direct-load reference kernel, 2 = LDS-tiled z-marching kernel).
+   Returns CGFD_ERR_ARG for an unknown variant. */
+int cgfd_core_set_variant(cgfd_core *core, int variant);
+
+/* The one-call drop-in: create + run nt_total steps + copy the final state
+   into the optional outputs (NULL to skip) + destroy. */
+int cgfd_drv_rk_curv_col_allstep(const cgfd_desc *desc, float *wave_out,
+                                 float *fault_out, float *pg_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGFD_CORE_H */

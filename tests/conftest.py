@@ -1,0 +1,25 @@
+"""Shared fixtures. GPU tests are marked ``gpu``; everything else runs on CPU."""
+import os
+import sys
+from pathlib import Path
+
+import pytest
+
+REPO = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(REPO / "gpu-cgfd3d-dynamic_amd"))
+sys.path.insert(0, str(REPO))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+    config.addinivalue_line("markers", "slow: longer CPU oracle runs")
+
+
+@pytest.fixture(scope="session")
+def built_libs():
+    from cgfd import build
+    if not build.LIB_PATH.exists():
+        build.build_core()
+    if not build.ORACLE_LIB.exists():
+        build.build_oracle()
+    return build.LIB_PATH, build.ORACLE_LIB

@@ -1,0 +1,142 @@
+"""GPU core vs the CPU oracle (FP32 restatement of the reference) on seeded
+small configurations that exercise every path of the core: interior stencil,
+CFS-PML on five or six faces, free-surface traction imaging and vlow,
+the split-node fault with slip-weakening friction (both dynamic methods),
+topography, two faults, and the fault2wave compatibility switch.
+
+Tolerances (stated by the target): relative L2 <= 1e-3 for wavefield
+snapshots, final slip and peak slip rate; rupture time within one dt.
+"""
+import numpy as np
+import pytest
+
+from helpers import fault_phys, interior, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+
+
+def _run_pair(sc, nsteps, compat=0, double_too=False):
+    from cgfd import model
+    import oracle
+    st = model.Setup(sc)
+    core = model.Core(st, compat_flags=compat)
+    orc = oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=False, fix_q1=bool(compat & 1))
+    core.run(nsteps)
+    orc.run(nsteps)
+    out = {"gpu": core, "ora": orc, "setup": st}
+    if double_too:
+        o64 = oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=True, fix_q1=bool(compat & 1))
+        o64.run(nsteps)
+        out["o64"] = o64
+    return out
+
+
+def _check(res, sc, label):
+    core, orc = res["gpu"], res["ora"]
+    wg = interior(core.read_wavefield())
+    wo = interior(orc.read_wavefield())
+    assert np.isfinite(wg).all(), f"{label}: non-finite GPU wavefield"
+    names = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]
+    for c in range(9):
+        r = rel_l2(wg[c], wo[c])
+        assert r <= TOL, f"{label}: {names[c]} rel L2 {r:.3e} > {TOL}"
+    for fid in range(len(sc.faults)):
+        fg = fault_phys(core.read_fault(fid))
+        fo = fault_phys(orc.read_fault(fid))
+        r_slip = rel_l2(fg[6], fo[6])
+        r_pv = rel_l2(fg[9], fo[9])
+        assert r_slip <= TOL, f"{label}: fault {fid} slip rel L2 {r_slip:.3e}"
+        assert r_pv <= TOL, f"{label}: fault {fid} peak slip rate rel L2 {r_pv:.3e}"
+        rup_o = fo[10] > 0
+        rup_g = fg[10] > 0
+        assert (rup_o == rup_g).mean() >= 0.995, f"{label}: ruptured-node sets differ"
+        both = rup_o & rup_g
+        if both.any():
+            dtt = np.abs(fg[10][both] - fo[10][both])
+            assert dtt.max() <= sc.dt * 1.0001 + 1e-6, f"{label}: rupture time off by {dtt.max():.4g} s"
+    return wg, wo
+
+
+def test_small_flat_method2(built_libs):
+    from cgfd import configs
+    sc = configs.small(imethod=2)
+    res = _run_pair(sc, 80)
+    _check(res, sc, "flat m2")
+    f = fault_phys(res["ora"].read_fault(0))
+    assert (f[10] > 0).sum() > 50, "the test case must rupture"
+
+
+def test_small_flat_method1(built_libs):
+    from cgfd import configs
+    sc = configs.small(imethod=1)
+    res = _run_pair(sc, 80)
+    _check(res, sc, "flat m1")
+
+
+def test_small_curvilinear(built_libs):
+    from cgfd import configs
+    sc = configs.small(imethod=2, curvilinear=True)
+    res = _run_pair(sc, 60)
+    _check(res, sc, "curvilinear")
+
+
+def test_no_free_surface_six_pml_faces(built_libs):
+    from cgfd import configs
+    sc = configs.small(imethod=2, free_top=False)
+    res = _run_pair(sc, 60)
+    _check(res, sc, "buried fault, 6 PML faces")
+
+
+def test_two_faults(built_libs):
+    from cgfd import configs
+    sc = configs.small(ni=48, imethod=2, nfault=2)
+    res = _run_pair(sc, 50)
+    _check(res, sc, "two faults")
+
+
+def test_fault2wave_swap_fix(built_libs):
+    from cgfd import configs
+    sc = configs.small(imethod=1)
+    res = _run_pair(sc, 40, compat=1)
+    _check(res, sc, "compat fix q1")
+
+
+def test_pg_and_stations(built_libs):
+    from cgfd import configs, model
+    import oracle
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    ijk = sc.stations + 3
+    core.set_stations(ijk, 64)
+    orc = oracle.Oracle(st.desc, sc.shape_ext, 1)
+    rec_o = []
+    for _ in range(40):
+        w = orc.read_wavefield()
+        rec_o.append([w[:, k, j, i] for (i, j, k) in ijk])
+        orc.run(1)
+    core.run(40)
+    rec_g = core.read_stations()
+    rec_o = np.transpose(np.array(rec_o), (1, 0, 2))
+    assert rec_g.shape == rec_o.shape
+    assert rel_l2(rec_g, rec_o) <= TOL
+    pg_g = core.read_pg()[:, 3:-3, 3:-3]
+    pg_o = orc.read_pg()[:, 3:-3, 3:-3]
+    for c in range(15):
+        assert rel_l2(pg_g[c], pg_o[c]) <= TOL, f"PG component {c}"
+
+
+def test_fp64_oracle_error_statement(built_libs):
+    """The FP32 GPU result vs the FP64 oracle: reported, bounded loosely."""
+    from cgfd import configs
+    sc = configs.small(imethod=2)
+    res = _run_pair(sc, 60, double_too=True)
+    wg = interior(res["gpu"].read_wavefield())
+    w64 = interior(res["o64"].read_wavefield())
+    w32 = interior(res["ora"].read_wavefield())
+    e_gpu = rel_l2(wg, w64)
+    e_ora = rel_l2(w32, w64)
+    print(f"rel L2 vs FP64: gpu {e_gpu:.3e}, fp32 oracle {e_ora:.3e}")
+    assert e_gpu <= TOL
