@@ -50,6 +50,24 @@ def read_traffic(profile_dir: Path):
     return float(m.group(1)) if m else None
 
 
+def calib_copy(mbytes: int) -> None:
+    """One device-to-device copy of a known size (the reference point that
+    scales the FETCH_SIZE/WRITE_SIZE counters of a profiling run)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mbytes << 20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), n) or hip.hipMalloc(C.byref(dst), n):
+        raise RuntimeError("hipMalloc failed in calib_copy")
+    hip.hipMemset(src, 0, n)
+    hip.hipDeviceSynchronize()
+    hip.hipMemcpy(dst, src, n, 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"[bench] calibration copy of {mbytes} MiB done")
+
+
 def cpu_baseline(threads: int, steps: int = 3):
     """The oracle (FP32 CPU restatement, OpenMP) on a bounded sample: the same
     configuration generator at 1/4 linear scale (150 x 300 x 100 nodes)."""
@@ -77,8 +95,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the workload (1.0 = 1200x600x400)")
-    ap.add_argument("--variant", type=int, default=0, help="volume kernel variant (0 = default)")
+    ap.add_argument("--variant", type=int, default=0, help="volume kernel variant (0 = autotune)")
+    ap.add_argument("--zchunk", type=int, default=0, help="z planes per block (0 = autotune/default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -87,6 +107,8 @@ def main() -> int:
         return 2
     rank = int(os.environ.get("RANK", "0"))
 
+    if args.calib_copy:
+        calib_copy(args.calib_copy)
     from cgfd import configs, model
     t0 = time.perf_counter()
     sc = configs.config2(scale=args.scale, steps=args.steps + args.warmup)
@@ -96,8 +118,16 @@ def main() -> int:
     log(f"[bench] host setup in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     core = model.Core(st, variant=args.variant)
+    if args.zchunk:
+        core.set_zchunk(args.zchunk)
     core.sync()
     log(f"[bench] device upload in {time.perf_counter() - t0:.1f} s")
+    tuned = None
+    if args.variant == 0:
+        t0 = time.perf_counter()
+        tuned = core.autotune(3)
+        log(f"[bench] autotune: variant {tuned[0]}, zchunk {tuned[1]}, {tuned[2]:.3f} ms per stage-0 launch "
+            f"({time.perf_counter() - t0:.1f} s)")
     # free host copies (the core holds everything in HBM)
     del sc.x, sc.y, sc.z, sc.lam, sc.mu, sc.rho
     if args.warmup:
@@ -138,7 +168,9 @@ def main() -> int:
         "config": {"workload": f"{sc.name}: plain-method rupture, {nj}x{ni}x{nk} nodes (strike x normal x depth) "
                                f"at 50 m, curvilinear, PML 20, free surface, 1 fault, FP32",
                    "grid": [ni, nj, nk], "parallelism": "single GPU", "hbm_frac_768B_model": round(hbm_frac, 4),
-                   "max_slip_m": round(slip_max, 4)},
+                   "max_slip_m": round(slip_max, 4),
+                   "volume_kernel": {"variant": tuned[0] if tuned else args.variant,
+                                     "zchunk": tuned[1] if tuned else args.zchunk, "autotuned": tuned is not None}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_vol (volume stencil + PML + RK update)",

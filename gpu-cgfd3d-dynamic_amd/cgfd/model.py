@@ -146,6 +146,15 @@ class Core:
     def set_variant(self, v: int):
         L.check(self.lib, self.lib.cgfd_core_set_variant(self.handle, v))
 
+    def set_zchunk(self, z: int):
+        L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
+
+    def autotune(self, reps: int = 3):
+        """Pick the fastest volume-kernel variant and z chunk for this grid."""
+        v, z, ms = C.c_int(), C.c_int(), C.c_float()
+        L.check(self.lib, self.lib.cgfd_core_autotune(self.handle, reps, C.byref(v), C.byref(z), C.byref(ms)))
+        return v.value, z.value, ms.value
+
     def run(self, nsteps: int):
         L.check(self.lib, self.lib.cgfd_core_run(self.handle, nsteps))
 

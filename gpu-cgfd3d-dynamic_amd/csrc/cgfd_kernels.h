@@ -40,7 +40,19 @@ struct FaultArgs {
   const int *it_dev;                        /* completed steps (device counter) */
 };
 
+/* tile lists of the z-queue volume kernel (interior tiles / tiles touching a
+   CFS-PML slab); tile id = bx + ntx*(by + nty*bz) */
+struct ZqTiles {
+  static constexpr int TX = 64, TY = 4;
+  int ntx = 0, nty = 0, ntz = 0;
+  int n_int = 0, n_pml = 0;
+  int *d_int = nullptr, *d_pml = nullptr;
+};
+
+void launch_vol_zq(const VolArgs &a, const ZqTiles &t, int minw, int idir, int jdir, int kdir, hipStream_t st);
+void launch_vol_pipe(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st);
 void launch_volume(int variant, const VolArgs &a, int idir, int jdir, int kdir, int top, hipStream_t st);
+void launch_vol_lds(const VolArgs &a, int idir, int jdir, int kdir, hipStream_t st);
 void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
                     hipStream_t st);
 void launch_pg(const Grid &g, const float *wend, const float *wpre, float *PG, float *Dis, float dt, hipStream_t st);

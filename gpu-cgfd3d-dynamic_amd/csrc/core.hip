@@ -15,7 +15,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
+#include <algorithm>
 #include <cstring>
+#include <map>
 #include <new>
 #include <vector>
 
@@ -56,7 +59,11 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
-  int variant = 1;
+  int variant = 4;
+  int zchunk = 32;
+  ZqTiles zq;                       /* tile lists for the current zchunk */
+  std::map<int, ZqTiles> zq_cache;  /* by zchunk */
+  int nvol = 0;
   DevBuf mem;
 
   float *wave = nullptr;   /* 4 levels x 9 x sicmp */
@@ -146,6 +153,52 @@ static int check_desc(const cgfd_desc *d) {
 static void soa9(std::vector<float> &dst, size_t off, const float *src, size_t nodes) {
   for (size_t p = 0; p < nodes; p++)
     for (int e = 0; e < 9; e++) dst[off + e * nodes + p] = src[p * 9 + e];
+}
+
+/* tile lists of the z-queue volume kernel over physical planes [0, nvol) */
+static int build_zq_tiles(cgfd_core *c, int nvol) {
+  auto it = c->zq_cache.find(c->zchunk);
+  if (it != c->zq_cache.end()) {
+    c->zq = it->second;
+    return CGFD_OK;
+  }
+  ZqTiles t;
+  const Grid &g = c->g;
+  const int TX = ZqTiles::TX, TY = ZqTiles::TY, zc = c->zchunk;
+  t.ntx = (g.ni + TX - 1) / TX;
+  t.nty = (g.nj + TY - 1) / TY;
+  t.ntz = nvol > 0 ? (nvol + zc - 1) / zc : 0;
+  std::vector<int> ti, tp;
+  for (int bz = 0; bz < t.ntz; bz++)
+    for (int by = 0; by < t.nty; by++)
+      for (int bx = 0; bx < t.ntx; bx++) {
+        int i0 = bx * TX + 3, i1 = std::min(bx * TX + TX, g.ni) - 1 + 3;
+        int j0 = by * TY + 3, j1 = std::min(by * TY + TY, g.nj) - 1 + 3;
+        int k0 = bz * zc + 3, k1 = std::min(bz * zc + zc, nvol) - 1 + 3;
+        bool pml = false;
+        for (int dim = 0; dim < 3; dim++)
+          for (int side = 0; side < 2; side++) {
+            const PmlFace &f = c->pml[dim][side];
+            if (!f.on) continue;
+            if (i1 >= f.i1 && i0 <= f.i2 && j1 >= f.j1 && j0 <= f.j2 && k1 >= f.k1 && k0 <= f.k2) pml = true;
+          }
+        int id = bx + t.ntx * (by + t.nty * bz);
+        (pml ? tp : ti).push_back(id);
+      }
+  t.n_int = (int)ti.size();
+  t.n_pml = (int)tp.size();
+  int rc;
+  if (t.n_int) {
+    if ((rc = c->alloc(&t.d_int, ti.size()))) return rc;
+    HIPCHK(hipMemcpy(t.d_int, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
+  }
+  if (t.n_pml) {
+    if ((rc = c->alloc(&t.d_pml, tp.size()))) return rc;
+    HIPCHK(hipMemcpy(t.d_pml, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
+  }
+  c->zq_cache[c->zchunk] = t;
+  c->zq = t;
+  return CGFD_OK;
 }
 
 static int upload_faults(cgfd_core *c, const cgfd_desc *d) {
@@ -327,6 +380,11 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
     if (hipMemset(c->fwav, 0, n * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
     if ((rc = upload_faults(c, d))) return fail(rc);
   }
+  {
+    const int ntop = c->isfree ? (g.nk < 4 ? g.nk : 4) : 0;
+    c->nvol = g.nk - ntop;
+    if ((rc = build_zq_tiles(c, c->nvol))) return fail(rc);
+  }
   if ((rc = c->alloc(&c->it_dev, 2))) return fail(rc);
   if (hipMemset(c->it_dev, 0, 8) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
   c->nan_flag = c->it_dev + 1;
@@ -373,7 +431,7 @@ void fill_vol(cgfd_core *c, VolArgs &v, int istage, const float *cur, float *tne
   v.cb = kRkB[istage] * c->dt;
   v.fs.nfault = c->nfault;
   for (int q = 0; q < c->nfault; q++) { v.fs.i0[q] = c->flt[q].i0; v.fs.united[q] = c->flt[q].united; }
-  v.zchunk = 0;
+  v.zchunk = c->zchunk;
 }
 
 void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int kdir, float *wcur, float *wnew,
@@ -392,6 +450,14 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.cb = kRkB[istage] * c->dt;
   f.rkb = kRkB[istage];
   f.it_dev = c->it_dev;
+}
+
+/* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
+   no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march */
+void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
+  if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
+  else if (c->variant >= 3) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
+  else launch_volume(c->variant, va, idir, jdir, kdir, 0, c->st);
 }
 
 /* one time step; if ev != nullptr, event pairs bracket the volume launches */
@@ -428,7 +494,7 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
       va.k_lo = 0;
       va.nk_run = nvol;
       if (ev) HIPCHK(hipEventRecord(ev[2 * (*nev)], c->st));
-      launch_volume(c->variant, va, idir, jdir, kdir, 0, c->st);
+      launch_vol(c, va, idir, jdir, kdir);
       if (ev) {
         HIPCHK(hipEventRecord(ev[2 * (*nev) + 1], c->st));
         (*nev)++;
@@ -609,9 +675,82 @@ extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
 
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
-  if (variant == 0) variant = 1;
-  if (variant != 1) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant == 0) variant = 4;
+  if (variant < 1 || variant > 6) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_zchunk(cgfd_core *c, int zchunk) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (zchunk < 4 || zchunk > 4096) return cgfd_set_error(CGFD_ERR_ARG, "zchunk %d out of range [4, 4096]", zchunk);
+  HIPCHK(hipSetDevice(c->dev));
+  c->zchunk = zchunk;
+  return build_zq_tiles(c, c->nvol);
+}
+
+extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int *zchunk_out, float *ms_out) {
+  if (!c || reps < 1) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  HIPCHK(hipSetDevice(c->dev));
+  if (c->nvol <= 0) {
+    if (variant_out) *variant_out = c->variant;
+    if (zchunk_out) *zchunk_out = c->zchunk;
+    if (ms_out) *ms_out = 0.0f;
+    return CGFD_OK;
+  }
+  const int ipair = c->it_host % 8;
+  const int idir = kPairs[ipair][0], jdir = kPairs[ipair][1], kdir = kPairs[ipair][2];
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int variants[6] = {1, 2, 3, 4, 5, 6};
+  const int chunks[4] = {16, 32, 64, 128};
+  float best = 1e30f;
+  int bv = c->variant, bz = c->zchunk;
+  for (int vi = 0; vi < 6; vi++)
+    for (int zi = 0; zi < 4; zi++) {
+      int v = variants[vi], z = chunks[zi];
+      if (v == 1 && zi > 0) continue;  /* the direct kernel has no z chunks */
+      c->variant = v;
+      c->zchunk = z;
+      int rc = build_zq_tiles(c, c->nvol);
+      if (rc) return rc;
+      /* the four stages' volume launches of one step: they read level pre
+         and write only levels A, B and end (and the PML aux tmp/end), which
+         the next real step rewrites before reading */
+      VolArgs va[4];
+      const int lc[4] = {c->lv_pre, c->lv_a, c->lv_b, c->lv_a};
+      const int ln[4] = {c->lv_a, c->lv_b, c->lv_a, -1};
+      for (int s = 0; s < 4; s++) {
+        fill_vol(c, va[s], s, c->wlev(lc[s]), ln[s] >= 0 ? c->wlev(ln[s]) : nullptr);
+        va[s].k_lo = 0;
+        va[s].nk_run = c->nvol;
+      }
+      auto one = [&]() {
+        for (int s = 0; s < 4; s++)
+          launch_vol(c, va[s], (idir + s) % 2, (jdir + s) % 2, (kdir + s) % 2);
+      };
+      one();
+      HIPCHK(hipEventRecord(e0, c->st));
+      for (int r = 0; r < reps; r++) one();
+      HIPCHK(hipEventRecord(e1, c->st));
+      HIPCHK(hipEventSynchronize(e1));
+      HIPCHK(hipGetLastError());
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      ms /= 4 * reps;
+      if (std::getenv("CGFD_AUTOTUNE_VERBOSE")) std::fprintf(stderr, "[autotune] variant %d zchunk %d: %.3f ms\n", v, z, ms);
+      if (ms < best) { best = ms; bv = v; bz = z; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->variant = bv;
+  c->zchunk = bz;
+  int rc = build_zq_tiles(c, c->nvol);
+  if (rc) return rc;
+  if (variant_out) *variant_out = bv;
+  if (zchunk_out) *zchunk_out = bz;
+  if (ms_out) *ms_out = best;
   return CGFD_OK;
 }
 

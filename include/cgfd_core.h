@@ -182,10 +182,22 @@ int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
                         float *ms_volume, int *n_volume);
 
-/* Kernel variant selection for the volume stencil (0 = default/tuned,
-   1 = direct-load reference kernel, 2 = LDS-tiled z-marching kernel).
+/* Kernel variant selection for the volume stencil (0 = default,
+   1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
+   4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
+   6 = software-pipelined LDS z-march).
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
+
+/* z planes per block for variants 2..6 (4..4096). */
+int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
+
+/* Autotuner: times every (variant, z-chunk) candidate on `reps` repetitions
+   of one step's four volume launches and keeps the fastest; *ms_best is the
+   average per launch. The launches read level "pre" and write only levels
+   that the next step rewrites before reading, so the simulation state is
+   unchanged. */
+int cgfd_core_autotune(cgfd_core *core, int reps, int *variant, int *zchunk, float *ms_best);
 
 /* The one-call drop-in: create + run nt_total steps + copy the final state
    into the optional outputs (NULL to skip) + destroy. */

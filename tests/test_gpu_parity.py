@@ -140,3 +140,19 @@ def test_fp64_oracle_error_statement(built_libs):
     e_ora = rel_l2(w32, w64)
     print(f"rel L2 vs FP64: gpu {e_gpu:.3e}, fp32 oracle {e_ora:.3e}")
     assert e_gpu <= TOL
+
+
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
+def test_volume_variants_match_oracle(built_libs, variant):
+    """Every volume-stencil implementation (direct loads, LDS z-march, z queue)."""
+    from cgfd import configs, model
+    import oracle
+    sc = configs.small(ni=70, nj=52, nk=40, imethod=2, curvilinear=True)
+    st = model.Setup(sc)
+    core = model.Core(st, variant=variant)
+    orc = oracle.Oracle(st.desc, sc.shape_ext, 1)
+    core.run(24)
+    orc.run(24)
+    wg = interior(core.read_wavefield())
+    wo = interior(orc.read_wavefield())
+    assert rel_l2(wg, wo) <= TOL
