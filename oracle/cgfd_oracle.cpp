@@ -146,6 +146,46 @@ template <class T> inline void load9(T M[][3], const T *src) {
 
 /* ======================================================================= */
 
+/* Linear slip-weakening friction at one node (trial_slipweakening.cu:296-384):
+   trial traction -> local traction -> normal/shear split, no opening,
+   friction coefficient, strength cap, |Ts| < 0.1 guard. vn must be the unit
+   normal. Writes the fault traction T1 (slot 3) and tTn/tTs1/tTs2; returns
+   flag_rup. */
+template <class T>
+int friction_point(const T *Trial, T jacvec, const T *vn, const T *T0, T mus, T mud, T Slip, T Dc, T C0,
+                   const T *vs1, const T *vs2, T *T1c, T &tTn, T &tTs1, T &tTs2) {
+  T TL[3], Ts[3];
+  for (int a = 0; a < 3; a++) TL[a] = Trial[a] / jacvec + T0[a];
+  T Tn0 = dot3(TL, vn);
+  for (int a = 0; a < 3; a++) Ts[a] = TL[a] - vn[a] * Tn0;
+  T Ts0 = norm3(Ts);
+  T Tau_n, Tau_s;
+  int ifchange = 0, flag;
+  if (Tn0 >= 0.0) { Tau_n = 0.0; ifchange = 1; }
+  else Tau_n = Tn0;
+  T friction;
+  if (Slip <= Dc) friction = mus - (mus - mud) * Slip / Dc;
+  else friction = mud;
+  T Tau_c = -friction * Tau_n + C0;
+  if (Ts0 >= Tau_c) { Tau_s = Tau_c; flag = 1; ifchange = 1; }
+  else { Tau_s = Ts0; flag = 0; }
+  T Tau[3];
+  if (ifchange == 1) {
+    if (std::fabs(Ts0) < 1e-1) {
+      for (int a = 0; a < 3; a++) Tau[a] = Tau_n * vn[a];
+    } else {
+      for (int a = 0; a < 3; a++) Tau[a] = Tau_n * vn[a] + (Tau_s / Ts0) * Ts[a];
+    }
+  } else {
+    for (int a = 0; a < 3; a++) Tau[a] = TL[a];
+  }
+  for (int a = 0; a < 3; a++) T1c[a] = ifchange == 1 ? (Tau[a] - T0[a]) * jacvec : Trial[a];
+  tTs1 = dot3(Tau, vs1);
+  tTs2 = dot3(Tau, vs2);
+  tTn = Tau_n;
+  return flag;
+}
+
 template <class T>
 struct Oracle {
   /* gd_t */
@@ -533,7 +573,7 @@ struct Oracle {
           T dVx = mVx[p1] - mVx[pf];
           T dVy = mVy[p1] - mVy[pf];
           T dVz = mVz[p1] - mVz[pf];
-          T Trial[3], Trial_local[3], Trial_s[3];
+          T Trial[3];
           Trial[0] = (T)(2.0 * (Mrho[0] * Mrho[1] * dVx / dt + Mrho[0] * Rx[1] - Mrho[1] * Rx[0]) / (a_0 * (Mrho[0] + Mrho[1])));
           Trial[1] = (T)(2.0 * (Mrho[0] * Mrho[1] * dVy / dt + Mrho[0] * Ry[1] - Mrho[1] * Ry[0]) / (a_0 * (Mrho[0] + Mrho[1])));
           Trial[2] = (T)(2.0 * (Mrho[0] * Mrho[1] * dVz / dt + Mrho[0] * Rz[1] - Mrho[1] * Rz[0]) / (a_0 * (Mrho[0] + Mrho[1])));
@@ -545,48 +585,15 @@ struct Oracle {
           T vn0 = norm3(vn);
           T jacvec = jac[iptr] * vn0;
           for (int a = 0; a < 3; a++) vn[a] /= vn0;
-          Trial_local[0] = Trial[0] / jacvec + F.T0x[pf];
-          Trial_local[1] = Trial[1] / jacvec + F.T0y[pf];
-          Trial_local[2] = Trial[2] / jacvec + F.T0z[pf];
-          T Trial_n0 = dot3(Trial_local, vn);
-          Trial_s[0] = Trial_local[0] - vn[0] * Trial_n0;
-          Trial_s[1] = Trial_local[1] - vn[1] * Trial_n0;
-          Trial_s[2] = Trial_local[2] - vn[2] * Trial_n0;
-          T Trial_s0 = norm3(Trial_s);
-          T Tau_n, Tau_s;
-          int ifchange = 0;
-          if (Trial_n0 >= 0.0) { Tau_n = 0.0; ifchange = 1; }
-          else { Tau_n = Trial_n0; }
-          T mus = F.mu_s[pf], mud = F.mu_d[pf], Slip = F.Slip[pf], Dc = F.Dc[pf], C0 = F.C0[pf];
-          T friction;
-          if (Slip <= Dc) friction = mus - (mus - mud) * Slip / Dc;
-          else friction = mud;
-          T Tau_c = -friction * Tau_n + C0;
-          if (Trial_s0 >= Tau_c) { Tau_s = Tau_c; F.flag_rup[pf] = 1; ifchange = 1; }
-          else { Tau_s = Trial_s0; F.flag_rup[pf] = 0; }
-          T Tau[3];
-          if (ifchange == 1) {
-            if (std::fabs(Trial_s0) < 1e-1) {
-              Tau[0] = Tau_n * vn[0]; Tau[1] = Tau_n * vn[1]; Tau[2] = Tau_n * vn[2];
-            } else {
-              Tau[0] = Tau_n * vn[0] + (Tau_s / Trial_s0) * Trial_s[0];
-              Tau[1] = Tau_n * vn[1] + (Tau_s / Trial_s0) * Trial_s[1];
-              Tau[2] = Tau_n * vn[2] + (Tau_s / Trial_s0) * Trial_s[2];
-            }
-          } else {
-            Tau[0] = Trial_local[0]; Tau[1] = Trial_local[1]; Tau[2] = Trial_local[2];
-          }
+          T T0v[3] = {F.T0x[pf], F.T0y[pf], F.T0z[pf]};
+          T T1c[3], tn, ts1, ts2;
+          F.flag_rup[pf] = friction_point<T>(Trial, jacvec, vn, T0v, F.mu_s[pf], F.mu_d[pf], F.Slip[pf], F.Dc[pf],
+                                             F.C0[pf], vs1, vs2, T1c, tn, ts1, ts2);
           size_t p3 = pf + 3 * sslice;
-          if (ifchange == 1) {
-            T1x[p3] = (Tau[0] - F.T0x[pf]) * jacvec;
-            T1y[p3] = (Tau[1] - F.T0y[pf]) * jacvec;
-            T1z[p3] = (Tau[2] - F.T0z[pf]) * jacvec;
-          } else {
-            T1x[p3] = Trial[0]; T1y[p3] = Trial[1]; T1z[p3] = Trial[2];
-          }
-          F.tTs1[pf] = dot3(Tau, vs1);
-          F.tTs2[pf] = dot3(Tau, vs2);
-          F.tTn[pf] = Tau_n;
+          T1x[p3] = T1c[0]; T1y[p3] = T1c[1]; T1z[p3] = T1c[2];
+          F.tTs1[pf] = ts1;
+          F.tTs2[pf] = ts2;
+          F.tTn[pf] = tn;
         }
     }
   }
@@ -1482,26 +1489,12 @@ int oracle_probe_friction(const double *in, double *out, int use_double) {
     T jacvec = (T)in[3];
     T vn[3] = {(T)in[4], (T)in[5], (T)in[6]};
     T T0[3] = {(T)in[7], (T)in[8], (T)in[9]};
-    T mus = (T)in[10], mud = (T)in[11], Slip = (T)in[12], Dc = (T)in[13], C0 = (T)in[14];
     T s1[3] = {(T)in[15], (T)in[16], (T)in[17]}, s2[3] = {(T)in[18], (T)in[19], (T)in[20]};
-    T TL[3], Ts[3];
-    for (int a = 0; a < 3; a++) TL[a] = Trial[a] / jacvec + T0[a];
-    T n0 = dot3(TL, vn);
-    for (int a = 0; a < 3; a++) Ts[a] = TL[a] - vn[a] * n0;
-    T s0 = norm3(Ts);
-    T Tau_n, Tau_s;
-    int ifchange = 0, flag = 0;
-    if (n0 >= 0.0) { Tau_n = 0.0; ifchange = 1; } else Tau_n = n0;
-    T fr = (Slip <= Dc) ? (T)(mus - (mus - mud) * Slip / Dc) : mud;
-    T Tau_c = -fr * Tau_n + C0;
-    if (s0 >= Tau_c) { Tau_s = Tau_c; flag = 1; ifchange = 1; } else Tau_s = s0;
-    T Tau[3];
-    if (ifchange == 1) {
-      if (std::fabs(s0) < 1e-1) for (int a = 0; a < 3; a++) Tau[a] = Tau_n * vn[a];
-      else for (int a = 0; a < 3; a++) Tau[a] = Tau_n * vn[a] + (Tau_s / s0) * Ts[a];
-    } else for (int a = 0; a < 3; a++) Tau[a] = TL[a];
-    for (int a = 0; a < 3; a++) out[a] = ifchange ? (double)((Tau[a] - T0[a]) * jacvec) : (double)Trial[a];
-    out[3] = Tau_n; out[4] = dot3(Tau, s1); out[5] = dot3(Tau, s2); out[6] = flag;
+    T T1c[3], tn, ts1, ts2;
+    int flag = friction_point<T>(Trial, jacvec, vn, T0, (T)in[10], (T)in[11], (T)in[12], (T)in[13], (T)in[14], s1, s2,
+                                 T1c, tn, ts1, ts2);
+    for (int a = 0; a < 3; a++) out[a] = (double)T1c[a];
+    out[3] = tn; out[4] = ts1; out[5] = ts2; out[6] = flag;
   };
   if (use_double) run(0.0); else run(0.0f);
   return 0;
