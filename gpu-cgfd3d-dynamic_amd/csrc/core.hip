@@ -453,10 +453,11 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
 }
 
 /* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
-   no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march */
+   no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march,
+   7 direct loads on an L2-aware block schedule */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
-  else if (c->variant >= 3) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
+  else if (c->variant >= 3 && c->variant <= 5) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
   else launch_volume(c->variant, va, idir, jdir, kdir, 0, c->st);
 }
 
@@ -676,7 +677,7 @@ extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant == 0) variant = 4;
-  if (variant < 1 || variant > 6) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant < 1 || variant > 7) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
   return CGFD_OK;
 }
@@ -703,11 +704,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[6] = {1, 2, 3, 4, 5, 6};
+  const int variants[7] = {1, 2, 3, 4, 5, 6, 7};
   const int chunks[4] = {16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 6; vi++)
+  for (int vi = 0; vi < 7; vi++)
     for (int zi = 0; zi < 4; zi++) {
       int v = variants[vi], z = chunks[zi];
       if (v == 1 && zi > 0) continue;  /* the direct kernel has no z chunks */

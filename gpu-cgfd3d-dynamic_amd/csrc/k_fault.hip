@@ -10,10 +10,12 @@
  *                 (fault_var_stage_update_gpu, fault_wav_t.cu:290-316), and
  *                 the split-node average written back onto the stage input
  *                 (fault2wave_gpu, transform.cu:292-376).
- *  k_fault_update<IDIR> : split-node velocity rhs (fault_velo, :191-429) and
- *                 stress rhs (fault_stress_F/B, :431-1879) on the six columns
- *                 around the plane, fused with the RK update of those columns,
- *                 of the 18 split-node fields (fault_wav_update, fault_wav_t.cu:
+ *  k_fault_cols<IDIR> : velocity and stress rhs (fault_velo, fault_gpu.cu:
+ *                 191-324; fault_stress_F/B, :431-1879) on the six columns
+ *                 around the plane, fused with their RK update (one lane per
+ *                 node and column).
+ *  k_fault_nodes<IDIR> : split-node velocity and traction rates, RK update of
+ *                 the 18 split-node fields (fault_wav_update, fault_wav_t.cu:
  *                 318-366), fault2wave onto the stage output, and at the last
  *                 stage the per-step accumulators (fault_var_update_gpu,
  *                 fault_wav_t.cu:197-264).
@@ -269,10 +271,159 @@ DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
   }
 }
 
-DEVI void mv_soa(const float *M, long st, const float *x, float *y) { mv3(M, st, x, y); }
+/* ---------------------------------------------------------------------- */
+/* The six wavefield columns i0-3..i0-1, i0+1..i0+3 of a split node: velocity
+   rhs (fault_gpu.cu:191-324) and stress rhs by Hooke's law (:1065-1155 and
+   :1694-1790) with the x differences next to the plane taken against the
+   split-node velocity (:518-543, :1020-1063, :1248-1274, :1694-1728), fused
+   with the RK update of the nine components. Lane layout: six lanes per
+   node, one per column, ten nodes (consecutive j) per wavefront, so the
+   loads of a node's x taps are adjacent addresses. */
+constexpr int kColNodes = 10;  /* nodes per wavefront in the column kernel */
 
 template <int IDIR>
-__global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
+__global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a) {
+  const int lane = threadIdx.x;
+  if (lane >= 6 * kColNodes) return;
+  const int iy = blockIdx.x * kColNodes + lane / 6;
+  const int cl = lane % 6;
+  const int iz = blockIdx.y * blockDim.y + threadIdx.y;
+  const int id = blockIdx.z;
+  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  const FaultDev &F = a.f[id];
+  const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
+  const long pf = (iy + 3) + (long)(iz + 3) * ny;
+  if (F.united[pf] != 0) return;
+  const int jdir = a.jdir, kdir = a.kdir;
+  const int i0 = F.i0;
+  const int mm = cl < 3 ? cl - 3 : cl - 2;  /* column offset from the plane */
+  const int i = i0 + mm;
+  const int n = -mm;
+  const long row = (long)(iy + 3) * sy + (long)(iz + 3) * sz;
+  const long p = i + row;
+  const int km = a.g.nk - (iz + 1);
+  const int n_free = km + 3;
+  const bool surf = (a.isfree == 1);
+  const float *met = a.met;
+  const float *W = a.w_cur;
+
+  /* ---- velocity rhs ---- */
+  const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
+  const float *Tyz = W + TYZ * sc, *Txz = W + TXZ * sc, *Txy = W + TXY * sc;
+  float v1[3][7], v2[3][7], v3[3][7];
+#pragma unroll
+  for (int l = -3; l <= 3; l++) {
+    long q = (i + l) + row;
+    float jq = met[q], a1 = met[sc + q], a2 = met[2 * sc + q], a3 = met[3 * sc + q];
+    v1[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+    v1[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+    v1[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    q = p + l * sy;
+    jq = met[q]; a1 = met[4 * sc + q]; a2 = met[5 * sc + q]; a3 = met[6 * sc + q];
+    v2[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+    v2[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+    v2[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    q = p + l * sz;
+    jq = met[q]; a1 = met[7 * sc + q]; a2 = met[8 * sc + q]; a3 = met[9 * sc + q];
+    v3[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+    v3[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+    v3[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+  }
+  float h[9];
+  const float rrhojac = a.slw[p] / met[p];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float *v = v1[c];
+    /* the tap on the plane is the fault traction; image beyond it
+       (fault_gpu.cu:266-291) */
+    const float tc = F.T1[(long)c * 7 * ss + 3 * ss + pf];
+#pragma unroll
+    for (int l = 0; l < 7; l++)
+      if (l == n + 3) v[l] = tc;
+    if (n == 2) { v[6] = (float)(2.0 * v[5] - v[4]); }
+    if (n == 1) { v[5] = (float)(2.0 * v[4] - v[3]); v[6] = (float)(2.0 * v[4] - v[2]); }
+    if (n == -1) { v[0] = (float)(2.0 * v[2] - v[4]); v[1] = (float)(2.0 * v[2] - v[3]); }
+    if (n == -2) { v[0] = (float)(2.0 * v[1] - v[2]); }
+    if (surf && km <= 3) {
+      float *u = v3[c];
+      u[n_free] = 0.0f;
+      for (int l = n_free + 1; l < 7; l++) u[l] = -u[2 * n_free - l];
+    }
+    float dx = vecc(v1[c] + 3, IDIR), dy = vecc(v2[c] + 3, jdir), dz = vecc(v3[c] + 3, kdir);
+    h[c] = (dx + dy + dz) * rrhojac;
+  }
+
+  /* ---- stress rhs ---- */
+  const float *V[3] = {W, W + sc, W + 2 * sc};
+  const float *fc = a.f_cur + (long)id * 18 * ss;
+  float dx[3], dy[3], dz[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const float *Vc = V[c] + row;
+    if (IDIR == 1) {
+      if (mm == 3) dx[c] = macdrp_f(Vc + (i0 + 3), 1);           /* A */
+      else if (mm == 2) dx[c] = mac24(Vc + (i0 + 2), 1, 1);      /* B */
+      else if (mm == 1) dx[c] = mac22(Vc + (i0 + 1), 1, 1);      /* C */
+      else {
+        const float vm = fc[fidx(c, 0, ss) + pf];
+        if (mm == -1) dx[c] = vm - Vc[i0 - 1];                    /* E */
+        else if (mm == -2) {                                       /* F */
+          float t3[3] = {Vc[i0 - 2], Vc[i0 - 1], vm};
+          dx[c] = vec24_f(t3);
+        } else {                                                   /* G */
+          float t5[5] = {Vc[i0 - 4], Vc[i0 - 3], Vc[i0 - 2], Vc[i0 - 1], vm};
+          dx[c] = vecdrp_f(t5);
+        }
+      }
+    } else {
+      if (mm == -3) dx[c] = macdrp_b(Vc + (i0 - 3), 1);          /* G */
+      else if (mm == -2) dx[c] = mac24(Vc + (i0 - 2), 1, 0);     /* F */
+      else if (mm == -1) dx[c] = mac22(Vc + (i0 - 1), 1, 0);     /* E */
+      else {
+        const float vp = fc[fidx(c, 1, ss) + pf];
+        if (mm == 1) dx[c] = Vc[i0 + 1] - vp;                     /* C */
+        else if (mm == 2) {                                        /* B */
+          float t3[3] = {vp, Vc[i0 + 1], Vc[i0 + 2]};
+          dx[c] = vec24_b(t3);
+        } else {                                                   /* A */
+          float t5[5] = {vp, Vc[i0 + 1], Vc[i0 + 2], Vc[i0 + 3], Vc[i0 + 4]};
+          dx[c] = vecdrp_b(t5);
+        }
+      }
+    }
+    dy[c] = macdrp(V[c] + p, sy, jdir);
+    dz[c] = macdrp(V[c] + p, sz, kdir);
+  }
+  if (surf && km == 0) {
+    long ij = ((long)i + (long)(iy + 3) * sy) * 9;
+    const float *X = a.matVx2Vz + ij, *Y = a.matVy2Vz + ij;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      dz[r] = X[3 * r] * dx[0] + X[3 * r + 1] * dx[1] + X[3 * r + 2] * dx[2] + Y[3 * r] * dy[0] + Y[3 * r + 1] * dy[1] +
+              Y[3 * r + 2] * dy[2];
+  }
+  if (surf && km == 1) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) dz[c] = mac22(V[c] + p, sz, kdir);
+  }
+  if (surf && km == 2) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) dz[c] = mac24(V[c] + p, sz, kdir);
+  }
+  hooke(met, sc, p, a.lam[p], a.mu[p], dx, dy, dz, h + 3);
+  rk_cols(a, p, 0, 9, h);
+}
+
+/* ---------------------------------------------------------------------- */
+/* The split nodes: velocity rhs from the fault traction (fault_gpu.cu:326-425),
+   traction rates from the split-node velocity gradient (method 1 or 2, free
+   surface rows; :431-1018, :1248-1517), RK update of the 18 split-node
+   fields (fault_wav_update, fault_wav_t.cu:318-366), split-node average onto
+   the stage output (drv_rk_curv_col.cu:356-359, 541-544) and at the last
+   stage the per-step accumulators (fault_var_update_gpu, fault_wav_t.cu:
+   197-264). One lane per node, lanes along j. */
+template <int IDIR>
+__global__ void __launch_bounds__(256) k_fault_nodes(FaultArgs a) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
   const int id = blockIdx.z;
@@ -294,56 +445,7 @@ __global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
   const int ry = F.rupy[pf], rz = F.rupz[pf];
   const float *T1 = F.T1;
 
-  /* ================= velocity rhs on the six columns ================= */
-  const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
-  const float *Tyz = W + TYZ * sc, *Txz = W + TXZ * sc, *Txy = W + TXY * sc;
-#pragma unroll 1
-  for (int i = i0 - 3; i <= i0 + 3; i++) {
-    const int n = i0 - i;
-    if (n == 0) continue;
-    float v1[3][7], v2[3][7], v3[3][7];
-#pragma unroll
-    for (int l = -3; l <= 3; l++) {
-      long q = (i + l) + row;
-      float jq = met[q], a1 = met[sc + q], a2 = met[2 * sc + q], a3 = met[3 * sc + q];
-      v1[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-      v1[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-      v1[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
-      q = i + row + l * sy;
-      jq = met[q]; a1 = met[4 * sc + q]; a2 = met[5 * sc + q]; a3 = met[6 * sc + q];
-      v2[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-      v2[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-      v2[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
-      q = i + row + l * sz;
-      jq = met[q]; a1 = met[7 * sc + q]; a2 = met[8 * sc + q]; a3 = met[9 * sc + q];
-      v3[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-      v3[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-      v3[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
-    }
-    float hv[3];
-    long p = i + row;
-    float rrhojac = a.slw[p] / met[p];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      float *v = v1[c];
-      v[n + 3] = T1[(long)c * 7 * ss + 3 * ss + pf];
-      /* traction image across the fault (fault_gpu.cu:266-291) */
-      if (n == 2) { v[6] = (float)(2.0 * v[5] - v[4]); }
-      if (n == 1) { v[5] = (float)(2.0 * v[4] - v[3]); v[6] = (float)(2.0 * v[4] - v[2]); }
-      if (n == -1) { v[0] = (float)(2.0 * v[2] - v[4]); v[1] = (float)(2.0 * v[2] - v[3]); }
-      if (n == -2) { v[0] = (float)(2.0 * v[1] - v[2]); }
-      if (surf && km <= 3) {
-        float *u = v3[c];
-        u[n_free] = 0.0f;
-        for (int l = n_free + 1; l < 7; l++) u[l] = -u[2 * n_free - l];
-      }
-      float dx = vecc(v1[c] + 3, IDIR), dy = vecc(v2[c] + 3, jdir), dz = vecc(v3[c] + 3, kdir);
-      hv[c] = (dx + dy + dz) * rrhojac;
-    }
-    rk_cols(a, p, 0, 3, hv);
-  }
-
-  /* split-node velocity rhs (fault_gpu.cu:326-425) */
+  /* split-node velocity rhs */
   float fhV[2][3];
 #pragma unroll
   for (int m = 0; m < 2; m++) {
@@ -366,28 +468,19 @@ __global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
     }
   }
 
-  /* ================= stress rhs ================= */
+  /* split-node velocity gradients: DV[0] minus side, DV[1] plus side */
   const float *V[3] = {W, W + sc, W + 2 * sc};
   const float *fV[3] = {fc + fidx(0, 0, ss), fc + fidx(1, 0, ss), fc + fidx(2, 0, ss)};
   const long pm = pf, pp = pf + ss;
-  float DxV[8][3], DyV[8][3], DzV[8][3];
-  constexpr int S_ = IDIR == 1 ? 4 : 3;  /* side with the one-sided x difference */
-  constexpr int O_ = 7 - S_;
+  float DxV[2][3], DyV[2][3], DzV[2][3];
+  constexpr int S_ = IDIR == 1 ? 1 : 0;  /* side with the one-sided x difference */
+  constexpr int O_ = 1 - S_;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    if (IDIR == 1) {  /* A, B, C (fault_gpu.cu:518-543) */
-      DxV[7][c] = macdrp_f(V[c] + (i0 + 3) + row, 1);
-      DxV[6][c] = mac24(V[c] + (i0 + 2) + row, 1, 1);
-      DxV[5][c] = mac22(V[c] + (i0 + 1) + row, 1, 1);
-    } else {  /* G, F, E (:1248-1274) */
-      DxV[0][c] = macdrp_b(V[c] + (i0 - 3) + row, 1);
-      DxV[1][c] = mac24(V[c] + (i0 - 2) + row, 1, 0);
-      DxV[2][c] = mac22(V[c] + (i0 - 1) + row, 1, 0);
-    }
-    if (ry == 1) { DyV[4][c] = mac22(fV[c] + pp, 1, jdir); DyV[3][c] = mac22(fV[c] + pm, 1, jdir); }
-    else { DyV[4][c] = macdrp(fV[c] + pp, 1, jdir); DyV[3][c] = macdrp(fV[c] + pm, 1, jdir); }
-    if (rz == 1) { DzV[4][c] = mac22(fV[c] + pp, ny, kdir); DzV[3][c] = mac22(fV[c] + pm, ny, kdir); }
-    else { DzV[4][c] = macdrp(fV[c] + pp, ny, kdir); DzV[3][c] = macdrp(fV[c] + pm, ny, kdir); }
+    if (ry == 1) { DyV[1][c] = mac22(fV[c] + pp, 1, jdir); DyV[0][c] = mac22(fV[c] + pm, 1, jdir); }
+    else { DyV[1][c] = macdrp(fV[c] + pp, 1, jdir); DyV[0][c] = macdrp(fV[c] + pm, 1, jdir); }
+    if (rz == 1) { DzV[1][c] = mac22(fV[c] + pp, ny, kdir); DzV[0][c] = mac22(fV[c] + pm, ny, kdir); }
+    else { DzV[1][c] = macdrp(fV[c] + pp, ny, kdir); DzV[0][c] = macdrp(fV[c] + pm, ny, kdir); }
   }
   const float *Ms = (IDIR == 1 ? F.P2M : F.M2P) + pf; /* 5 matrices, stride 9*ss between them */
   const float *M2 = F.M2 + pf;                        /* T1toVx,VytoVx,VztoVx (minus), then plus */
@@ -399,25 +492,25 @@ __global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
   float o1[3], o2[3], o3[3], o4[3], o5[3];
   /* method 2 on both sides with the full matrices (fault_gpu.cu:662-707) */
   auto method2 = [&]() {
-    mv_soa(M2 + 3 * mq, ms, dtT1, o1); mv_soa(M2 + 4 * mq, ms, DyV[4], o2); mv_soa(M2 + 5 * mq, ms, DzV[4], o3);
+    mv3(M2 + 3 * mq, ms, dtT1, o1); mv3(M2 + 4 * mq, ms, DyV[1], o2); mv3(M2 + 5 * mq, ms, DzV[1], o3);
 #pragma unroll
-    for (int c = 0; c < 3; c++) DxV[4][c] = o1[c] - o2[c] - o3[c];
-    mv_soa(M2 + 0 * mq, ms, dtT1, o1); mv_soa(M2 + 1 * mq, ms, DyV[3], o2); mv_soa(M2 + 2 * mq, ms, DzV[3], o3);
+    for (int c = 0; c < 3; c++) DxV[1][c] = o1[c] - o2[c] - o3[c];
+    mv3(M2 + 0 * mq, ms, dtT1, o1); mv3(M2 + 1 * mq, ms, DyV[0], o2); mv3(M2 + 2 * mq, ms, DzV[0], o3);
 #pragma unroll
-    for (int c = 0; c < 3; c++) DxV[3][c] = o1[c] - o2[c] - o3[c];
+    for (int c = 0; c < 3; c++) DxV[0][c] = o1[c] - o2[c] - o3[c];
   };
   /* method 1: one-sided difference on side s, continuity maps it to side o */
   auto method1 = [&]() {
-    mv_soa(Ms + 0 * mq, ms, DxV[S_], o1); mv_soa(Ms + 1 * mq, ms, DyV[S_], o2); mv_soa(Ms + 2 * mq, ms, DzV[S_], o3);
-    mv_soa(Ms + 3 * mq, ms, DyV[O_], o4); mv_soa(Ms + 4 * mq, ms, DzV[O_], o5);
+    mv3(Ms + 0 * mq, ms, DxV[S_], o1); mv3(Ms + 1 * mq, ms, DyV[S_], o2); mv3(Ms + 2 * mq, ms, DzV[S_], o3);
+    mv3(Ms + 3 * mq, ms, DyV[O_], o4); mv3(Ms + 4 * mq, ms, DzV[O_], o5);
 #pragma unroll
     for (int c = 0; c < 3; c++) DxV[O_][c] = o1[c] + o2[c] + o3[c] - o4[c] - o5[c];
   };
   if (a.imethod == 1) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      if (IDIR == 1) DxV[4][c] = V[c][(i0 + 1) + row] - fV[c][pp];
-      else DxV[3][c] = fV[c][pm] - V[c][(i0 - 1) + row];
+      if (IDIR == 1) DxV[1][c] = V[c][(i0 + 1) + row] - fV[c][pp];
+      else DxV[0][c] = fV[c][pm] - V[c][(i0 - 1) + row];
     }
     method1();
   } else {
@@ -430,29 +523,29 @@ __global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
     const long fq = 9 * ny;
     if (a.imethod == 1) {
       const float *Mf = Fm + (IDIR == 1 ? 4 : 7) * fq;
-      mv_soa(Mf, fs_, DxV[S_], o1); mv_soa(Mf + fq, fs_, DyV[S_], o2); mv_soa(Mf + 2 * fq, fs_, DyV[O_], o3);
+      mv3(Mf, fs_, DxV[S_], o1); mv3(Mf + fq, fs_, DyV[S_], o2); mv3(Mf + 2 * fq, fs_, DyV[O_], o3);
 #pragma unroll
       for (int c = 0; c < 3; c++) DxV[O_][c] = o1[c] + o2[c] - o3[c];
     } else {
-      mv_soa(Fm + 12 * fq, fs_, dtT1, o1); mv_soa(Fm + 13 * fq, fs_, DyV[4], o2);
+      mv3(Fm + 12 * fq, fs_, dtT1, o1); mv3(Fm + 13 * fq, fs_, DyV[1], o2);
 #pragma unroll
-      for (int c = 0; c < 3; c++) DxV[4][c] = o1[c] - o2[c];
-      mv_soa(Fm + 10 * fq, fs_, dtT1, o1); mv_soa(Fm + 11 * fq, fs_, DyV[3], o2);
+      for (int c = 0; c < 3; c++) DxV[1][c] = o1[c] - o2[c];
+      mv3(Fm + 10 * fq, fs_, dtT1, o1); mv3(Fm + 11 * fq, fs_, DyV[0], o2);
 #pragma unroll
-      for (int c = 0; c < 3; c++) DxV[3][c] = o1[c] - o2[c];
+      for (int c = 0; c < 3; c++) DxV[0][c] = o1[c] - o2[c];
     }
-    mv_soa(Fm + 2 * fq, fs_, DxV[4], o1); mv_soa(Fm + 3 * fq, fs_, DyV[4], o2);
+    mv3(Fm + 2 * fq, fs_, DxV[1], o1); mv3(Fm + 3 * fq, fs_, DyV[1], o2);
 #pragma unroll
-    for (int c = 0; c < 3; c++) DzV[4][c] = o1[c] + o2[c];
-    mv_soa(Fm + 0 * fq, fs_, DxV[3], o1); mv_soa(Fm + 1 * fq, fs_, DyV[3], o2);
+    for (int c = 0; c < 3; c++) DzV[1][c] = o1[c] + o2[c];
+    mv3(Fm + 0 * fq, fs_, DxV[0], o1); mv3(Fm + 1 * fq, fs_, DyV[0], o2);
 #pragma unroll
-    for (int c = 0; c < 3; c++) DzV[3][c] = o1[c] + o2[c];
+    for (int c = 0; c < 3; c++) DzV[0][c] = o1[c] + o2[c];
   }
   if (surf && (km == 1 || km == 2)) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      if (km == 1) { DzV[4][c] = mac22(fV[c] + pp, ny, kdir); DzV[3][c] = mac22(fV[c] + pm, ny, kdir); }
-      else { DzV[4][c] = mac24(fV[c] + pp, ny, kdir); DzV[3][c] = mac24(fV[c] + pm, ny, kdir); }
+      if (km == 1) { DzV[1][c] = mac22(fV[c] + pp, ny, kdir); DzV[0][c] = mac22(fV[c] + pm, ny, kdir); }
+      else { DzV[1][c] = mac24(fV[c] + pp, ny, kdir); DzV[0][c] = mac24(fV[c] + pm, ny, kdir); }
     }
     if (a.imethod == 1) method1();
     else method2();
@@ -461,91 +554,41 @@ __global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
   float fhT2[2][3], fhT3[2][3];
 #pragma unroll
   for (int m = 0; m < 2; m++) {
-    const int q = m == 1 ? 4 : 3;
     const float *Db = F.D + (long)(m * 6) * mq + pf;
     float g1[3], g2[3], g3[3];
-    mv_soa(Db + 0 * mq, ms, DxV[q], g1); mv_soa(Db + 1 * mq, ms, DyV[q], g2); mv_soa(Db + 2 * mq, ms, DzV[q], g3);
+    mv3(Db + 0 * mq, ms, DxV[m], g1); mv3(Db + 1 * mq, ms, DyV[m], g2); mv3(Db + 2 * mq, ms, DzV[m], g3);
 #pragma unroll
     for (int c = 0; c < 3; c++) fhT2[m][c] = g1[c] + g2[c] + g3[c];
-    mv_soa(Db + 3 * mq, ms, DxV[q], g1); mv_soa(Db + 4 * mq, ms, DyV[q], g2); mv_soa(Db + 5 * mq, ms, DzV[q], g3);
+    mv3(Db + 3 * mq, ms, DxV[m], g1); mv3(Db + 4 * mq, ms, DyV[m], g2); mv3(Db + 5 * mq, ms, DzV[m], g3);
 #pragma unroll
     for (int c = 0; c < 3; c++) fhT3[m][c] = g1[c] + g2[c] + g3[c];
   }
-  /* x differences next to the fault on side o, using that side's velocity */
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    if (IDIR == 1) {  /* E, F, G (fault_gpu.cu:1020-1063) */
-      float vm = fV[c][pm];
-      DxV[2][c] = vm - V[c][(i0 - 1) + row];
-      float t3[3] = {V[c][(i0 - 2) + row], V[c][(i0 - 1) + row], vm};
-      DxV[1][c] = vec24_f(t3);
-      float t5[5] = {V[c][(i0 - 4) + row], V[c][(i0 - 3) + row], V[c][(i0 - 2) + row], V[c][(i0 - 1) + row], vm};
-      DxV[0][c] = vecdrp_f(t5);
-    } else {  /* C, B, A (:1694-1728) */
-      float vp = fV[c][pp];
-      DxV[5][c] = V[c][(i0 + 1) + row] - vp;
-      float t3[3] = {vp, V[c][(i0 + 1) + row], V[c][(i0 + 2) + row]};
-      DxV[6][c] = vec24_b(t3);
-      float t5[5] = {vp, V[c][(i0 + 1) + row], V[c][(i0 + 2) + row], V[c][(i0 + 3) + row], V[c][(i0 + 4) + row]};
-      DxV[7][c] = vecdrp_b(t5);
-    }
-  }
-  /* Hooke on the six columns (:1065-1155) with the RK update */
-#pragma unroll 1
-  for (int n = 0; n < 8; n++) {
-    const int mm = n < 4 ? n - 3 : n - 4;
-    if (mm == 0) continue;
-    const long p = (i0 + mm) + row;
-    float dx[3], dy[3], dz[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      dx[c] = DxV[n][c];
-      dy[c] = macdrp(V[c] + p, sy, jdir);
-      dz[c] = macdrp(V[c] + p, sz, kdir);
-    }
-    if (surf && km == 0) {
-      long ij = ((long)(i0 + mm) + (long)(iy + 3) * sy) * 9;
-      const float *X = a.matVx2Vz + ij, *Y = a.matVy2Vz + ij;
-#pragma unroll
-      for (int r = 0; r < 3; r++)
-        dz[r] = X[3 * r] * dx[0] + X[3 * r + 1] * dx[1] + X[3 * r + 2] * dx[2] + Y[3 * r] * dy[0] + Y[3 * r + 1] * dy[1] +
-                Y[3 * r + 2] * dy[2];
-    }
-    if (surf && km == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) dz[c] = mac22(V[c] + p, sz, kdir);
-    }
-    if (surf && km == 2) {
-#pragma unroll
-      for (int c = 0; c < 3; c++) dz[c] = mac24(V[c] + p, sz, kdir);
-    }
-    float ht[6];
-    hooke(met, sc, p, a.lam[p], a.mu[p], dx, dy, dz, ht);
-    rk_cols(a, p, TXX, 6, ht);
-  }
 
-  /* ================= split-node state RK update ================= */
+  /* split-node state RK update */
   float *fpre = a.f_pre + (long)id * 18 * ss;
   float *fnew = a.istage < 3 ? a.f_tnew + (long)id * 18 * ss : nullptr;
   float *fend = a.f_end + (long)id * 18 * ss;
+  float fe[2][3];
 #pragma unroll
   for (int m = 0; m < 2; m++)
 #pragma unroll
     for (int c = 0; c < 9; c++) {
       float r = c < 3 ? fhV[m][c] : (c < 6 ? fhT2[m][c - 3] : fhT3[m][c - 6]);
       long o = fidx(c, m, ss) + pf;
+      float e;
       if (a.istage == 0) {
         float pv = fpre[o];
         fnew[o] = pv + a.ca * r;
-        fend[o] = pv + a.cb * r;
+        e = pv + a.cb * r;
       } else if (a.istage < 3) {
         fnew[o] = fpre[o] + a.ca * r;
-        fend[o] += a.cb * r;
+        e = fend[o] + a.cb * r;
       } else {
-        fend[o] += a.cb * r;
+        e = fend[o] + a.cb * r;
       }
+      fend[o] = e;
+      if (c < 3) fe[m][c] = e;
     }
-  /* split-node average onto the stage output (drv_rk_curv_col.cu:356-359, 541-544) */
   if (a.istage < 3) fault2wave_node(a, F, pf, p0, a.w_tnew, fnew);
   else fault2wave_node(a, F, pf, p0, a.w_end, fend);
 
@@ -553,7 +596,7 @@ __global__ void __launch_bounds__(64) k_fault_update(FaultArgs a) {
     /* per-step accumulators (fault_var_update_gpu, fault_wav_t.cu:219-262) */
     float dV[3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) dV[c] = fend[fidx(c, 1, ss) + pf] - fend[fidx(c, 0, ss) + pf];
+    for (int c = 0; c < 3; c++) dV[c] = fe[1][c] - fe[0][c];
     float Vs1 = dV[0] * F.vs[pf] + dV[1] * F.vs[ss + pf] + dV[2] * F.vs[2 * ss + pf];
     float Vs2 = dV[0] * F.vs[3 * ss + pf] + dV[1] * F.vs[4 * ss + pf] + dV[2] * F.vs[5 * ss + pf];
     float Vs = sqrtf(Vs2 * Vs2 + Vs1 * Vs1);
@@ -624,9 +667,15 @@ void launch_trial(const FaultArgs &a, hipStream_t st) {
 }
 void launch_fault_update(const FaultArgs &a, hipStream_t st) {
   if (a.nfault <= 0) return;
-  dim3 blk(64, 1, 1), grd((a.g.nj + 63) / 64, a.g.nk, a.nfault);
-  if (a.idir == 1) hipLaunchKernelGGL(k_fault_update<1>, grd, blk, 0, st, a);
-  else hipLaunchKernelGGL(k_fault_update<0>, grd, blk, 0, st, a);
+  dim3 cblk(64, 4, 1), cgrd((a.g.nj + kColNodes - 1) / kColNodes, (a.g.nk + 3) / 4, a.nfault);
+  dim3 nblk(64, 4, 1), ngrd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault);
+  if (a.idir == 1) {
+    hipLaunchKernelGGL(k_fault_cols<1>, cgrd, cblk, 0, st, a);
+    hipLaunchKernelGGL(k_fault_nodes<1>, ngrd, nblk, 0, st, a);
+  } else {
+    hipLaunchKernelGGL(k_fault_cols<0>, cgrd, cblk, 0, st, a);
+    hipLaunchKernelGGL(k_fault_nodes<0>, ngrd, nblk, 0, st, a);
+  }
 }
 void launch_step_end(int *it_dev, hipStream_t st) { hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, it_dev); }
 void launch_zero(float *p, long n, hipStream_t st) {

@@ -47,6 +47,51 @@ __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a) {
 }
 
 /* ------------------------------------------------------------------------
+ * The same per-point kernel on an L2-aware block schedule. Each XCD (block
+ * ids L with L % 8 == x) takes a contiguous range of work items; an item is
+ * one 64 x 4 tile of one plane, ordered so that the blocks resident on an
+ * XCD at the same time cover zchunk consecutive planes of kSchedG y-adjacent
+ * tiles. The z and y neighbours a block reads were then fetched into that
+ * XCD's L2 by its neighbours moments earlier, instead of coming again from
+ * HBM or the MALL.
+ * ---------------------------------------------------------------------- */
+constexpr int kSchedG = 8;
+
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_sched(VolArgs a) {
+  const int L = blockIdx.x;
+  const int per = (int)(gridDim.x / 8);
+  int t = (L % 8) * per + L / 8;
+  const int Kc = a.zchunk;
+  const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + 3) / 4;
+  const int nchunk = (a.nk_run + Kc - 1) / Kc;
+  const int ngy = (nty + kSchedG - 1) / kSchedG;
+  const int kk = t % Kc; t /= Kc;
+  const int cy = t % kSchedG; t /= kSchedG;
+  const int ch = t % nchunk; t /= nchunk;
+  const int gy = t % ngy;
+  const int bx = t / ngy;
+  const int by = gy * kSchedG + cy;
+  const int kz = ch * Kc + kk;
+  if (bx >= ntx || by >= nty || kz >= a.nk_run) return;
+  const int ix = bx * 64 + (int)threadIdx.x;
+  const int iy = by * 4 + (int)threadIdx.y;
+  if (ix >= a.g.ni || iy >= a.g.nj) return;
+  const int i = ix + 3, j = iy + 3, k = a.k_lo + kz + 3;
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  float Dx[9], Dy[9], Dz[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float *q = a.cur + c * sc + p;
+    Dx[c] = dfd<ID>(q, 1);
+    Dy[c] = dfd<JD>(q, sy);
+    Dz[c] = dfd<KD>(q, sz);
+  }
+  vol_point_update(a, i, j, k, p, Dx, Dy, Dz);
+}
+
+/* ------------------------------------------------------------------------
  * Top-layer kernel (free surface): physical planes nk-4 .. nk-1, reproducing
  * the overwrite order inner -> timg -> vlow -> cfspml of the reference.
  * ---------------------------------------------------------------------- */
@@ -252,6 +297,15 @@ static void launch_dir(int variant, const VolArgs &a, int top, hipStream_t st) {
   }
   if (variant == 2) {
     launch_vol_lds(a, ID, JD, KD, st);
+    return;
+  }
+  if (variant == 7) {
+    const long ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + 3) / 4;
+    const long nchunk = (a.nk_run + a.zchunk - 1) / a.zchunk;
+    const long ngy = (nty + kSchedG - 1) / kSchedG;
+    const long items = ntx * ngy * nchunk * kSchedG * a.zchunk;
+    const long nb = ((items + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a);
     return;
   }
   hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), grd, blk, 0, st, a);

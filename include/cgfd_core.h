@@ -185,11 +185,12 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
 /* Kernel variant selection for the volume stencil (0 = default,
    1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
-   6 = software-pipelined LDS z-march).
+   6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
+   block schedule).
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
-/* z planes per block for variants 2..6 (4..4096). */
+/* z planes per block (variants 2..6) or per schedule chunk (variant 7), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
 /* Autotuner: times every (variant, z-chunk) candidate on `reps` repetitions
