@@ -146,6 +146,9 @@ class Core:
     def set_variant(self, v: int):
         L.check(self.lib, self.lib.cgfd_core_set_variant(self.handle, v))
 
+    def set_graph(self, on: bool):
+        L.check(self.lib, self.lib.cgfd_core_set_graph(self.handle, int(on)))
+
     def set_zchunk(self, z: int):
         L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
 

@@ -100,6 +100,25 @@ DEVI float dfd(const float *__restrict__ p, long s) {
     return CGFD_CB0 * p[-3 * s] + CGFD_CB1 * p[-2 * s] + CGFD_CB2 * p[-s] + CGFD_CB3 * p[0] + CGFD_CB4 * p[s];
 }
 
+/* Loads and stores at a 32-bit byte offset from a wave-uniform base pointer.
+   The address is base (SGPRs) + zero-extended offset (one VGPR), which the
+   compiler emits as the saddr form of global_load/store: one offset register
+   serves all nine fields and all metric arrays, instead of a 64-bit address
+   pair per load. Valid while one field array is below 4 GiB (checked at
+   create). */
+DEVI float ld32(const float *base, unsigned off) { return *(const float *)((const char *)base + off); }
+DEVI void st32(float *base, unsigned off, float v) { *(float *)((char *)base + off) = v; }
+
+template <int DIR>
+DEVI float dfd32(const float *__restrict__ b, unsigned o, unsigned s) {
+  if (DIR == 1)
+    return CGFD_CF0 * ld32(b, o - s) + CGFD_CF1 * ld32(b, o) + CGFD_CF2 * ld32(b, o + s) +
+           CGFD_CF3 * ld32(b, o + 2 * s) + CGFD_CF4 * ld32(b, o + 3 * s);
+  else
+    return CGFD_CB0 * ld32(b, o - 3 * s) + CGFD_CB1 * ld32(b, o - 2 * s) + CGFD_CB2 * ld32(b, o - s) +
+           CGFD_CB3 * ld32(b, o) + CGFD_CB4 * ld32(b, o + s);
+}
+
 /* the same operator on five loaded taps t[0..4] at offsets (-1..3) or (-3..1) */
 template <int DIR>
 DEVI float dfd5(float t0, float t1, float t2, float t3, float t4) {

@@ -44,13 +44,18 @@ struct FaultArgs {
    CFS-PML slab); tile id = bx + ntx*(by + nty*bz) */
 struct ZqTiles {
   static constexpr int TX = 64, TY = 4;
+  static constexpr int G = 8;  /* y-adjacent tiles per group of the block schedule */
   int ntx = 0, nty = 0, ntz = 0;
   int n_int = 0, n_pml = 0;
   int *d_int = nullptr, *d_pml = nullptr;
+  /* the same tiles in schedule order: x tile, group of G y tiles, z chunk,
+     y tile within the group (slowest to fastest) */
+  int *d_sint = nullptr, *d_spml = nullptr;
 };
 
 void launch_vol_zq(const VolArgs &a, const ZqTiles &t, int minw, int idir, int jdir, int kdir, hipStream_t st);
 void launch_vol_pipe(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st);
+void launch_vol_sched(const VolArgs &a, const ZqTiles &t, int minw, int idir, int jdir, int kdir, hipStream_t st);
 void launch_volume(int variant, const VolArgs &a, int idir, int jdir, int kdir, int top, hipStream_t st);
 void launch_vol_lds(const VolArgs &a, int idir, int jdir, int kdir, hipStream_t st);
 void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,

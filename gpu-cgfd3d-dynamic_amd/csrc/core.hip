@@ -87,6 +87,16 @@ struct cgfd_core {
   float *st_buf = nullptr;
   int st_n = 0, st_nt = 0, st_it0 = 0;
 
+  /* eight steps (one cycle of the operator pairs; an even number of level
+     swaps) captured once and replayed */
+  static constexpr int kGraphSteps = 8;
+  int use_graph = 1;
+  hipGraphExec_t gexec = nullptr;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
+
   float *wlev(int l) { return wave + (long)l * 9 * g.sicmp; }
   float *flev(int l) { return fwav + (long)l * nfault * 18 * g.sslice; }
 
@@ -108,6 +118,10 @@ static int check_desc(const cgfd_desc *d) {
   if (!d) return cgfd_set_error(CGFD_ERR_ARG, "null descriptor");
   if (d->ni < 8 || d->nj < 8 || d->nk < 8)
     return cgfd_set_error(CGFD_ERR_ARG, "grid too small: ni=%d nj=%d nk=%d (each must be >= 8)", d->ni, d->nj, d->nk);
+  /* the volume kernels address one field array with 32-bit byte offsets */
+  if ((double)(d->ni + 6) * (d->nj + 6) * (d->nk + 6) * 4.0 >= 4294967296.0)
+    return cgfd_set_error(CGFD_ERR_ARG, "one field array of %dx%dx%d points exceeds 4 GiB; split the domain",
+                          d->ni + 6, d->nj + 6, d->nk + 6);
   if (!d->metric || !d->lambda || !d->mu || !d->rho) return cgfd_set_error(CGFD_ERR_ARG, "metric/media pointers missing");
   if (!(d->dt > 0.0f)) return cgfd_set_error(CGFD_ERR_ARG, "dt must be > 0 (got %g)", (double)d->dt);
   if (d->number_fault < 0 || d->number_fault > CGFD_MAX_FAULTS)
@@ -187,14 +201,32 @@ static int build_zq_tiles(cgfd_core *c, int nvol) {
       }
   t.n_int = (int)ti.size();
   t.n_pml = (int)tp.size();
+  /* schedule order of the same two sets */
+  std::vector<char> is_pml((size_t)t.ntx * t.nty * std::max(t.ntz, 1), 0);
+  for (int id : tp) is_pml[id] = 1;
+  std::vector<int> si, sp;
+  const int G = ZqTiles::G, ngy = (t.nty + G - 1) / G;
+  for (int bx = 0; bx < t.ntx; bx++)
+    for (int gy = 0; gy < ngy; gy++)
+      for (int bz = 0; bz < t.ntz; bz++)
+        for (int cy = 0; cy < G; cy++) {
+          int by = gy * G + cy;
+          if (by >= t.nty) continue;
+          int id = bx + t.ntx * (by + t.nty * bz);
+          (is_pml[id] ? sp : si).push_back(id);
+        }
   int rc;
   if (t.n_int) {
     if ((rc = c->alloc(&t.d_int, ti.size()))) return rc;
     HIPCHK(hipMemcpy(t.d_int, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = c->alloc(&t.d_sint, si.size()))) return rc;
+    HIPCHK(hipMemcpy(t.d_sint, si.data(), si.size() * 4, hipMemcpyHostToDevice));
   }
   if (t.n_pml) {
     if ((rc = c->alloc(&t.d_pml, tp.size()))) return rc;
     HIPCHK(hipMemcpy(t.d_pml, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = c->alloc(&t.d_spml, sp.size()))) return rc;
+    HIPCHK(hipMemcpy(t.d_spml, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
   }
   c->zq_cache[c->zchunk] = t;
   c->zq = t;
@@ -403,10 +435,9 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
 extern "C" int cgfd_core_destroy(cgfd_core *c) {
   if (!c) return CGFD_OK;
   (void)hipSetDevice(c->dev);
-  if (c->st) {
-    (void)hipStreamSynchronize(c->st);
-    (void)hipStreamDestroy(c->st);
-  }
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  c->drop_graph();
+  if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return CGFD_OK;
 }
@@ -454,9 +485,11 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
 
 /* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
    no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march,
-   7 direct loads on an L2-aware block schedule */
+   7..9 direct loads on an L2-aware block schedule with the interior tiles
+   at the compiler's occupancy / >= 4 / >= 6 waves per SIMD */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
+  else if (c->variant >= 7) launch_vol_sched(va, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
   else if (c->variant >= 3 && c->variant <= 5) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
   else launch_volume(c->variant, va, idir, jdir, kdir, 0, c->st);
 }
@@ -533,14 +566,63 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
 
 }  // namespace
 
+namespace {
+/* Captures kGraphSteps steps starting at a step index that is a multiple of
+   kGraphSteps. Capturing only records the launches: the host-side level
+   rotation is advanced by the recording and, after an even number of swaps,
+   is back where it started; the step counter is restored. */
+int capture_graph(cgfd_core *c) {
+  const int it_save = c->it_host;
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+  int rc = CGFD_OK;
+  for (int s = 0; s < cgfd_core::kGraphSteps && rc == CGFD_OK; s++) rc = do_step(c, nullptr, nullptr);
+  hipError_t e = hipStreamEndCapture(c->st, &graph);
+  c->it_host = it_save;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    c->gexec = nullptr;
+    return cgfd_set_error(CGFD_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  return CGFD_OK;
+}
+}  // namespace
+
 extern "C" int cgfd_core_run(cgfd_core *c, int nsteps) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (nsteps < 0) return cgfd_set_error(CGFD_ERR_ARG, "nsteps must be >= 0");
   HIPCHK(hipSetDevice(c->dev));
-  for (int s = 0; s < nsteps; s++) {
+  const int G = cgfd_core::kGraphSteps;
+  const bool graph = c->use_graph && c->nan_every <= 0;
+  int left = nsteps;
+  while (left > 0) {
+    if (graph && left >= G && c->it_host % G == 0) {
+      if (!c->gexec) {
+        int rc = capture_graph(c);
+        if (rc) return rc;
+      }
+      HIPCHK(hipGraphLaunch(c->gexec, c->st));
+      c->it_host += G;
+      left -= G;
+      continue;
+    }
     int rc = do_step(c, nullptr, nullptr);
     if (rc) return rc;
+    left--;
   }
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_graph(cgfd_core *c, int on) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  c->use_graph = on ? 1 : 0;
+  if (!on) c->drop_graph();
   return CGFD_OK;
 }
 
@@ -656,6 +738,7 @@ extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int n
   c->st_n = n;
   c->st_nt = nt_max;
   c->st_it0 = c->it_host;
+  c->drop_graph();
   return CGFD_OK;
 }
 
@@ -677,8 +760,9 @@ extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant == 0) variant = 4;
-  if (variant < 1 || variant > 7) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant < 1 || variant > 9) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
+  c->drop_graph();
   return CGFD_OK;
 }
 
@@ -687,12 +771,14 @@ extern "C" int cgfd_core_set_zchunk(cgfd_core *c, int zchunk) {
   if (zchunk < 4 || zchunk > 4096) return cgfd_set_error(CGFD_ERR_ARG, "zchunk %d out of range [4, 4096]", zchunk);
   HIPCHK(hipSetDevice(c->dev));
   c->zchunk = zchunk;
+  c->drop_graph();
   return build_zq_tiles(c, c->nvol);
 }
 
 extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int *zchunk_out, float *ms_out) {
   if (!c || reps < 1) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
   HIPCHK(hipSetDevice(c->dev));
+  c->drop_graph();
   if (c->nvol <= 0) {
     if (variant_out) *variant_out = c->variant;
     if (zchunk_out) *zchunk_out = c->zchunk;
@@ -704,11 +790,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[7] = {1, 2, 3, 4, 5, 6, 7};
+  const int variants[9] = {1, 2, 3, 4, 5, 6, 7, 8, 9};
   const int chunks[4] = {16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 7; vi++)
+  for (int vi = 0; vi < 9; vi++)
     for (int zi = 0; zi < 4; zi++) {
       int v = variants[vi], z = chunks[zi];
       if (v == 1 && zi > 0) continue;  /* the direct kernel has no z chunks */

@@ -141,6 +141,45 @@ DEVI void rk_point(const VolArgs &a, long p, const float *h) {
   }
 }
 
+/* rk_point with 32-bit byte offsets (o = 4 p) */
+DEVI void rk_point32(const VolArgs &a, unsigned o, const float *h) {
+  const long sc = a.g.sicmp;
+  if (a.istage == 0) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      float pv = ld32(a.pre + c * sc, o);
+      st32(a.tnew + c * sc, o, pv + a.ca * h[c]);
+      st32(a.end + c * sc, o, pv + a.cb * h[c]);
+    }
+  } else if (a.istage < 3) {
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      st32(a.tnew + c * sc, o, ld32(a.pre + c * sc, o) + a.ca * h[c]);
+      st32(a.end + c * sc, o, ld32(a.end + c * sc, o) + a.cb * h[c]);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 9; c++) st32(a.end + c * sc, o, ld32(a.end + c * sc, o) + a.cb * h[c]);
+  }
+}
+
+/* vol_point_update with 32-bit byte offsets for the per-point arrays */
+template <bool PML = true>
+DEVI void vol_point_update32(const VolArgs &a, int i, int j, int k, long p, unsigned o, const float *Dx,
+                             const float *Dy, const float *Dz) {
+  const long sc = a.g.sicmp;
+  const float *M = a.met;
+  float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
+  float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
+  float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
+  float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+  float h[9];
+  rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
+  if (PML && a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h);
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  rk_point32(a, o, h);
+}
+
 /* rhs + PML + RK update of one point from its 27 derivatives */
 DEVI void vol_point_update(const VolArgs &a, int i, int j, int k, long p, const float *Dx, const float *Dy,
                            const float *Dz) {

@@ -23,6 +23,25 @@
 
 namespace cgfd {
 
+/* one point of the direct kernels: 27 derivatives from the cache hierarchy,
+   then rhs + PML + RK */
+template <int ID, int JD, int KD, bool PML = true>
+DEVI void direct_point(const VolArgs &a, int i, int j, int k) {
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  const unsigned o = (unsigned)p * 4u;
+  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
+  float Dx[9], Dy[9], Dz[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float *b = a.cur + c * sc;
+    Dx[c] = dfd32<ID>(b, o, 4u);
+    Dy[c] = dfd32<JD>(b, o, by);
+    Dz[c] = dfd32<KD>(b, o, bz);
+  }
+  vol_point_update32<PML>(a, i, j, k, p, o, Dx, Dy, Dz);
+}
+
 /* ------------------------------------------------------------------------
  * Direct-load volume kernel: one thread per point, neighbours from the
  * cache hierarchy. Handles physical planes k_lo .. k_lo+nk_run-1.
@@ -33,62 +52,77 @@ __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a) {
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
   const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
-  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
-  const long p = i + j * sy + (long)k * sz;
-  float Dx[9], Dy[9], Dz[9];
-#pragma unroll
-  for (int c = 0; c < 9; c++) {
-    const float *q = a.cur + c * sc + p;
-    Dx[c] = dfd<ID>(q, 1);
-    Dy[c] = dfd<JD>(q, sy);
-    Dz[c] = dfd<KD>(q, sz);
-  }
-  vol_point_update(a, i, j, k, p, Dx, Dy, Dz);
+  direct_point<ID, JD, KD>(a, i, j, k);
 }
 
 /* ------------------------------------------------------------------------
- * The same per-point kernel on an L2-aware block schedule. Each XCD (block
- * ids L with L % 8 == x) takes a contiguous range of work items; an item is
- * one 64 x 4 tile of one plane, ordered so that the blocks resident on an
- * XCD at the same time cover zchunk consecutive planes of kSchedG y-adjacent
- * tiles. The z and y neighbours a block reads were then fetched into that
- * XCD's L2 by its neighbours moments earlier, instead of coming again from
- * HBM or the MALL.
+ * The same per-point kernel on an L2-aware block schedule. A work item is one
+ * 64 x 4 tile of one plane; items run in the order of a host-built tile list
+ * (x tile, group of 8 y-adjacent tiles, z chunk, y tile in the group) with
+ * the planes of a chunk fastest. Each XCD (block ids L with L % 8 == x) takes
+ * a contiguous range of items, so the blocks resident on one XCD cover a z
+ * chunk of 8 y-adjacent tiles: the z and y neighbours a block reads were
+ * fetched into that XCD's L2 by its neighbours moments earlier. Interior
+ * tiles run without the CFS-PML code (fewer registers); MINW > 0 asks the
+ * compiler for at least MINW waves per SIMD.
  * ---------------------------------------------------------------------- */
-constexpr int kSchedG = 8;
-
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256) k_vol_sched(VolArgs a) {
-  const int L = blockIdx.x;
-  const int per = (int)(gridDim.x / 8);
-  int t = (L % 8) * per + L / 8;
+template <int ID, int JD, int KD, bool PML, int MINW>
+__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
+    k_vol_sched(VolArgs a, const int *__restrict__ tiles, long nitems, int ntx, int nty) {
+  const long L = blockIdx.x;
+  const long per = (long)(gridDim.x / 8);
+  const long item = (L % 8) * per + L / 8;
+  if (item >= nitems) return;
   const int Kc = a.zchunk;
-  const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + 3) / 4;
-  const int nchunk = (a.nk_run + Kc - 1) / Kc;
-  const int ngy = (nty + kSchedG - 1) / kSchedG;
-  const int kk = t % Kc; t /= Kc;
-  const int cy = t % kSchedG; t /= kSchedG;
-  const int ch = t % nchunk; t /= nchunk;
-  const int gy = t % ngy;
-  const int bx = t / ngy;
-  const int by = gy * kSchedG + cy;
-  const int kz = ch * Kc + kk;
-  if (bx >= ntx || by >= nty || kz >= a.nk_run) return;
+  const int tile = tiles[item / Kc];
+  const int kk = (int)(item % Kc);
+  const int bx = tile % ntx;
+  const int by = (tile / ntx) % nty;
+  const int bz = tile / (ntx * nty);
+  const int kz = bz * Kc + kk;
+  if (kz >= a.nk_run) return;
   const int ix = bx * 64 + (int)threadIdx.x;
   const int iy = by * 4 + (int)threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
-  const int i = ix + 3, j = iy + 3, k = a.k_lo + kz + 3;
-  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
-  const long p = i + j * sy + (long)k * sz;
-  float Dx[9], Dy[9], Dz[9];
-#pragma unroll
-  for (int c = 0; c < 9; c++) {
-    const float *q = a.cur + c * sc + p;
-    Dx[c] = dfd<ID>(q, 1);
-    Dy[c] = dfd<JD>(q, sy);
-    Dz[c] = dfd<KD>(q, sz);
+  direct_point<ID, JD, KD, PML>(a, ix + 3, iy + 3, a.k_lo + kz + 3);
+}
+
+template <int ID, int JD, int KD, int MINW>
+static void launch_sched_t(const VolArgs &a, const ZqTiles &t, hipStream_t st) {
+  dim3 blk(64, 4, 1);
+  if (t.n_int > 0) {
+    const long items = (long)t.n_int * a.zchunk;
+    const long nb = ((items + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, false, MINW>), dim3((unsigned)nb), blk, 0, st, a, t.d_sint, items,
+                       t.ntx, t.nty);
   }
-  vol_point_update(a, i, j, k, p, Dx, Dy, Dz);
+  if (t.n_pml > 0) {
+    const long items = (long)t.n_pml * a.zchunk;
+    const long nb = ((items + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, true, 0>), dim3((unsigned)nb), blk, 0, st, a, t.d_spml, items,
+                       t.ntx, t.nty);
+  }
+}
+
+template <int MINW>
+static void launch_sched_w(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st) {
+  switch (idir * 4 + jdir * 2 + kdir) {
+    case 0: launch_sched_t<0, 0, 0, MINW>(a, t, st); break;
+    case 1: launch_sched_t<0, 0, 1, MINW>(a, t, st); break;
+    case 2: launch_sched_t<0, 1, 0, MINW>(a, t, st); break;
+    case 3: launch_sched_t<0, 1, 1, MINW>(a, t, st); break;
+    case 4: launch_sched_t<1, 0, 0, MINW>(a, t, st); break;
+    case 5: launch_sched_t<1, 0, 1, MINW>(a, t, st); break;
+    case 6: launch_sched_t<1, 1, 0, MINW>(a, t, st); break;
+    default: launch_sched_t<1, 1, 1, MINW>(a, t, st); break;
+  }
+}
+
+/* w: 0 = the compiler's choice, 1 = >= 4 waves/SIMD, 2 = >= 6 waves/SIMD */
+void launch_vol_sched(const VolArgs &a, const ZqTiles &t, int w, int idir, int jdir, int kdir, hipStream_t st) {
+  if (w == 1) launch_sched_w<4>(a, t, idir, jdir, kdir, st);
+  else if (w >= 2) launch_sched_w<6>(a, t, idir, jdir, kdir, st);
+  else launch_sched_w<0>(a, t, idir, jdir, kdir, st);
 }
 
 /* ------------------------------------------------------------------------
@@ -297,15 +331,6 @@ static void launch_dir(int variant, const VolArgs &a, int top, hipStream_t st) {
   }
   if (variant == 2) {
     launch_vol_lds(a, ID, JD, KD, st);
-    return;
-  }
-  if (variant == 7) {
-    const long ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + 3) / 4;
-    const long nchunk = (a.nk_run + a.zchunk - 1) / a.zchunk;
-    const long ngy = (nty + kSchedG - 1) / kSchedG;
-    const long items = ntx * ngy * nchunk * kSchedG * a.zchunk;
-    const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a);
     return;
   }
   hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), grd, blk, 0, st, a);

@@ -140,6 +140,15 @@ int cgfd_core_destroy(cgfd_core *core);
    time loop, drv_rk_curv_col.cu:209-636, without the I/O hooks). */
 int cgfd_core_run(cgfd_core *core, int nsteps);
 
+/* Graph replay (default on): cgfd_core_run captures eight consecutive steps
+   (one cycle of the operator-direction pairs) into a HIP graph the first time
+   it reaches a step index divisible by 8 and replays it for every further
+   block of eight steps; other steps are launched directly. Changing the
+   variant, z chunk or stations drops the graph. Off when the NaN check is on.
+   No counterpart in the reference, which launches every kernel from the host
+   (drv_rk_curv_col.cu:209-636). */
+int cgfd_core_set_graph(cgfd_core *core, int on);
+
 /* Current step counter (number of completed steps). */
 int cgfd_core_step_index(const cgfd_core *core);
 
@@ -186,11 +195,12 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
    6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
-   block schedule).
+   block schedule, 8 = the same with >= 4 waves/SIMD on interior tiles,
+   9 = the same with >= 6 waves/SIMD).
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
-/* z planes per block (variants 2..6) or per schedule chunk (variant 7), 4..4096. */
+/* z planes per block (variants 2..6) or per schedule chunk (variants 7..9), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
 /* Autotuner: times every (variant, z-chunk) candidate on `reps` repetitions

@@ -142,7 +142,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation (direct loads, LDS z-march, z queue)."""
     from cgfd import configs, model
@@ -156,3 +156,27 @@ def test_volume_variants_match_oracle(built_libs, variant):
     wg = interior(core.read_wavefield())
     wo = interior(orc.read_wavefield())
     assert rel_l2(wg, wo) <= TOL
+
+
+def test_graph_replay_matches_eager(built_libs):
+    """Eight-step graphs replayed several times, with eager steps and
+    synchronisations in between, against direct launches: bit-identical
+    wavefield, fault outputs and stations after every replay."""
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    ijk = sc.stations + 3
+    cg = model.Core(st)
+    ce = model.Core(st)
+    ce.set_graph(False)
+    for c in (cg, ce):
+        c.set_stations(ijk, 64)
+    for rep, n in enumerate((3, 13, 8, 16, 1)):
+        cg.run(n)
+        ce.run(n)
+        cg.sync()
+        ce.sync()
+        assert cg.step_index == ce.step_index
+        assert np.array_equal(cg.read_wavefield(), ce.read_wavefield()), f"wavefield differs after block {rep}"
+        assert np.array_equal(cg.read_fault(0), ce.read_fault(0)), f"fault differs after block {rep}"
+    assert np.array_equal(cg.read_stations(), ce.read_stations())
