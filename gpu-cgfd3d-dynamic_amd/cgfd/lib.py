@@ -104,9 +104,19 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_set_graph": (I32, [VP, I32]),
         "cgfd_core_autotune": (I32, [VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(F32)]),
         "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),
+        "cgfd_batch_create": (I32, [C.POINTER(Desc), I32, C.POINTER(VP)]),
+        "cgfd_batch_active": (I32, [VP]),
+        "cgfd_batch_steps": (I32, [VP, I32]),
+        "cgfd_batch_read_wavefield": (I32, [VP, I32, VP]),
+        "cgfd_batch_write_wavefield": (I32, [VP, I32, VP]),
+        "cgfd_batch_read_fault": (I32, [VP, I32, I32, VP]),
+        "cgfd_batch_read_fault_state": (I32, [VP, I32, I32, VP]),
+        "cgfd_batch_read_pg": (I32, [VP, I32, VP]),
+        "cgfd_batch_read_stations": (I32, [VP, I32, VP, C.POINTER(I32)]),
         "cgfd_setup_create": (I32, [C.POINTER(SetupParams), C.POINTER(VP)]),
         "cgfd_setup_destroy": (I32, [VP]),
         "cgfd_setup_get_desc": (I32, [VP, C.POINTER(Desc)]),
+        "cgfd_setup_scenario_desc": (I32, [VP, C.POINTER(VP), I32, C.POINTER(Desc)]),
         "cgfd_setup_coords": (C.POINTER(F32), [VP]),
         "cgfd_setup_vec_n": (C.POINTER(F32), [VP, I32]),
         "cgfd_setup_metric": (I32, [I32, I32, I32, VP, VP, VP, VP]),

@@ -225,3 +225,88 @@ class Core:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Several rupture scenarios on one grid (include/cgfd_core.h, batched
+    ensembles). ``scenarios`` is a list of (faults, nt) where ``faults`` is a
+    list of FaultSpec-like field dicts (one per fault of the setup, keys
+    FAULT_FIELDS) and nt the scenario's number of steps; the first scenario
+    may be None to reuse the setup's own fault inputs."""
+
+    def __init__(self, setup: Setup, scenarios, variant: int = 0):
+        self.lib = L.load_library()
+        self.setup = setup
+        nf = len(setup.sc.faults)
+        descs = (L.Desc * len(scenarios))()
+        self._keep = []
+        for s, (fields, nt) in enumerate(scenarios):
+            if fields is None:
+                d = L.Desc.from_buffer_copy(setup.desc)
+                d.nt_total = nt
+            else:
+                arrs = []
+                for fid in range(nf):
+                    for name in FAULT_FIELDS:
+                        a = np.ascontiguousarray(fields[fid][name], np.float32)
+                        arrs.append(a)
+                self._keep.append(arrs)
+                ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+                d = L.Desc()
+                L.check(self.lib, self.lib.cgfd_setup_scenario_desc(setup.handle, ptrs, nt, C.byref(d)))
+            descs[s] = d
+        h = C.c_void_p()
+        L.check(self.lib, self.lib.cgfd_batch_create(descs, len(scenarios), C.byref(h)))
+        self.handle = h
+        self.n = len(scenarios)
+        if variant:
+            L.check(self.lib, self.lib.cgfd_core_set_variant(h, variant))
+        self.shape_ext = setup.sc.shape_ext
+
+    def run(self, nsteps: int):
+        L.check(self.lib, self.lib.cgfd_core_run(self.handle, nsteps))
+
+    def sync(self):
+        L.check(self.lib, self.lib.cgfd_core_sync(self.handle))
+
+    def set_graph(self, on: bool):
+        L.check(self.lib, self.lib.cgfd_core_set_graph(self.handle, int(on)))
+
+    def set_variant(self, v: int):
+        L.check(self.lib, self.lib.cgfd_core_set_variant(self.handle, v))
+
+    @property
+    def active(self) -> int:
+        return self.lib.cgfd_batch_active(self.handle)
+
+    def steps(self, s: int) -> int:
+        return self.lib.cgfd_batch_steps(self.handle, s)
+
+    def read_wavefield(self, s: int) -> np.ndarray:
+        nz, ny, nx = self.shape_ext
+        out = np.empty((9, nz, ny, nx), np.float32)
+        L.check(self.lib, self.lib.cgfd_batch_read_wavefield(self.handle, s, L.ptr(out)))
+        return out
+
+    def read_fault(self, s: int, fid: int = 0) -> np.ndarray:
+        nz, ny, _ = self.shape_ext
+        out = np.empty((11, nz, ny), np.float32)
+        L.check(self.lib, self.lib.cgfd_batch_read_fault(self.handle, s, fid, L.ptr(out)))
+        return out
+
+    def read_pg(self, s: int) -> np.ndarray:
+        _, ny, nx = self.shape_ext
+        out = np.empty((15, ny, nx), np.float32)
+        L.check(self.lib, self.lib.cgfd_batch_read_pg(self.handle, s, L.ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.cgfd_core_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -91,6 +91,25 @@ struct VolArgs {
 };
 
 /* ---------------------------------------------------------------------- */
+/* Scenarios of a batch share the grid, metric, media, CFS-PML profiles and
+   fault coefficients; each has its own wavefield levels, PML auxiliaries,
+   fault state, fault parameters and outputs, stored as equal-sized blocks
+   one after another. A kernel loops over the active scenarios, loading the
+   shared data of a point once. A plain run is a batch of one. */
+constexpr int kMaxScen = 64;
+
+struct BatArgs {
+  int nact;              /* active scenarios of this launch */
+  int act[kMaxScen];     /* their indices */
+  long wstride;          /* floats between scenarios: wavefield levels */
+  long pstride;          /* CFS-PML auxiliaries */
+  long fwstride;         /* fault split-node state levels */
+  long fpf, fpi;         /* fault parameters/outputs (floats), flags (ints) */
+  long pgstride;         /* PG + displacement */
+  long ststride;         /* station records */
+};
+
+/* ---------------------------------------------------------------------- */
 /* volume operators with the FP32 table; DIR 1 forward, 0 backward */
 template <int DIR>
 DEVI float dfd(const float *__restrict__ p, long s) {

@@ -11,6 +11,8 @@ namespace cgfd {
 /* one fault in device memory (fault_one_t + fault_coef_one_t + the T1
    traction columns of fault_wav_t; fault_info.h:11-119, fault_wav_t.h:18-63) */
 struct FaultDev {
+  /* per scenario (block of 45 floats per node: T0 3, fric 4, out 11, T1 21,
+     hT1 3, mT1 3; and 2 ints: flag_rup, init_t0_flag); the rest is shared */
   int i0;                        /* ghost-inclusive x index of the fault plane */
   const float *T0;               /* [3][sslice] T0x,T0y,T0z */
   const float *fric;             /* [4][sslice] mu_s, mu_d, Dc, C0 */
@@ -33,12 +35,46 @@ struct FaultArgs {
   float *w_cur, *w_pre, *w_tnew, *w_end;
   float *f_cur, *f_pre, *f_tnew, *f_end;   /* level bases; fault id at +id*18*sslice */
   int nfault;
-  FaultDev f[5];
+  const FaultDev *ftab;                     /* [nfault] in device memory (scenario 0) */
+  long sfp, sfi;                            /* this scenario's offsets into the per-node blocks */
   int istage, idir, jdir, kdir;
   int isfree, imethod, fix_q1;
   float dt, ca, cb, rkb;
   const int *it_dev;                        /* completed steps (device counter) */
 };
+
+/* the fault arguments of scenario s: its wavefield and fault-state levels,
+   and the offsets of its per-node parameters, outputs and traction columns
+   (one block of 45 floats and 2 ints per node and fault, see FaultDev) */
+DEVI FaultArgs fault_at(const FaultArgs &a, const BatArgs &b, int s) {
+  FaultArgs r = a;
+  const long w = (long)s * b.wstride, f = (long)s * b.fwstride;
+  r.w_cur += w;
+  r.w_pre += w;
+  if (r.w_tnew) r.w_tnew += w;
+  r.w_end += w;
+  r.f_cur += f;
+  r.f_pre += f;
+  if (r.f_tnew) r.f_tnew += f;
+  r.f_end += f;
+  r.sfp = (long)s * b.fpf;
+  r.sfi = (long)s * b.fpi;
+  return r;
+}
+
+/* fault `id` as seen by the scenario of `a` */
+DEVI FaultDev fdev(const FaultArgs &a, int id) {
+  FaultDev F = a.ftab[id];
+  F.T0 += a.sfp;
+  F.fric += a.sfp;
+  F.out += a.sfp;
+  F.T1 += a.sfp;
+  F.hT1 += a.sfp;
+  F.mT1 += a.sfp;
+  F.flag_rup += a.sfi;
+  F.init_t0_flag += a.sfi;
+  return F;
+}
 
 /* tile lists of the z-queue volume kernel (interior tiles / tiles touching a
    CFS-PML slab); tile id = bx + ntx*(by + nty*bz) */
@@ -55,21 +91,24 @@ struct ZqTiles {
 
 void launch_vol_zq(const VolArgs &a, const ZqTiles &t, int minw, int idir, int jdir, int kdir, hipStream_t st);
 void launch_vol_pipe(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st);
-void launch_vol_sched(const VolArgs &a, const ZqTiles &t, int minw, int idir, int jdir, int kdir, hipStream_t st);
-void launch_volume(int variant, const VolArgs &a, int idir, int jdir, int kdir, int top, hipStream_t st);
+void launch_vol_sched(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int minw, int idir, int jdir, int kdir,
+                      hipStream_t st);
+void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
+                   hipStream_t st);
 void launch_vol_lds(const VolArgs &a, int idir, int jdir, int kdir, hipStream_t st);
 void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
                     hipStream_t st);
-void launch_pg(const Grid &g, const float *wend, const float *wpre, float *PG, float *Dis, float dt, hipStream_t st);
+void launch_pg(const Grid &g, const BatArgs &b, const float *wend, const float *wpre, float *PG, float *Dis, float dt,
+               hipStream_t st);
 
-void launch_wave2fault(const FaultArgs &a, hipStream_t st);
-void launch_trial(const FaultArgs &a, hipStream_t st);
-void launch_fault_update(const FaultArgs &a, hipStream_t st);
+void launch_wave2fault(const FaultArgs &a, const BatArgs &b, hipStream_t st);
+void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st);
+void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st);
 void launch_step_end(int *it_dev, hipStream_t st);
 void launch_zero(float *p, long n, hipStream_t st);
 void launch_nan_check(const float *p, long n, int *flag, hipStream_t st);
-void launch_record_stations(const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max,
-                            const int *it_dev, int it0, hipStream_t st);
+void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
+                            int nt_max, const int *it_dev, int it0, hipStream_t st);
 
 }  // namespace cgfd
 

@@ -59,7 +59,7 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
-  int variant = 4;
+  int variant = 1;
   int zchunk = 32;
   ZqTiles zq;                       /* tile lists for the current zchunk */
   std::map<int, ZqTiles> zq_cache;  /* by zchunk */
@@ -77,10 +77,25 @@ struct cgfd_core {
   float *fwav = nullptr;   /* 4 levels x nfault x 18 x sslice */
   int fl_pre = 0, fl_a = 1, fl_b = 2, fl_end = 3;
   FaultDev flt[5]{};
+  FaultDev *d_ftab = nullptr;       /* flt[] in device memory */
 
   int *it_dev = nullptr;
   int *nan_flag = nullptr;
   int it_host = 0;
+
+  /* scenarios (a plain run is a batch of one) */
+  int nscen = 1;
+  int scen_nt[kMaxScen]{};           /* steps each scenario runs (batches only) */
+  BatArgs bat{};                     /* strides + the active list of the next launch */
+  bool active(int s) const { return nscen == 1 || it_host < scen_nt[s]; }
+  int refresh_active() {             /* returns the number of active scenarios */
+    int n = 0;
+    for (int s = 0; s < nscen; s++)
+      if (active(s)) bat.act[n++] = s;
+    if (n != bat.nact) drop_graph();
+    bat.nact = n;
+    return n;
+  }
   float *PG = nullptr, *Dis = nullptr;
 
   long *st_idx = nullptr;
@@ -233,8 +248,13 @@ static int build_zq_tiles(cgfd_core *c, int nvol) {
   return CGFD_OK;
 }
 
-static int upload_faults(cgfd_core *c, const cgfd_desc *d) {
+/* fault arrays: the per-scenario block (T0 3, fric 4, out 11, T1 21, hT1 3,
+   mT1 3 floats and flag_rup, init_t0_flag ints per node) for every scenario,
+   and the shared masks and coefficients from the first descriptor */
+static int upload_faults(cgfd_core *c, const cgfd_desc *ds) {
   const size_t ss = c->g.sslice, ny = c->g.ny;
+  const cgfd_desc *d = &ds[0];
+  const int B = c->nscen;
   for (int id = 0; id < c->nfault; id++) {
     const cgfd_fault_desc &f = d->fault[id];
     FaultDev &F = c->flt[id];
@@ -243,18 +263,34 @@ static int upload_faults(cgfd_core *c, const cgfd_desc *d) {
     float *dp;
     int *ip;
     int rc;
-    /* T0 [3][ss] */
-    h.assign(3 * ss, 0.f);
-    for (size_t p = 0; p < ss; p++) { h[p] = f.T0x[p]; h[ss + p] = f.T0y[p]; h[2 * ss + p] = f.T0z[p]; }
-    if ((rc = c->alloc(&dp, h.size()))) return rc;
-    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
-    F.T0 = dp;
-    /* friction [4][ss] */
-    h.assign(4 * ss, 0.f);
-    for (size_t p = 0; p < ss; p++) { h[p] = f.mu_s[p]; h[ss + p] = f.mu_d[p]; h[2 * ss + p] = f.Dc[p]; h[3 * ss + p] = f.C0[p]; }
-    if ((rc = c->alloc(&dp, h.size()))) return rc;
-    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
-    F.fric = dp;
+    /* per-scenario blocks */
+    float *fb;
+    int *ib;
+    if ((rc = c->alloc(&fb, (size_t)B * 45 * ss))) return rc;
+    if ((rc = c->alloc(&ib, (size_t)B * 2 * ss))) return rc;
+    HIPCHK(hipMemset(ib, 0, (size_t)B * 2 * ss * 4));  /* flag_rup = 0 (fault_info.cu:1069), init_t0_flag = 0 (:960) */
+    h.assign(45 * ss, 0.f);
+    for (int sn = 0; sn < B; sn++) {
+      const cgfd_fault_desc &fs = ds[sn].fault[id];
+      std::fill(h.begin(), h.end(), 0.f);  /* T1, hT1, mT1 start at zero (fault_wav_t.cu:59-78) */
+      for (size_t p = 0; p < ss; p++) {
+        h[p] = fs.T0x[p]; h[ss + p] = fs.T0y[p]; h[2 * ss + p] = fs.T0z[p];
+        h[3 * ss + p] = fs.mu_s[p]; h[4 * ss + p] = fs.mu_d[p]; h[5 * ss + p] = fs.Dc[p]; h[6 * ss + p] = fs.C0[p];
+        h[7 * ss + p] = fs.Tn0 ? fs.Tn0[p] : 0.f;
+        h[8 * ss + p] = fs.Ts10 ? fs.Ts10[p] : 0.f;
+        h[9 * ss + p] = fs.Ts20 ? fs.Ts20[p] : 0.f;
+        h[17 * ss + p] = -9999.9f;  /* Init_t0 (fault_info.cu:1068) */
+      }
+      HIPCHK(hipMemcpy(fb + (size_t)sn * 45 * ss, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    }
+    F.T0 = fb;
+    F.fric = fb + 3 * ss;
+    F.out = fb + 7 * ss;
+    F.T1 = fb + 18 * ss;
+    F.hT1 = fb + 39 * ss;
+    F.mT1 = fb + 42 * ss;
+    F.flag_rup = ib;
+    F.init_t0_flag = ib + ss;
     /* masks */
     const int *msk[4] = {f.united, f.faultgrid, f.rup_index_y, f.rup_index_z};
     const int *dmsk[4];
@@ -264,21 +300,6 @@ static int upload_faults(cgfd_core *c, const cgfd_desc *d) {
       dmsk[q] = ip;
     }
     F.united = dmsk[0]; F.faultgrid = dmsk[1]; F.rupy = dmsk[2]; F.rupz = dmsk[3];
-    if ((rc = c->alloc(&ip, 2 * ss))) return rc;
-    HIPCHK(hipMemset(ip, 0, 2 * ss * 4));  /* flag_rup = 0 (fault_info.cu:1069), init_t0_flag = 0 (:960) */
-    F.flag_rup = ip;
-    F.init_t0_flag = ip + ss;
-    /* outputs [11][ss] */
-    h.assign(11 * ss, 0.f);
-    for (size_t p = 0; p < ss; p++) {
-      h[p] = f.Tn0 ? f.Tn0[p] : 0.f;
-      h[ss + p] = f.Ts10 ? f.Ts10[p] : 0.f;
-      h[2 * ss + p] = f.Ts20 ? f.Ts20[p] : 0.f;
-      h[10 * ss + p] = -9999.9f;  /* Init_t0 (fault_info.cu:1068) */
-    }
-    if ((rc = c->alloc(&dp, h.size()))) return rc;
-    HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
-    F.out = dp;
     /* rho_f [2][ss] */
     if ((rc = c->alloc(&dp, 2 * ss))) return rc;
     HIPCHK(hipMemcpy(dp, f.rho_f, 2 * ss * 4, hipMemcpyHostToDevice));
@@ -321,21 +342,45 @@ static int upload_faults(cgfd_core *c, const cgfd_desc *d) {
     if ((rc = c->alloc(&dp, h.size()))) return rc;
     HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     F.Fm = dp;
-    /* T1 [3][7][ss], hT1/mT1 [3][ss]: zero (fault_wav_t.cu:59-78) */
-    if ((rc = c->alloc(&dp, 3 * 7 * ss + 6 * ss))) return rc;
-    HIPCHK(hipMemset(dp, 0, (3 * 7 * ss + 6 * ss) * 4));
-    F.T1 = dp;
-    F.hT1 = dp + 21 * ss;
-    F.mT1 = dp + 24 * ss;
   }
   return CGFD_OK;
 }
 
-extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
+/* scenarios of a batch must agree on everything the kernels share */
+static int check_batch(const cgfd_desc *ds, int nscen) {
+  const cgfd_desc &a = ds[0];
+  for (int s = 1; s < nscen; s++) {
+    const cgfd_desc &b = ds[s];
+    int rc = check_desc(&b);
+    if (rc) return rc;
+    bool same = a.ni == b.ni && a.nj == b.nj && a.nk == b.nk && a.free_top == b.free_top && a.imethod == b.imethod &&
+                a.number_fault == b.number_fault && a.dt == b.dt && a.compat_flags == b.compat_flags &&
+                a.device_id == b.device_id;
+    for (int dim = 0; dim < 3 && same; dim++)
+      for (int side = 0; side < 2; side++)
+        same = same && a.pml_is_sides[dim][side] == b.pml_is_sides[dim][side] &&
+               a.pml_ni1[dim][side] == b.pml_ni1[dim][side] && a.pml_ni2[dim][side] == b.pml_ni2[dim][side] &&
+               a.pml_nj1[dim][side] == b.pml_nj1[dim][side] && a.pml_nj2[dim][side] == b.pml_nj2[dim][side] &&
+               a.pml_nk1[dim][side] == b.pml_nk1[dim][side] && a.pml_nk2[dim][side] == b.pml_nk2[dim][side];
+    for (int id = 0; id < a.number_fault && same; id++)
+      same = a.fault[id].fault_x_index == b.fault[id].fault_x_index;
+    if (!same)
+      return cgfd_set_error(CGFD_ERR_ARG, "scenario %d differs from scenario 0 in grid, boundaries, method, faults or dt",
+                            s);
+    if (b.nt_total < 0) return cgfd_set_error(CGFD_ERR_ARG, "scenario %d: nt_total < 0", s);
+  }
+  return CGFD_OK;
+}
+
+static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   if (!out) return cgfd_set_error(CGFD_ERR_ARG, "null output pointer");
   *out = nullptr;
+  if (!ds) return cgfd_set_error(CGFD_ERR_ARG, "null descriptor");
+  if (nscen < 1 || nscen > kMaxScen) return cgfd_set_error(CGFD_ERR_ARG, "nscen=%d out of range [1,%d]", nscen, kMaxScen);
+  const cgfd_desc *d = &ds[0];
   int rc = check_desc(d);
   if (rc) return rc;
+  if ((rc = check_batch(ds, nscen))) return rc;
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return cgfd_set_error(CGFD_ERR_NODEV, "no HIP device available");
@@ -361,10 +406,17 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
   c->fix_q1 = (d->compat_flags & CGFD_COMPAT_FIX_F2W_SWAP) ? 1 : 0;
   c->nan_every = d->qc_check_nan_every;
   const long sc = g.sicmp;
+  const int B = nscen;
+  c->nscen = B;
+  for (int sn = 0; sn < B; sn++) c->scen_nt[sn] = ds[sn].nt_total;
+  c->bat.nact = B;
+  for (int sn = 0; sn < B; sn++) c->bat.act[sn] = sn;
 
-  /* wavefield: 4 levels, zeroed (alloc.cu:448-457) */
-  if ((rc = c->alloc(&c->wave, (size_t)4 * 9 * sc))) return fail(rc);
-  if (hipMemset(c->wave, 0, (size_t)4 * 9 * sc * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+  /* wavefield: 4 levels per scenario, zeroed (alloc.cu:448-457) */
+  c->bat.wstride = 4L * 9 * sc;
+  if ((rc = c->alloc(&c->wave, (size_t)B * 4 * 9 * sc))) return fail(rc);
+  if (hipMemset(c->wave, 0, (size_t)B * 4 * 9 * sc * 4) != hipSuccess)
+    return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
   /* metric and media */
   if ((rc = c->alloc(&c->met, (size_t)10 * sc))) return fail(rc);
   if ((rc = c->alloc(&c->lam, (size_t)3 * sc))) return fail(rc);
@@ -379,7 +431,23 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
   if ((rc = c->alloc(&c->matVx2Vz, (size_t)2 * 9 * g.siz))) return fail(rc);
   c->matVy2Vz = c->matVx2Vz + 9 * g.siz;
   if (hipMemset(c->matVx2Vz, 0, (size_t)2 * 9 * g.siz * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
-  /* PML */
+  /* PML: profiles shared; auxiliaries (3 levels x 9 x slab) of all faces in
+     one block per scenario */
+  long paux = 0;
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++) {
+      if (!d->pml_is_sides[dim][side]) continue;
+      long n = (long)(d->pml_ni2[dim][side] - d->pml_ni1[dim][side] + 1) *
+               (d->pml_nj2[dim][side] - d->pml_nj1[dim][side] + 1) * (d->pml_nk2[dim][side] - d->pml_nk1[dim][side] + 1);
+      paux += 27 * n;
+    }
+  float *auxblk = nullptr;
+  c->bat.pstride = paux;
+  if (paux > 0) {
+    if ((rc = c->alloc(&auxblk, (size_t)B * paux))) return fail(rc);
+    if (hipMemset(auxblk, 0, (size_t)B * paux * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+  }
+  long paoff = 0;
   for (int dim = 0; dim < 3; dim++)
     for (int side = 0; side < 2; side++) {
       PmlFace &f = c->pml[dim][side];
@@ -400,17 +468,22 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
           hipMemcpy(cf + 2 * np, d->pml_D[dim][side], np * 4, hipMemcpyHostToDevice) != hipSuccess)
         return fail(cgfd_set_error(CGFD_ERR_HIP, "PML coefficient upload failed"));
       f.A = cf; f.B = cf + np; f.D = cf + 2 * np;
-      float *aux;
-      if ((rc = c->alloc(&aux, (size_t)3 * 9 * f.n))) return fail(rc);
-      if (hipMemset(aux, 0, (size_t)3 * 9 * f.n * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+      float *aux = auxblk + paoff;
+      paoff += 27 * f.n;
       f.pre = aux; f.tmp = aux + 9 * f.n; f.end = aux + 18 * f.n;
     }
   /* faults */
   if (c->nfault > 0) {
-    size_t n = (size_t)4 * c->nfault * 18 * g.sslice;
+    c->bat.fwstride = 4L * c->nfault * 18 * g.sslice;
+    c->bat.fpf = 45L * g.sslice;
+    c->bat.fpi = 2L * g.sslice;
+    size_t n = (size_t)B * c->bat.fwstride;
     if ((rc = c->alloc(&c->fwav, n))) return fail(rc);
     if (hipMemset(c->fwav, 0, n * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
-    if ((rc = upload_faults(c, d))) return fail(rc);
+    if ((rc = upload_faults(c, ds))) return fail(rc);
+    if ((rc = c->alloc(&c->d_ftab, (size_t)c->nfault))) return fail(rc);
+    if (hipMemcpy(c->d_ftab, c->flt, sizeof(FaultDev) * c->nfault, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(cgfd_set_error(CGFD_ERR_HIP, "fault table upload failed"));
   }
   {
     const int ntop = c->isfree ? (g.nk < 4 ? g.nk : 4) : 0;
@@ -421,8 +494,10 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
   if (hipMemset(c->it_dev, 0, 8) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
   c->nan_flag = c->it_dev + 1;
   if (c->isfree) {
-    if ((rc = c->alloc(&c->PG, (size_t)18 * g.siz))) return fail(rc);
-    if (hipMemset(c->PG, 0, (size_t)18 * g.siz * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+    c->bat.pgstride = 18L * g.siz;
+    if ((rc = c->alloc(&c->PG, (size_t)B * 18 * g.siz))) return fail(rc);
+    if (hipMemset(c->PG, 0, (size_t)B * 18 * g.siz * 4) != hipSuccess)
+      return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
     c->Dis = c->PG + 15 * g.siz;
     launch_dvh2dvz(g, c->met, c->lam, c->mu, c->matVx2Vz, c->matVy2Vz, c->st);
   }
@@ -430,6 +505,12 @@ extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) {
     return fail(cgfd_set_error(CGFD_ERR_HIP, "setup kernels failed"));
   *out = c;
   return CGFD_OK;
+}
+
+extern "C" int cgfd_core_create(const cgfd_desc *d, cgfd_core **out) { return core_create(d, 1, out); }
+
+extern "C" int cgfd_batch_create(const cgfd_desc *descs, int nscen, cgfd_core **out) {
+  return core_create(descs, nscen, out);
 }
 
 extern "C" int cgfd_core_destroy(cgfd_core *c) {
@@ -473,7 +554,9 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.w_cur = wcur; f.w_pre = c->wlev(c->lv_pre); f.w_tnew = wnew; f.w_end = c->wlev(c->lv_end);
   f.f_cur = fcur; f.f_pre = c->flev(c->fl_pre); f.f_tnew = fnew; f.f_end = c->flev(c->fl_end);
   f.nfault = c->nfault;
-  for (int q = 0; q < c->nfault; q++) f.f[q] = c->flt[q];
+  f.ftab = c->d_ftab;
+  f.sfp = 0;
+  f.sfi = 0;
   f.istage = istage; f.idir = idir; f.jdir = jdir; f.kdir = kdir;
   f.isfree = c->isfree; f.imethod = c->imethod; f.fix_q1 = c->fix_q1;
   f.dt = c->dt;
@@ -489,9 +572,9 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
    at the compiler's occupancy / >= 4 / >= 6 waves per SIMD */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
-  else if (c->variant >= 7) launch_vol_sched(va, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
+  else if (c->variant >= 7) launch_vol_sched(va, c->bat, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
   else if (c->variant >= 3 && c->variant <= 5) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
-  else launch_volume(c->variant, va, idir, jdir, kdir, 0, c->st);
+  else launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st);
 }
 
 /* one time step; if ev != nullptr, event pairs bracket the volume launches */
@@ -500,9 +583,10 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
   const int nk = c->g.nk;
   const int ntop = c->isfree ? (nk < 4 ? nk : 4) : 0;
   const int nvol = nk - ntop;
+  if (c->refresh_active() == 0) return CGFD_OK;  /* every scenario of the batch has retired */
   if (c->st_n > 0)
-    launch_record_stations(c->wlev(c->lv_pre), c->g.sicmp, c->st_idx, c->st_n, c->st_buf, c->st_nt, c->it_dev,
-                           c->st_it0, c->st);
+    launch_record_stations(c->bat, c->wlev(c->lv_pre), c->g.sicmp, c->st_idx, c->st_n, c->st_buf, c->st_nt,
+                           c->it_dev, c->st_it0, c->st);
   for (int s = 0; s < 4; s++) {
     const int idir = (kPairs[ipair][0] + s) % 2;
     const int jdir = (kPairs[ipair][1] + s) % 2;
@@ -519,8 +603,8 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
       fnew = s < 3 ? c->flev(fnewl) : nullptr;
       FaultArgs fa;
       fill_fault(c, fa, s, idir, jdir, kdir, wcur, wnew, fcur, fnew);
-      launch_wave2fault(fa, c->st);
-      launch_trial(fa, c->st);
+      launch_wave2fault(fa, c->bat, c->st);
+      launch_trial(fa, c->bat, c->st);
     }
     VolArgs va;
     fill_vol(c, va, s, wcur, wnew);
@@ -537,15 +621,15 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
     if (ntop > 0) {
       va.k_lo = nk - ntop;
       va.nk_run = ntop;
-      launch_volume(c->variant, va, idir, jdir, kdir, 1, c->st);
+      launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 1, c->st);
     }
     if (c->nfault > 0) {
       FaultArgs fa;
       fill_fault(c, fa, s, idir, jdir, kdir, wcur, wnew, fcur, fnew);
-      launch_fault_update(fa, c->st);
+      launch_fault_update(fa, c->bat, c->st);
     }
   }
-  if (c->isfree) launch_pg(c->g, c->wlev(c->lv_end), c->wlev(c->lv_pre), c->PG, c->Dis, c->dt, c->st);
+  if (c->isfree) launch_pg(c->g, c->bat, c->wlev(c->lv_end), c->wlev(c->lv_pre), c->PG, c->Dis, c->dt, c->st);
   launch_step_end(c->it_dev, c->st);
   HIPCHK(hipGetLastError());
   /* swap pre <-> end (drv_rk_curv_col.cu:622-635) */
@@ -555,7 +639,8 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
     for (int side = 0; side < 2; side++) std::swap(c->pml[dim][side].pre, c->pml[dim][side].end);
   c->it_host++;
   if (c->nan_every > 0 && (c->it_host % c->nan_every) == 0) {
-    launch_nan_check(c->wlev(c->lv_pre), 9 * c->g.sicmp, c->nan_flag, c->st);
+    for (int q = 0; q < c->bat.nact; q++)
+      launch_nan_check(c->wlev(c->lv_pre) + c->bat.act[q] * c->bat.wstride, 9 * c->g.sicmp, c->nan_flag, c->st);
     int flag = 0;
     HIPCHK(hipMemcpyAsync(&flag, c->nan_flag, 4, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
@@ -602,7 +687,11 @@ extern "C" int cgfd_core_run(cgfd_core *c, int nsteps) {
   const bool graph = c->use_graph && c->nan_every <= 0;
   int left = nsteps;
   while (left > 0) {
-    if (graph && left >= G && c->it_host % G == 0) {
+    if (c->refresh_active() == 0) break;  /* every scenario of the batch has retired */
+    bool stable = true;                   /* no scenario retires inside the next G steps */
+    for (int sn = 0; sn < c->nscen && c->nscen > 1; sn++)
+      if (c->it_host < c->scen_nt[sn] && c->scen_nt[sn] < c->it_host + G) stable = false;
+    if (graph && stable && left >= G && c->it_host % G == 0) {
       if (!c->gexec) {
         int rc = capture_graph(c);
         if (rc) return rc;
@@ -661,16 +750,41 @@ extern "C" int cgfd_core_sync(cgfd_core *c) {
   return CGFD_OK;
 }
 
-extern "C" int cgfd_core_read_wavefield(cgfd_core *c, float *dst) {
-  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
-  HIPCHK(hipSetDevice(c->dev));
-  HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(dst, c->wlev(c->lv_pre), (size_t)9 * c->g.sicmp * 4, hipMemcpyDeviceToHost));
+/* steps scenario s has taken */
+static int scen_steps(const cgfd_core *c, int s) {
+  return c->nscen == 1 ? c->it_host : std::min(c->it_host, c->scen_nt[s]);
+}
+
+/* The levels holding a scenario's current state: "pre" and "end" swap every
+   step, so a scenario that retired an odd number of steps ago has its final
+   state in today's "end" level. */
+static int scen_wlev(const cgfd_core *c, int s) {
+  return ((c->it_host - scen_steps(c, s)) % 2 == 0) ? c->lv_pre : c->lv_end;
+}
+static int scen_flev(const cgfd_core *c, int s) {
+  return ((c->it_host - scen_steps(c, s)) % 2 == 0) ? c->fl_pre : c->fl_end;
+}
+
+static int check_scen(const cgfd_core *c, int s) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (s < 0 || s >= c->nscen) return cgfd_set_error(CGFD_ERR_ARG, "scenario %d out of range [0,%d)", s, c->nscen);
   return CGFD_OK;
 }
 
-extern "C" int cgfd_core_write_wavefield(cgfd_core *c, const float *src) {
-  if (!c || !src) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+extern "C" int cgfd_batch_read_wavefield(cgfd_core *c, int s, float *dst) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(dst, c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, (size_t)9 * c->g.sicmp * 4,
+                   hipMemcpyDeviceToHost));
+  return CGFD_OK;
+}
+extern "C" int cgfd_core_read_wavefield(cgfd_core *c, float *dst) { return cgfd_batch_read_wavefield(c, 0, dst); }
+
+extern "C" int cgfd_batch_write_wavefield(cgfd_core *c, int s, const float *src) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!src) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   const Grid &g = c->g;
@@ -681,31 +795,39 @@ extern "C" int cgfd_core_write_wavefield(cgfd_core *c, const float *src) {
         size_t o = (size_t)cmp * g.sicmp + (size_t)k * g.siz + (size_t)j * g.siy;
         std::memcpy(&h[o + 3], &src[o + 3], (size_t)g.ni * 4);
       }
-  HIPCHK(hipMemcpy(c->wlev(c->lv_pre), h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, h.data(), h.size() * 4, hipMemcpyHostToDevice));
   return CGFD_OK;
 }
+extern "C" int cgfd_core_write_wavefield(cgfd_core *c, const float *src) { return cgfd_batch_write_wavefield(c, 0, src); }
 
-extern "C" int cgfd_core_read_fault(cgfd_core *c, int id, float *dst) {
-  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+extern "C" int cgfd_batch_read_fault(cgfd_core *c, int s, int id, float *dst) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   if (id < 0 || id >= c->nfault) return cgfd_set_error(CGFD_ERR_ARG, "fault id %d out of range", id);
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(dst, c->flt[id].out, (size_t)11 * c->g.sslice * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dst, c->flt[id].out + s * c->bat.fpf, (size_t)11 * c->g.sslice * 4, hipMemcpyDeviceToHost));
   return CGFD_OK;
 }
+extern "C" int cgfd_core_read_fault(cgfd_core *c, int id, float *dst) { return cgfd_batch_read_fault(c, 0, id, dst); }
 
-extern "C" int cgfd_core_read_fault_state(cgfd_core *c, int id, float *dst) {
-  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+extern "C" int cgfd_batch_read_fault_state(cgfd_core *c, int s, int id, float *dst) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   if (id < 0 || id >= c->nfault) return cgfd_set_error(CGFD_ERR_ARG, "fault id %d out of range", id);
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  const float *src = c->flev(c->fl_pre) + (long)id * 18 * c->g.sslice;
+  const float *src = c->flev(scen_flev(c, s)) + s * c->bat.fwstride + (long)id * 18 * c->g.sslice;
   HIPCHK(hipMemcpy(dst, src, (size_t)18 * c->g.sslice * 4, hipMemcpyDeviceToHost));
   return CGFD_OK;
 }
+extern "C" int cgfd_core_read_fault_state(cgfd_core *c, int id, float *dst) {
+  return cgfd_batch_read_fault_state(c, 0, id, dst);
+}
 
-extern "C" int cgfd_core_read_pg(cgfd_core *c, float *dst) {
-  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+extern "C" int cgfd_batch_read_pg(cgfd_core *c, int s, float *dst) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   size_t n = (size_t)15 * c->g.siz;
@@ -713,9 +835,10 @@ extern "C" int cgfd_core_read_pg(cgfd_core *c, float *dst) {
     std::memset(dst, 0, n * 4);
     return CGFD_OK;
   }
-  HIPCHK(hipMemcpy(dst, c->PG, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(dst, c->PG + s * c->bat.pgstride, n * 4, hipMemcpyDeviceToHost));
   return CGFD_OK;
 }
+extern "C" int cgfd_core_read_pg(cgfd_core *c, float *dst) { return cgfd_batch_read_pg(c, 0, dst); }
 
 extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int nt_max) {
   if (!c || n < 0 || (n > 0 && (!ijk || nt_max <= 0))) return cgfd_set_error(CGFD_ERR_ARG, "bad station arguments");
@@ -729,11 +852,12 @@ extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int n
     idx[s] = i + (long)j * c->g.siy + (long)k * c->g.siz;
   }
   int rc;
+  c->bat.ststride = (long)n * nt_max * 9;
   if (n > 0) {
     if ((rc = c->alloc(&c->st_idx, (size_t)n))) return rc;
     HIPCHK(hipMemcpy(c->st_idx, idx.data(), (size_t)n * sizeof(long), hipMemcpyHostToDevice));
-    if ((rc = c->alloc(&c->st_buf, (size_t)n * nt_max * 9))) return rc;
-    HIPCHK(hipMemset(c->st_buf, 0, (size_t)n * nt_max * 9 * 4));
+    if ((rc = c->alloc(&c->st_buf, (size_t)c->nscen * n * nt_max * 9))) return rc;
+    HIPCHK(hipMemset(c->st_buf, 0, (size_t)c->nscen * n * nt_max * 9 * 4));
   }
   c->st_n = n;
   c->st_nt = nt_max;
@@ -742,25 +866,44 @@ extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int n
   return CGFD_OK;
 }
 
-extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
-  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+extern "C" int cgfd_batch_read_stations(cgfd_core *c, int s, float *dst, int *nt) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  int rec = c->it_host - c->st_it0;
+  int rec = scen_steps(c, s) - c->st_it0;
   if (rec > c->st_nt) rec = c->st_nt;
+  if (rec < 0) rec = 0;
   if (nt) *nt = rec;
   if (c->st_n == 0) return CGFD_OK;
   std::vector<float> h((size_t)c->st_n * c->st_nt * 9);
-  HIPCHK(hipMemcpy(h.data(), c->st_buf, h.size() * 4, hipMemcpyDeviceToHost));
-  for (int s = 0; s < c->st_n; s++)
-    std::memcpy(dst + (size_t)s * rec * 9, &h[(size_t)s * c->st_nt * 9], (size_t)rec * 9 * 4);
+  HIPCHK(hipMemcpy(h.data(), c->st_buf + s * c->bat.ststride, h.size() * 4, hipMemcpyDeviceToHost));
+  for (int q = 0; q < c->st_n; q++)
+    std::memcpy(dst + (size_t)q * rec * 9, &h[(size_t)q * c->st_nt * 9], (size_t)rec * 9 * 4);
   return CGFD_OK;
+}
+extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
+  return cgfd_batch_read_stations(c, 0, dst, nt);
+}
+
+extern "C" int cgfd_batch_steps(const cgfd_core *c, int s) {
+  if (check_scen(c, s)) return -1;
+  return scen_steps(c, s);
+}
+
+extern "C" int cgfd_batch_active(cgfd_core *c) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  int n = 0;
+  for (int s = 0; s < c->nscen; s++) n += c->active(s) ? 1 : 0;
+  return n;
 }
 
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
-  if (variant == 0) variant = 4;
+  if (variant == 0) variant = 1;
   if (variant < 1 || variant > 9) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (c->nscen > 1 && variant != 1 && variant < 7)
+    return cgfd_set_error(CGFD_ERR_ARG, "variant %d does not run batches (use 1 or 7..9)", variant);
   c->variant = variant;
   c->drop_graph();
   return CGFD_OK;
@@ -798,6 +941,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
     for (int zi = 0; zi < 4; zi++) {
       int v = variants[vi], z = chunks[zi];
       if (v == 1 && zi > 0) continue;  /* the direct kernel has no z chunks */
+      if (c->nscen > 1 && v != 1 && v < 7) continue;  /* single-scenario kernels */
       c->variant = v;
       c->zchunk = z;
       int rc = build_zq_tiles(c, c->nvol);

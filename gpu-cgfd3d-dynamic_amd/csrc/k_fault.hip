@@ -42,12 +42,12 @@ DEVI void metric_cols(const float *met, long sc, long p, float M[3][3]) {
 }
 
 /* ---------------------------------------------------------------------- */
-__global__ void __launch_bounds__(256) k_wave2fault(FaultArgs a) {
+DEVI void k_wave2fault_body(const FaultArgs &a) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
   const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
-  const FaultDev &F = a.f[id];
+  const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp;
   const long pf = (iy + 3) + (long)(iz + 3) * a.g.ny;
   if (F.united[pf] != 1) return;
@@ -79,6 +79,12 @@ __global__ void __launch_bounds__(256) k_wave2fault(FaultArgs a) {
       fc[fidx(6 + c, m, ss) + pf] = Tr[c][2];
     }
   }
+}
+
+template <bool MULTI>
+__global__ void __launch_bounds__(256) k_wave2fault(FaultArgs a, BatArgs b) {
+  const int nq = MULTI ? b.nact : 1;
+  for (int q = 0; q < nq; q++) k_wave2fault_body(fault_at(a, b, b.act[q]));
 }
 
 /* split-node average back onto the wavefield column i0 (transform.cu:331-372),
@@ -125,12 +131,12 @@ DEVI float dz_free_image(const float *col, long ny, int n_free, int kdir) {
 }
 
 /* ---------------------------------------------------------------------- */
-__global__ void __launch_bounds__(128) k_trial(FaultArgs a) {
+DEVI void k_trial_body(const FaultArgs &a) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
   const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
-  const FaultDev &F = a.f[id];
+  const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny;
   const long pf = (iy + 3) + (long)(iz + 3) * ny;
   if (F.united[pf] != 0) return;
@@ -238,6 +244,12 @@ __global__ void __launch_bounds__(128) k_trial(FaultArgs a) {
   fault2wave_node(a, F, pf, p0, a.w_cur, fc);
 }
 
+template <bool MULTI>
+__global__ void __launch_bounds__(128) k_trial(FaultArgs a, BatArgs b) {
+  const int nq = MULTI ? b.nact : 1;
+  for (int q = 0; q < nq; q++) k_trial_body(fault_at(a, b, b.act[q]));
+}
+
 /* Hooke's law on one column point (sv_curv_col_el_iso_fault_gpu.cu:1126-1154) */
 DEVI void hooke(const float *met, long sc, long p, float l, float m, const float *Dx, const float *Dy, const float *Dz,
                 float *h) {
@@ -282,7 +294,7 @@ DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
 constexpr int kColNodes = 10;  /* nodes per wavefront in the column kernel */
 
 template <int IDIR>
-__global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a) {
+DEVI void k_fault_cols_body(const FaultArgs &a) {
   const int lane = threadIdx.x;
   if (lane >= 6 * kColNodes) return;
   const int iy = blockIdx.x * kColNodes + lane / 6;
@@ -290,7 +302,7 @@ __global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a) {
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
   const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
-  const FaultDev &F = a.f[id];
+  const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
   const long pf = (iy + 3) + (long)(iz + 3) * ny;
   if (F.united[pf] != 0) return;
@@ -414,6 +426,12 @@ __global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a) {
   rk_cols(a, p, 0, 9, h);
 }
 
+template <int IDIR, bool MULTI>
+__global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a, BatArgs b) {
+  const int nq = MULTI ? b.nact : 1;
+  for (int q = 0; q < nq; q++) k_fault_cols_body<IDIR>(fault_at(a, b, b.act[q]));
+}
+
 /* ---------------------------------------------------------------------- */
 /* The split nodes: velocity rhs from the fault traction (fault_gpu.cu:326-425),
    traction rates from the split-node velocity gradient (method 1 or 2, free
@@ -423,12 +441,12 @@ __global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a) {
    stage the per-step accumulators (fault_var_update_gpu, fault_wav_t.cu:
    197-264). One lane per node, lanes along j. */
 template <int IDIR>
-__global__ void __launch_bounds__(256) k_fault_nodes(FaultArgs a) {
+DEVI void k_fault_nodes_body(const FaultArgs &a) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
   const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
-  const FaultDev &F = a.f[id];
+  const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
   const long pf = (iy + 3) + (long)(iz + 3) * ny;
   if (F.united[pf] != 0) return;
@@ -624,6 +642,12 @@ __global__ void __launch_bounds__(256) k_fault_nodes(FaultArgs a) {
   }
 }
 
+template <int IDIR, bool MULTI>
+__global__ void __launch_bounds__(256) k_fault_nodes(FaultArgs a, BatArgs b) {
+  const int nq = MULTI ? b.nact : 1;
+  for (int q = 0; q < nq; q++) k_fault_nodes_body<IDIR>(fault_at(a, b, b.act[q]));
+}
+
 /* ---------------------------------------------------------------------- */
 __global__ void k_step_end(int *it_dev) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *it_dev += 1;
@@ -644,37 +668,49 @@ __global__ void k_nan_check(const float *p, long n, int *flag) {
 }
 
 /* receivers (io_recv_keep, drv_rk_curv_col.cu:215): 9 components of level pre */
-__global__ void k_record(const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max, const int *it_dev,
-                         int it0) {
+__global__ void k_record(BatArgs b, const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max,
+                         const int *it_dev, int it0) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 9) return;
   int s = t / 9, c = t % 9;
   int it = *it_dev - it0;
   if (it < 0 || it >= nt_max) return;
-  buf[((long)s * nt_max + it) * 9 + c] = w[c * sicmp + idx[s]];
+  for (int q = 0; q < b.nact; q++) {
+    const long sc = b.act[q];
+    buf[sc * b.ststride + ((long)s * nt_max + it) * 9 + c] = w[sc * b.wstride + c * sicmp + idx[s]];
+  }
 }
 
 /* ---------------------------------------------------------------------- */
-void launch_wave2fault(const FaultArgs &a, hipStream_t st) {
+void launch_wave2fault(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
   dim3 blk(64, 4, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault);
-  hipLaunchKernelGGL(k_wave2fault, grd, blk, 0, st, a);
+  if (b.nact > 1) hipLaunchKernelGGL(k_wave2fault<true>, grd, blk, 0, st, a, b);
+  else hipLaunchKernelGGL(k_wave2fault<false>, grd, blk, 0, st, a, b);
 }
-void launch_trial(const FaultArgs &a, hipStream_t st) {
+void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
   dim3 blk(64, 2, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 1) / 2, a.nfault);
-  hipLaunchKernelGGL(k_trial, grd, blk, 0, st, a);
+  if (b.nact > 1) hipLaunchKernelGGL(k_trial<true>, grd, blk, 0, st, a, b);
+  else hipLaunchKernelGGL(k_trial<false>, grd, blk, 0, st, a, b);
 }
-void launch_fault_update(const FaultArgs &a, hipStream_t st) {
+void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
   dim3 cblk(64, 4, 1), cgrd((a.g.nj + kColNodes - 1) / kColNodes, (a.g.nk + 3) / 4, a.nfault);
   dim3 nblk(64, 4, 1), ngrd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault);
-  if (a.idir == 1) {
-    hipLaunchKernelGGL(k_fault_cols<1>, cgrd, cblk, 0, st, a);
-    hipLaunchKernelGGL(k_fault_nodes<1>, ngrd, nblk, 0, st, a);
+  const bool multi = b.nact > 1;
+  if (a.idir == 1 && multi) {
+    hipLaunchKernelGGL((k_fault_cols<1, true>), cgrd, cblk, 0, st, a, b);
+    hipLaunchKernelGGL((k_fault_nodes<1, true>), ngrd, nblk, 0, st, a, b);
+  } else if (a.idir == 1) {
+    hipLaunchKernelGGL((k_fault_cols<1, false>), cgrd, cblk, 0, st, a, b);
+    hipLaunchKernelGGL((k_fault_nodes<1, false>), ngrd, nblk, 0, st, a, b);
+  } else if (multi) {
+    hipLaunchKernelGGL((k_fault_cols<0, true>), cgrd, cblk, 0, st, a, b);
+    hipLaunchKernelGGL((k_fault_nodes<0, true>), ngrd, nblk, 0, st, a, b);
   } else {
-    hipLaunchKernelGGL(k_fault_cols<0>, cgrd, cblk, 0, st, a);
-    hipLaunchKernelGGL(k_fault_nodes<0>, ngrd, nblk, 0, st, a);
+    hipLaunchKernelGGL((k_fault_cols<0, false>), cgrd, cblk, 0, st, a, b);
+    hipLaunchKernelGGL((k_fault_nodes<0, false>), ngrd, nblk, 0, st, a, b);
   }
 }
 void launch_step_end(int *it_dev, hipStream_t st) { hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, it_dev); }
@@ -689,11 +725,12 @@ void launch_nan_check(const float *p, long n, int *flag, hipStream_t st) {
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_nan_check, dim3((unsigned)blocks), dim3(256), 0, st, p, n, flag);
 }
-void launch_record_stations(const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max,
-                            const int *it_dev, int it0, hipStream_t st) {
+void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
+                            int nt_max, const int *it_dev, int it0, hipStream_t st) {
   if (n <= 0) return;
   int t = n * 9;
-  hipLaunchKernelGGL(k_record, dim3((t + 255) / 256), dim3(256), 0, st, w, sicmp, idx, n, buf, nt_max, it_dev, it0);
+  hipLaunchKernelGGL(k_record, dim3((t + 255) / 256), dim3(256), 0, st, b, w, sicmp, idx, n, buf, nt_max, it_dev,
+                     it0);
 }
 
 }  // namespace cgfd

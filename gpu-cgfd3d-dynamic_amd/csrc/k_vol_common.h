@@ -31,7 +31,7 @@ DEVI void rhs_interior(const float *Dx, const float *Dy, const float *Dz, float 
 /* CFS-PML correction of every face containing (i,j,k) plus the auxiliary
    RK update (sv_curv_col_el_iso_gpu.cu:916-1288, drv_rk_curv_col.cu:368-567) */
 DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *Dx, const float *Dy, const float *Dz,
-                    float l, float m, float s, float *h) {
+                    float l, float m, float s, float *h, long poff = 0) {
   const long sc = a.g.sicmp;
 #pragma unroll 1
   for (int dim = 0; dim < 3; dim++) {
@@ -58,7 +58,8 @@ DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *
       r[TXY] = m * (e2 * D[VX] + e1 * D[VY]);
       r[TXZ] = m * (e3 * D[VX] + e1 * D[VZ]);
       r[TYZ] = m * (e3 * D[VY] + e2 * D[VZ]);
-      float *auxc = (a.istage == 0 ? f.pre : f.tmp);
+      float *const fpre = f.pre + poff, *const ftmp = f.tmp + poff, *const fend = f.end + poff;
+      const float *auxc = (a.istage == 0 ? fpre : ftmp);
       float av[9], ar[9];
 #pragma unroll
       for (int c = 0; c < 9; c++) av[c] = auxc[c * f.n + ia];
@@ -92,19 +93,19 @@ DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *
       if (a.istage == 0) {
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-          float pv = f.pre[c * f.n + ia];
-          f.tmp[c * f.n + ia] = pv + a.ca * ar[c];
-          f.end[c * f.n + ia] = pv + a.cb * ar[c];
+          float pv = fpre[c * f.n + ia];
+          ftmp[c * f.n + ia] = pv + a.ca * ar[c];
+          fend[c * f.n + ia] = pv + a.cb * ar[c];
         }
       } else if (a.istage < 3) {
 #pragma unroll
         for (int c = 0; c < 9; c++) {
-          f.tmp[c * f.n + ia] = f.pre[c * f.n + ia] + a.ca * ar[c];
-          f.end[c * f.n + ia] += a.cb * ar[c];
+          ftmp[c * f.n + ia] = fpre[c * f.n + ia] + a.ca * ar[c];
+          fend[c * f.n + ia] += a.cb * ar[c];
         }
       } else {
 #pragma unroll
-        for (int c = 0; c < 9; c++) f.end[c * f.n + ia] += a.cb * ar[c];
+        for (int c = 0; c < 9; c++) fend[c * f.n + ia] += a.cb * ar[c];
       }
     }
   }
@@ -120,8 +121,9 @@ DEVI bool fault_column(const FaultSkip &fs, int i, long pf) {
 }
 
 /* RK stage update of the nine wavefield components at p */
-DEVI void rk_point(const VolArgs &a, long p, const float *h) {
+DEVI void rk_point(const VolArgs &a, long p, const float *h, long woff = 0) {
   const long sc = a.g.sicmp;
+  p += woff;
   if (a.istage == 0) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
@@ -141,43 +143,28 @@ DEVI void rk_point(const VolArgs &a, long p, const float *h) {
   }
 }
 
-/* rk_point with 32-bit byte offsets (o = 4 p) */
-DEVI void rk_point32(const VolArgs &a, unsigned o, const float *h) {
+/* rk_point with 32-bit byte offsets (o = 4 p) from the scenario's levels */
+DEVI void rk_point32(const VolArgs &a, unsigned o, const float *h, long woff = 0) {
   const long sc = a.g.sicmp;
+  const float *pre = a.pre + woff;
+  float *tnew = a.tnew + woff, *end = a.end + woff;
   if (a.istage == 0) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      float pv = ld32(a.pre + c * sc, o);
-      st32(a.tnew + c * sc, o, pv + a.ca * h[c]);
-      st32(a.end + c * sc, o, pv + a.cb * h[c]);
+      float pv = ld32(pre + c * sc, o);
+      st32(tnew + c * sc, o, pv + a.ca * h[c]);
+      st32(end + c * sc, o, pv + a.cb * h[c]);
     }
   } else if (a.istage < 3) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      st32(a.tnew + c * sc, o, ld32(a.pre + c * sc, o) + a.ca * h[c]);
-      st32(a.end + c * sc, o, ld32(a.end + c * sc, o) + a.cb * h[c]);
+      st32(tnew + c * sc, o, ld32(pre + c * sc, o) + a.ca * h[c]);
+      st32(end + c * sc, o, ld32(end + c * sc, o) + a.cb * h[c]);
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < 9; c++) st32(a.end + c * sc, o, ld32(a.end + c * sc, o) + a.cb * h[c]);
+    for (int c = 0; c < 9; c++) st32(end + c * sc, o, ld32(end + c * sc, o) + a.cb * h[c]);
   }
-}
-
-/* vol_point_update with 32-bit byte offsets for the per-point arrays */
-template <bool PML = true>
-DEVI void vol_point_update32(const VolArgs &a, int i, int j, int k, long p, unsigned o, const float *Dx,
-                             const float *Dy, const float *Dz) {
-  const long sc = a.g.sicmp;
-  const float *M = a.met;
-  float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
-  float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
-  float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
-  float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
-  float h[9];
-  rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
-  if (PML && a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h);
-  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
-  rk_point32(a, o, h);
 }
 
 /* rhs + PML + RK update of one point from its 27 derivatives */

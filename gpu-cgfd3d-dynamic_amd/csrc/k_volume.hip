@@ -23,36 +23,51 @@
 
 namespace cgfd {
 
-/* one point of the direct kernels: 27 derivatives from the cache hierarchy,
-   then rhs + PML + RK */
-template <int ID, int JD, int KD, bool PML = true>
-DEVI void direct_point(const VolArgs &a, int i, int j, int k) {
+/* one point of the direct kernels: the shared metric and media are loaded
+   once, then for every active scenario 27 derivatives from the cache
+   hierarchy, rhs + PML + RK */
+template <int ID, int JD, int KD, bool PML = true, bool MULTI = false>
+DEVI void direct_point(const VolArgs &a, const BatArgs &b, int i, int j, int k) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   const unsigned o = (unsigned)p * 4u;
   const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
-  float Dx[9], Dy[9], Dz[9];
+  const float *M = a.met;
+  const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
+  const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
+  const float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
+  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+  const bool fault_owned = a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny);
+  const int nq = MULTI ? b.nact : 1;  /* one scenario: the loop disappears */
+  for (int q = 0; q < nq; q++) {
+    const long sn = b.act[q];
+    const long woff = sn * b.wstride;
+    float Dx[9], Dy[9], Dz[9];
 #pragma unroll
-  for (int c = 0; c < 9; c++) {
-    const float *b = a.cur + c * sc;
-    Dx[c] = dfd32<ID>(b, o, 4u);
-    Dy[c] = dfd32<JD>(b, o, by);
-    Dz[c] = dfd32<KD>(b, o, bz);
+    for (int c = 0; c < 9; c++) {
+      const float *bc = a.cur + woff + c * sc;
+      Dx[c] = dfd32<ID>(bc, o, 4u);
+      Dy[c] = dfd32<JD>(bc, o, by);
+      Dz[c] = dfd32<KD>(bc, o, bz);
+    }
+    float h[9];
+    rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
+    if (PML && a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h, sn * b.pstride);
+    if (!fault_owned) rk_point32(a, o, h, woff);
   }
-  vol_point_update32<PML>(a, i, j, k, p, o, Dx, Dy, Dz);
 }
 
 /* ------------------------------------------------------------------------
  * Direct-load volume kernel: one thread per point, neighbours from the
  * cache hierarchy. Handles physical planes k_lo .. k_lo+nk_run-1.
  * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256) k_vol_direct(VolArgs a) {
+template <int ID, int JD, int KD, bool MULTI>
+__global__ void __launch_bounds__(256) k_vol_direct(VolArgs a, BatArgs b) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
   const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
-  direct_point<ID, JD, KD>(a, i, j, k);
+  direct_point<ID, JD, KD, true, MULTI>(a, b, i, j, k);
 }
 
 /* ------------------------------------------------------------------------
@@ -66,9 +81,9 @@ __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a) {
  * tiles run without the CFS-PML code (fewer registers); MINW > 0 asks the
  * compiler for at least MINW waves per SIMD.
  * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD, bool PML, int MINW>
+template <int ID, int JD, int KD, bool PML, int MINW, bool MULTI>
 __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
-    k_vol_sched(VolArgs a, const int *__restrict__ tiles, long nitems, int ntx, int nty) {
+    k_vol_sched(VolArgs a, BatArgs b, const int *__restrict__ tiles, long nitems, int ntx, int nty) {
   const long L = blockIdx.x;
   const long per = (long)(gridDim.x / 8);
   const long item = (L % 8) * per + L / 8;
@@ -84,45 +99,55 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
   const int ix = bx * 64 + (int)threadIdx.x;
   const int iy = by * 4 + (int)threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, PML>(a, ix + 3, iy + 3, a.k_lo + kz + 3);
+  direct_point<ID, JD, KD, PML, MULTI>(a, b, ix + 3, iy + 3, a.k_lo + kz + 3);
 }
 
-template <int ID, int JD, int KD, int MINW>
-static void launch_sched_t(const VolArgs &a, const ZqTiles &t, hipStream_t st) {
+template <int ID, int JD, int KD, int MINW, bool MULTI>
+static void launch_sched_m(const VolArgs &a, const BatArgs &b, const ZqTiles &t, hipStream_t st) {
   dim3 blk(64, 4, 1);
   if (t.n_int > 0) {
     const long items = (long)t.n_int * a.zchunk;
     const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, false, MINW>), dim3((unsigned)nb), blk, 0, st, a, t.d_sint, items,
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, false, MINW, MULTI>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_sint,
+                       items,
                        t.ntx, t.nty);
   }
   if (t.n_pml > 0) {
     const long items = (long)t.n_pml * a.zchunk;
     const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, true, 0>), dim3((unsigned)nb), blk, 0, st, a, t.d_spml, items,
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, true, 0, MULTI>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_spml,
+                       items,
                        t.ntx, t.nty);
   }
 }
 
+template <int ID, int JD, int KD, int MINW>
+static void launch_sched_t(const VolArgs &a, const BatArgs &b, const ZqTiles &t, hipStream_t st) {
+  if (b.nact > 1) launch_sched_m<ID, JD, KD, MINW, true>(a, b, t, st);
+  else launch_sched_m<ID, JD, KD, MINW, false>(a, b, t, st);
+}
+
 template <int MINW>
-static void launch_sched_w(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st) {
+static void launch_sched_w(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int idir, int jdir, int kdir,
+                           hipStream_t st) {
   switch (idir * 4 + jdir * 2 + kdir) {
-    case 0: launch_sched_t<0, 0, 0, MINW>(a, t, st); break;
-    case 1: launch_sched_t<0, 0, 1, MINW>(a, t, st); break;
-    case 2: launch_sched_t<0, 1, 0, MINW>(a, t, st); break;
-    case 3: launch_sched_t<0, 1, 1, MINW>(a, t, st); break;
-    case 4: launch_sched_t<1, 0, 0, MINW>(a, t, st); break;
-    case 5: launch_sched_t<1, 0, 1, MINW>(a, t, st); break;
-    case 6: launch_sched_t<1, 1, 0, MINW>(a, t, st); break;
-    default: launch_sched_t<1, 1, 1, MINW>(a, t, st); break;
+    case 0: launch_sched_t<0, 0, 0, MINW>(a, b, t, st); break;
+    case 1: launch_sched_t<0, 0, 1, MINW>(a, b, t, st); break;
+    case 2: launch_sched_t<0, 1, 0, MINW>(a, b, t, st); break;
+    case 3: launch_sched_t<0, 1, 1, MINW>(a, b, t, st); break;
+    case 4: launch_sched_t<1, 0, 0, MINW>(a, b, t, st); break;
+    case 5: launch_sched_t<1, 0, 1, MINW>(a, b, t, st); break;
+    case 6: launch_sched_t<1, 1, 0, MINW>(a, b, t, st); break;
+    default: launch_sched_t<1, 1, 1, MINW>(a, b, t, st); break;
   }
 }
 
 /* w: 0 = the compiler's choice, 1 = >= 4 waves/SIMD, 2 = >= 6 waves/SIMD */
-void launch_vol_sched(const VolArgs &a, const ZqTiles &t, int w, int idir, int jdir, int kdir, hipStream_t st) {
-  if (w == 1) launch_sched_w<4>(a, t, idir, jdir, kdir, st);
-  else if (w >= 2) launch_sched_w<6>(a, t, idir, jdir, kdir, st);
-  else launch_sched_w<0>(a, t, idir, jdir, kdir, st);
+void launch_vol_sched(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int w, int idir, int jdir, int kdir,
+                      hipStream_t st) {
+  if (w == 1) launch_sched_w<4>(a, b, t, idir, jdir, kdir, st);
+  else if (w >= 2) launch_sched_w<6>(a, b, t, idir, jdir, kdir, st);
+  else launch_sched_w<0>(a, b, t, idir, jdir, kdir, st);
 }
 
 /* ------------------------------------------------------------------------
@@ -132,18 +157,15 @@ void launch_vol_sched(const VolArgs &a, const ZqTiles &t, int w, int idir, int j
 __device__ const int kOpIdx[2][5] = {{-3, -2, -1, 0, 1}, {-1, 0, 1, 2, 3}};
 
 template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256) k_vol_top(VolArgs a) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-  if (ix >= a.g.ni || iy >= a.g.nj) return;
-  const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
+DEVI void top_point(const VolArgs &a, int i, int j, int k, long woff, long poff) {
+  const float *__restrict__ const cur = a.cur + woff;
   const int nk2 = a.g.nk2;
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   float Dx[9], Dy[9], Dz[9];
 #pragma unroll
   for (int c = 0; c < 9; c++) {
-    const float *q = a.cur + c * sc + p;
+    const float *q = cur + c * sc + p;
     Dx[c] = dfd<ID>(q, 1);
     Dy[c] = dfd<JD>(q, sy);
     Dz[c] = dfd<KD>(q, sz);
@@ -165,7 +187,7 @@ __global__ void __launch_bounds__(256) k_vol_top(VolArgs a) {
       /* timg: conservative tractions with an odd image above the surface */
       const int n_free = nk2 - k - fz[0];
       const float slwjac = a.slw[p] / jac[p];
-      const float *T = a.cur;
+      const float *T = cur;
       const float *Ta[3][3] = {{T + TXX * sc, T + TXY * sc, T + TXZ * sc},
                                {T + TXY * sc, T + TYY * sc, T + TYZ * sc},
                                {T + TXZ * sc, T + TYZ * sc, T + TZZ * sc}};
@@ -197,7 +219,7 @@ __global__ void __launch_bounds__(256) k_vol_top(VolArgs a) {
     }
     if (k >= nk2 - 2) {
       /* vlow: reduced-order z derivative of V near the surface */
-      const float *Vx = a.cur + p, *Vy = a.cur + sc + p, *Vz = a.cur + 2 * sc + p;
+      const float *Vx = cur + p, *Vy = cur + sc + p, *Vz = cur + 2 * sc + p;
       float dv[3][3];  /* [axis][comp] */
       dv[0][0] = macdrp(Vx, 1, ID); dv[1][0] = macdrp(Vx, sy, JD);
       dv[0][1] = macdrp(Vy, 1, ID); dv[1][1] = macdrp(Vy, sy, JD);
@@ -226,9 +248,22 @@ __global__ void __launch_bounds__(256) k_vol_top(VolArgs a) {
       h[TYZ] = m * (xz * DxVy + xy * DxVz + ez * DyVy + ey * DyVz + zz * DzVy + zy * DzVz);
     }
   }
-  if (a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h);
+  if (a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h, poff);
   if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
-  rk_point(a, p, h);
+  rk_point(a, p, h, woff);
+}
+
+template <int ID, int JD, int KD, bool MULTI>
+__global__ void __launch_bounds__(256) k_vol_top(VolArgs a, BatArgs b) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ix >= a.g.ni || iy >= a.g.nj) return;
+  const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
+  const int nq = MULTI ? b.nact : 1;
+  for (int q = 0; q < nq; q++) {
+    const long sn = b.act[q];
+    top_point<ID, JD, KD>(a, i, j, k, sn * b.wstride, sn * b.pstride);
+  }
 }
 
 /* ------------------------------------------------------------------------
@@ -289,11 +324,8 @@ __global__ void k_dvh2dvz(Grid g, const float *__restrict__ met, const float *__
 }
 
 /* surface PGV/PGA/PGD (PG_calcu_gpu, wav_t.cu:137-216) */
-__global__ void k_pg(Grid g, const float *__restrict__ wend, const float *__restrict__ wpre, float *PG, float *Dis,
-                     float dt) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-  if (ix >= g.ni || iy >= g.nj) return;
+DEVI void pg_point(const Grid &g, int ix, int iy, const float *__restrict__ wend, const float *__restrict__ wpre,
+                   float *PG, float *Dis, float dt) {
   const long sc = g.sicmp, siz = g.siz;
   long p = (ix + 3) + (long)(iy + 3) * g.siy + (long)g.nk2 * siz;
   long q = (ix + 3) + (long)(iy + 3) * g.siy;
@@ -319,34 +351,48 @@ __global__ void k_pg(Grid g, const float *__restrict__ wend, const float *__rest
   }
 }
 
+__global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const float *__restrict__ wpre, float *PG,
+                     float *Dis, float dt) {
+  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ix >= g.ni || iy >= g.nj) return;
+  for (int q = 0; q < b.nact; q++) {
+    const long s = b.act[q];
+    pg_point(g, ix, iy, wend + s * b.wstride, wpre + s * b.wstride, PG + s * b.pgstride, Dis + s * b.pgstride, dt);
+  }
+}
+
 /* ---------------------------------------------------------------------- */
 /* host launchers */
 template <int ID, int JD, int KD>
-static void launch_dir(int variant, const VolArgs &a, int top, hipStream_t st) {
+static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top, hipStream_t st) {
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.ni + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);
   if (top) {
-    hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), grd, blk, 0, st, a);
+    if (b.nact > 1) hipLaunchKernelGGL((k_vol_top<ID, JD, KD, true>), grd, blk, 0, st, a, b);
+    else hipLaunchKernelGGL((k_vol_top<ID, JD, KD, false>), grd, blk, 0, st, a, b);
     return;
   }
   if (variant == 2) {
     launch_vol_lds(a, ID, JD, KD, st);
     return;
   }
-  hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), grd, blk, 0, st, a);
+  if (b.nact > 1) hipLaunchKernelGGL((k_vol_direct<ID, JD, KD, true>), grd, blk, 0, st, a, b);
+  else hipLaunchKernelGGL((k_vol_direct<ID, JD, KD, false>), grd, blk, 0, st, a, b);
 }
 
-void launch_volume(int variant, const VolArgs &a, int idir, int jdir, int kdir, int top, hipStream_t st) {
+void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
+                   hipStream_t st) {
   int key = idir * 4 + jdir * 2 + kdir;
   switch (key) {
-    case 0: launch_dir<0, 0, 0>(variant, a, top, st); break;
-    case 1: launch_dir<0, 0, 1>(variant, a, top, st); break;
-    case 2: launch_dir<0, 1, 0>(variant, a, top, st); break;
-    case 3: launch_dir<0, 1, 1>(variant, a, top, st); break;
-    case 4: launch_dir<1, 0, 0>(variant, a, top, st); break;
-    case 5: launch_dir<1, 0, 1>(variant, a, top, st); break;
-    case 6: launch_dir<1, 1, 0>(variant, a, top, st); break;
-    default: launch_dir<1, 1, 1>(variant, a, top, st); break;
+    case 0: launch_dir<0, 0, 0>(variant, a, b, top, st); break;
+    case 1: launch_dir<0, 0, 1>(variant, a, b, top, st); break;
+    case 2: launch_dir<0, 1, 0>(variant, a, b, top, st); break;
+    case 3: launch_dir<0, 1, 1>(variant, a, b, top, st); break;
+    case 4: launch_dir<1, 0, 0>(variant, a, b, top, st); break;
+    case 5: launch_dir<1, 0, 1>(variant, a, b, top, st); break;
+    case 6: launch_dir<1, 1, 0>(variant, a, b, top, st); break;
+    default: launch_dir<1, 1, 1>(variant, a, b, top, st); break;
   }
 }
 
@@ -357,10 +403,11 @@ void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const flo
   hipLaunchKernelGGL(k_dvh2dvz, grd, blk, 0, st, g, met, lam, mu, mx, my);
 }
 
-void launch_pg(const Grid &g, const float *wend, const float *wpre, float *PG, float *Dis, float dt, hipStream_t st) {
+void launch_pg(const Grid &g, const BatArgs &b, const float *wend, const float *wpre, float *PG, float *Dis, float dt,
+               hipStream_t st) {
   dim3 blk(64, 4, 1);
   dim3 grd((g.ni + 63) / 64, (g.nj + 3) / 4, 1);
-  hipLaunchKernelGGL(k_pg, grd, blk, 0, st, g, wend, wpre, PG, Dis, dt);
+  hipLaunchKernelGGL(k_pg, grd, blk, 0, st, g, b, wend, wpre, PG, Dis, dt);
 }
 
 }  // namespace cgfd

@@ -13,6 +13,7 @@
 #include <new>
 #include <string>
 #include <vector>
+#include <memory>
 
 namespace {
 
@@ -267,6 +268,12 @@ struct cgfd_setup {
     std::vector<float> rho_f, lam_f, mu_f, vec_n, vec_s1, vec_s2, x_et, y_et, z_et;
     std::vector<float> D[12], P2M[5], M2P[5], M2[6], Fm[14];
   } f[CGFD_MAX_FAULTS];
+  /* further scenarios on the same grid: fault inputs and initial tractions */
+  struct Scen {
+    std::vector<float> in[CGFD_MAX_FAULTS][7];
+    std::vector<float> Tn0[CGFD_MAX_FAULTS], Ts10[CGFD_MAX_FAULTS], Ts20[CGFD_MAX_FAULTS];
+  };
+  std::vector<std::unique_ptr<Scen>> scen;
 };
 
 extern "C" {
@@ -633,6 +640,48 @@ int cgfd_setup_get_desc(const cgfd_setup *s, cgfd_desc *d) {
   }
   d->dt = s->p.dt; d->t0 = s->p.t0; d->nt_total = s->p.nt_total;
   d->device_id = s->p.device_id;
+  return CGFD_OK;
+}
+
+int cgfd_setup_scenario_desc(cgfd_setup *s, const float *const *init_stress, int nt_total, cgfd_desc *d) {
+  if (!s || !d || (s->p.number_fault > 0 && !init_stress))
+    return cgfd_set_error(CGFD_ERR_ARG, "cgfd_setup_scenario_desc: null argument");
+  const Grid &g = s->g;
+  const size_t ns = (size_t)g.ny * g.nz;
+  for (int q = 0; q < 7 * s->p.number_fault; q++)
+    if (!init_stress[q]) return cgfd_set_error(CGFD_ERR_ARG, "cgfd_setup_scenario_desc: input array %d missing", q);
+  std::unique_ptr<cgfd_setup::Scen> sc(new (std::nothrow) cgfd_setup::Scen());
+  if (!sc) return cgfd_set_error(CGFD_ERR_NOMEM, "host allocation failed");
+  for (int id = 0; id < s->p.number_fault; id++) {
+    const cgfd_setup::F &F = s->f[id];
+    for (int q = 0; q < 7; q++) sc->in[id][q].assign(ns, 0.f);
+    for (int k = 0; k < g.nk; k++)
+      for (int j = 0; j < g.nj; j++)
+        for (int q = 0; q < 7; q++)
+          sc->in[id][q][(j + 3) + (size_t)(k + 3) * g.ny] = init_stress[7 * id + q][j + (size_t)k * g.nj];
+    sc->Tn0[id].assign(ns, 0.f); sc->Ts10[id].assign(ns, 0.f); sc->Ts20[id].assign(ns, 0.f);
+    /* the same projections as fault_set (fault_info.cu:1015-1023) */
+    for (int k = 0; k < g.nk; k++)
+      for (int j = 0; j < g.nj; j++) {
+        size_t pf = (j + 3) + (size_t)(k + 3) * g.ny;
+        const float *vn = &F.vec_n[pf * 3], *s1 = &F.vec_s1[pf * 3], *s2 = &F.vec_s2[pf * 3];
+        float T0x = sc->in[id][0][pf], T0y = sc->in[id][1][pf], T0z = sc->in[id][2][pf];
+        sc->Tn0[id][pf] = T0x * vn[0] + T0y * vn[1] + T0z * vn[2];
+        sc->Ts10[id][pf] = T0x * s1[0] + T0y * s1[1] + T0z * s1[2];
+        sc->Ts20[id][pf] = T0x * s2[0] + T0y * s2[1] + T0z * s2[2];
+      }
+  }
+  int rc = cgfd_setup_get_desc(s, d);
+  if (rc) return rc;
+  for (int id = 0; id < s->p.number_fault; id++) {
+    cgfd_fault_desc &fd = d->fault[id];
+    fd.T0x = sc->in[id][0].data(); fd.T0y = sc->in[id][1].data(); fd.T0z = sc->in[id][2].data();
+    fd.mu_s = sc->in[id][3].data(); fd.mu_d = sc->in[id][4].data(); fd.Dc = sc->in[id][5].data();
+    fd.C0 = sc->in[id][6].data();
+    fd.Tn0 = sc->Tn0[id].data(); fd.Ts10 = sc->Ts10[id].data(); fd.Ts20 = sc->Ts20[id].data();
+  }
+  d->nt_total = nt_total;
+  s->scen.push_back(std::move(sc));
   return CGFD_OK;
 }
 

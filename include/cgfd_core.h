@@ -191,7 +191,7 @@ int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
                         float *ms_volume, int *n_volume);
 
-/* Kernel variant selection for the volume stencil (0 = default,
+/* Kernel variant selection for the volume stencil (0 = default = 1,
    1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
    6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
@@ -209,6 +209,28 @@ int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
    that the next step rewrites before reading, so the simulation state is
    unchanged. */
 int cgfd_core_autotune(cgfd_core *core, int reps, int *variant, int *zchunk, float *ms_best);
+
+/* ---------------------------------------------------------------------
+ * Batched ensembles: nscen rupture scenarios on one grid. The scenarios
+ * share the grid, metric, media, CFS-PML, free surface, fault geometry and
+ * fault coefficient matrices (taken from descs[0]; the other descriptors
+ * must agree on sizes, boundaries, method, fault indices and dt) and differ
+ * in the per-node fault inputs (T0x/y/z, mu_s, mu_d, Dc, C0, Tn0/Ts10/Ts20)
+ * and in nt_total. The kernels load the shared data of a point once for all
+ * scenarios. cgfd_core_run advances every scenario that has not yet taken
+ * its nt_total steps; a scenario that has retires and is no longer touched.
+ * All cgfd_core_* functions apply to a batch, reading scenario 0.
+ * No counterpart in the reference, which runs one scenario per process.
+ * --------------------------------------------------------------------- */
+int cgfd_batch_create(const cgfd_desc *descs, int nscen, cgfd_core **out);
+int cgfd_batch_active(cgfd_core *core);                 /* scenarios still running */
+int cgfd_batch_steps(const cgfd_core *core, int s);     /* steps scenario s has taken */
+int cgfd_batch_read_wavefield(cgfd_core *core, int s, float *host_dst);
+int cgfd_batch_write_wavefield(cgfd_core *core, int s, const float *host_src);
+int cgfd_batch_read_fault(cgfd_core *core, int s, int id, float *host_dst);
+int cgfd_batch_read_fault_state(cgfd_core *core, int s, int id, float *host_dst);
+int cgfd_batch_read_pg(cgfd_core *core, int s, float *host_dst);
+int cgfd_batch_read_stations(cgfd_core *core, int s, float *host_dst, int *nt);
 
 /* The one-call drop-in: create + run nt_total steps + copy the final state
    into the optional outputs (NULL to skip) + destroy. */

@@ -53,6 +53,14 @@ int cgfd_setup_destroy(cgfd_setup *s);
 /* Fills *d with the boundary descriptor (pointers into the setup object). */
 int cgfd_setup_get_desc(const cgfd_setup *s, cgfd_desc *d);
 
+/* A further rupture scenario on the setup's grid, for cgfd_batch_create:
+   init_stress[7*id + q] holds field q (Tx,Ty,Tz,mu_s,mu_d,Dc,C0, layout as in
+   cgfd_setup_params) of fault id. The setup keeps copies of these inputs and
+   the initial tractions projected on the fault frame (fault_info.cu:1015-1023)
+   and fills *d with a descriptor that shares everything else with the
+   setup's own. The descriptor stays valid until cgfd_setup_destroy. */
+int cgfd_setup_scenario_desc(cgfd_setup *s, const float *const *init_stress, int nt_total, cgfd_desc *d);
+
 /* Extended-grid coordinates x3d,y3d,z3d (3 x nx*ny*nz) after geometric_symmetry. */
 const float *cgfd_setup_coords(const cgfd_setup *s);
 

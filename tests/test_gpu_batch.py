@@ -1,0 +1,77 @@
+"""Batched ensembles on the GPU: scenarios with their own friction and
+stress fields and their own run lengths, on one shared grid, against the
+same scenarios run one by one through the plain core (and the oracle for
+one of them)."""
+import numpy as np
+import pytest
+
+from helpers import fault_phys, interior, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenario_fields(sc, seed):
+    """Perturbed copies of the configuration's fault inputs."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    f = {k: v.copy() for k, v in sc.faults[0].fields.items()}
+    f["Ty"] = f["Ty"] * np.float32(1.0 + 0.02 * rng.uniform(-1, 1))
+    f["Dc"] = f["Dc"] * np.float32(rng.uniform(0.8, 1.2))
+    f["mu_d"] = f["mu_d"] + np.float32(rng.uniform(-0.02, 0.02))
+    return [f]
+
+
+def _plain_with_fields(sc, fields):
+    from cgfd import model
+    import copy
+    sc2 = copy.copy(sc)
+    sc2.faults = [copy.copy(sc.faults[0])]
+    sc2.faults[0].fields = fields[0]
+    return model.Setup(sc2)
+
+
+@pytest.mark.parametrize("variant", [1, 7])
+def test_batch_matches_plain_runs(built_libs, variant):
+    from cgfd import configs, model
+    import oracle
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    nts = [40, 25, 40, 33]
+    fields = [None] + [_scenario_fields(sc, 100 + s) for s in range(1, len(nts))]
+    bat = model.Batch(st, list(zip(fields, nts)), variant=variant)
+    bat.run(max(nts) + 5)
+    bat.sync()
+    assert bat.active == 0
+    for s, nt in enumerate(nts):
+        assert bat.steps(s) == nt
+        st_s = st if fields[s] is None else _plain_with_fields(sc, fields[s])
+        core = model.Core(st_s, variant=variant)
+        core.run(nt)
+        wg, wp = bat.read_wavefield(s), core.read_wavefield()
+        fg, fp = bat.read_fault(s), core.read_fault(0)
+        r = rel_l2(interior(wg), interior(wp))
+        assert r <= 1e-6, f"scenario {s}: batched vs plain wavefield rel L2 {r:.3e}"
+        assert rel_l2(fault_phys(fg)[6], fault_phys(fp)[6]) <= 1e-6
+        if s == 1:
+            orc = oracle.Oracle(st_s.desc, sc.shape_ext, 1)
+            orc.run(nt)
+            assert rel_l2(interior(wg), interior(orc.read_wavefield())) <= 1e-3
+            assert rel_l2(fault_phys(fg)[6], fault_phys(orc.read_fault())[6]) <= 1e-3
+        core.close()
+    bat.close()
+
+
+def test_batch_graph_replay_matches_eager(built_libs):
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    nts = [37, 21, 37]
+    fields = [None] + [_scenario_fields(sc, 200 + s) for s in range(1, len(nts))]
+    bg = model.Batch(st, list(zip(fields, nts)))
+    be = model.Batch(st, list(zip(fields, nts)))
+    be.set_graph(False)
+    for n in (5, 16, 3, 16):
+        bg.run(n)
+        be.run(n)
+        for s in range(len(nts)):
+            assert np.array_equal(bg.read_wavefield(s), be.read_wavefield(s))
+            assert np.array_equal(bg.read_fault(s), be.read_fault(s))
