@@ -68,7 +68,7 @@ def calib_copy(mbytes: int) -> None:
     log(f"[bench] calibration copy of {mbytes} MiB done")
 
 
-def cpu_baseline(threads: int, steps: int = 3):
+def cpu_baseline(threads: int, steps: int = 24):
     """The oracle (FP32 CPU restatement, OpenMP) on a bounded sample: the same
     configuration generator at 1/4 linear scale (150 x 300 x 100 nodes)."""
     from cgfd import configs, model
@@ -89,15 +89,55 @@ def cpu_baseline(threads: int, steps: int = 3):
                       f"{steps} steps after 1 warm-up, {n} OpenMP threads, {el:.1f} s"}
 
 
+def batched_leg(steps: int, nscen: int):
+    """Config-3 ensemble: nscen scenarios on one 400x200x150 grid, batched,
+    against scenario 0 alone through the plain path on the same grid."""
+    from cgfd import configs, model
+    t0 = time.perf_counter()
+    base, fields, nts = configs.config3(nscen=nscen)
+    st = model.Setup(base)
+    log(f"[bench] config3 {base.ni}x{base.nj}x{base.nk}, {nscen} scenarios, setup {time.perf_counter() - t0:.1f} s")
+    cells = base.ni * base.nj * base.nk
+    # plain: scenario 0 alone
+    core = model.Core(st, variant=1)
+    core.run(8)
+    core.sync()
+    t = time.perf_counter()
+    core.run(steps)
+    core.sync()
+    plain = cells * steps / (time.perf_counter() - t) / 1e9
+    core.close()
+    # batched: every scenario active for the timed steps (run lengths >= 1000)
+    big = [max(n, steps + 16) for n in nts]
+    bat = model.Batch(st, [(None if s == 0 else [fields[s]], big[s]) for s in range(nscen)], variant=1)
+    bat.run(8)
+    bat.sync()
+    t = time.perf_counter()
+    bat.run(steps)
+    bat.sync()
+    el = time.perf_counter() - t
+    batched = cells * nscen * steps / el / 1e9
+    bat.close()
+    st.close()
+    return {"value": round(batched, 4), "unit": "scenario-Gcell-updates/s", "nscen": nscen,
+            "grid": [base.ni, base.nj, base.nk], "steps": steps, "ms_per_step": round(el / steps * 1e3, 3),
+            "plain_same_grid": round(plain, 4), "per_scenario_vs_plain": round(batched / plain, 3),
+            "note": "all scenarios active in the timed steps; retirement is exercised by the GPU tests"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the workload (1.0 = 1200x600x400)")
-    ap.add_argument("--variant", type=int, default=0, help="volume kernel variant (0 = autotune)")
+    ap.add_argument("--variant", type=int, default=1,
+                    help="volume kernel variant (1 = direct loads, the one profiles/ covers; 0 = autotune)")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per block (0 = autotune/default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=16)
+    ap.add_argument("--nscen", type=int, default=32)
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
@@ -178,6 +218,11 @@ def main() -> int:
     }
     core.close()
     st.close()
+    del core, st, sc
+    if not args.no_batched:
+        log("[bench] batched ensemble ...")
+        out["batched"] = batched_leg(args.batched_steps, args.nscen)
+    out["adjoint_vs_forward"] = None
     if rank == 0 and not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_threads)

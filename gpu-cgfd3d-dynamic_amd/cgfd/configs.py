@@ -243,3 +243,65 @@ def config2(scale: float = 1.0, steps: int = 4000, seed: int = 2002, imethod: in
                     faults=[FaultSpec(x_index=i0, grid=(j1, j2, k1, k2), fields=f)],
                     united_margin=pml + 3, dt=0.004, nt_total=steps + 1, stations=st,
                     name=f"config2x{scale:g}" if scale != 1.0 else "config2")
+
+
+def config3(nscen: int = 32, scale: float = 1.0, steps_scale: float = 1.0, seed: int = 3000, imethod: int = 2):
+    """Batched ensemble: ``nscen`` rupture scenarios on one 400x200x150 grid
+    at 100 m (reference axes ni=200 fault-normal, nj=400 strike, nk=150),
+    3 seeded Gaussian hills, layered medium as in config2, PML 20, dt 0.008.
+    Scenario s (seed 3000+s) draws mu_s~U(0.62,0.70), mu_d=mu_s-0.15,
+    Dc~U(0.3,0.5) m and a von Karman shear-stress field (mean 0.6 sigma_n,
+    std 0.05 sigma_n), a fault length L~U(10,30) km set by locking the rest
+    of the plane (mu_s = 1000), a 1-3 km nucleation patch at a random strike
+    position and depth inside its fault, and a run length U(1000,3000) steps
+    (times ``steps_scale``). Returns (base Scenario, [fault field dicts],
+    [nt]); the base scenario carries scenario 0's fields and the widest
+    fault grid."""
+    rng = _rng(seed)
+    dh = 100.0 / scale
+    ni, nj, nk = int(200 * scale), int(400 * scale), int(150 * scale)
+    pml = max(int(round(20 * scale)), 5)
+    i0 = ni // 2
+    xs = ((np.arange(ni) - i0) * dh)[None, :]
+    ys = ((np.arange(nj) - nj // 2) * dh)[:, None]
+    topo = hills(np.broadcast_to(xs, (nj, ni)), np.broadcast_to(ys, (nj, ni)), rng, 3, 800.0, 2000.0, 10000.0)
+    x, y, z = curvilinear_coords(ni, nj, nk, dh, i0, topo, dh, 1.5 * dh)
+    lam, mu, rho = layered_media(z, topo, rng, 0.05)
+    nlen_max = min(int(round(30000 / dh)) + 1, nj - 2 * (pml + 3) - 8)
+    ndep = min(int(round(18000 / dh)) + 1, nk - pml - 10)
+    jc = nj // 2
+    j1, j2 = jc - nlen_max // 2, jc - nlen_max // 2 + nlen_max - 1
+    k2 = nk - 1
+    k1 = k2 - ndep + 1
+    depth_km = (topo[:, i0][None, :] - z[:, :, i0]) / 1000.0
+    sn = np.minimum(-np.minimum(7.378 * depth_km, 120.0) * MPA, -1.0 * MPA)
+    jj, kk = np.meshgrid(np.arange(nj), np.arange(nk))
+    fields, nts = [], []
+    for s in range(nscen):
+        r = _rng(seed + s)
+        f = _fault_fields(nj, nk)
+        nlen = int(round(r.uniform(10000, 30000) / dh)) + 1
+        nlen = min(nlen, nlen_max)
+        a1 = jc - nlen // 2
+        inside = (jj >= a1) & (jj < a1 + nlen) & (kk >= k1) & (kk <= k2)
+        mus = r.uniform(0.62, 0.70, size=(nk, nj))
+        f["mu_s"][:] = np.where(inside, mus, 1000.0)
+        f["mu_d"][:] = mus - 0.15
+        f["Dc"][:] = r.uniform(0.3, 0.5, size=(nk, nj))
+        vk = von_karman(nj, nk, dh, 2000.0, 0.75, r)
+        f["Tx"][:] = sn
+        f["Ty"][:] = (0.6 + 0.05 * vk) * (-sn)
+        npatch = int(round(r.uniform(1000, 3000) / dh)) + 1
+        pj = int(r.integers(a1, a1 + max(nlen - npatch, 1)))
+        pk = int(r.integers(k1, max(k2 - npatch, k1 + 1)))
+        patch = (jj >= pj) & (jj < pj + npatch) & (kk >= pk) & (kk < pk + npatch) & inside
+        f["Ty"][patch] = mus[patch] * (-sn[patch]) + 1.0 * MPA
+        fields.append(f)
+        nts.append(max(int(round(r.uniform(1000, 3000) * steps_scale)), 1))
+    layers = [[pml, pml], [pml, pml], [pml, 0]]
+    base = Scenario(ni=ni, nj=nj, nk=nk, x=x, y=y, z=z, lam=lam, mu=mu, rho=rho, free_top=True,
+                    pml_layers=layers, imethod=imethod,
+                    faults=[FaultSpec(x_index=i0, grid=(j1, j2, k1, k2), fields=fields[0])],
+                    united_margin=pml + 3, dt=0.008 / scale, nt_total=nts[0], stations=None,
+                    name="config3" if scale == 1.0 else f"config3x{scale:g}")
+    return base, fields, nts

@@ -569,10 +569,12 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
 /* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
    no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march,
    7..9 direct loads on an L2-aware block schedule with the interior tiles
-   at the compiler's occupancy / >= 4 / >= 6 waves per SIMD */
+   at the compiler's occupancy / >= 4 / >= 6 waves per SIMD, 10 direct loads
+   with row-major thread order */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
-  if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
-  else if (c->variant >= 7) launch_vol_sched(va, c->bat, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
+  if (c->bat.nact > 1) launch_volume(1, va, c->bat, idir, jdir, kdir, 0, c->st);  /* batch kernel */
+  else if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
+  else if (c->variant >= 7 && c->variant <= 9) launch_vol_sched(va, c->bat, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
   else if (c->variant >= 3 && c->variant <= 5) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
   else launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st);
 }
@@ -901,9 +903,7 @@ extern "C" int cgfd_batch_active(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant == 0) variant = 1;
-  if (variant < 1 || variant > 9) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
-  if (c->nscen > 1 && variant != 1 && variant < 7)
-    return cgfd_set_error(CGFD_ERR_ARG, "variant %d does not run batches (use 1 or 7..9)", variant);
+  if (variant < 1 || variant > 10) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
   c->drop_graph();
   return CGFD_OK;
@@ -922,7 +922,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   if (!c || reps < 1) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
   HIPCHK(hipSetDevice(c->dev));
   c->drop_graph();
-  if (c->nvol <= 0) {
+  if (c->nvol <= 0 || c->nscen > 1) {  /* batches have one volume kernel */
     if (variant_out) *variant_out = c->variant;
     if (zchunk_out) *zchunk_out = c->zchunk;
     if (ms_out) *ms_out = 0.0f;
@@ -933,15 +933,14 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[9] = {1, 2, 3, 4, 5, 6, 7, 8, 9};
+  const int variants[10] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
   const int chunks[4] = {16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 9; vi++)
+  for (int vi = 0; vi < 10; vi++)
     for (int zi = 0; zi < 4; zi++) {
       int v = variants[vi], z = chunks[zi];
-      if (v == 1 && zi > 0) continue;  /* the direct kernel has no z chunks */
-      if (c->nscen > 1 && v != 1 && v < 7) continue;  /* single-scenario kernels */
+      if ((v == 1 || v == 10) && zi > 0) continue;  /* the direct kernels have no z chunks */
       c->variant = v;
       c->zchunk = z;
       int rc = build_zq_tiles(c, c->nvol);

@@ -42,10 +42,9 @@ DEVI void metric_cols(const float *met, long sc, long p, float M[3][3]) {
 }
 
 /* ---------------------------------------------------------------------- */
-DEVI void k_wave2fault_body(const FaultArgs &a) {
+DEVI void k_wave2fault_body(const FaultArgs &a, int id) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
-  const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
   const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp;
@@ -81,10 +80,9 @@ DEVI void k_wave2fault_body(const FaultArgs &a) {
   }
 }
 
-template <bool MULTI>
 __global__ void __launch_bounds__(256) k_wave2fault(FaultArgs a, BatArgs b) {
-  const int nq = MULTI ? b.nact : 1;
-  for (int q = 0; q < nq; q++) k_wave2fault_body(fault_at(a, b, b.act[q]));
+  const int id = (int)blockIdx.z % a.nfault;
+  k_wave2fault_body(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
 }
 
 /* split-node average back onto the wavefield column i0 (transform.cu:331-372),
@@ -131,10 +129,9 @@ DEVI float dz_free_image(const float *col, long ny, int n_free, int kdir) {
 }
 
 /* ---------------------------------------------------------------------- */
-DEVI void k_trial_body(const FaultArgs &a) {
+DEVI void k_trial_body(const FaultArgs &a, int id) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
-  const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
   const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny;
@@ -244,10 +241,9 @@ DEVI void k_trial_body(const FaultArgs &a) {
   fault2wave_node(a, F, pf, p0, a.w_cur, fc);
 }
 
-template <bool MULTI>
 __global__ void __launch_bounds__(128) k_trial(FaultArgs a, BatArgs b) {
-  const int nq = MULTI ? b.nact : 1;
-  for (int q = 0; q < nq; q++) k_trial_body(fault_at(a, b, b.act[q]));
+  const int id = (int)blockIdx.z % a.nfault;
+  k_trial_body(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
 }
 
 /* Hooke's law on one column point (sv_curv_col_el_iso_fault_gpu.cu:1126-1154) */
@@ -294,13 +290,12 @@ DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
 constexpr int kColNodes = 10;  /* nodes per wavefront in the column kernel */
 
 template <int IDIR>
-DEVI void k_fault_cols_body(const FaultArgs &a) {
+DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
   const int lane = threadIdx.x;
   if (lane >= 6 * kColNodes) return;
   const int iy = blockIdx.x * kColNodes + lane / 6;
   const int cl = lane % 6;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
-  const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
   const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
@@ -426,10 +421,10 @@ DEVI void k_fault_cols_body(const FaultArgs &a) {
   rk_cols(a, p, 0, 9, h);
 }
 
-template <int IDIR, bool MULTI>
+template <int IDIR>
 __global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a, BatArgs b) {
-  const int nq = MULTI ? b.nact : 1;
-  for (int q = 0; q < nq; q++) k_fault_cols_body<IDIR>(fault_at(a, b, b.act[q]));
+  const int id = (int)blockIdx.z % a.nfault;
+  k_fault_cols_body<IDIR>(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -441,10 +436,9 @@ __global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a, BatArgs b) {
    stage the per-step accumulators (fault_var_update_gpu, fault_wav_t.cu:
    197-264). One lane per node, lanes along j. */
 template <int IDIR>
-DEVI void k_fault_nodes_body(const FaultArgs &a) {
+DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
-  const int id = blockIdx.z;
   if (iy >= a.g.nj || iz >= a.g.nk) return;
   const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
@@ -642,10 +636,10 @@ DEVI void k_fault_nodes_body(const FaultArgs &a) {
   }
 }
 
-template <int IDIR, bool MULTI>
+template <int IDIR>
 __global__ void __launch_bounds__(256) k_fault_nodes(FaultArgs a, BatArgs b) {
-  const int nq = MULTI ? b.nact : 1;
-  for (int q = 0; q < nq; q++) k_fault_nodes_body<IDIR>(fault_at(a, b, b.act[q]));
+  const int id = (int)blockIdx.z % a.nfault;
+  k_fault_nodes_body<IDIR>(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -684,33 +678,24 @@ __global__ void k_record(BatArgs b, const float *w, long sicmp, const long *idx,
 /* ---------------------------------------------------------------------- */
 void launch_wave2fault(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
-  dim3 blk(64, 4, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault);
-  if (b.nact > 1) hipLaunchKernelGGL(k_wave2fault<true>, grd, blk, 0, st, a, b);
-  else hipLaunchKernelGGL(k_wave2fault<false>, grd, blk, 0, st, a, b);
+  dim3 blk(64, 4, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault * b.nact);
+  hipLaunchKernelGGL(k_wave2fault, grd, blk, 0, st, a, b);
 }
 void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
-  dim3 blk(64, 2, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 1) / 2, a.nfault);
-  if (b.nact > 1) hipLaunchKernelGGL(k_trial<true>, grd, blk, 0, st, a, b);
-  else hipLaunchKernelGGL(k_trial<false>, grd, blk, 0, st, a, b);
+  dim3 blk(64, 2, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 1) / 2, a.nfault * b.nact);
+  hipLaunchKernelGGL(k_trial, grd, blk, 0, st, a, b);
 }
 void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
-  dim3 cblk(64, 4, 1), cgrd((a.g.nj + kColNodes - 1) / kColNodes, (a.g.nk + 3) / 4, a.nfault);
-  dim3 nblk(64, 4, 1), ngrd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault);
-  const bool multi = b.nact > 1;
-  if (a.idir == 1 && multi) {
-    hipLaunchKernelGGL((k_fault_cols<1, true>), cgrd, cblk, 0, st, a, b);
-    hipLaunchKernelGGL((k_fault_nodes<1, true>), ngrd, nblk, 0, st, a, b);
-  } else if (a.idir == 1) {
-    hipLaunchKernelGGL((k_fault_cols<1, false>), cgrd, cblk, 0, st, a, b);
-    hipLaunchKernelGGL((k_fault_nodes<1, false>), ngrd, nblk, 0, st, a, b);
-  } else if (multi) {
-    hipLaunchKernelGGL((k_fault_cols<0, true>), cgrd, cblk, 0, st, a, b);
-    hipLaunchKernelGGL((k_fault_nodes<0, true>), ngrd, nblk, 0, st, a, b);
+  dim3 cblk(64, 4, 1), cgrd((a.g.nj + kColNodes - 1) / kColNodes, (a.g.nk + 3) / 4, a.nfault * b.nact);
+  dim3 nblk(64, 4, 1), ngrd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault * b.nact);
+  if (a.idir == 1) {
+    hipLaunchKernelGGL(k_fault_cols<1>, cgrd, cblk, 0, st, a, b);
+    hipLaunchKernelGGL(k_fault_nodes<1>, ngrd, nblk, 0, st, a, b);
   } else {
-    hipLaunchKernelGGL((k_fault_cols<0, false>), cgrd, cblk, 0, st, a, b);
-    hipLaunchKernelGGL((k_fault_nodes<0, false>), ngrd, nblk, 0, st, a, b);
+    hipLaunchKernelGGL(k_fault_cols<0>, cgrd, cblk, 0, st, a, b);
+    hipLaunchKernelGGL(k_fault_nodes<0>, ngrd, nblk, 0, st, a, b);
   }
 }
 void launch_step_end(int *it_dev, hipStream_t st) { hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, it_dev); }

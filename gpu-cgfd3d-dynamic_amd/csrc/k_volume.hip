@@ -26,48 +26,84 @@ namespace cgfd {
 /* one point of the direct kernels: the shared metric and media are loaded
    once, then for every active scenario 27 derivatives from the cache
    hierarchy, rhs + PML + RK */
-template <int ID, int JD, int KD, bool PML = true, bool MULTI = false>
-DEVI void direct_point(const VolArgs &a, const BatArgs &b, int i, int j, int k) {
+template <int ID, int JD, int KD, bool PML = true>
+DEVI void direct_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   const unsigned o = (unsigned)p * 4u;
   const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
   const float *M = a.met;
+  const long woff = (long)sn * b.wstride;
+  float Dx[9], Dy[9], Dz[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) {
+    const float *bc = a.cur + woff + c * sc;
+    Dx[c] = dfd32<ID>(bc, o, 4u);
+    Dy[c] = dfd32<JD>(bc, o, by);
+    Dz[c] = dfd32<KD>(bc, o, bz);
+  }
   const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
   const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
   const float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
   const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
-  const bool fault_owned = a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny);
-  const int nq = MULTI ? b.nact : 1;  /* one scenario: the loop disappears */
-  for (int q = 0; q < nq; q++) {
-    const long sn = b.act[q];
-    const long woff = sn * b.wstride;
-    float Dx[9], Dy[9], Dz[9];
-#pragma unroll
-    for (int c = 0; c < 9; c++) {
-      const float *bc = a.cur + woff + c * sc;
-      Dx[c] = dfd32<ID>(bc, o, 4u);
-      Dy[c] = dfd32<JD>(bc, o, by);
-      Dz[c] = dfd32<KD>(bc, o, bz);
-    }
-    float h[9];
-    rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
-    if (PML && a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h, sn * b.pstride);
-    if (!fault_owned) rk_point32(a, o, h, woff);
-  }
+  float h[9];
+  rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
+  if (PML && a.pml_on) pml_point(a, i, j, k, p, Dx, Dy, Dz, l, m, s, h, (long)sn * b.pstride);
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  rk_point32(a, o, h, woff);
+}
+
+/* ------------------------------------------------------------------------
+ * Batched volume kernel: one 64 x 4 tile of one plane for one scenario per
+ * block. Block ids map XCD-contiguously onto (tile, scenario) pairs with the
+ * scenario fastest, so the blocks of all scenarios of a tile run together on
+ * one XCD and the tile's metric and media lines come from HBM once for the
+ * whole batch.
+ * ---------------------------------------------------------------------- */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_bat(VolArgs a, BatArgs b, long nitems) {
+  const long L = blockIdx.x;
+  const long per = (long)(gridDim.x / 8);
+  const long item = (L % 8) * per + L / 8;
+  if (item >= nitems) return;
+  const int q = (int)(item % b.nact);
+  const long tile = item / b.nact;
+  const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + 3) / 4;
+  const int bx = (int)(tile % ntx);
+  const int by = (int)((tile / ntx) % nty);
+  const int kz = (int)(tile / ((long)ntx * nty));
+  const int ix = bx * 64 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
+  if (ix >= a.g.ni || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD>(a, b, ix + 3, iy + 3, a.k_lo + kz + 3, b.act[q]);
+}
+
+/* ------------------------------------------------------------------------
+ * Row-major direct kernel: consecutive threads take consecutive physical
+ * points of a plane (i fastest, wrapping to the next j), so every array is
+ * read and written in runs of whole rows instead of 256-byte tile rows,
+ * which keeps HBM pages open. grid = (ceil(ni*nj / 256), nk_run).
+ * ---------------------------------------------------------------------- */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_rows(VolArgs a, BatArgs b) {
+  const unsigned t = blockIdx.x * 256u + threadIdx.x;
+  const unsigned ni = (unsigned)a.g.ni;
+  if (t >= ni * (unsigned)a.g.nj) return;
+  const int iy = (int)(t / ni);
+  const int ix = (int)(t - (unsigned)iy * ni);
+  direct_point<ID, JD, KD>(a, b, ix + 3, iy + 3, a.k_lo + (int)blockIdx.y + 3, b.act[0]);
 }
 
 /* ------------------------------------------------------------------------
  * Direct-load volume kernel: one thread per point, neighbours from the
  * cache hierarchy. Handles physical planes k_lo .. k_lo+nk_run-1.
  * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD, bool MULTI>
+template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a, BatArgs b) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
   const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
-  direct_point<ID, JD, KD, true, MULTI>(a, b, i, j, k);
+  direct_point<ID, JD, KD>(a, b, i, j, k, b.act[0]);
 }
 
 /* ------------------------------------------------------------------------
@@ -81,7 +117,7 @@ __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a, BatArgs b) {
  * tiles run without the CFS-PML code (fewer registers); MINW > 0 asks the
  * compiler for at least MINW waves per SIMD.
  * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD, bool PML, int MINW, bool MULTI>
+template <int ID, int JD, int KD, bool PML, int MINW>
 __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
     k_vol_sched(VolArgs a, BatArgs b, const int *__restrict__ tiles, long nitems, int ntx, int nty) {
   const long L = blockIdx.x;
@@ -99,32 +135,26 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
   const int ix = bx * 64 + (int)threadIdx.x;
   const int iy = by * 4 + (int)threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, PML, MULTI>(a, b, ix + 3, iy + 3, a.k_lo + kz + 3);
+  direct_point<ID, JD, KD, PML>(a, b, ix + 3, iy + 3, a.k_lo + kz + 3, b.act[0]);
 }
 
-template <int ID, int JD, int KD, int MINW, bool MULTI>
-static void launch_sched_m(const VolArgs &a, const BatArgs &b, const ZqTiles &t, hipStream_t st) {
+template <int ID, int JD, int KD, int MINW>
+static void launch_sched_t(const VolArgs &a, const BatArgs &b, const ZqTiles &t, hipStream_t st) {
   dim3 blk(64, 4, 1);
   if (t.n_int > 0) {
     const long items = (long)t.n_int * a.zchunk;
     const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, false, MINW, MULTI>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_sint,
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, false, MINW>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_sint,
                        items,
                        t.ntx, t.nty);
   }
   if (t.n_pml > 0) {
     const long items = (long)t.n_pml * a.zchunk;
     const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, true, 0, MULTI>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_spml,
+    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, true, 0>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_spml,
                        items,
                        t.ntx, t.nty);
   }
-}
-
-template <int ID, int JD, int KD, int MINW>
-static void launch_sched_t(const VolArgs &a, const BatArgs &b, const ZqTiles &t, hipStream_t st) {
-  if (b.nact > 1) launch_sched_m<ID, JD, KD, MINW, true>(a, b, t, st);
-  else launch_sched_m<ID, JD, KD, MINW, false>(a, b, t, st);
 }
 
 template <int MINW>
@@ -253,17 +283,15 @@ DEVI void top_point(const VolArgs &a, int i, int j, int k, long woff, long poff)
   rk_point(a, p, h, woff);
 }
 
-template <int ID, int JD, int KD, bool MULTI>
+/* grid z = plane x scenario */
+template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256) k_vol_top(VolArgs a, BatArgs b) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
-  const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
-  const int nq = MULTI ? b.nact : 1;
-  for (int q = 0; q < nq; q++) {
-    const long sn = b.act[q];
-    top_point<ID, JD, KD>(a, i, j, k, sn * b.wstride, sn * b.pstride);
-  }
+  const int kz = (int)blockIdx.z % a.nk_run;
+  const long sn = b.act[blockIdx.z / a.nk_run];
+  top_point<ID, JD, KD>(a, ix + 3, iy + 3, a.k_lo + kz + 3, sn * b.wstride, sn * b.pstride);
 }
 
 /* ------------------------------------------------------------------------
@@ -356,10 +384,8 @@ __global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const fl
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   if (ix >= g.ni || iy >= g.nj) return;
-  for (int q = 0; q < b.nact; q++) {
-    const long s = b.act[q];
-    pg_point(g, ix, iy, wend + s * b.wstride, wpre + s * b.wstride, PG + s * b.pgstride, Dis + s * b.pgstride, dt);
-  }
+  const long s = b.act[blockIdx.z];
+  pg_point(g, ix, iy, wend + s * b.wstride, wpre + s * b.wstride, PG + s * b.pgstride, Dis + s * b.pgstride, dt);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -369,16 +395,26 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.ni + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);
   if (top) {
-    if (b.nact > 1) hipLaunchKernelGGL((k_vol_top<ID, JD, KD, true>), grd, blk, 0, st, a, b);
-    else hipLaunchKernelGGL((k_vol_top<ID, JD, KD, false>), grd, blk, 0, st, a, b);
+    dim3 tgrd(grd.x, grd.y, (unsigned)(a.nk_run * b.nact));
+    hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
+    return;
+  }
+  if (b.nact > 1) {  /* batches: one kernel, tiles x scenarios */
+    const long items = (long)grd.x * grd.y * a.nk_run * b.nact;
+    const long nb = ((items + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_vol_bat<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b, items);
     return;
   }
   if (variant == 2) {
     launch_vol_lds(a, ID, JD, KD, st);
     return;
   }
-  if (b.nact > 1) hipLaunchKernelGGL((k_vol_direct<ID, JD, KD, true>), grd, blk, 0, st, a, b);
-  else hipLaunchKernelGGL((k_vol_direct<ID, JD, KD, false>), grd, blk, 0, st, a, b);
+  if (variant == 10) {
+    dim3 rgrd((unsigned)(((long)a.g.ni * a.g.nj + 255) / 256), a.nk_run, 1);
+    hipLaunchKernelGGL((k_vol_rows<ID, JD, KD>), rgrd, dim3(256), 0, st, a, b);
+    return;
+  }
+  hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), grd, blk, 0, st, a, b);
 }
 
 void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
@@ -406,7 +442,7 @@ void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const flo
 void launch_pg(const Grid &g, const BatArgs &b, const float *wend, const float *wpre, float *PG, float *Dis, float dt,
                hipStream_t st) {
   dim3 blk(64, 4, 1);
-  dim3 grd((g.ni + 63) / 64, (g.nj + 3) / 4, 1);
+  dim3 grd((g.ni + 63) / 64, (g.nj + 3) / 4, b.nact);
   hipLaunchKernelGGL(k_pg, grd, blk, 0, st, g, b, wend, wpre, PG, Dis, dt);
 }
 

@@ -196,7 +196,9 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
    6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
    block schedule, 8 = the same with >= 4 waves/SIMD on interior tiles,
-   9 = the same with >= 6 waves/SIMD).
+   9 = the same with >= 6 waves/SIMD, 10 = direct loads, row-major thread
+   order). The variant applies to single-scenario launches; batches with
+   more than one active scenario use their own tile x scenario kernel.
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
