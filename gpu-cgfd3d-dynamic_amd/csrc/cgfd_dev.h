@@ -44,8 +44,9 @@ constexpr double kC1 = -0.291647047, kC2 = -0.666745142, kC3 = 1.250117713, kC4 
 struct Grid {
   int ni, nj, nk, nx, ny, nz;
   int nk2;           /* last physical k (ghost-inclusive) */
-  long siy, siz;     /* strides */
-  long sicmp;        /* points per component (nx*ny*nz) */
+  long siy, siz;     /* strides: siy is the device row pitch, nx rounded up to
+                        32 floats so that every row starts on a 128-byte line */
+  long sicmp;        /* points per component (siy*ny*nz) */
   long sslice;       /* ny*nz */
 };
 

@@ -106,6 +106,7 @@ void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st);
 void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st);
 void launch_step_end(int *it_dev, hipStream_t st);
 void launch_zero(float *p, long n, hipStream_t st);
+void launch_repitch(float *dst, long dpitch, const float *src, long spitch, long width, long rows, hipStream_t st);
 void launch_nan_check(const float *p, long n, int *flag, hipStream_t st);
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
                             int nt_max, const int *it_dev, int it0, hipStream_t st);

@@ -97,6 +97,7 @@ struct cgfd_core {
     return n;
   }
   float *PG = nullptr, *Dis = nullptr;
+  float *stage = nullptr;  /* nx*ny*nz floats: host-layout staging for volume copies */
 
   long *st_idx = nullptr;
   float *st_buf = nullptr;
@@ -134,7 +135,7 @@ static int check_desc(const cgfd_desc *d) {
   if (d->ni < 8 || d->nj < 8 || d->nk < 8)
     return cgfd_set_error(CGFD_ERR_ARG, "grid too small: ni=%d nj=%d nk=%d (each must be >= 8)", d->ni, d->nj, d->nk);
   /* the volume kernels address one field array with 32-bit byte offsets */
-  if ((double)(d->ni + 6) * (d->nj + 6) * (d->nk + 6) * 4.0 >= 4294967296.0)
+  if ((double)(((d->ni + 6 + 31) / 32) * 32) * (d->nj + 6) * (d->nk + 6) * 4.0 >= 4294967296.0)
     return cgfd_set_error(CGFD_ERR_ARG, "one field array of %dx%dx%d points exceeds 4 GiB; split the domain",
                           d->ni + 6, d->nj + 6, d->nk + 6);
   if (!d->metric || !d->lambda || !d->mu || !d->rho) return cgfd_set_error(CGFD_ERR_ARG, "metric/media pointers missing");
@@ -178,6 +179,34 @@ static int check_desc(const cgfd_desc *d) {
   return CGFD_OK;
 }
 
+/* Volume arrays: host rows of nx floats (the reference's layout), device
+   rows of siy floats. Each component goes through a contiguous device
+   staging buffer (one full-speed copy) and a re-pitch kernel. */
+static int host_to_dev(cgfd_core *c, float *dev, const float *host, int ncmp) {
+  const Grid &g = c->g;
+  const size_t hc = (size_t)g.nx * g.ny * g.nz;
+  for (int q = 0; q < ncmp; q++) {
+    HIPCHK(hipMemcpyAsync(c->stage, host + q * hc, hc * 4, hipMemcpyHostToDevice, c->st));
+    launch_repitch(dev + q * g.sicmp, g.siy, c->stage, g.nx, g.nx, (long)g.ny * g.nz, c->st);
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipGetLastError());
+  return CGFD_OK;
+}
+/* rows of nx floats from a device array with pitch siy to the host */
+static int dev_to_host(cgfd_core *c, float *host, const float *dev, long rows) {
+  const Grid &g = c->g;
+  const long per = (long)g.ny * g.nz;  /* rows per staging pass */
+  for (long r0 = 0; r0 < rows; r0 += per) {
+    const long nr = std::min(per, rows - r0);
+    launch_repitch(c->stage, g.nx, dev + r0 * g.siy, g.siy, g.nx, nr, c->st);
+    HIPCHK(hipMemcpyAsync(host + r0 * g.nx, c->stage, (size_t)nr * g.nx * 4, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipGetLastError());
+  return CGFD_OK;
+}
+
 /* host [node][9] -> device SoA [9][node] for a list of matrices */
 static void soa9(std::vector<float> &dst, size_t off, const float *src, size_t nodes) {
   for (size_t p = 0; p < nodes; p++)
@@ -194,14 +223,16 @@ static int build_zq_tiles(cgfd_core *c, int nvol) {
   ZqTiles t;
   const Grid &g = c->g;
   const int TX = ZqTiles::TX, TY = ZqTiles::TY, zc = c->zchunk;
-  t.ntx = (g.ni + TX - 1) / TX;
+  t.ntx = (g.ni + 3 + TX - 1) / TX;  /* tiles from ghost-inclusive column 0 (aligned kernels) */
   t.nty = (g.nj + TY - 1) / TY;
   t.ntz = nvol > 0 ? (nvol + zc - 1) / zc : 0;
   std::vector<int> ti, tp;
   for (int bz = 0; bz < t.ntz; bz++)
     for (int by = 0; by < t.nty; by++)
       for (int bx = 0; bx < t.ntx; bx++) {
-        int i0 = bx * TX + 3, i1 = std::min(bx * TX + TX, g.ni) - 1 + 3;
+        /* columns of tile bx for the aligned kernels ([64 bx, 64 bx + 63]) and for
+           the ones with origin at the first physical column ([64 bx + 3, 64 bx + 66]) */
+        int i0 = std::max(bx * TX, 3), i1 = std::min(bx * TX + TX + 2, g.ni + 2);
         int j0 = by * TY + 3, j1 = std::min(by * TY + TY, g.nj) - 1 + 3;
         int k0 = bz * zc + 3, k1 = std::min(bz * zc + zc, nvol) - 1 + 3;
         bool pml = false;
@@ -398,7 +429,8 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   g.ni = d->ni; g.nj = d->nj; g.nk = d->nk;
   g.nx = g.ni + 6; g.ny = g.nj + 6; g.nz = g.nk + 6;
   g.nk2 = 3 + g.nk - 1;
-  g.siy = g.nx; g.siz = (long)g.nx * g.ny; g.sicmp = g.siz * g.nz; g.sslice = (long)g.ny * g.nz;
+  g.siy = ((g.nx + 31) / 32) * 32;  /* 128-byte aligned rows on the device */
+  g.siz = g.siy * g.ny; g.sicmp = g.siz * g.nz; g.sslice = (long)g.ny * g.nz;
   c->dt = d->dt; c->t0 = d->t0; c->nt_total = d->nt_total;
   c->isfree = d->free_top ? 1 : 0;
   c->imethod = d->imethod;
@@ -422,11 +454,11 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   if ((rc = c->alloc(&c->lam, (size_t)3 * sc))) return fail(rc);
   c->mu = c->lam + sc;
   c->slw = c->lam + 2 * sc;
-  if (hipMemcpy(c->met, d->metric, (size_t)10 * sc * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->lam, d->lambda, (size_t)sc * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->mu, d->mu, (size_t)sc * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(c->slw, d->rho, (size_t)sc * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return fail(cgfd_set_error(CGFD_ERR_HIP, "upload of metric/media failed"));
+  /* staging buffer for host <-> device copies of volume arrays */
+  if ((rc = c->alloc(&c->stage, (size_t)g.nx * g.ny * g.nz))) return fail(rc);
+  if ((rc = host_to_dev(c, c->met, d->metric, 10)) || (rc = host_to_dev(c, c->lam, d->lambda, 1)) ||
+      (rc = host_to_dev(c, c->mu, d->mu, 1)) || (rc = host_to_dev(c, c->slw, d->rho, 1)))
+    return fail(rc);
   /* free-surface matrices (bdry_free_set calloc, bdry_t.cu:60-71) */
   if ((rc = c->alloc(&c->matVx2Vz, (size_t)2 * 9 * g.siz))) return fail(rc);
   c->matVy2Vz = c->matVx2Vz + 9 * g.siz;
@@ -778,8 +810,7 @@ extern "C" int cgfd_batch_read_wavefield(cgfd_core *c, int s, float *dst) {
   if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  HIPCHK(hipMemcpy(dst, c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, (size_t)9 * c->g.sicmp * 4,
-                   hipMemcpyDeviceToHost));
+  if (int rc = dev_to_host(c, dst, c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, 9L * c->g.ny * c->g.nz)) return rc;
   return CGFD_OK;
 }
 extern "C" int cgfd_core_read_wavefield(cgfd_core *c, float *dst) { return cgfd_batch_read_wavefield(c, 0, dst); }
@@ -790,14 +821,15 @@ extern "C" int cgfd_batch_write_wavefield(cgfd_core *c, int s, const float *src)
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   const Grid &g = c->g;
-  std::vector<float> h((size_t)9 * g.sicmp, 0.0f);
+  const size_t hz = (size_t)g.nx * g.ny, hc = hz * g.nz;  /* host strides */
+  std::vector<float> h((size_t)9 * hc, 0.0f);
   for (int cmp = 0; cmp < 9; cmp++)
     for (int k = 3; k < 3 + g.nk; k++)
       for (int j = 3; j < 3 + g.nj; j++) {
-        size_t o = (size_t)cmp * g.sicmp + (size_t)k * g.siz + (size_t)j * g.siy;
+        size_t o = (size_t)cmp * hc + (size_t)k * hz + (size_t)j * g.nx;
         std::memcpy(&h[o + 3], &src[o + 3], (size_t)g.ni * 4);
       }
-  HIPCHK(hipMemcpy(c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  if (int rc = host_to_dev(c, c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, h.data(), 9)) return rc;
   return CGFD_OK;
 }
 extern "C" int cgfd_core_write_wavefield(cgfd_core *c, const float *src) { return cgfd_batch_write_wavefield(c, 0, src); }
@@ -832,12 +864,12 @@ extern "C" int cgfd_batch_read_pg(cgfd_core *c, int s, float *dst) {
   if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  size_t n = (size_t)15 * c->g.siz;
+  size_t n = (size_t)15 * c->g.nx * c->g.ny;
   if (!c->isfree) {
     std::memset(dst, 0, n * 4);
     return CGFD_OK;
   }
-  HIPCHK(hipMemcpy(dst, c->PG + s * c->bat.pgstride, n * 4, hipMemcpyDeviceToHost));
+  if (int rc = dev_to_host(c, dst, c->PG + s * c->bat.pgstride, 15L * c->g.ny)) return rc;
   return CGFD_OK;
 }
 extern "C" int cgfd_core_read_pg(cgfd_core *c, float *dst) { return cgfd_batch_read_pg(c, 0, dst); }
@@ -934,11 +966,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const int variants[10] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
-  const int chunks[4] = {16, 32, 64, 128};
+  const int chunks[6] = {4, 8, 16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
   for (int vi = 0; vi < 10; vi++)
-    for (int zi = 0; zi < 4; zi++) {
+    for (int zi = 0; zi < 6; zi++) {
       int v = variants[vi], z = chunks[zi];
       if ((v == 1 || v == 10) && zi > 0) continue;  /* the direct kernels have no z chunks */
       c->variant = v;

@@ -647,6 +647,16 @@ __global__ void k_step_end(int *it_dev) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *it_dev += 1;
 }
 
+/* copy `rows` rows of `width` floats between two pitches */
+__global__ void k_repitch(float *dst, long dpitch, const float *src, long spitch, long width, long rows) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = rows * width, stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const long r = i / width, x = i - r * width;
+    dst[r * dpitch + x] = src[r * spitch + x];
+  }
+}
+
 __global__ void k_zero(float *p, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long stride = (long)gridDim.x * blockDim.x;
@@ -699,6 +709,13 @@ void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   }
 }
 void launch_step_end(int *it_dev, hipStream_t st) { hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, st, it_dev); }
+void launch_repitch(float *dst, long dpitch, const float *src, long spitch, long width, long rows, hipStream_t st) {
+  if (rows <= 0 || width <= 0) return;
+  long blocks = (rows * width + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_repitch, dim3((unsigned)blocks), dim3(256), 0, st, dst, dpitch, src, spitch, width, rows);
+}
+
 void launch_zero(float *p, long n, hipStream_t st) {
   if (n <= 0) return;
   long blocks = (n + 255) / 256;

@@ -68,13 +68,13 @@ __global__ void __launch_bounds__(256) k_vol_bat(VolArgs a, BatArgs b, long nite
   if (item >= nitems) return;
   const int q = (int)(item % b.nact);
   const long tile = item / b.nact;
-  const int ntx = (a.g.ni + 63) / 64, nty = (a.g.nj + 3) / 4;
+  const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
   const int bx = (int)(tile % ntx);
   const int by = (int)((tile / ntx) % nty);
   const int kz = (int)(tile / ((long)ntx * nty));
-  const int ix = bx * 64 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
-  if (ix >= a.g.ni || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD>(a, b, ix + 3, iy + 3, a.k_lo + kz + 3, b.act[q]);
+  const int i = bx * 64 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;  /* i ghost-inclusive */
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD>(a, b, i, iy + 3, a.k_lo + kz + 3, b.act[q]);
 }
 
 /* ------------------------------------------------------------------------
@@ -99,10 +99,12 @@ __global__ void __launch_bounds__(256) k_vol_rows(VolArgs a, BatArgs b) {
  * ---------------------------------------------------------------------- */
 template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a, BatArgs b) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+  /* tiles start at ghost-inclusive column 0, so a 64-wide tile row is two
+     whole 128-byte lines of every array */
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-  if (ix >= a.g.ni || iy >= a.g.nj) return;
-  const int i = ix + 3, j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  const int j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
   direct_point<ID, JD, KD>(a, b, i, j, k, b.act[0]);
 }
 
@@ -132,10 +134,10 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
   const int bz = tile / (ntx * nty);
   const int kz = bz * Kc + kk;
   if (kz >= a.nk_run) return;
-  const int ix = bx * 64 + (int)threadIdx.x;
+  const int i = bx * 64 + (int)threadIdx.x;  /* ghost-inclusive, aligned tiles */
   const int iy = by * 4 + (int)threadIdx.y;
-  if (ix >= a.g.ni || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, PML>(a, b, ix + 3, iy + 3, a.k_lo + kz + 3, b.act[0]);
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD, PML>(a, b, i, iy + 3, a.k_lo + kz + 3, b.act[0]);
 }
 
 template <int ID, int JD, int KD, int MINW>
@@ -394,13 +396,14 @@ template <int ID, int JD, int KD>
 static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top, hipStream_t st) {
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.ni + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);
+  dim3 agrd((a.g.ni + 3 + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);  /* aligned tiles */
   if (top) {
     dim3 tgrd(grd.x, grd.y, (unsigned)(a.nk_run * b.nact));
     hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
     return;
   }
   if (b.nact > 1) {  /* batches: one kernel, tiles x scenarios */
-    const long items = (long)grd.x * grd.y * a.nk_run * b.nact;
+    const long items = (long)agrd.x * agrd.y * a.nk_run * b.nact;
     const long nb = ((items + 7) / 8) * 8;
     hipLaunchKernelGGL((k_vol_bat<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b, items);
     return;
@@ -414,7 +417,7 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_rows<ID, JD, KD>), rgrd, dim3(256), 0, st, a, b);
     return;
   }
-  hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), grd, blk, 0, st, a, b);
+  hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), agrd, blk, 0, st, a, b);
 }
 
 void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
