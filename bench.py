@@ -131,9 +131,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the workload (1.0 = 1200x600x400)")
-    ap.add_argument("--variant", type=int, default=1,
-                    help="volume kernel variant (1 = direct loads, the one profiles/ covers; 0 = autotune)")
-    ap.add_argument("--zchunk", type=int, default=0, help="z planes per block (0 = autotune/default)")
+    ap.add_argument("--variant", type=int, default=16,
+                    help="volume kernel variant (16 = split interior/PML kernels in slab order, the one the "
+                         "committed profiles cover; 0 = autotune)")
+    ap.add_argument("--zchunk", type=int, default=4, help="z planes per block / y tiles per slab (0 = default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=16)

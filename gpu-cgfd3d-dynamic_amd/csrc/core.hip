@@ -59,8 +59,8 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
-  int variant = 1;
-  int zchunk = 32;
+  int variant = 16;
+  int zchunk = 4;
   ZqTiles zq;                       /* tile lists for the current zchunk */
   std::map<int, ZqTiles> zq_cache;  /* by zchunk */
   int nvol = 0;
@@ -602,7 +602,12 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
    no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march,
    7..9 direct loads on an L2-aware block schedule with the interior tiles
    at the compiler's occupancy / >= 4 / >= 6 waves per SIMD, 10 direct loads
-   with row-major thread order */
+   with row-major thread order, 11 direct loads in slab order (zchunk y tiles
+   per slab), 12..14 split interior/PML direct kernels (interior with all
+   loads free to move / a fence every 3 fields / >= 8 waves per SIMD and a
+   fence every field; a fence makes a field's taps be consumed before the
+   next field's loads), 15 = 14 with fences in the PML blocks too, 16 = 14
+   with the interior blocks in slab order (zchunk y tiles per slab) */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   if (c->bat.nact > 1) launch_volume(1, va, c->bat, idir, jdir, kdir, 0, c->st);  /* batch kernel */
   else if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
@@ -934,8 +939,8 @@ extern "C" int cgfd_batch_active(cgfd_core *c) {
 
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
-  if (variant == 0) variant = 1;
-  if (variant < 1 || variant > 10) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant == 0) variant = 16;
+  if (variant < 1 || variant > 16) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
   c->drop_graph();
   return CGFD_OK;
@@ -965,14 +970,14 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[10] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+  const int variants[16] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
   const int chunks[6] = {4, 8, 16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 10; vi++)
+  for (int vi = 0; vi < 16; vi++)
     for (int zi = 0; zi < 6; zi++) {
       int v = variants[vi], z = chunks[zi];
-      if ((v == 1 || v == 10) && zi > 0) continue;  /* the direct kernels have no z chunks */
+      if ((v == 1 || v == 10 || (v >= 12 && v <= 15)) && zi > 0) continue;  /* no z chunks */
       c->variant = v;
       c->zchunk = z;
       int rc = build_zq_tiles(c, c->nvol);

@@ -129,6 +129,42 @@ DEVI float dz_free_image(const float *col, long ny, int n_free, int kdir) {
 }
 
 /* ---------------------------------------------------------------------- */
+/* Traction vectors J (grad xi . sigma) on the three columns each side of
+   the plane (the image columns of trial_slipweakening.cu:151-165), written
+   into the T1 slots 0-2 and 4-6. Six lanes per node (one per column), ten
+   nodes per wavefront, so a node's loads are adjacent addresses. */
+constexpr int kTrialNodes = 10;
+
+DEVI void k_trial_t1_body(const FaultArgs &a, int id) {
+  const int lane = threadIdx.x;
+  if (lane >= 6 * kTrialNodes) return;
+  const int iy = blockIdx.x * kTrialNodes + lane / 6;
+  const int cl = lane % 6;
+  const int iz = blockIdx.y * blockDim.y + threadIdx.y;
+  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  const FaultDev F = fdev(a, id);
+  const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny;
+  const long pf = (iy + 3) + (long)(iz + 3) * ny;
+  if (F.united[pf] != 0) return;
+  const int m = cl / 3, l = cl % 3 + 1;
+  const float *W = a.w_cur;
+  const float *met = a.met;
+  const long q = F.i0 + (2 * m - 1) * l + (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
+  float x1 = met[sc + q], x2 = met[2 * sc + q], x3 = met[3 * sc + q], jl = met[q];
+  float txx = W[TXX * sc + q], tyy = W[TYY * sc + q], tzz = W[TZZ * sc + q];
+  float tyz = W[TYZ * sc + q], txz = W[TXZ * sc + q], txy = W[TXY * sc + q];
+  const long slot = 3 + (2 * m - 1) * l;
+  F.T1[slot * ss + pf] = jl * (x1 * txx + x2 * txy + x3 * txz);
+  F.T1[7 * ss + slot * ss + pf] = jl * (x1 * txy + x2 * tyy + x3 * tyz);
+  F.T1[14 * ss + slot * ss + pf] = jl * (x1 * txz + x2 * tyz + x3 * tzz);
+}
+
+__global__ void __launch_bounds__(256) k_trial_t1(FaultArgs a, BatArgs b) {
+  const int id = (int)blockIdx.z % a.nfault;
+  k_trial_t1_body(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
+}
+
+/* ---------------------------------------------------------------------- */
 DEVI void k_trial_body(const FaultArgs &a, int id) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
   const int iz = blockIdx.y * blockDim.y + threadIdx.y;
@@ -138,7 +174,6 @@ DEVI void k_trial_body(const FaultArgs &a, int id) {
   const long pf = (iy + 3) + (long)(iz + 3) * ny;
   if (F.united[pf] != 0) return;
   const int jdir = a.jdir, kdir = a.kdir;
-  const float *W = a.w_cur;
   const float *met = a.met;
   const long row = (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
   const long p0 = F.i0 + row;
@@ -154,19 +189,12 @@ DEVI void k_trial_body(const FaultArgs &a, int id) {
     float jac0 = met[p0];
     float rho = F.rho_f[m * ss + pf];
     Mrho[m] = (float)(0.5 * jac0 * rho);
-    float t1[3][3];  /* [comp][l-1] */
+    float t1[3][3];  /* [comp][l-1]: the traction columns k_trial_t1 wrote */
 #pragma unroll
     for (int l = 1; l <= 3; l++) {
-      long q = p0 + (2 * m - 1) * l;
-      float x1 = met[sc + q], x2 = met[2 * sc + q], x3 = met[3 * sc + q], jl = met[q];
-      float txx = W[TXX * sc + q], tyy = W[TYY * sc + q], tzz = W[TZZ * sc + q];
-      float tyz = W[TYZ * sc + q], txz = W[TXZ * sc + q], txy = W[TXY * sc + q];
-      t1[0][l - 1] = jl * (x1 * txx + x2 * txy + x3 * txz);
-      t1[1][l - 1] = jl * (x1 * txy + x2 * tyy + x3 * tyz);
-      t1[2][l - 1] = jl * (x1 * txz + x2 * tyz + x3 * tzz);
-      long slot = 3 + (2 * m - 1) * l;
+      const long slot = 3 + (2 * m - 1) * l;
 #pragma unroll
-      for (int c = 0; c < 3; c++) F.T1[(long)c * 7 * ss + slot * ss + pf] = t1[c][l - 1];
+      for (int c = 0; c < 3; c++) t1[c][l - 1] = F.T1[(long)c * 7 * ss + slot * ss + pf];
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -693,6 +721,8 @@ void launch_wave2fault(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
 }
 void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
+  dim3 tblk(64, 4, 1), tgrd((a.g.nj + kTrialNodes - 1) / kTrialNodes, (a.g.nk + 3) / 4, a.nfault * b.nact);
+  hipLaunchKernelGGL(k_trial_t1, tgrd, tblk, 0, st, a, b);
   dim3 blk(64, 2, 1), grd((a.g.nj + 63) / 64, (a.g.nk + 1) / 2, a.nfault * b.nact);
   hipLaunchKernelGGL(k_trial, grd, blk, 0, st, a, b);
 }

@@ -36,9 +36,10 @@ __global__ void __launch_bounds__(TX *TY, MINW > 0 ? MINW : 1)
 
   const Grid &g = a.g;
   const long sc = g.sicmp, sy = g.siy, sz = g.siz;
-  const int gx = bx * TX + (int)threadIdx.x, gy = by * TY + (int)threadIdx.y;
-  if (gx >= g.ni || gy >= g.nj) return;  /* no barriers: inactive lanes may leave */
-  const long col = (long)(gx + 3) + (long)(gy + 3) * sy;
+  /* tiles start at ghost-inclusive column 0: whole 128-byte lines per row */
+  const int ie = bx * TX + (int)threadIdx.x, gy = by * TY + (int)threadIdx.y;
+  if (ie < 3 || ie >= g.ni + 3 || gy >= g.nj) return;  /* no barriers: inactive lanes may leave */
+  const long col = (long)ie + (long)(gy + 3) * sy;
   const int k_first = a.k_lo + bz * a.zchunk;
   int k_last = k_first + a.zchunk;
   if (k_last > a.k_lo + a.nk_run) k_last = a.k_lo + a.nk_run;
@@ -82,8 +83,8 @@ __global__ void __launch_bounds__(TX *TY, MINW > 0 ? MINW : 1)
       }
       float h[9];
       rhs_interior(Dx, Dy, Dz, xx, xy, xz, ex, ey, ez, zx, zy, zz, l, m, s, h);
-      pml_point(a, gx + 3, gy + 3, k + 3, p, Dx, Dy, Dz, l, m, s, h);
-      if (!(a.fs.nfault > 0 && fault_column(a.fs, gx + 3, (gy + 3) + (long)(k + 3) * g.ny))) rk_point(a, p, h);
+      pml_point(a, ie, gy + 3, k + 3, p, Dx, Dy, Dz, l, m, s, h);
+      if (!(a.fs.nfault > 0 && fault_column(a.fs, ie, (gy + 3) + (long)(k + 3) * g.ny))) rk_point(a, p, h);
     } else {
       /* physical gradient of each field, accumulated into the rhs */
       float gvx[3], gvy[3], gvz[3];
@@ -114,7 +115,7 @@ __global__ void __launch_bounds__(TX *TY, MINW > 0 ? MINW : 1)
           case TXY: hv0 += g1; hv1 += g0; break;
         }
       }
-      if (a.fs.nfault > 0 && fault_column(a.fs, gx + 3, (gy + 3) + (long)(k + 3) * g.ny)) {
+      if (a.fs.nfault > 0 && fault_column(a.fs, ie, (gy + 3) + (long)(k + 3) * g.ny)) {
         /* owned by the fault kernel */
       } else {
         const float l2m = l + 2.0f * m;

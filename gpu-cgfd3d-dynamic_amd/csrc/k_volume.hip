@@ -26,7 +26,9 @@ namespace cgfd {
 /* one point of the direct kernels: the shared metric and media are loaded
    once, then for every active scenario 27 derivatives from the cache
    hierarchy, rhs + PML + RK */
-template <int ID, int JD, int KD, bool PML = true>
+/* FENCE: each field's three derivatives are materialised before the next
+   field's taps are loaded (fewer loads in flight, far fewer registers) */
+template <int ID, int JD, int KD, bool PML = true, bool FENCE = false>
 DEVI void direct_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
@@ -41,6 +43,7 @@ DEVI void direct_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, 
     Dx[c] = dfd32<ID>(bc, o, 4u);
     Dy[c] = dfd32<JD>(bc, o, by);
     Dz[c] = dfd32<KD>(bc, o, bz);
+    if (FENCE) __asm__ volatile("" : "+v"(Dx[c]), "+v"(Dy[c]), "+v"(Dz[c]) : : "memory");
   }
   const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
   const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
@@ -75,6 +78,147 @@ __global__ void __launch_bounds__(256) k_vol_bat(VolArgs a, BatArgs b, long nite
   const int i = bx * 64 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;  /* i ghost-inclusive */
   if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
   direct_point<ID, JD, KD>(a, b, i, iy + 3, a.k_lo + kz + 3, b.act[q]);
+}
+
+/* ------------------------------------------------------------------------
+ * Slab-ordered direct kernel: block ids run x tile fastest, then the S
+ * y-tiles of a slab, then the plane, then the slab (S = zchunk). The blocks
+ * resident at one time cover a few planes of one slab of 4 S rows, so the
+ * planes above and below a block are read by blocks running at the same
+ * time (the z-neighbour block is ntx*S ids away, on the same XCD when that
+ * is a multiple of 8), and each array is touched in contiguous runs of 4 S
+ * rows per plane.
+ * ---------------------------------------------------------------------- */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_slab(VolArgs a, BatArgs b) {
+  const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
+  const int S = a.zchunk;
+  long L = blockIdx.x;
+  const int bx = (int)(L % ntx);
+  L /= ntx;
+  const int cy = (int)(L % S);
+  L /= S;
+  const int kz = (int)(L % a.nk_run);
+  const int by = (int)(L / a.nk_run) * S + cy;
+  const int i = bx * 64 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
+  if (by >= nty || i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD>(a, b, i, iy + 3, a.k_lo + kz + 3, b.act[0]);
+}
+
+/* ------------------------------------------------------------------------
+ * Split direct kernels: blocks whose tile-plane touches no CFS-PML slab run
+ * an interior kernel that turns each field's three derivatives into its
+ * physical gradient at once and accumulates the rhs (the sums of
+ * sv_curv_col_el_iso_gpu.cu:349-386 grouped per field), so only a handful of
+ * values stay live between fields and the kernel fits many waves per SIMD;
+ * the other blocks run the full per-point path with CFS-PML. Both launches
+ * cover the whole grid (same block order as the direct kernel) and leave
+ * the other class's blocks at once.
+ * ---------------------------------------------------------------------- */
+DEVI bool block_touches_pml(const VolArgs &a, int i0, int i1, int j0, int j1, int k) {
+  if (!a.pml_on) return false;
+  for (int d = 0; d < 3; d++)
+    for (int e = 0; e < 2; e++) {
+      const PmlFace &f = a.pml[d][e];
+      if (f.on && i1 >= f.i1 && i0 <= f.i2 && j1 >= f.j1 && j0 <= f.j2 && k >= f.k1 && k <= f.k2) return true;
+    }
+  return false;
+}
+
+/* G > 0: the scheduler may not move instructions across the end of every
+   G-th field, which bounds the loads in flight (and the registers) per wave */
+template <int ID, int JD, int KD, int G>
+DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  const unsigned o = (unsigned)p * 4u;
+  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
+  const float *M = a.met;
+  const long woff = (long)sn * b.wstride;
+  const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
+  const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
+  const float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
+  float gvx[3], gvy[3], gvz[3];
+  float hv0 = 0.f, hv1 = 0.f, hv2 = 0.f;
+#pragma unroll
+  for (int f = 0; f < 9; f++) {
+    const float *bc = a.cur + woff + f * sc;
+    const float dx = dfd32<ID>(bc, o, 4u), dy = dfd32<JD>(bc, o, by), dz = dfd32<KD>(bc, o, bz);
+    float g0 = xx * dx + ex * dy + zx * dz;  /* d/dx */
+    float g1 = xy * dx + ey * dy + zy * dz;  /* d/dy */
+    float g2 = xz * dx + ez * dy + zz * dz;  /* d/dz */
+    if (G > 0 && (f + 1) % G == 0) {
+      /* the gradient exists in registers here and later loads stay below:
+         this field's taps are consumed before the next field's are issued */
+      __asm__ volatile("" : "+v"(g0), "+v"(g1), "+v"(g2) : : "memory");
+    }
+    switch (f) {
+      case VX: gvx[0] = g0; gvx[1] = g1; gvx[2] = g2; break;
+      case VY: gvy[0] = g0; gvy[1] = g1; gvy[2] = g2; break;
+      case VZ: gvz[0] = g0; gvz[1] = g1; gvz[2] = g2; break;
+      case TXX: hv0 += g0; break;
+      case TYY: hv1 += g1; break;
+      case TZZ: hv2 += g2; break;
+      case TYZ: hv1 += g2; hv2 += g1; break;
+      case TXZ: hv0 += g2; hv2 += g0; break;
+      case TXY: hv0 += g1; hv1 += g0; break;
+    }
+  }
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+  const float l2m = l + 2.0f * m;
+  float h[9];
+  h[VX] = s * hv0;
+  h[VY] = s * hv1;
+  h[VZ] = s * hv2;
+  h[TXX] = l2m * gvx[0] + l * (gvy[1] + gvz[2]);
+  h[TYY] = l2m * gvy[1] + l * (gvx[0] + gvz[2]);
+  h[TZZ] = l2m * gvz[2] + l * (gvx[0] + gvy[1]);
+  h[TXY] = m * (gvx[1] + gvy[0]);
+  h[TXZ] = m * (gvx[2] + gvz[0]);
+  h[TYZ] = m * (gvy[2] + gvz[1]);
+  rk_point32(a, o, h, woff);
+}
+
+template <int ID, int JD, int KD, int MINW, int G>
+__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) k_vol_acc(VolArgs a, BatArgs b) {
+  const int k = a.k_lo + (int)blockIdx.z + 3;
+  const int bi0 = blockIdx.x * 64, bj0 = blockIdx.y * 4 + 3;
+  if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = blockIdx.y * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  acc_point<ID, JD, KD, G>(a, b, i, iy + 3, k, b.act[0]);
+}
+
+/* the interior kernel of variant 14 in slab order (see k_vol_slab) */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256, 8) k_vol_acc_slab(VolArgs a, BatArgs b) {
+  const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
+  const int S = a.zchunk;
+  long L = blockIdx.x;
+  const int bx = (int)(L % ntx);
+  L /= ntx;
+  const int cy = (int)(L % S);
+  L /= S;
+  const int kz = (int)(L % a.nk_run);
+  const int by = (int)(L / a.nk_run) * S + cy;
+  if (by >= nty) return;
+  const int k = a.k_lo + kz + 3;
+  const int bi0 = bx * 64, bj0 = by * 4 + 3;
+  if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  acc_point<ID, JD, KD, 1>(a, b, i, iy + 3, k, b.act[0]);
+}
+
+template <int ID, int JD, int KD, bool FENCE>
+__global__ void __launch_bounds__(256) k_vol_pmlblk(VolArgs a, BatArgs b) {
+  const int k = a.k_lo + (int)blockIdx.z + 3;
+  const int bi0 = blockIdx.x * 64, bj0 = blockIdx.y * 4 + 3;
+  if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = blockIdx.y * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD, true, FENCE>(a, b, i, iy + 3, k, b.act[0]);
 }
 
 /* ------------------------------------------------------------------------
@@ -410,6 +554,26 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
   }
   if (variant == 2) {
     launch_vol_lds(a, ID, JD, KD, st);
+    return;
+  }
+  if (variant >= 12 && variant <= 16) {
+    if (variant == 16) {
+      const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
+      const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
+      hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
+    } else if (variant == 12) hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 0, 0>), agrd, blk, 0, st, a, b);
+    else if (variant == 13) hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 0, 3>), agrd, blk, 0, st, a, b);
+    else hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 8, 1>), agrd, blk, 0, st, a, b);
+    if (a.pml_on) {
+      if (variant == 15) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, true>), agrd, blk, 0, st, a, b);
+      else hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, false>), agrd, blk, 0, st, a, b);
+    }
+    return;
+  }
+  if (variant == 11) {
+    const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
+    const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
+    hipLaunchKernelGGL((k_vol_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
     return;
   }
   if (variant == 10) {

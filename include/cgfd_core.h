@@ -191,18 +191,23 @@ int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
                         float *ms_volume, int *n_volume);
 
-/* Kernel variant selection for the volume stencil (0 = default = 1,
+/* Kernel variant selection for the volume stencil (0 = default = 16,
    1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
    6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
    block schedule, 8 = the same with >= 4 waves/SIMD on interior tiles,
    9 = the same with >= 6 waves/SIMD, 10 = direct loads, row-major thread
-   order). The variant applies to single-scenario launches; batches with
+   order, 11 = direct loads in slab order with zchunk y-tiles per slab,
+   12..14 = interior blocks on a per-field accumulating kernel (loads free
+   to move / a fence every 3 fields / >= 8 waves per SIMD with a fence every
+   field) and PML blocks on the direct kernel, 15 = 14 with the PML blocks
+   fenced as well, 16 = 14 with the interior blocks in slab order). The variant applies to single-scenario launches; batches with
    more than one active scenario use their own tile x scenario kernel.
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
-/* z planes per block (variants 2..6) or per schedule chunk (variants 7..9), 4..4096. */
+/* z planes per block (variants 2..6), per schedule chunk (variants 7..9) or
+   y-tiles per slab (variant 11), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
 /* Autotuner: times every (variant, z-chunk) candidate on `reps` repetitions

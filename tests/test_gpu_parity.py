@@ -142,7 +142,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation (direct loads, LDS z-march, z queue)."""
     from cgfd import configs, model
