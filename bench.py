@@ -89,9 +89,10 @@ def cpu_baseline(threads: int, steps: int = 24):
                       f"{steps} steps after 1 warm-up, {n} OpenMP threads, {el:.1f} s"}
 
 
-def batched_leg(steps: int, nscen: int):
+def batched_leg(steps: int, nscen: int, variant: int, overlap: bool = False):
     """Config-3 ensemble: nscen scenarios on one 400x200x150 grid, batched,
-    against scenario 0 alone through the plain path on the same grid."""
+    against scenario 0 alone through the plain path on the same grid, both
+    with the same volume-kernel variant."""
     from cgfd import configs, model
     t0 = time.perf_counter()
     base, fields, nts = configs.config3(nscen=nscen)
@@ -99,7 +100,8 @@ def batched_leg(steps: int, nscen: int):
     log(f"[bench] config3 {base.ni}x{base.nj}x{base.nk}, {nscen} scenarios, setup {time.perf_counter() - t0:.1f} s")
     cells = base.ni * base.nj * base.nk
     # plain: scenario 0 alone
-    core = model.Core(st, variant=1)
+    core = model.Core(st, variant=variant)
+    core.set_overlap(overlap)
     core.run(8)
     core.sync()
     t = time.perf_counter()
@@ -109,7 +111,8 @@ def batched_leg(steps: int, nscen: int):
     core.close()
     # batched: every scenario active for the timed steps (run lengths >= 1000)
     big = [max(n, steps + 16) for n in nts]
-    bat = model.Batch(st, [(None if s == 0 else [fields[s]], big[s]) for s in range(nscen)], variant=1)
+    bat = model.Batch(st, [(None if s == 0 else [fields[s]], big[s]) for s in range(nscen)], variant=variant)
+    bat.set_overlap(overlap)
     bat.run(8)
     bat.sync()
     t = time.perf_counter()
@@ -122,6 +125,7 @@ def batched_leg(steps: int, nscen: int):
     return {"value": round(batched, 4), "unit": "scenario-Gcell-updates/s", "nscen": nscen,
             "grid": [base.ni, base.nj, base.nk], "steps": steps, "ms_per_step": round(el / steps * 1e3, 3),
             "plain_same_grid": round(plain, 4), "per_scenario_vs_plain": round(batched / plain, 3),
+            "volume_kernel_variant": variant,
             "note": "all scenarios active in the timed steps; retirement is exercised by the GPU tests"}
 
 
@@ -135,10 +139,14 @@ def main() -> int:
                     help="volume kernel variant (16 = split interior/PML kernels in slab order, the one the "
                          "committed profiles cover; 0 = autotune)")
     ap.add_argument("--zchunk", type=int, default=4, help="z planes per block / y tiles per slab (0 = default)")
+    ap.add_argument("--overlap", action="store_true",
+                    help="stage fork: interior, PML + top and fault kernels on three streams")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (profiling)")
     ap.add_argument("--batched-steps", type=int, default=16)
     ap.add_argument("--nscen", type=int, default=32)
+    ap.add_argument("--batched-variant", type=int, default=16)
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
@@ -150,6 +158,10 @@ def main() -> int:
 
     if args.calib_copy:
         calib_copy(args.calib_copy)
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args.batched_steps, args.nscen, args.batched_variant,
+                                                 args.overlap)}), flush=True)
+        return 0
     from cgfd import configs, model
     t0 = time.perf_counter()
     sc = configs.config2(scale=args.scale, steps=args.steps + args.warmup)
@@ -161,6 +173,8 @@ def main() -> int:
     core = model.Core(st, variant=args.variant)
     if args.zchunk:
         core.set_zchunk(args.zchunk)
+    if args.overlap:
+        core.set_overlap(True)
     core.sync()
     log(f"[bench] device upload in {time.perf_counter() - t0:.1f} s")
     tuned = None
@@ -209,7 +223,7 @@ def main() -> int:
         "config": {"workload": f"{sc.name}: plain-method rupture, {nj}x{ni}x{nk} nodes (strike x normal x depth) "
                                f"at 50 m, curvilinear, PML 20, free surface, 1 fault, FP32",
                    "grid": [ni, nj, nk], "parallelism": "single GPU", "hbm_frac_768B_model": round(hbm_frac, 4),
-                   "max_slip_m": round(slip_max, 4),
+                   "max_slip_m": round(slip_max, 4), "stage_fork": args.overlap,
                    "volume_kernel": {"variant": tuned[0] if tuned else args.variant,
                                      "zchunk": tuned[1] if tuned else args.zchunk, "autotuned": tuned is not None}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -222,7 +236,7 @@ def main() -> int:
     del core, st, sc
     if not args.no_batched:
         log("[bench] batched ensemble ...")
-        out["batched"] = batched_leg(args.batched_steps, args.nscen)
+        out["batched"] = batched_leg(args.batched_steps, args.nscen, args.batched_variant, args.overlap)
     out["adjoint_vs_forward"] = None
     if rank == 0 and not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")

@@ -149,6 +149,9 @@ class Core:
     def set_graph(self, on: bool):
         L.check(self.lib, self.lib.cgfd_core_set_graph(self.handle, int(on)))
 
+    def set_overlap(self, on: bool):
+        L.check(self.lib, self.lib.cgfd_core_set_overlap(self.handle, int(on)))
+
     def set_zchunk(self, z: int):
         L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
 
@@ -271,6 +274,9 @@ class Batch:
 
     def set_graph(self, on: bool):
         L.check(self.lib, self.lib.cgfd_core_set_graph(self.handle, int(on)))
+
+    def set_overlap(self, on: bool):
+        L.check(self.lib, self.lib.cgfd_core_set_overlap(self.handle, int(on)))
 
     def set_variant(self, v: int):
         L.check(self.lib, self.lib.cgfd_core_set_variant(self.handle, v))

@@ -93,8 +93,10 @@ void launch_vol_zq(const VolArgs &a, const ZqTiles &t, int minw, int idir, int j
 void launch_vol_pipe(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st);
 void launch_vol_sched(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int minw, int idir, int jdir, int kdir,
                       hipStream_t st);
+/* variants with separate interior and CFS-PML block kernels launch the PML
+   blocks on st_pml (nullptr = st); the two write disjoint points */
 void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
-                   hipStream_t st);
+                   hipStream_t st, hipStream_t st_pml = nullptr);
 void launch_vol_lds(const VolArgs &a, int idir, int jdir, int kdir, hipStream_t st);
 void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
                     hipStream_t st);

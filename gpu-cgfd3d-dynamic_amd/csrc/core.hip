@@ -107,6 +107,33 @@ struct cgfd_core {
      swaps) captured once and replayed */
   static constexpr int kGraphSteps = 8;
   int use_graph = 1;
+
+  /* stage fork: after the trial kernel, the interior volume blocks run on
+     st, the CFS-PML blocks and the top layers on side[0], the fault columns
+     and split nodes on side[1]; all of them read the stage input level and
+     write disjoint points of the output levels, so they only join at the
+     end of the stage. All streams have the default priority. Off by
+     default: on the 1200 x 600 x 400 rupture the volume blocks fill the GPU
+     and the step takes as long either way. */
+  int overlap = 0;
+  hipStream_t side[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[2] = {nullptr, nullptr};
+  int fork() {
+    if (!overlap) return CGFD_OK;
+    HIPCHK(hipEventRecord(ev_fork, st));
+    for (int q = 0; q < 2; q++) HIPCHK(hipStreamWaitEvent(side[q], ev_fork, 0));
+    return CGFD_OK;
+  }
+  int join() {
+    if (!overlap) return CGFD_OK;
+    for (int q = 0; q < 2; q++) {
+      HIPCHK(hipEventRecord(ev_join[q], side[q]));
+      HIPCHK(hipStreamWaitEvent(st, ev_join[q], 0));
+    }
+    return CGFD_OK;
+  }
+  hipStream_t st_pml() const { return overlap ? side[0] : st; }
+  hipStream_t st_fault() const { return overlap ? side[1] : st; }
   hipGraphExec_t gexec = nullptr;
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -424,6 +451,12 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   if (hipSetDevice(c->dev) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "hipSetDevice failed"));
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
     return fail(cgfd_set_error(CGFD_ERR_HIP, "hipStreamCreate failed"));
+  for (int q = 0; q < 2; q++)
+    if (hipStreamCreateWithFlags(&c->side[q], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming) != hipSuccess)
+      return fail(cgfd_set_error(CGFD_ERR_HIP, "side stream / event creation failed"));
+  if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return fail(cgfd_set_error(CGFD_ERR_HIP, "hipEventCreate failed"));
   Grid &g = c->g;
   c->ni = d->ni; c->nj = d->nj; c->nk = d->nk;
   g.ni = d->ni; g.nj = d->nj; g.nk = d->nk;
@@ -549,7 +582,14 @@ extern "C" int cgfd_core_destroy(cgfd_core *c) {
   if (!c) return CGFD_OK;
   (void)hipSetDevice(c->dev);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  for (int q = 0; q < 2; q++)
+    if (c->side[q]) (void)hipStreamSynchronize(c->side[q]);
   c->drop_graph();
+  for (int q = 0; q < 2; q++) {
+    if (c->ev_join[q]) (void)hipEventDestroy(c->ev_join[q]);
+    if (c->side[q]) (void)hipStreamDestroy(c->side[q]);
+  }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
   return CGFD_OK;
@@ -607,13 +647,18 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
    loads free to move / a fence every 3 fields / >= 8 waves per SIMD and a
    fence every field; a fence makes a field's taps be consumed before the
    next field's loads), 15 = 14 with fences in the PML blocks too, 16 = 14
-   with the interior blocks in slab order (zchunk y tiles per slab) */
+   with the interior blocks in slab order (zchunk y tiles per slab), 17 = 16
+   with XCD-contiguous block ids for both the interior and the PML blocks,
+   18 = 16 with the interior x taps taken from neighbouring lanes (DPP) */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
-  if (c->bat.nact > 1) launch_volume(1, va, c->bat, idir, jdir, kdir, 0, c->st);  /* batch kernel */
+  if (c->bat.nact > 1) {  /* batch kernels: split interior/PML for 16 and 17, else one direct kernel */
+    const int bv = (c->variant == 16 || c->variant == 17) ? c->variant : 1;
+    launch_volume(bv, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
+  }
   else if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
   else if (c->variant >= 7 && c->variant <= 9) launch_vol_sched(va, c->bat, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
   else if (c->variant >= 3 && c->variant <= 5) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
-  else launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st);
+  else launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
 }
 
 /* one time step; if ev != nullptr, event pairs bracket the volume launches */
@@ -647,12 +692,15 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
     }
     VolArgs va;
     fill_vol(c, va, s, wcur, wnew);
+    /* timed region: the volume kernels; with the stage fork on, from the
+       fork to the join (the top-layer and fault kernels run beside them) */
+    if (ev && nvol > 0) HIPCHK(hipEventRecord(ev[2 * (*nev)], c->st));
+    if (int rc = c->fork()) return rc;
     if (nvol > 0) {
       va.k_lo = 0;
       va.nk_run = nvol;
-      if (ev) HIPCHK(hipEventRecord(ev[2 * (*nev)], c->st));
       launch_vol(c, va, idir, jdir, kdir);
-      if (ev) {
+      if (ev && !c->overlap) {
         HIPCHK(hipEventRecord(ev[2 * (*nev) + 1], c->st));
         (*nev)++;
       }
@@ -660,12 +708,17 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
     if (ntop > 0) {
       va.k_lo = nk - ntop;
       va.nk_run = ntop;
-      launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 1, c->st);
+      launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 1, c->st_pml());
     }
     if (c->nfault > 0) {
       FaultArgs fa;
       fill_fault(c, fa, s, idir, jdir, kdir, wcur, wnew, fcur, fnew);
-      launch_fault_update(fa, c->bat, c->st);
+      launch_fault_update(fa, c->bat, c->st_fault());
+    }
+    if (int rc = c->join()) return rc;
+    if (ev && nvol > 0 && c->overlap) {
+      HIPCHK(hipEventRecord(ev[2 * (*nev) + 1], c->st));
+      (*nev)++;
     }
   }
   if (c->isfree) launch_pg(c->g, c->bat, c->wlev(c->lv_end), c->wlev(c->lv_pre), c->PG, c->Dis, c->dt, c->st);
@@ -744,6 +797,15 @@ extern "C" int cgfd_core_run(cgfd_core *c, int nsteps) {
     if (rc) return rc;
     left--;
   }
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_overlap(cgfd_core *c, int on) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->drop_graph();
+  c->overlap = on ? 1 : 0;
   return CGFD_OK;
 }
 
@@ -940,7 +1002,7 @@ extern "C" int cgfd_batch_active(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant == 0) variant = 16;
-  if (variant < 1 || variant > 16) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant < 1 || variant > 18) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
   c->drop_graph();
   return CGFD_OK;
@@ -970,11 +1032,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[16] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16};
+  const int variants[18] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18};
   const int chunks[6] = {4, 8, 16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 16; vi++)
+  for (int vi = 0; vi < 18; vi++)
     for (int zi = 0; zi < 6; zi++) {
       int v = variants[vi], z = chunks[zi];
       if ((v == 1 || v == 10 || (v >= 12 && v <= 15)) && zi > 0) continue;  /* no z chunks */
@@ -994,8 +1056,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
         va[s].nk_run = c->nvol;
       }
       auto one = [&]() {
-        for (int s = 0; s < 4; s++)
+        for (int s = 0; s < 4; s++) {
+          (void)c->fork();
           launch_vol(c, va[s], (idir + s) % 2, (jdir + s) % 2, (kdir + s) % 2);
+          (void)c->join();
+        }
       };
       one();
       HIPCHK(hipEventRecord(e0, c->st));

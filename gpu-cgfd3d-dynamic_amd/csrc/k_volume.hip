@@ -190,25 +190,220 @@ __global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) k_vol_acc(VolArgs a,
   acc_point<ID, JD, KD, G>(a, b, i, iy + 3, k, b.act[0]);
 }
 
-/* the interior kernel of variant 14 in slab order (see k_vol_slab) */
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256, 8) k_vol_acc_slab(VolArgs a, BatArgs b) {
+/* block id -> (x tile, y tile, plane) in slab order (see k_vol_slab). XCD:
+   the grid is a multiple of 8 blocks and block ids L with L % 8 == x (the
+   ones the dispatcher sends to XCD x) take the x-th contiguous eighth of the
+   items, so one XCD's L2 sees whole rows of a slab and the planes above and
+   below: the x, y and z halos of a tile are fetched into that L2 by the
+   tile's own neighbours. false: padding item past the last slab. */
+DEVI bool slab_item(const VolArgs &a, bool xcd, int &bx, int &by, int &kz) {
   const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
   const int S = a.zchunk;
   long L = blockIdx.x;
-  const int bx = (int)(L % ntx);
+  if (xcd) L = (L % 8) * (long)(gridDim.x / 8) + L / 8;
+  bx = (int)(L % ntx);
   L /= ntx;
   const int cy = (int)(L % S);
   L /= S;
-  const int kz = (int)(L % a.nk_run);
-  const int by = (int)(L / a.nk_run) * S + cy;
-  if (by >= nty) return;
+  kz = (int)(L % a.nk_run);
+  by = (int)(L / a.nk_run) * S + cy;
+  return by < nty;
+}
+
+/* the interior kernel of variant 14 in slab order */
+template <int ID, int JD, int KD, bool XCD>
+__global__ void __launch_bounds__(256, 8) k_vol_acc_slab(VolArgs a, BatArgs b) {
+  int bx, by, kz;
+  if (!slab_item(a, XCD, bx, by, kz)) return;
   const int k = a.k_lo + kz + 3;
   const int bi0 = bx * 64, bj0 = by * 4 + 3;
   if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
   const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
   if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
   acc_point<ID, JD, KD, 1>(a, b, i, iy + 3, k, b.act[0]);
+}
+
+/* ------------------------------------------------------------------------
+ * x taps from the wave's own lanes. A 64 x 4 block is four waves, one row
+ * each, lane l at x = bi0 + l, so the x neighbours of a point sit in the
+ * neighbouring lanes: DPP wave shifts move the centre values one lane, and
+ * the lane shifted in at the wave's edge keeps the `old` operand (bound_ctrl
+ * off), which carries the halo value. One load per row brings the four halo
+ * values of all nine fields (lane 4f+t: field f, halo t); v_readlane makes
+ * them wave-uniform. This replaces four of the five x-tap loads per field.
+ * ---------------------------------------------------------------------- */
+DEVI float wave_shl1(float v, float edge) {  /* lane l <- lane l+1; lane 63 <- edge */
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+DEVI float wave_shr1(float v, float edge) {  /* lane l <- lane l-1; lane 0 <- edge */
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+DEVI float lane_val(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+
+/* halo t of a wave starting at x0: forward taps (-1..3) need x0-1 and
+   x0+64..66, backward taps (-3..1) x0-3..x0-1 and x0+64 */
+template <int ID>
+DEVI int halo_x(int x0, int t) { return ID ? (t == 0 ? x0 - 1 : x0 + 63 + t) : (t == 3 ? x0 + 64 : x0 - 3 + t); }
+
+template <int ID>
+DEVI float dx_wave(float c, float h0, float h1, float h2, float h3) {
+  if (ID == 1) {
+    const float m1 = wave_shr1(c, h0), p1 = wave_shl1(c, h1), p2 = wave_shl1(p1, h2), p3 = wave_shl1(p2, h3);
+    return dfd5<1>(m1, c, p1, p2, p3);
+  } else {
+    const float m1 = wave_shr1(c, h2), m2 = wave_shr1(m1, h1), m3 = wave_shr1(m2, h0), p1 = wave_shl1(c, h3);
+    return dfd5<0>(m3, m2, m1, c, p1);
+  }
+}
+
+/* acc_point with x taps from DPP; every lane of the wave takes part (lanes
+   outside the physical columns load clamped addresses and store nothing) */
+template <int ID, int JD, int KD>
+DEVI void acc_dpp_point(const VolArgs &a, const BatArgs &b, int x0, int j, int k, int sn) {
+  const int lane = (int)threadIdx.x;
+  const int i = x0 + lane;
+  const int nx = a.g.nx;
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long row = j * sy + (long)k * sz;
+  const unsigned o = (unsigned)(row + (i < nx ? i : nx - 1)) * 4u;
+  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
+  const float *M = a.met;
+  const long woff = (long)sn * b.wstride;
+  int hx = halo_x<ID>(x0, lane & 3);
+  hx = hx < 0 ? 0 : (hx > nx - 1 ? nx - 1 : hx);
+  const int hf = (lane >> 2) < 8 ? (lane >> 2) : 8;
+  const float hv = ld32(a.cur + woff + hf * sc, (unsigned)(row + hx) * 4u);
+  const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
+  const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
+  const float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
+  float gvx[3], gvy[3], gvz[3];
+  float hv0 = 0.f, hv1 = 0.f, hv2 = 0.f;
+#pragma unroll
+  for (int f = 0; f < 9; f++) {
+    const float *bc = a.cur + woff + f * sc;
+    const float c = ld32(bc, o);
+    const float dx = dx_wave<ID>(c, lane_val(hv, 4 * f), lane_val(hv, 4 * f + 1), lane_val(hv, 4 * f + 2),
+                                 lane_val(hv, 4 * f + 3));
+    const float dy = dfd32<JD>(bc, o, by), dz = dfd32<KD>(bc, o, bz);
+    float g0 = xx * dx + ex * dy + zx * dz;
+    float g1 = xy * dx + ey * dy + zy * dz;
+    float g2 = xz * dx + ez * dy + zz * dz;
+    __asm__ volatile("" : "+v"(g0), "+v"(g1), "+v"(g2) : : "memory");
+    switch (f) {
+      case VX: gvx[0] = g0; gvx[1] = g1; gvx[2] = g2; break;
+      case VY: gvy[0] = g0; gvy[1] = g1; gvy[2] = g2; break;
+      case VZ: gvz[0] = g0; gvz[1] = g1; gvz[2] = g2; break;
+      case TXX: hv0 += g0; break;
+      case TYY: hv1 += g1; break;
+      case TZZ: hv2 += g2; break;
+      case TYZ: hv1 += g2; hv2 += g1; break;
+      case TXZ: hv0 += g2; hv2 += g0; break;
+      case TXY: hv0 += g1; hv1 += g0; break;
+    }
+  }
+  if (i < 3 || i >= a.g.ni + 3) return;
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+  const float l2m = l + 2.0f * m;
+  float h[9];
+  h[VX] = s * hv0;
+  h[VY] = s * hv1;
+  h[VZ] = s * hv2;
+  h[TXX] = l2m * gvx[0] + l * (gvy[1] + gvz[2]);
+  h[TYY] = l2m * gvy[1] + l * (gvx[0] + gvz[2]);
+  h[TZZ] = l2m * gvz[2] + l * (gvx[0] + gvy[1]);
+  h[TXY] = m * (gvx[1] + gvy[0]);
+  h[TXZ] = m * (gvx[2] + gvz[0]);
+  h[TYZ] = m * (gvy[2] + gvz[1]);
+  rk_point32(a, o, h, woff);
+}
+
+/* the slab-ordered interior kernel with x taps from DPP (whole waves: a
+   wave leaves only when its row is past the last y row) */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256, 8) k_vol_acc_dpp(VolArgs a, BatArgs b) {
+  int bx, by, kz;
+  if (!slab_item(a, false, bx, by, kz)) return;
+  const int k = a.k_lo + kz + 3;
+  const int bi0 = bx * 64, bj0 = by * 4 + 3;
+  if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int iy = by * 4 + (int)threadIdx.y;
+  if (iy >= a.g.nj) return;
+  acc_dpp_point<ID, JD, KD>(a, b, bi0, iy + 3, k, b.act[0]);
+}
+
+/* the CFS-PML blocks in the same order */
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_pml_slab(VolArgs a, BatArgs b) {
+  int bx, by, kz;
+  if (!slab_item(a, true, bx, by, kz)) return;
+  const int k = a.k_lo + kz + 3;
+  const int bi0 = bx * 64, bj0 = by * 4 + 3;
+  if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[0]);
+}
+
+/* ------------------------------------------------------------------------
+ * Batches on the split interior / CFS-PML kernels. ORDER 0: (tile, scenario)
+ * pairs with the scenario fastest, XCD-contiguous, so a tile's metric and
+ * media lines serve every scenario from one L2. ORDER 1: one slab-ordered
+ * sweep per scenario (scenario slowest), so each scenario's x, y and z halos
+ * are reused from L2 as in the single-scenario kernel and the shared metric
+ * and media come back from the last-level cache. false: padding block.
+ * ---------------------------------------------------------------------- */
+DEVI bool bat_item(const VolArgs &a, const BatArgs &b, int order, long nitems, int &bx, int &by, int &kz,
+                   int &q) {
+  const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
+  long L = blockIdx.x;
+  if (order == 0) {
+    const long item = (L % 8) * (long)(gridDim.x / 8) + L / 8;
+    if (item >= nitems) return false;
+    q = (int)(item % b.nact);
+    const long tile = item / b.nact;
+    bx = (int)(tile % ntx);
+    by = (int)((tile / ntx) % nty);
+    kz = (int)(tile / ((long)ntx * nty));
+    return true;
+  }
+  const int S = a.zchunk;
+  const long nslab = (nty + S - 1) / S;
+  const long per = (long)ntx * S * a.nk_run * nslab;
+  q = (int)(L / per);
+  if (q >= b.nact) return false;
+  L %= per;
+  bx = (int)(L % ntx);
+  L /= ntx;
+  const int cy = (int)(L % S);
+  L /= S;
+  kz = (int)(L % a.nk_run);
+  by = (int)(L / a.nk_run) * S + cy;
+  return by < nty;
+}
+
+template <int ID, int JD, int KD, int ORDER>
+__global__ void __launch_bounds__(256, 8) k_vol_bat_acc(VolArgs a, BatArgs b, long nitems) {
+  int bx, by, kz, q;
+  if (!bat_item(a, b, ORDER, nitems, bx, by, kz, q)) return;
+  const int k = a.k_lo + kz + 3;
+  const int bi0 = bx * 64, bj0 = by * 4 + 3;
+  if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  acc_point<ID, JD, KD, 1>(a, b, i, iy + 3, k, b.act[q]);
+}
+
+template <int ID, int JD, int KD, int ORDER>
+__global__ void __launch_bounds__(256) k_vol_bat_pml(VolArgs a, BatArgs b, long nitems) {
+  int bx, by, kz, q;
+  if (!bat_item(a, b, ORDER, nitems, bx, by, kz, q)) return;
+  const int k = a.k_lo + kz + 3;
+  const int bi0 = bx * 64, bj0 = by * 4 + 3;
+  if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[q]);
 }
 
 template <int ID, int JD, int KD, bool FENCE>
@@ -537,13 +732,28 @@ __global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const fl
 /* ---------------------------------------------------------------------- */
 /* host launchers */
 template <int ID, int JD, int KD>
-static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top, hipStream_t st) {
+static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top, hipStream_t st, hipStream_t st_pml) {
   dim3 blk(64, 4, 1);
   dim3 grd((a.g.ni + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);
   dim3 agrd((a.g.ni + 3 + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);  /* aligned tiles */
   if (top) {
     dim3 tgrd(grd.x, grd.y, (unsigned)(a.nk_run * b.nact));
     hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
+    return;
+  }
+  if (b.nact > 1 && (variant == 16 || variant == 17)) {  /* batches, split kernels */
+    long items, nb;
+    if (variant == 17) {
+      items = (long)agrd.x * agrd.y * a.nk_run * b.nact;
+      nb = ((items + 7) / 8) * 8;
+      hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 0>), dim3((unsigned)nb), blk, 0, st, a, b, items);
+      if (a.pml_on) hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+    } else {
+      const long nslab = ((long)agrd.y + a.zchunk - 1) / a.zchunk;
+      nb = items = (long)agrd.x * a.zchunk * a.nk_run * nslab * b.nact;
+      hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 1>), dim3((unsigned)nb), blk, 0, st, a, b, items);
+      if (a.pml_on) hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+    }
     return;
   }
   if (b.nact > 1) {  /* batches: one kernel, tiles x scenarios */
@@ -556,17 +766,31 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     launch_vol_lds(a, ID, JD, KD, st);
     return;
   }
+  if (variant == 18) {
+    const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
+    const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
+    hipLaunchKernelGGL((k_vol_acc_dpp<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
+    if (a.pml_on) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, false>), agrd, blk, 0, st_pml, a, b);
+    return;
+  }
+  if (variant == 17) {
+    const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
+    const long nb = (((long)agrd.x * a.zchunk * a.nk_run * nslab + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, true>), dim3((unsigned)nb), blk, 0, st, a, b);
+    if (a.pml_on) hipLaunchKernelGGL((k_vol_pml_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st_pml, a, b);
+    return;
+  }
   if (variant >= 12 && variant <= 16) {
     if (variant == 16) {
       const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
       const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
-      hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
+      hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, false>), dim3((unsigned)nb), blk, 0, st, a, b);
     } else if (variant == 12) hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 0, 0>), agrd, blk, 0, st, a, b);
     else if (variant == 13) hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 0, 3>), agrd, blk, 0, st, a, b);
     else hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 8, 1>), agrd, blk, 0, st, a, b);
     if (a.pml_on) {
-      if (variant == 15) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, true>), agrd, blk, 0, st, a, b);
-      else hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, false>), agrd, blk, 0, st, a, b);
+      if (variant == 15) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, true>), agrd, blk, 0, st_pml, a, b);
+      else hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, false>), agrd, blk, 0, st_pml, a, b);
     }
     return;
   }
@@ -585,17 +809,18 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
 }
 
 void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
-                   hipStream_t st) {
+                   hipStream_t st, hipStream_t st_pml) {
+  if (!st_pml) st_pml = st;
   int key = idir * 4 + jdir * 2 + kdir;
   switch (key) {
-    case 0: launch_dir<0, 0, 0>(variant, a, b, top, st); break;
-    case 1: launch_dir<0, 0, 1>(variant, a, b, top, st); break;
-    case 2: launch_dir<0, 1, 0>(variant, a, b, top, st); break;
-    case 3: launch_dir<0, 1, 1>(variant, a, b, top, st); break;
-    case 4: launch_dir<1, 0, 0>(variant, a, b, top, st); break;
-    case 5: launch_dir<1, 0, 1>(variant, a, b, top, st); break;
-    case 6: launch_dir<1, 1, 0>(variant, a, b, top, st); break;
-    default: launch_dir<1, 1, 1>(variant, a, b, top, st); break;
+    case 0: launch_dir<0, 0, 0>(variant, a, b, top, st, st_pml); break;
+    case 1: launch_dir<0, 0, 1>(variant, a, b, top, st, st_pml); break;
+    case 2: launch_dir<0, 1, 0>(variant, a, b, top, st, st_pml); break;
+    case 3: launch_dir<0, 1, 1>(variant, a, b, top, st, st_pml); break;
+    case 4: launch_dir<1, 0, 0>(variant, a, b, top, st, st_pml); break;
+    case 5: launch_dir<1, 0, 1>(variant, a, b, top, st, st_pml); break;
+    case 6: launch_dir<1, 1, 0>(variant, a, b, top, st, st_pml); break;
+    default: launch_dir<1, 1, 1>(variant, a, b, top, st, st_pml); break;
   }
 }
 

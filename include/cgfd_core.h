@@ -149,6 +149,15 @@ int cgfd_core_run(cgfd_core *core, int nsteps);
    (drv_rk_curv_col.cu:209-636). */
 int cgfd_core_set_graph(cgfd_core *core, int on);
 
+/* Stage fork (default off): within each RK stage, after the trial traction,
+   the interior volume blocks, the CFS-PML blocks with the top layers, and
+   the fault columns with the split nodes run on three default-priority
+   streams and join at the end of the stage; they read the stage input and
+   write disjoint points, so results are identical either way. Off: every
+   kernel on one stream in the reference's order (sv_curv_col_el_iso_gpu.cu:
+   22-225 then the fault kernels, drv_rk_curv_col.cu:329-568). */
+int cgfd_core_set_overlap(cgfd_core *core, int on);
+
 /* Current step counter (number of completed steps). */
 int cgfd_core_step_index(const cgfd_core *core);
 
@@ -201,13 +210,18 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    12..14 = interior blocks on a per-field accumulating kernel (loads free
    to move / a fence every 3 fields / >= 8 waves per SIMD with a fence every
    field) and PML blocks on the direct kernel, 15 = 14 with the PML blocks
-   fenced as well, 16 = 14 with the interior blocks in slab order). The variant applies to single-scenario launches; batches with
-   more than one active scenario use their own tile x scenario kernel.
+   fenced as well, 16 = 14 with the interior blocks in slab order,
+   17 = 16 with XCD-contiguous block ids for the interior and the PML
+   blocks, 18 = 16 with the interior x taps moved between lanes by DPP
+   wave shifts instead of loaded). Batches with more than one active scenario run split interior /
+   PML kernels for variants 16 (one slab-ordered sweep per scenario) and 17
+   (the scenarios of a tile back to back, XCD-contiguous), and one direct
+   tile x scenario kernel for every other variant.
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
 /* z planes per block (variants 2..6), per schedule chunk (variants 7..9) or
-   y-tiles per slab (variant 11), 4..4096. */
+   y-tiles per slab (variants 11, 16, 17), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
 /* Autotuner: times every (variant, z-chunk) candidate on `reps` repetitions

@@ -142,7 +142,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation (direct loads, LDS z-march, z queue)."""
     from cgfd import configs, model
@@ -180,3 +180,28 @@ def test_graph_replay_matches_eager(built_libs):
         assert np.array_equal(cg.read_wavefield(), ce.read_wavefield()), f"wavefield differs after block {rep}"
         assert np.array_equal(cg.read_fault(0), ce.read_fault(0)), f"fault differs after block {rep}"
     assert np.array_equal(cg.read_stations(), ce.read_stations())
+
+
+@pytest.mark.parametrize("variant", [16, 17, 18, 1])
+def test_stage_fork_matches_serial(built_libs, variant):
+    """The stage fork (interior blocks, CFS-PML blocks + top layers, fault
+    kernels on three streams) against every kernel on one stream: the
+    kernels write disjoint points, so the results are bit-identical, eager
+    and replayed from graphs."""
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    cf = model.Core(st, variant=variant)
+    cf.set_overlap(True)
+    cs = model.Core(st, variant=variant)
+    cs.set_overlap(False)
+    cs.set_graph(False)
+    for n in (5, 16, 3):
+        cf.run(n)
+        cs.run(n)
+        assert np.array_equal(cf.read_wavefield(), cs.read_wavefield())
+        assert np.array_equal(cf.read_fault(0), cs.read_fault(0))
+    assert np.array_equal(cf.read_pg(), cs.read_pg())
+    cf.close()
+    cs.close()
+    st.close()
