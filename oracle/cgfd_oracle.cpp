@@ -15,6 +15,19 @@
  * use it; the product never links or calls it.
  *
  * Each function names the reference lines it restates.
+ *
+ * What pins it (tests/test_oracle_pins.py, tests/test_abi.py): the interior
+ * rhs, FD operators and RK4 composition by the closed-form step of cubic
+ * fields; the metric by affine grids; friction by closed-form values at its
+ * kinks; the free surface by sigma.n = 0 after surface rupture; the CFS-PML
+ * by absorbed energy; the fault frame by orthonormality; FP32 vs FP64.
+ * Parity unpinned (no golden vectors exist and the reference cannot be
+ * built here; these rest on line-by-line review and on agreement with the
+ * independently written GPU kernels): the split-node fault velocity/stress
+ * formulas and their free-surface rows (fault_velo/stress_F/B), the
+ * method-1/2 split-node matrices (fault_coef_cal), the trial traction
+ * assembly before friction, wave2fault/fault2wave including the Txz/Tyz
+ * swap, the one-step lag of hT1, and the exact stage ordering.
  */
 #include "../include/cgfd_core.h"
 
