@@ -135,8 +135,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the workload (1.0 = 1200x600x400)")
-    ap.add_argument("--variant", type=int, default=16,
-                    help="volume kernel variant (16 = split interior/PML kernels in slab order, the one the "
+    ap.add_argument("--variant", type=int, default=19,
+                    help="volume kernel variant (19 = interior kernel in slab order + lean PML-tile kernel, the one the "
                          "committed profiles cover; 0 = autotune)")
     ap.add_argument("--zchunk", type=int, default=4, help="z planes per block / y tiles per slab (0 = default)")
     ap.add_argument("--overlap", action="store_true",
@@ -146,7 +146,7 @@ def main() -> int:
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (profiling)")
     ap.add_argument("--batched-steps", type=int, default=16)
     ap.add_argument("--nscen", type=int, default=32)
-    ap.add_argument("--batched-variant", type=int, default=16)
+    ap.add_argument("--batched-variant", type=int, default=19)
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()

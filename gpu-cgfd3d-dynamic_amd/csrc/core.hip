@@ -59,7 +59,7 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
-  int variant = 16;
+  int variant = 19;
   int zchunk = 4;
   ZqTiles zq;                       /* tile lists for the current zchunk */
   std::map<int, ZqTiles> zq_cache;  /* by zchunk */
@@ -649,10 +649,12 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
    next field's loads), 15 = 14 with fences in the PML blocks too, 16 = 14
    with the interior blocks in slab order (zchunk y tiles per slab), 17 = 16
    with XCD-contiguous block ids for both the interior and the PML blocks,
-   18 = 16 with the interior x taps taken from neighbouring lanes (DPP) */
+   18 = 16 with the interior x taps taken from neighbouring lanes (DPP),
+   19 = 16 with the lean PML-tile kernel (directional rhs parts accumulated
+   per field) */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   if (c->bat.nact > 1) {  /* batch kernels: split interior/PML for 16 and 17, else one direct kernel */
-    const int bv = (c->variant == 16 || c->variant == 17) ? c->variant : 1;
+    const int bv = (c->variant == 16 || c->variant == 17 || c->variant == 19) ? c->variant : 1;
     launch_volume(bv, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
   }
   else if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
@@ -1001,8 +1003,8 @@ extern "C" int cgfd_batch_active(cgfd_core *c) {
 
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
-  if (variant == 0) variant = 16;
-  if (variant < 1 || variant > 18) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant == 0) variant = 19;
+  if (variant < 1 || variant > 19) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
   c->drop_graph();
   return CGFD_OK;
@@ -1032,11 +1034,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[18] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18};
+  const int variants[19] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19};
   const int chunks[6] = {4, 8, 16, 32, 64, 128};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 18; vi++)
+  for (int vi = 0; vi < 19; vi++)
     for (int zi = 0; zi < 6; zi++) {
       int v = variants[vi], z = chunks[zi];
       if ((v == 1 || v == 10 || (v >= 12 && v <= 15)) && zi > 0) continue;  /* no z chunks */

@@ -346,6 +346,126 @@ __global__ void __launch_bounds__(256) k_vol_pml_slab(VolArgs a, BatArgs b) {
 }
 
 /* ------------------------------------------------------------------------
+ * Lean CFS-PML point. The correction of face `dim` needs the part of every
+ * rhs component that comes from the derivatives along `dim` only (r below;
+ * the r[] of sv_curv_col_el_iso_gpu.cu:916-1288). Those parts are
+ * accumulated field by field, like the interior kernel's gradient sums, with
+ * a register fence per field, so only 27 partial sums, the metric and the
+ * media stay live; the full rhs is their sum over the three directions.
+ * Then each face containing the point applies its correction and advances
+ * its auxiliary variables in the reference's face order. The free-surface
+ * term of the x/y faces (k == nk2) never occurs here: with a free surface
+ * the top four layers belong to k_vol_top.
+ * ---------------------------------------------------------------------- */
+template <int ID, int JD, int KD>
+DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
+  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  const unsigned o = (unsigned)p * 4u;
+  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
+  const float *M = a.met;
+  const long woff = (long)sn * b.wstride;
+  float e[3][3];  /* rows: xi, eta, zeta derivatives (x, y, z components) */
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) e[d][c] = ld32(M + (1 + 3 * d + c) * sc, o);
+  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+  const float l2m = l + 2.0f * m;
+  float r[3][9];
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int c = 0; c < 9; c++) r[d][c] = 0.0f;
+#pragma unroll
+  for (int f = 0; f < 9; f++) {
+    const float *bc = a.cur + woff + f * sc;
+    float D0 = dfd32<ID>(bc, o, 4u), D1 = dfd32<JD>(bc, o, by), D2 = dfd32<KD>(bc, o, bz);
+    __asm__ volatile("" : "+v"(D0), "+v"(D1), "+v"(D2) : : "memory");
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const float Dd = d == 0 ? D0 : (d == 1 ? D1 : D2);
+      const float e1 = e[d][0], e2 = e[d][1], e3 = e[d][2];
+      switch (f) {
+        case TXX: r[d][VX] += s * e1 * Dd; break;
+        case TYY: r[d][VY] += s * e2 * Dd; break;
+        case TZZ: r[d][VZ] += s * e3 * Dd; break;
+        case TXY: r[d][VX] += s * e2 * Dd; r[d][VY] += s * e1 * Dd; break;
+        case TXZ: r[d][VX] += s * e3 * Dd; r[d][VZ] += s * e1 * Dd; break;
+        case TYZ: r[d][VY] += s * e3 * Dd; r[d][VZ] += s * e2 * Dd; break;
+        case VX:
+          r[d][TXX] += l2m * e1 * Dd; r[d][TYY] += l * e1 * Dd; r[d][TZZ] += l * e1 * Dd;
+          r[d][TXY] += m * e2 * Dd; r[d][TXZ] += m * e3 * Dd;
+          break;
+        case VY:
+          r[d][TXX] += l * e2 * Dd; r[d][TYY] += l2m * e2 * Dd; r[d][TZZ] += l * e2 * Dd;
+          r[d][TXY] += m * e1 * Dd; r[d][TYZ] += m * e3 * Dd;
+          break;
+        case VZ:
+          r[d][TXX] += l * e3 * Dd; r[d][TYY] += l * e3 * Dd; r[d][TZZ] += l2m * e3 * Dd;
+          r[d][TXZ] += m * e1 * Dd; r[d][TYZ] += m * e2 * Dd;
+          break;
+      }
+    }
+  }
+  float h[9];
+#pragma unroll
+  for (int c = 0; c < 9; c++) h[c] = r[0][c] + r[1][c] + r[2][c];
+  const long poff = (long)sn * b.pstride;
+#pragma unroll
+  for (int dim = 0; dim < 3; dim++) {
+#pragma unroll 1
+    for (int side = 0; side < 2; side++) {
+      const PmlFace &F = a.pml[dim][side];
+      if (!F.on) continue;
+      if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
+      const long ia = (long)(k - F.k1) * ((long)F.bn * F.an) + (long)(j - F.j1) * F.an + (i - F.i1);
+      const int ic = dim == 0 ? i - F.i1 : (dim == 1 ? j - F.j1 : k - F.k1);
+      const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
+      const float cB1 = cB - 1.0f;
+      float *const fpre = F.pre + poff, *const ftmp = F.tmp + poff, *const fend = F.end + poff;
+      const float *auxc = (a.istage == 0 ? fpre : ftmp);
+      float ar[9];
+#pragma unroll
+      for (int c = 0; c < 9; c++) {
+        const float av = auxc[c * F.n + ia];
+        h[c] += cB1 * r[dim][c] - cB * av;
+        ar[c] = cD * r[dim][c] - cA * av;
+      }
+      if (a.istage == 0) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          const float pv = fpre[c * F.n + ia];
+          ftmp[c * F.n + ia] = pv + a.ca * ar[c];
+          fend[c * F.n + ia] = pv + a.cb * ar[c];
+        }
+      } else if (a.istage < 3) {
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+          ftmp[c * F.n + ia] = fpre[c * F.n + ia] + a.ca * ar[c];
+          fend[c * F.n + ia] += a.cb * ar[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 9; c++) fend[c * F.n + ia] += a.cb * ar[c];
+      }
+    }
+  }
+  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
+  rk_point32(a, o, h, woff);
+}
+
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256) k_vol_pml_lean(VolArgs a, BatArgs b) {
+  const int k = a.k_lo + (int)blockIdx.z + 3;
+  const int bi0 = blockIdx.x * 64, bj0 = blockIdx.y * 4 + 3;
+  if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
+  const int i = bi0 + (int)threadIdx.x, iy = blockIdx.y * 4 + (int)threadIdx.y;
+  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
+  pml_acc_point<ID, JD, KD>(a, b, i, iy + 3, k, b.act[0]);
+}
+
+/* ------------------------------------------------------------------------
  * Batches on the split interior / CFS-PML kernels. ORDER 0: (tile, scenario)
  * pairs with the scenario fastest, XCD-contiguous, so a tile's metric and
  * media lines serve every scenario from one L2. ORDER 1: one slab-ordered
@@ -394,7 +514,7 @@ __global__ void __launch_bounds__(256, 8) k_vol_bat_acc(VolArgs a, BatArgs b, lo
   acc_point<ID, JD, KD, 1>(a, b, i, iy + 3, k, b.act[q]);
 }
 
-template <int ID, int JD, int KD, int ORDER>
+template <int ID, int JD, int KD, int ORDER, bool LEAN>
 __global__ void __launch_bounds__(256) k_vol_bat_pml(VolArgs a, BatArgs b, long nitems) {
   int bx, by, kz, q;
   if (!bat_item(a, b, ORDER, nitems, bx, by, kz, q)) return;
@@ -403,7 +523,8 @@ __global__ void __launch_bounds__(256) k_vol_bat_pml(VolArgs a, BatArgs b, long 
   if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
   const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
   if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[q]);
+  if (LEAN) pml_acc_point<ID, JD, KD>(a, b, i, iy + 3, k, b.act[q]);
+  else direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[q]);
 }
 
 template <int ID, int JD, int KD, bool FENCE>
@@ -741,18 +862,22 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
     return;
   }
-  if (b.nact > 1 && (variant == 16 || variant == 17)) {  /* batches, split kernels */
+  if (b.nact > 1 && variant >= 16 && variant != 18) {  /* batches, split kernels */
     long items, nb;
     if (variant == 17) {
       items = (long)agrd.x * agrd.y * a.nk_run * b.nact;
       nb = ((items + 7) / 8) * 8;
       hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 0>), dim3((unsigned)nb), blk, 0, st, a, b, items);
-      if (a.pml_on) hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+      if (a.pml_on)
+        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0, false>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
     } else {
       const long nslab = ((long)agrd.y + a.zchunk - 1) / a.zchunk;
       nb = items = (long)agrd.x * a.zchunk * a.nk_run * nslab * b.nact;
       hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 1>), dim3((unsigned)nb), blk, 0, st, a, b, items);
-      if (a.pml_on) hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+      if (a.pml_on && variant == 19)
+        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1, true>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+      else if (a.pml_on)
+        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1, false>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
     }
     return;
   }
@@ -764,6 +889,13 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
   }
   if (variant == 2) {
     launch_vol_lds(a, ID, JD, KD, st);
+    return;
+  }
+  if (variant == 19) {
+    const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
+    const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
+    hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, false>), dim3((unsigned)nb), blk, 0, st, a, b);
+    if (a.pml_on) hipLaunchKernelGGL((k_vol_pml_lean<ID, JD, KD>), agrd, blk, 0, st_pml, a, b);
     return;
   }
   if (variant == 18) {

@@ -200,7 +200,7 @@ int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
                         float *ms_volume, int *n_volume);
 
-/* Kernel variant selection for the volume stencil (0 = default = 16,
+/* Kernel variant selection for the volume stencil (0 = default = 19,
    1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
    6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
@@ -213,9 +213,11 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    fenced as well, 16 = 14 with the interior blocks in slab order,
    17 = 16 with XCD-contiguous block ids for the interior and the PML
    blocks, 18 = 16 with the interior x taps moved between lanes by DPP
-   wave shifts instead of loaded). Batches with more than one active scenario run split interior /
-   PML kernels for variants 16 (one slab-ordered sweep per scenario) and 17
-   (the scenarios of a tile back to back, XCD-contiguous), and one direct
+   wave shifts instead of loaded, 19 = 16 with a lean PML-tile kernel that
+   accumulates the directional rhs parts field by field). Batches with more than one active scenario run split interior /
+   PML kernels for variants 16 and 19 (one slab-ordered sweep per scenario;
+   19 with the lean PML kernel) and 17 (the scenarios of a tile back to
+   back, XCD-contiguous), and one direct
    tile x scenario kernel for every other variant.
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);

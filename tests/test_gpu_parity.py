@@ -142,7 +142,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation (direct loads, LDS z-march, z queue)."""
     from cgfd import configs, model
@@ -182,7 +182,7 @@ def test_graph_replay_matches_eager(built_libs):
     assert np.array_equal(cg.read_stations(), ce.read_stations())
 
 
-@pytest.mark.parametrize("variant", [16, 17, 18, 1])
+@pytest.mark.parametrize("variant", [16, 17, 18, 19, 1])
 def test_stage_fork_matches_serial(built_libs, variant):
     """The stage fork (interior blocks, CFS-PML blocks + top layers, fault
     kernels on three streams) against every kernel on one stream: the
