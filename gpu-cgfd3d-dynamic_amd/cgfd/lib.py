@@ -103,6 +103,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_set_zchunk": (I32, [VP, I32]),
         "cgfd_core_set_graph": (I32, [VP, I32]),
         "cgfd_core_set_overlap": (I32, [VP, I32]),
+        "cgfd_core_state_bytes": (I32, [VP, C.POINTER(C.c_size_t)]),
+        "cgfd_core_save_state": (I32, [VP, VP, I32]),
+        "cgfd_core_load_state": (I32, [VP, VP, I32]),
         "cgfd_core_autotune": (I32, [VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(F32)]),
         "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),
         "cgfd_batch_create": (I32, [C.POINTER(Desc), I32, C.POINTER(VP)]),

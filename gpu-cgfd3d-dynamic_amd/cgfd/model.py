@@ -152,6 +152,30 @@ class Core:
     def set_overlap(self, on: bool):
         L.check(self.lib, self.lib.cgfd_core_set_overlap(self.handle, int(on)))
 
+    def state_bytes(self) -> int:
+        n = C.c_size_t()
+        L.check(self.lib, self.lib.cgfd_core_state_bytes(self.handle, C.byref(n)))
+        return int(n.value)
+
+    def save_state(self, dev_ptr: int = 0) -> np.ndarray | None:
+        """Step-boundary state into a new host array, or into device memory
+        at dev_ptr (state_bytes() bytes) when given."""
+        if dev_ptr:
+            L.check(self.lib, self.lib.cgfd_core_save_state(self.handle, C.c_void_p(dev_ptr), 1))
+            return None
+        out = np.empty(self.state_bytes(), np.uint8)
+        L.check(self.lib, self.lib.cgfd_core_save_state(self.handle, L.ptr(out), 0))
+        return out
+
+    def load_state(self, state: np.ndarray | None = None, dev_ptr: int = 0):
+        if dev_ptr:
+            L.check(self.lib, self.lib.cgfd_core_load_state(self.handle, C.c_void_p(dev_ptr), 1))
+            return
+        state = np.ascontiguousarray(state, np.uint8)
+        if state.size != self.state_bytes():
+            raise ValueError(f"state has {state.size} bytes, this core needs {self.state_bytes()}")
+        L.check(self.lib, self.lib.cgfd_core_load_state(self.handle, L.ptr(state), 0))
+
     def set_zchunk(self, z: int):
         L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
 

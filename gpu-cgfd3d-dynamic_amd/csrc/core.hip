@@ -802,6 +802,86 @@ extern "C" int cgfd_core_run(cgfd_core *c, int nsteps) {
   return CGFD_OK;
 }
 
+/* ---- step-boundary state (checkpoint / restart) ----
+   What a step reads from earlier steps (drv_rk_curv_col.cu:118-127,161-166,
+   fault_info.h:97-118, fault_wav_t.h:30-38): the wavefield and fault-state
+   "pre" levels, the PML auxiliaries' "pre" level, the fault outputs and
+   accumulators (Tn .. Init_t0), hT1 and mT1, the rupture flags, the surface
+   PG and displacement, and the step counter. Copied part by part in this
+   order into one caller buffer. */
+namespace {
+struct StatePart {
+  void *p;
+  size_t bytes;
+};
+std::vector<StatePart> state_parts(cgfd_core *c) {
+  std::vector<StatePart> v;
+  const Grid &g = c->g;
+  const size_t ss = (size_t)g.sslice;
+  v.push_back({c->wlev(c->lv_pre), (size_t)9 * g.sicmp * 4});
+  if (c->nfault > 0) {
+    v.push_back({c->flev(c->fl_pre), (size_t)c->nfault * 18 * ss * 4});
+    for (int id = 0; id < c->nfault; id++) {
+      const FaultDev &F = c->flt[id];
+      v.push_back({F.out, 11 * ss * 4});
+      v.push_back({F.hT1, 3 * ss * 4});
+      v.push_back({F.mT1, 3 * ss * 4});
+      v.push_back({F.flag_rup, ss * 4});
+      v.push_back({F.init_t0_flag, ss * 4});
+    }
+  }
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++)
+      if (c->pml[dim][side].on) v.push_back({c->pml[dim][side].pre, (size_t)9 * c->pml[dim][side].n * 4});
+  if (c->PG) v.push_back({c->PG, (size_t)18 * g.siz * 4});
+  v.push_back({c->it_dev, 4});
+  return v;
+}
+}  // namespace
+
+extern "C" int cgfd_core_state_bytes(cgfd_core *c, size_t *bytes) {
+  if (!c || !bytes) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "state save/load covers single-scenario cores");
+  size_t n = 0;
+  for (const StatePart &q : state_parts(c)) n += q.bytes;
+  *bytes = n;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_save_state(cgfd_core *c, void *dst, int dst_on_device) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "state save/load covers single-scenario cores");
+  HIPCHK(hipSetDevice(c->dev));
+  const hipMemcpyKind kind = dst_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  size_t off = 0;
+  for (const StatePart &q : state_parts(c)) {
+    HIPCHK(hipMemcpyAsync((char *)dst + off, q.p, q.bytes, kind, c->st));
+    off += q.bytes;
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_device) {
+  if (!c || !src) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "state save/load covers single-scenario cores");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->drop_graph();
+  const hipMemcpyKind kind = src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  size_t off = 0;
+  for (const StatePart &q : state_parts(c)) {
+    HIPCHK(hipMemcpyAsync(q.p, (const char *)src + off, q.bytes, kind, c->st));
+    off += q.bytes;
+  }
+  int it = 0;
+  HIPCHK(hipMemcpyAsync(&it, c->it_dev, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (it < 0) return cgfd_set_error(CGFD_ERR_ARG, "loaded state has a negative step counter");
+  c->it_host = it;
+  return CGFD_OK;
+}
+
 extern "C" int cgfd_core_set_overlap(cgfd_core *c, int on) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   HIPCHK(hipSetDevice(c->dev));

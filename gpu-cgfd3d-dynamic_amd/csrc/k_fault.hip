@@ -345,48 +345,76 @@ DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
   /* ---- velocity rhs ---- */
   const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
   const float *Tyz = W + TYZ * sc, *Txz = W + TXZ * sc, *Txy = W + TXY * sc;
-  float v1[3][7], v2[3][7], v3[3][7];
+  /* J (grad xi_d . T) at the seven points along direction d, one direction
+     at a time and reduced to its derivative before the next direction's
+     loads (a register fence keeps the three groups apart) */
+  float dvx[3], dvy[3], dvz[3];
+  {
+    float v[3][7];
 #pragma unroll
-  for (int l = -3; l <= 3; l++) {
-    long q = (i + l) + row;
-    float jq = met[q], a1 = met[sc + q], a2 = met[2 * sc + q], a3 = met[3 * sc + q];
-    v1[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-    v1[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-    v1[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
-    q = p + l * sy;
-    jq = met[q]; a1 = met[4 * sc + q]; a2 = met[5 * sc + q]; a3 = met[6 * sc + q];
-    v2[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-    v2[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-    v2[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
-    q = p + l * sz;
-    jq = met[q]; a1 = met[7 * sc + q]; a2 = met[8 * sc + q]; a3 = met[9 * sc + q];
-    v3[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-    v3[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-    v3[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    for (int l = -3; l <= 3; l++) {
+      const long q = (i + l) + row;
+      const float jq = met[q], a1 = met[sc + q], a2 = met[2 * sc + q], a3 = met[3 * sc + q];
+      v[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+      v[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+      v[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float *u = v[c];
+      /* the tap on the plane is the fault traction; image beyond it
+         (fault_gpu.cu:266-291) */
+      const float tc = F.T1[(long)c * 7 * ss + 3 * ss + pf];
+#pragma unroll
+      for (int l = 0; l < 7; l++)
+        if (l == n + 3) u[l] = tc;
+      if (n == 2) { u[6] = (float)(2.0 * u[5] - u[4]); }
+      if (n == 1) { u[5] = (float)(2.0 * u[4] - u[3]); u[6] = (float)(2.0 * u[4] - u[2]); }
+      if (n == -1) { u[0] = (float)(2.0 * u[2] - u[4]); u[1] = (float)(2.0 * u[2] - u[3]); }
+      if (n == -2) { u[0] = (float)(2.0 * u[1] - u[2]); }
+      dvx[c] = vecc(u + 3, IDIR);
+    }
+    __asm__ volatile("" : "+v"(dvx[0]), "+v"(dvx[1]), "+v"(dvx[2]) : : "memory");
+  }
+  {
+    float v[3][7];
+#pragma unroll
+    for (int l = -3; l <= 3; l++) {
+      const long q = p + l * sy;
+      const float jq = met[q], a1 = met[4 * sc + q], a2 = met[5 * sc + q], a3 = met[6 * sc + q];
+      v[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+      v[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+      v[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) dvy[c] = vecc(v[c] + 3, jdir);
+    __asm__ volatile("" : "+v"(dvy[0]), "+v"(dvy[1]), "+v"(dvy[2]) : : "memory");
+  }
+  {
+    float v[3][7];
+#pragma unroll
+    for (int l = -3; l <= 3; l++) {
+      const long q = p + l * sz;
+      const float jq = met[q], a1 = met[7 * sc + q], a2 = met[8 * sc + q], a3 = met[9 * sc + q];
+      v[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+      v[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+      v[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      float *u = v[c];
+      if (surf && km <= 3) {
+        u[n_free] = 0.0f;
+        for (int l = n_free + 1; l < 7; l++) u[l] = -u[2 * n_free - l];
+      }
+      dvz[c] = vecc(u + 3, kdir);
+    }
   }
   float h[9];
   const float rrhojac = a.slw[p] / met[p];
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    float *v = v1[c];
-    /* the tap on the plane is the fault traction; image beyond it
-       (fault_gpu.cu:266-291) */
-    const float tc = F.T1[(long)c * 7 * ss + 3 * ss + pf];
-#pragma unroll
-    for (int l = 0; l < 7; l++)
-      if (l == n + 3) v[l] = tc;
-    if (n == 2) { v[6] = (float)(2.0 * v[5] - v[4]); }
-    if (n == 1) { v[5] = (float)(2.0 * v[4] - v[3]); v[6] = (float)(2.0 * v[4] - v[2]); }
-    if (n == -1) { v[0] = (float)(2.0 * v[2] - v[4]); v[1] = (float)(2.0 * v[2] - v[3]); }
-    if (n == -2) { v[0] = (float)(2.0 * v[1] - v[2]); }
-    if (surf && km <= 3) {
-      float *u = v3[c];
-      u[n_free] = 0.0f;
-      for (int l = n_free + 1; l < 7; l++) u[l] = -u[2 * n_free - l];
-    }
-    float dx = vecc(v1[c] + 3, IDIR), dy = vecc(v2[c] + 3, jdir), dz = vecc(v3[c] + 3, kdir);
-    h[c] = (dx + dy + dz) * rrhojac;
-  }
+  for (int c = 0; c < 3; c++) h[c] = (dvx[c] + dvy[c] + dvz[c]) * rrhojac;
+  __asm__ volatile("" : "+v"(h[0]), "+v"(h[1]), "+v"(h[2]) : : "memory");
 
   /* ---- stress rhs ---- */
   const float *V[3] = {W, W + sc, W + 2 * sc};
@@ -507,6 +535,9 @@ DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
       fhV[m][c] = (dx + dy + dz) * rrhojac;
     }
   }
+  /* register fences between the phases keep each phase's loads together */
+  __asm__ volatile("" : "+v"(fhV[0][0]), "+v"(fhV[0][1]), "+v"(fhV[0][2]), "+v"(fhV[1][0]), "+v"(fhV[1][1]),
+                   "+v"(fhV[1][2]) : : "memory");
 
   /* split-node velocity gradients: DV[0] minus side, DV[1] plus side */
   const float *V[3] = {W, W + sc, W + 2 * sc};
@@ -590,6 +621,10 @@ DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
     if (a.imethod == 1) method1();
     else method2();
   }
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+    __asm__ volatile("" : "+v"(DxV[m][0]), "+v"(DxV[m][1]), "+v"(DxV[m][2]), "+v"(DyV[m][0]), "+v"(DyV[m][1]),
+                     "+v"(DyV[m][2]), "+v"(DzV[m][0]), "+v"(DzV[m][1]), "+v"(DzV[m][2]) : : "memory");
   /* traction rates of the split nodes (fault_gpu.cu:936-1018) */
   float fhT2[2][3], fhT3[2][3];
 #pragma unroll

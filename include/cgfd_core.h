@@ -149,6 +149,25 @@ int cgfd_core_run(cgfd_core *core, int nsteps);
    (drv_rk_curv_col.cu:209-636). */
 int cgfd_core_set_graph(cgfd_core *core, int on);
 
+/* Step-boundary state, for checkpoint-and-recompute and restart files. No
+   counterpart in the reference, which has no restart (its state between
+   steps is drv_rk_curv_col.cu:118-127,161-166 plus the fault accumulators of
+   fault_info.h:97-118 and hT1/mT1 of fault_wav_t.h:30-38, which is exactly
+   what is saved: wavefield and fault-state levels at the step boundary, PML
+   auxiliaries, fault outputs and accumulators, hT1, mT1, rupture flags,
+   surface PG and displacement, and the step counter).
+   cgfd_core_state_bytes: size of the state in bytes.
+   cgfd_core_save_state: copies it into dst (device memory if dst_on_device,
+   else host memory); the core is unchanged.
+   cgfd_core_load_state: overwrites the core's state from src (as written by
+   cgfd_core_save_state of a core created from the same descriptor) and
+   resets the step index to the saved one; the next steps are bit-identical
+   to the steps that followed the save. Single-scenario cores only
+   (CGFD_ERR_ARG for batches). */
+int cgfd_core_state_bytes(cgfd_core *core, size_t *bytes);
+int cgfd_core_save_state(cgfd_core *core, void *dst, int dst_on_device);
+int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
+
 /* Stage fork (default off): within each RK stage, after the trial traction,
    the interior volume blocks, the CFS-PML blocks with the top layers, and
    the fault columns with the split nodes run on three default-priority
