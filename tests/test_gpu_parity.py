@@ -212,23 +212,27 @@ def test_state_save_load_restarts_bit_identically(built_libs):
     continued, the state loaded back (also into a second core) and the same
     steps run again: bit-identical wavefield, fault outputs, PG and step
     index. The restored run crosses a graph block boundary."""
-    import torch
+    import ctypes as C
     from cgfd import configs, model
     sc = configs.small(imethod=2)
     st = model.Setup(sc)
     core = model.Core(st)
     core.run(13)
     host = core.save_state()
-    dev = torch.empty(core.state_bytes(), dtype=torch.uint8, device="cuda")
-    torch.cuda.synchronize()
-    core.save_state(dev.data_ptr())
+    # device buffer from the HIP runtime the core library is linked against
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    dptr = C.c_void_p()
+    assert hip.hipMalloc(C.byref(dptr), core.state_bytes()) == 0
+    core.save_state(dptr.value)
     core.run(11)
     ref = (core.read_wavefield(), core.read_fault(0), core.read_pg(), core.step_index)
     for how in ("host", "device"):
         if how == "host":
             core.load_state(host)
         else:
-            core.load_state(dev_ptr=dev.data_ptr())
+            core.load_state(dev_ptr=dptr.value)
         assert core.step_index == 13
         core.run(11)
         got = (core.read_wavefield(), core.read_fault(0), core.read_pg(), core.step_index)
@@ -242,4 +246,5 @@ def test_state_save_load_restarts_bit_identically(built_libs):
     assert np.array_equal(other.read_fault(0), ref[1])
     other.close()
     core.close()
+    assert hip.hipFree(dptr) == 0
     st.close()
