@@ -152,6 +152,18 @@ class Core:
     def set_overlap(self, on: bool):
         L.check(self.lib, self.lib.cgfd_core_set_overlap(self.handle, int(on)))
 
+    def adjoint_initial(self, residual: np.ndarray) -> np.ndarray:
+        """dJ/dw at the first recorded step for J = sum residual . traces
+        (residual shaped like read_stations(): [nsta][nt][9])."""
+        residual = np.ascontiguousarray(residual, np.float32)
+        nsta, nt, nc = residual.shape
+        if nc != 9:
+            raise ValueError("residual must be [nsta][nt][9]")
+        nz, ny, nx = self.shape_ext
+        out = np.zeros((9, nz, ny, nx), np.float32)
+        L.check(self.lib, self.lib.cgfd_adj_backward(self.handle, L.ptr(residual), nt, L.ptr(out)))
+        return out
+
     def state_bytes(self) -> int:
         n = C.c_size_t()
         L.check(self.lib, self.lib.cgfd_core_state_bytes(self.handle, C.byref(n)))

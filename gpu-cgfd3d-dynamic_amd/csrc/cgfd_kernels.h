@@ -110,6 +110,13 @@ void launch_step_end(int *it_dev, hipStream_t st);
 void launch_zero(float *p, long n, hipStream_t st);
 void launch_repitch(float *dst, long dpitch, const float *src, long spitch, long width, long rows, hipStream_t st);
 void launch_nan_check(const float *p, long n, int *flag, hipStream_t st);
+/* discrete adjoint of the volume update (k_adjoint.hip) */
+void launch_adj_prep(const Grid &g, const float *lam, const float *mu, const float *slw, const float *X, float alpha,
+                     const float *E, float beta, float *SG, hipStream_t st);
+void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
+                      const float *E, float *A, float *B, float *P, hipStream_t st);
+void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int n, long sicmp, float *P,
+                       hipStream_t st);
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
                             int nt_max, const int *it_dev, int it0, hipStream_t st);
 

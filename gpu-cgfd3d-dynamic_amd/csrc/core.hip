@@ -140,6 +140,10 @@ struct cgfd_core {
     gexec = nullptr;
   }
 
+  /* discrete adjoint: 4 levels of 9 fields (E, A, B, P) and the 12 fields
+     of the adjoint gradient terms; allocated by the first backward call */
+  float *adj = nullptr, *adj_sg = nullptr;
+
   float *wlev(int l) { return wave + (long)l * 9 * g.sicmp; }
   float *flev(int l) { return fwav + (long)l * nfault * 18 * g.sslice; }
 
@@ -879,6 +883,62 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
   HIPCHK(hipStreamSynchronize(c->st));
   if (it < 0) return cgfd_set_error(CGFD_ERR_ARG, "loaded state has a negative step counter");
   c->it_host = it;
+  return CGFD_OK;
+}
+
+/* ---- discrete adjoint of the volume update (no PML, free surface or
+   fault): gradient of J = sum_n sum_s r_n(s) . w_n(s) with respect to the
+   state at the first recorded step, for the residuals r of the recorded
+   station traces; the gradient of a least-squares trace misfit is this with
+   r = recorded - target. Reverse sweep over the recorded steps, one
+   reverse RK4 step per recorded step (kernels in k_adjoint.hip). ---- */
+extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
+  if (!c || !residual || !grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
+  if (c->nfault > 0 || c->pml_on || c->isfree)
+    return cgfd_set_error(CGFD_ERR_ARG,
+                          "adjoint: this build transposes the interior update only (no fault, CFS-PML or free surface)");
+  if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "adjoint: no stations set");
+  if (nt < 1 || nt > c->st_nt) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: nt %d outside 1..%d", nt, c->st_nt);
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const Grid &g = c->g;
+  const long sc = g.sicmp, lv = 9 * sc;
+  int rc;
+  if (!c->adj) {
+    if ((rc = c->alloc(&c->adj, (size_t)4 * lv))) return rc;
+    if ((rc = c->alloc(&c->adj_sg, (size_t)12 * sc))) return rc;
+    HIPCHK(hipMemset(c->adj_sg, 0, (size_t)12 * sc * 4));  /* ghost entries stay zero */
+  }
+  HIPCHK(hipMemsetAsync(c->adj, 0, (size_t)4 * lv * 4, c->st));
+  float *res = nullptr;
+  const size_t nres = (size_t)c->st_n * nt * 9;
+  HIPCHK(hipMalloc(&res, nres * 4));
+  hipError_t e = hipMemcpyAsync(res, residual, nres * 4, hipMemcpyHostToDevice, c->st);
+  float *E = c->adj, *A = c->adj + lv, *B = c->adj + 2 * lv, *P = c->adj + 3 * lv;
+  for (int n = nt - 1; n >= 0 && e == hipSuccess; n--) {
+    if (n < nt - 1) {
+      const int it = c->st_it0 + n;  /* the forward step that took w_n to w_{n+1} */
+      const int ipair = it % 8;
+      for (int s = 3; s >= 0; s--) {
+        const int idir = (kPairs[ipair][0] + s) % 2, jdir = (kPairs[ipair][1] + s) % 2, kdir = (kPairs[ipair][2] + s) % 2;
+        const float cb = kRkB[s] * c->dt, ca = s < 3 ? kRkA[s] * c->dt : 0.0f;
+        const float *X = s == 3 ? nullptr : (s == 1 ? B : A);
+        launch_adj_prep(g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, c->st);
+        launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, c->st);
+      }
+    }
+    launch_adj_inject(c->st_idx, c->st_n, res, nt, n, sc, P, c->st);
+    std::swap(E, P);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  (void)hipFree(res);
+  if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "adjoint sweep: %s", hipGetErrorString(e));
+  /* E holds the adjoint of the first recorded state */
+  const size_t hc = (size_t)g.nx * g.ny * g.nz;
+  for (int f = 0; f < 9; f++)
+    if ((rc = dev_to_host(c, grad + f * hc, E + f * sc, (long)g.ny * g.nz))) return rc;
   return CGFD_OK;
 }
 

@@ -1,0 +1,181 @@
+/*
+ * k_adjoint.hip -- discrete adjoint of the RK4 volume update (interior
+ * stencil, no CFS-PML, no free surface, no fault).
+ *
+ * One forward stage computes, at every physical point p,
+ *   h(p) = H(p) . [D_d f(p)],  D_d f(p) = sum_t c_t f(p + (o_d + t) e_d)
+ * (the 5-point one-sided operators of fd_t.cu:85-109 along each axis d,
+ * o_d = -1 forward / -3 backward), where H(p) forms the physical gradients
+ * g(f,a) = sum_d M[d][a] D_d f with the metric rows M[d] = (xi, eta, zeta)_{x,y,z}
+ * and then momentum and Hooke's law (sv_curv_col_el_iso_gpu.cu:349-386).
+ * Its transpose applied to an adjoint rhs c is
+ *   lambda_f(q) = sum_d sum_t c_t kappa(f,d)(q - (o_d + t) e_d),
+ *   kappa(f,d)(p) = sum_a M[d][a](p) gamma(f,a)(p),
+ * where gamma are the adjoints of the physical gradients, formed pointwise
+ * from c(p) and the media at p:
+ *   stress fields: gamma from S = (1/rho) c_V (3 values),
+ *   velocity fields: gamma(V,a) from the stress adjoints through Hooke's law
+ *   (9 values G).
+ * So each reverse stage is two kernels: k_adj_prep writes S and G at the
+ * physical points (ghost entries stay zero: the forward never uses h at a
+ * ghost), and k_adj_stage gathers the transposed operators with the metric
+ * at each tap and does the pointwise RK-adjoint accumulations.
+ *
+ * RK4 (drv_rk_curv_col.cu:329-568) with a = {1/2, 1/2, 1}, b = {1/6, 1/3,
+ * 1/3, 1/6}, ca = a dt, cb = b dt, reversed (E = adjoint of the step's
+ * output, P = adjoint of its input):
+ *   stage 3: A  = L3^T (cb3 E)
+ *   stage 2: B  = L2^T (ca2 A + cb2 E),  P  = E + A
+ *   stage 1: A  = L1^T (ca1 B + cb1 E),  P += B
+ *   stage 0: P += A + L0^T (ca0 A + cb0 E)
+ */
+#include "cgfd_dev.h"
+#include "cgfd_kernels.h"
+#include "k_vol_common.h"
+
+namespace cgfd {
+
+/* S (3) and G (9) from c = alpha X + beta E at physical points; X may be null */
+__global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restrict__ lam, const float *__restrict__ mu,
+                                                  const float *__restrict__ slw, const float *__restrict__ X,
+                                                  float alpha, const float *__restrict__ E, float beta,
+                                                  float *__restrict__ SG) {
+  const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
+  const int k = (int)blockIdx.z + 3;
+  if (i >= g.ni + 3 || j >= g.nj + 3) return;
+  const long sc = g.sicmp;
+  const long p = i + j * g.siy + (long)k * g.siz;
+  float c[9];
+#pragma unroll
+  for (int f = 0; f < 9; f++) c[f] = beta * E[f * sc + p] + (X ? alpha * X[f * sc + p] : 0.0f);
+  const float l = lam[p], m = mu[p], s = slw[p];
+  const float l2m = l + 2.0f * m;
+  float o[12];
+  o[0] = s * c[VX];
+  o[1] = s * c[VY];
+  o[2] = s * c[VZ];
+  o[3] = l2m * c[TXX] + l * c[TYY] + l * c[TZZ];  /* gamma(Vx, x) */
+  o[4] = m * c[TXY];                              /* gamma(Vx, y) */
+  o[5] = m * c[TXZ];                              /* gamma(Vx, z) */
+  o[6] = m * c[TXY];                              /* gamma(Vy, x) */
+  o[7] = l * c[TXX] + l2m * c[TYY] + l * c[TZZ];  /* gamma(Vy, y) */
+  o[8] = m * c[TYZ];                              /* gamma(Vy, z) */
+  o[9] = m * c[TXZ];                              /* gamma(Vz, x) */
+  o[10] = m * c[TYZ];                             /* gamma(Vz, y) */
+  o[11] = l * c[TXX] + l * c[TYY] + l2m * c[TZZ]; /* gamma(Vz, z) */
+#pragma unroll
+  for (int q = 0; q < 12; q++) SG[q * sc + p] = o[q];
+}
+
+template <int DIR>
+DEVI float tap_coef(int t) {
+  return DIR ? (t == 0 ? CGFD_CF0 : t == 1 ? CGFD_CF1 : t == 2 ? CGFD_CF2 : t == 3 ? CGFD_CF3 : CGFD_CF4)
+             : (t == 0 ? CGFD_CB0 : t == 1 ? CGFD_CB1 : t == 2 ? CGFD_CB2 : t == 3 ? CGFD_CB3 : CGFD_CB4);
+}
+
+/* add w * kappa(., d)(p) for all nine fields; M: metric row d at p */
+DEVI void add_kappa(float *lm, float w, const float *__restrict__ SG, long sc, long p, float M0, float M1, float M2) {
+  const float Sx = SG[p], Sy = SG[sc + p], Sz = SG[2 * sc + p];
+  const float *G = SG + 3 * sc + p;
+  lm[VX] += w * (M0 * G[0] + M1 * G[sc] + M2 * G[2 * sc]);
+  lm[VY] += w * (M0 * G[3 * sc] + M1 * G[4 * sc] + M2 * G[5 * sc]);
+  lm[VZ] += w * (M0 * G[6 * sc] + M1 * G[7 * sc] + M2 * G[8 * sc]);
+  lm[TXX] += w * (M0 * Sx);
+  lm[TYY] += w * (M1 * Sy);
+  lm[TZZ] += w * (M2 * Sz);
+  lm[TXY] += w * (M0 * Sy + M1 * Sx);
+  lm[TXZ] += w * (M0 * Sz + M2 * Sx);
+  lm[TYZ] += w * (M1 * Sz + M2 * Sy);
+}
+
+/* MODE 3: A = L^T;  2: B = L^T, P = E + A;  1: A = L^T, P += B;  0: P += A + L^T */
+template <int ID, int JD, int KD, int MODE>
+__global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restrict__ met,
+                                                   const float *__restrict__ SG, const float *__restrict__ E,
+                                                   float *A, float *B, float *P) {
+  const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
+  const int k = (int)blockIdx.z + 3;
+  if (i >= g.ni + 3 || j >= g.nj + 3) return;
+  const long sc = g.sicmp, sy = g.siy, sz = g.siz;
+  const long q = i + j * sy + (long)k * sz;
+  float lm[9];
+#pragma unroll
+  for (int f = 0; f < 9; f++) lm[f] = 0.0f;
+  constexpr int ox = ID ? -1 : -3, oy = JD ? -1 : -3, oz = KD ? -1 : -3;
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const long p = q - (ox + t);
+    add_kappa(lm, tap_coef<ID>(t), SG, sc, p, met[sc + p], met[2 * sc + p], met[3 * sc + p]);
+  }
+  __asm__ volatile("" : "+v"(lm[0]), "+v"(lm[1]), "+v"(lm[2]), "+v"(lm[3]), "+v"(lm[4]), "+v"(lm[5]),
+                   "+v"(lm[6]), "+v"(lm[7]), "+v"(lm[8]) : : "memory");
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const long p = q - (long)(oy + t) * sy;
+    add_kappa(lm, tap_coef<JD>(t), SG, sc, p, met[4 * sc + p], met[5 * sc + p], met[6 * sc + p]);
+  }
+  __asm__ volatile("" : "+v"(lm[0]), "+v"(lm[1]), "+v"(lm[2]), "+v"(lm[3]), "+v"(lm[4]), "+v"(lm[5]),
+                   "+v"(lm[6]), "+v"(lm[7]), "+v"(lm[8]) : : "memory");
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const long p = q - (long)(oz + t) * sz;
+    add_kappa(lm, tap_coef<KD>(t), SG, sc, p, met[7 * sc + p], met[8 * sc + p], met[9 * sc + p]);
+  }
+#pragma unroll
+  for (int f = 0; f < 9; f++) {
+    const long o = f * sc + q;
+    if (MODE == 3) A[o] = lm[f];
+    else if (MODE == 2) { B[o] = lm[f]; P[o] = E[o] + A[o]; }
+    else if (MODE == 1) { const float b = B[o]; A[o] = lm[f]; P[o] += b; }
+    else P[o] += A[o] + lm[f];
+  }
+}
+
+/* P += residual row n at the stations (all nine components) */
+__global__ void k_adj_inject(const long *__restrict__ idx, int nsta, const float *__restrict__ res, int nt, int n,
+                             long sicmp, float *P) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsta * 9) return;
+  const int s = t / 9, c = t % 9;
+  atomicAdd(P + c * sicmp + idx[s], res[((long)s * nt + n) * 9 + c]);
+}
+
+void launch_adj_prep(const Grid &g, const float *lam, const float *mu, const float *slw, const float *X, float alpha,
+                     const float *E, float beta, float *SG, hipStream_t st) {
+  dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk);
+  hipLaunchKernelGGL(k_adj_prep, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG);
+}
+
+template <int ID, int JD, int KD>
+static void adj_stage_dir(int mode, const Grid &g, const float *met, const float *SG, const float *E, float *A,
+                          float *B, float *P, hipStream_t st) {
+  dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk);
+  switch (mode) {
+    case 3: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 3>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
+    case 2: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 2>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
+    case 1: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 1>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
+    default: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 0>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
+  }
+}
+
+void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
+                      const float *E, float *A, float *B, float *P, hipStream_t st) {
+  switch (idir * 4 + jdir * 2 + kdir) {
+    case 0: adj_stage_dir<0, 0, 0>(mode, g, met, SG, E, A, B, P, st); break;
+    case 1: adj_stage_dir<0, 0, 1>(mode, g, met, SG, E, A, B, P, st); break;
+    case 2: adj_stage_dir<0, 1, 0>(mode, g, met, SG, E, A, B, P, st); break;
+    case 3: adj_stage_dir<0, 1, 1>(mode, g, met, SG, E, A, B, P, st); break;
+    case 4: adj_stage_dir<1, 0, 0>(mode, g, met, SG, E, A, B, P, st); break;
+    case 5: adj_stage_dir<1, 0, 1>(mode, g, met, SG, E, A, B, P, st); break;
+    case 6: adj_stage_dir<1, 1, 0>(mode, g, met, SG, E, A, B, P, st); break;
+    default: adj_stage_dir<1, 1, 1>(mode, g, met, SG, E, A, B, P, st); break;
+  }
+}
+
+void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int n, long sicmp, float *P,
+                       hipStream_t st) {
+  const int nthr = nsta * 9;
+  hipLaunchKernelGGL(k_adj_inject, dim3((nthr + 255) / 256), dim3(256), 0, st, idx, nsta, res, nt, n, sicmp, P);
+}
+
+}  // namespace cgfd

@@ -168,6 +168,19 @@ int cgfd_core_state_bytes(cgfd_core *core, size_t *bytes);
 int cgfd_core_save_state(cgfd_core *core, void *dst, int dst_on_device);
 int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
 
+/* Discrete adjoint of the time loop, first part: the interior velocity-stress
+   update and RK4 (cores without fault, CFS-PML or free surface; other cores
+   get CGFD_ERR_ARG). The reference has no adjoint; this is the transpose of
+   the discrete operator of sv_curv_col_el_iso_gpu.cu:231-388 composed with
+   the RK4 stages of drv_rk_curv_col.cu:329-568.
+   With stations set before the forward run (cgfd_core_set_stations) and the
+   core advanced over the recorded steps, residual[nsta][nt][9] (the layout
+   of cgfd_core_read_stations) defines J = sum_n sum_s residual[s][n] .
+   w_n(s); grad[9][nz][ny][nx] (host layout, ghosts zero) receives dJ/dw at
+   the first recorded step. For a least-squares trace misfit pass
+   residual = recorded - target. The core's forward state is not touched. */
+int cgfd_adj_backward(cgfd_core *core, const float *residual, int nt, float *grad);
+
 /* Stage fork (default off): within each RK stage, after the trial traction,
    the interior volume blocks, the CFS-PML blocks with the top layers, and
    the fault columns with the split nodes run on three default-priority

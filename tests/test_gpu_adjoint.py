@@ -1,0 +1,133 @@
+"""Discrete adjoint of the interior velocity-stress update with RK4 (cores
+without fault, CFS-PML or free surface), through the C ABI.
+
+* Linear identity: the station traces are a linear map T of the initial
+  wavefield, so for any residual r the adjoint g = T^T r satisfies
+  <g, w0> = sum r . T(w0) (both sides in FP64 from FP32 data).
+* Least-squares misfit J = 1/2 sum (T(w0) - target)^2: the adjoint gradient
+  against FP64 central differences of J computed by the FP64 oracle along
+  seeded random directions (J is quadratic, so the central difference has no
+  truncation error); agreement within 1e-3 relative.
+* Cores outside the supported subset are refused with CGFD_ERR_ARG.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NT = 24
+
+
+def _scenario():
+    from cgfd import configs
+    sc = configs.small(imethod=2)
+    sc.faults = []
+    sc.pml_layers = [[0, 0], [0, 0], [0, 0]]
+    sc.free_top = False
+    rng = np.random.default_rng(41)
+    extra = np.stack([rng.integers(2, sc.ni - 2, 6), rng.integers(2, sc.nj - 2, 6), rng.integers(2, sc.nk - 2, 6)], 1)
+    sc.stations = np.concatenate([sc.stations, extra]).astype(np.int32)
+    return sc
+
+
+def _smooth_field(sc, seed, amp=1.0):
+    """Random wavefield on the physical points (ghosts zero), smoothed so a
+    short run stays well resolved."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = sc.shape_ext
+    w = np.zeros((9, nz, ny, nx), np.float32)
+    core = rng.standard_normal((9, sc.nk, sc.nj, sc.ni))
+    for ax in (1, 2, 3):
+        core = (np.roll(core, 1, ax) + 2 * core + np.roll(core, -1, ax)) / 4
+    scale = np.array([1.0] * 3 + [1e6] * 6)[:, None, None, None]
+    w[:, 3:-3, 3:-3, 3:-3] = (amp * scale * core).astype(np.float32)
+    return w
+
+
+def _traces_gpu(st, sc, w0):
+    from cgfd import model
+    core = model.Core(st)
+    core.set_stations(sc.stations + 3, NT)
+    core.write_wavefield(w0)
+    core.run(NT)
+    tr = core.read_stations()
+    return core, tr
+
+
+def test_adjoint_linear_identity(built_libs):
+    from cgfd import model
+    sc = _scenario()
+    st = model.Setup(sc)
+    w0 = _smooth_field(sc, 5)
+    core, tr = _traces_gpu(st, sc, w0)
+    rng = np.random.default_rng(9)
+    r = rng.standard_normal(tr.shape).astype(np.float32)
+    r[:, :, 3:] *= 1e-6  # stress residuals on the stress scale
+    g = core.adjoint_initial(r)
+    lhs = float(np.sum(g.astype(np.float64) * w0.astype(np.float64)))
+    rhs = float(np.sum(r.astype(np.float64) * tr.astype(np.float64)))
+    print(f"[adjoint] <g, w0> = {lhs:.9e}, sum r.T(w0) = {rhs:.9e}", flush=True)
+    assert np.isfinite(g).all()
+    assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
+    # ghosts carry no gradient
+    assert not g[:, :3].any() and not g[:, -3:].any()
+    core.close()
+    st.close()
+
+
+def test_adjoint_least_squares_vs_fp64_central_differences(built_libs):
+    from cgfd import model
+    import oracle
+    sc = _scenario()
+    st = model.Setup(sc)
+    w0 = _smooth_field(sc, 6)
+    target = _smooth_field(sc, 7, amp=0.5)
+    _, tr_target = _traces_gpu(st, sc, target)
+    core, tr = _traces_gpu(st, sc, w0)
+    g = core.adjoint_initial(tr - tr_target).astype(np.float64)
+    ijk = sc.stations + 3
+
+    def j_fp64(w):
+        orc = oracle.Oracle(st.desc, sc.shape_ext, 0, double=True)
+        orc.write_wavefield(w)
+        rec = []
+        for _ in range(NT):
+            x = orc.read_wavefield()
+            rec.append([x[:, k, j, i] for (i, j, k) in ijk])
+            orc.run(1)
+        orc.close()
+        rec = np.transpose(np.array(rec, np.float64), (1, 0, 2))
+        return 0.5 * float(np.sum((rec - tr_target.astype(np.float64)) ** 2))
+
+    worst = 0.0
+    for d_seed in (101, 102, 103):
+        d = _smooth_field(sc, d_seed, amp=0.1).astype(np.float64)
+        eps = 1.0
+        fd = (j_fp64(w0 + eps * d) - j_fp64(w0 - eps * d)) / (2 * eps)
+        ad = float(np.sum(g * d))
+        rel = abs(ad - fd) / abs(fd)
+        worst = max(worst, rel)
+        print(f"[adjoint] direction {d_seed}: adjoint {ad:.8e}, FP64 central difference {fd:.8e}, rel {rel:.2e}",
+              flush=True)
+    assert worst <= 1e-3
+    core.close()
+    st.close()
+
+
+def test_adjoint_refuses_unsupported_cores(built_libs):
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)  # fault + PML + free surface
+    st = model.Setup(sc)
+    core = model.Core(st)
+    core.set_stations(sc.stations + 3, 4)
+    core.run(4)
+    r = np.zeros((len(sc.stations), 4, 9), np.float32)
+    nz, ny, nx = sc.shape_ext
+    out = np.zeros((9, nz, ny, nx), np.float32)
+    rc = core.lib.cgfd_adj_backward(core.handle, r.ctypes.data_as(C.c_void_p), 4, out.ctypes.data_as(C.c_void_p))
+    assert rc == -1
+    assert b"interior update only" in core.lib.cgfd_last_error()
+    core.close()
+    st.close()
