@@ -111,10 +111,23 @@ void launch_zero(float *p, long n, hipStream_t st);
 void launch_repitch(float *dst, long dpitch, const float *src, long spitch, long width, long rows, hipStream_t st);
 void launch_nan_check(const float *p, long n, int *flag, hipStream_t st);
 /* discrete adjoint of the volume update (k_adjoint.hip) */
-void launch_adj_prep(const Grid &g, const float *lam, const float *mu, const float *slw, const float *X, float alpha,
-                     const float *E, float beta, float *SG, hipStream_t st);
+struct AdjFace {          /* one CFS-PML face of the adjoint */
+  int on, dim;
+  int i1, i2, j1, j2, k1, k2, an, bn;
+  long n;                 /* slab points */
+  const float *A, *B, *D; /* profiles along the face's axis */
+  float *lev;             /* auxiliary adjoint levels E, A, B, P (by index), [9][n] each */
+  float *dsg;             /* extra S and G of the face, [12][n] */
+};
+struct AdjPml {
+  int on;
+  int le, lp;             /* level indices of E and P in every face's lev (A = 1, B = 2) */
+  AdjFace f[6];           /* dim-major, side-minor */
+};
+void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
+                     float alpha, const float *E, float beta, float *SG, const AdjPml &pm, hipStream_t st);
 void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
-                      const float *E, float *A, float *B, float *P, hipStream_t st);
+                      const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st);
 void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int n, long sicmp, float *P,
                        hipStream_t st);
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,

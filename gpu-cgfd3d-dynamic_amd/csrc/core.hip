@@ -143,6 +143,7 @@ struct cgfd_core {
   /* discrete adjoint: 4 levels of 9 fields (E, A, B, P) and the 12 fields
      of the adjoint gradient terms; allocated by the first backward call */
   float *adj = nullptr, *adj_sg = nullptr;
+  float *adj_pml = nullptr;  /* per CFS-PML face: 4 adjoint levels [9][n] + extra S, G [12][n] */
 
   float *wlev(int l) { return wave + (long)l * 9 * g.sicmp; }
   float *flev(int l) { return fwav + (long)l * nfault * 18 * g.sslice; }
@@ -886,8 +887,8 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
   return CGFD_OK;
 }
 
-/* ---- discrete adjoint of the volume update (no PML, free surface or
-   fault): gradient of J = sum_n sum_s r_n(s) . w_n(s) with respect to the
+/* ---- discrete adjoint of the volume update with CFS-PML (no free surface
+   or fault): gradient of J = sum_n sum_s r_n(s) . w_n(s) with respect to the
    state at the first recorded step, for the residuals r of the recorded
    station traces; the gradient of a least-squares trace misfit is this with
    r = recorded - target. Reverse sweep over the recorded steps, one
@@ -895,9 +896,9 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
 extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
   if (!c || !residual || !grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
-  if (c->nfault > 0 || c->pml_on || c->isfree)
+  if (c->nfault > 0 || c->isfree)
     return cgfd_set_error(CGFD_ERR_ARG,
-                          "adjoint: this build transposes the interior update only (no fault, CFS-PML or free surface)");
+                          "adjoint: this build transposes the interior update and CFS-PML only (no fault or free surface)");
   if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "adjoint: no stations set");
   if (nt < 1 || nt > c->st_nt) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: nt %d outside 1..%d", nt, c->st_nt);
   HIPCHK(hipSetDevice(c->dev));
@@ -905,12 +906,39 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
   const Grid &g = c->g;
   const long sc = g.sicmp, lv = 9 * sc;
   int rc;
+  long npml = 0;  /* floats of the PML adjoint block */
+  for (int dim = 0; dim < 3; dim++)
+    for (int side = 0; side < 2; side++)
+      if (c->pml[dim][side].on) npml += 48 * c->pml[dim][side].n;
   if (!c->adj) {
     if ((rc = c->alloc(&c->adj, (size_t)4 * lv))) return rc;
     if ((rc = c->alloc(&c->adj_sg, (size_t)12 * sc))) return rc;
     HIPCHK(hipMemset(c->adj_sg, 0, (size_t)12 * sc * 4));  /* ghost entries stay zero */
+    if (npml > 0 && (rc = c->alloc(&c->adj_pml, (size_t)npml))) return rc;
   }
   HIPCHK(hipMemsetAsync(c->adj, 0, (size_t)4 * lv * 4, c->st));
+  AdjPml pm{};
+  pm.on = c->pml_on;
+  pm.le = 0;
+  pm.lp = 3;
+  {
+    long off = 0;
+    for (int dim = 0; dim < 3; dim++)
+      for (int side = 0; side < 2; side++) {
+        const PmlFace &f = c->pml[dim][side];
+        AdjFace &a = pm.f[2 * dim + side];
+        a.on = f.on;
+        a.dim = dim;
+        if (!f.on) continue;
+        a.i1 = f.i1; a.i2 = f.i2; a.j1 = f.j1; a.j2 = f.j2; a.k1 = f.k1; a.k2 = f.k2;
+        a.an = f.an; a.bn = f.bn; a.n = f.n;
+        a.A = f.A; a.B = f.B; a.D = f.D;
+        a.lev = c->adj_pml + off;
+        a.dsg = c->adj_pml + off + 36 * f.n;
+        HIPCHK(hipMemsetAsync(a.lev, 0, (size_t)36 * f.n * 4, c->st));
+        off += 48 * f.n;
+      }
+  }
   float *res = nullptr;
   const size_t nres = (size_t)c->st_n * nt * 9;
   HIPCHK(hipMalloc(&res, nres * 4));
@@ -924,12 +952,13 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
         const int idir = (kPairs[ipair][0] + s) % 2, jdir = (kPairs[ipair][1] + s) % 2, kdir = (kPairs[ipair][2] + s) % 2;
         const float cb = kRkB[s] * c->dt, ca = s < 3 ? kRkA[s] * c->dt : 0.0f;
         const float *X = s == 3 ? nullptr : (s == 1 ? B : A);
-        launch_adj_prep(g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, c->st);
-        launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, c->st);
+        launch_adj_prep(s, g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, pm, c->st);
+        launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, pm, c->st);
       }
     }
     launch_adj_inject(c->st_idx, c->st_n, res, nt, n, sc, P, c->st);
     std::swap(E, P);
+    std::swap(pm.le, pm.lp);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);

@@ -1,6 +1,6 @@
 /*
  * k_adjoint.hip -- discrete adjoint of the RK4 volume update (interior
- * stencil, no CFS-PML, no free surface, no fault).
+ * stencil and CFS-PML; no free surface, no fault).
  *
  * One forward stage computes, at every physical point p,
  *   h(p) = H(p) . [D_d f(p)],  D_d f(p) = sum_t c_t f(p + (o_d + t) e_d)
@@ -21,6 +21,14 @@
  * ghost), and k_adj_stage gathers the transposed operators with the metric
  * at each tap and does the pointwise RK-adjoint accumulations.
  *
+ * CFS-PML (sv_curv_col_el_iso_gpu.cu:916-1288): at a point of face (d, side)
+ * the forward adds (B-1) r_d - B a to the rhs and advances the auxiliary a
+ * with rhs D r_d - A a, r_d being the part of the rhs from the derivatives
+ * along d. Its transpose: the adjoint of r_d gains (B-1) c + D c_a, kept per
+ * face as extra S and G (added by the gather at taps inside the face), and
+ * the auxiliary's stage adjoint is -B c - A c_a, pointwise, with the same
+ * reverse RK structure as the wavefield (c_a: the auxiliary rhs adjoint).
+ *
  * RK4 (drv_rk_curv_col.cu:329-568) with a = {1/2, 1/2, 1}, b = {1/6, 1/3,
  * 1/3, 1/6}, ca = a dt, cb = b dt, reversed (E = adjoint of the step's
  * output, P = adjoint of its input):
@@ -35,22 +43,9 @@
 
 namespace cgfd {
 
-/* S (3) and G (9) from c = alpha X + beta E at physical points; X may be null */
-__global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restrict__ lam, const float *__restrict__ mu,
-                                                  const float *__restrict__ slw, const float *__restrict__ X,
-                                                  float alpha, const float *__restrict__ E, float beta,
-                                                  float *__restrict__ SG) {
-  const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
-  const int k = (int)blockIdx.z + 3;
-  if (i >= g.ni + 3 || j >= g.nj + 3) return;
-  const long sc = g.sicmp;
-  const long p = i + j * g.siy + (long)k * g.siz;
-  float c[9];
-#pragma unroll
-  for (int f = 0; f < 9; f++) c[f] = beta * E[f * sc + p] + (X ? alpha * X[f * sc + p] : 0.0f);
-  const float l = lam[p], m = mu[p], s = slw[p];
+/* S (3) and G (9) of an adjoint rhs c with the media at the point */
+DEVI void sg_of(const float *c, float l, float m, float s, float *o) {
   const float l2m = l + 2.0f * m;
-  float o[12];
   o[0] = s * c[VX];
   o[1] = s * c[VY];
   o[2] = s * c[VZ];
@@ -63,8 +58,64 @@ __global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restric
   o[9] = m * c[TXZ];                              /* gamma(Vz, x) */
   o[10] = m * c[TYZ];                             /* gamma(Vz, y) */
   o[11] = l * c[TXX] + l * c[TYY] + l2m * c[TZZ]; /* gamma(Vz, z) */
+}
+
+DEVI bool in_face(const AdjFace &F, int i, int j, int k) {
+  return F.on && i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2;
+}
+DEVI long face_idx(const AdjFace &F, int i, int j, int k) {
+  return (long)(k - F.k1) * ((long)F.bn * F.an) + (long)(j - F.j1) * F.an + (i - F.i1);
+}
+
+/* S and G from c = alpha X + beta E at physical points (X may be null);
+   at CFS-PML points also the per-face extra S and G and the auxiliaries'
+   reverse stage (MODE as k_adj_stage) */
+template <int MODE>
+__global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restrict__ lam, const float *__restrict__ mu,
+                                                  const float *__restrict__ slw, const float *__restrict__ X,
+                                                  float alpha, const float *__restrict__ E, float beta,
+                                                  float *__restrict__ SG, AdjPml pm) {
+  const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
+  const int k = (int)blockIdx.z + 3;
+  if (i >= g.ni + 3 || j >= g.nj + 3) return;
+  const long sc = g.sicmp;
+  const long p = i + j * g.siy + (long)k * g.siz;
+  float c[9];
+#pragma unroll
+  for (int f = 0; f < 9; f++) c[f] = beta * E[f * sc + p] + (X ? alpha * X[f * sc + p] : 0.0f);
+  const float l = lam[p], m = mu[p], s = slw[p];
+  float o[12];
+  sg_of(c, l, m, s, o);
 #pragma unroll
   for (int q = 0; q < 12; q++) SG[q * sc + p] = o[q];
+  if (!pm.on) return;
+#pragma unroll 1
+  for (int fc = 0; fc < 6; fc++) {
+    const AdjFace &F = pm.f[fc];
+    if (!in_face(F, i, j, k)) continue;
+    const long ia = face_idx(F, i, j, k), n = F.n;
+    const int ic = F.dim == 0 ? i - F.i1 : (F.dim == 1 ? j - F.j1 : k - F.k1);
+    const float cA = F.A[ic], cB = F.B[ic], cD = F.D[ic];
+    const float *Ea = F.lev + (long)pm.le * 9 * n, *Aa = F.lev + 9 * n, *Ba = F.lev + 18 * n;
+    float *Pa = F.lev + (long)pm.lp * 9 * n;
+    float *Aw = F.lev + 9 * n, *Bw = F.lev + 18 * n;
+    const float *Xa = MODE == 3 ? nullptr : (MODE == 1 ? Ba : Aa);
+    float dc[9];
+#pragma unroll
+    for (int f = 0; f < 9; f++) {
+      const long o2 = f * n + ia;
+      const float ca_ = beta * Ea[o2] + (Xa ? alpha * Xa[o2] : 0.0f);  /* auxiliary rhs adjoint */
+      dc[f] = (cB - 1.0f) * c[f] + cD * ca_;
+      const float la = -cB * c[f] - cA * ca_;                          /* auxiliary stage adjoint */
+      if (MODE == 3) Aw[o2] = la;
+      else if (MODE == 2) { Bw[o2] = la; Pa[o2] = Ea[o2] + Aa[o2]; }
+      else if (MODE == 1) { const float b = Ba[o2]; Aw[o2] = la; Pa[o2] += b; }
+      else Pa[o2] += Aa[o2] + la;
+    }
+    sg_of(dc, l, m, s, o);
+#pragma unroll
+    for (int q = 0; q < 12; q++) F.dsg[q * n + ia] = o[q];
+  }
 }
 
 template <int DIR>
@@ -88,11 +139,22 @@ DEVI void add_kappa(float *lm, float w, const float *__restrict__ SG, long sc, l
   lm[TYZ] += w * (M1 * Sz + M2 * Sy);
 }
 
+/* the per-face extra S and G of the faces along `dim` at tap (ti, tj, tk) */
+DEVI void add_face_kappa(float *lm, float w, const AdjPml &pm, int dim, int ti, int tj, int tk, float M0, float M1,
+                         float M2) {
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const AdjFace &F = pm.f[2 * dim + side];
+    if (!in_face(F, ti, tj, tk)) continue;
+    add_kappa(lm, w, F.dsg, F.n, face_idx(F, ti, tj, tk), M0, M1, M2);
+  }
+}
+
 /* MODE 3: A = L^T;  2: B = L^T, P = E + A;  1: A = L^T, P += B;  0: P += A + L^T */
 template <int ID, int JD, int KD, int MODE>
 __global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restrict__ met,
                                                    const float *__restrict__ SG, const float *__restrict__ E,
-                                                   float *A, float *B, float *P) {
+                                                   float *A, float *B, float *P, AdjPml pm) {
   const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
   const int k = (int)blockIdx.z + 3;
   if (i >= g.ni + 3 || j >= g.nj + 3) return;
@@ -105,21 +167,27 @@ __global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restri
 #pragma unroll
   for (int t = 0; t < 5; t++) {
     const long p = q - (ox + t);
-    add_kappa(lm, tap_coef<ID>(t), SG, sc, p, met[sc + p], met[2 * sc + p], met[3 * sc + p]);
+    const float M0 = met[sc + p], M1 = met[2 * sc + p], M2 = met[3 * sc + p];
+    add_kappa(lm, tap_coef<ID>(t), SG, sc, p, M0, M1, M2);
+    if (pm.on) add_face_kappa(lm, tap_coef<ID>(t), pm, 0, i - (ox + t), j, k, M0, M1, M2);
   }
   __asm__ volatile("" : "+v"(lm[0]), "+v"(lm[1]), "+v"(lm[2]), "+v"(lm[3]), "+v"(lm[4]), "+v"(lm[5]),
                    "+v"(lm[6]), "+v"(lm[7]), "+v"(lm[8]) : : "memory");
 #pragma unroll
   for (int t = 0; t < 5; t++) {
     const long p = q - (long)(oy + t) * sy;
-    add_kappa(lm, tap_coef<JD>(t), SG, sc, p, met[4 * sc + p], met[5 * sc + p], met[6 * sc + p]);
+    const float M0 = met[4 * sc + p], M1 = met[5 * sc + p], M2 = met[6 * sc + p];
+    add_kappa(lm, tap_coef<JD>(t), SG, sc, p, M0, M1, M2);
+    if (pm.on) add_face_kappa(lm, tap_coef<JD>(t), pm, 1, i, j - (oy + t), k, M0, M1, M2);
   }
   __asm__ volatile("" : "+v"(lm[0]), "+v"(lm[1]), "+v"(lm[2]), "+v"(lm[3]), "+v"(lm[4]), "+v"(lm[5]),
                    "+v"(lm[6]), "+v"(lm[7]), "+v"(lm[8]) : : "memory");
 #pragma unroll
   for (int t = 0; t < 5; t++) {
     const long p = q - (long)(oz + t) * sz;
-    add_kappa(lm, tap_coef<KD>(t), SG, sc, p, met[7 * sc + p], met[8 * sc + p], met[9 * sc + p]);
+    const float M0 = met[7 * sc + p], M1 = met[8 * sc + p], M2 = met[9 * sc + p];
+    add_kappa(lm, tap_coef<KD>(t), SG, sc, p, M0, M1, M2);
+    if (pm.on) add_face_kappa(lm, tap_coef<KD>(t), pm, 2, i, j, k - (oz + t), M0, M1, M2);
   }
 #pragma unroll
   for (int f = 0; f < 9; f++) {
@@ -140,35 +208,40 @@ __global__ void k_adj_inject(const long *__restrict__ idx, int nsta, const float
   atomicAdd(P + c * sicmp + idx[s], res[((long)s * nt + n) * 9 + c]);
 }
 
-void launch_adj_prep(const Grid &g, const float *lam, const float *mu, const float *slw, const float *X, float alpha,
-                     const float *E, float beta, float *SG, hipStream_t st) {
+void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
+                     float alpha, const float *E, float beta, float *SG, const AdjPml &pm, hipStream_t st) {
   dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk);
-  hipLaunchKernelGGL(k_adj_prep, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG);
+  switch (mode) {
+    case 3: hipLaunchKernelGGL(k_adj_prep<3>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
+    case 2: hipLaunchKernelGGL(k_adj_prep<2>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
+    case 1: hipLaunchKernelGGL(k_adj_prep<1>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
+    default: hipLaunchKernelGGL(k_adj_prep<0>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
+  }
 }
 
 template <int ID, int JD, int KD>
 static void adj_stage_dir(int mode, const Grid &g, const float *met, const float *SG, const float *E, float *A,
-                          float *B, float *P, hipStream_t st) {
+                          float *B, float *P, const AdjPml &pm, hipStream_t st) {
   dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk);
   switch (mode) {
-    case 3: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 3>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
-    case 2: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 2>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
-    case 1: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 1>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
-    default: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 0>), grd, blk, 0, st, g, met, SG, E, A, B, P); break;
+    case 3: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 3>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    case 2: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 2>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    case 1: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 1>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    default: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 0>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
   }
 }
 
 void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
-                      const float *E, float *A, float *B, float *P, hipStream_t st) {
+                      const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st) {
   switch (idir * 4 + jdir * 2 + kdir) {
-    case 0: adj_stage_dir<0, 0, 0>(mode, g, met, SG, E, A, B, P, st); break;
-    case 1: adj_stage_dir<0, 0, 1>(mode, g, met, SG, E, A, B, P, st); break;
-    case 2: adj_stage_dir<0, 1, 0>(mode, g, met, SG, E, A, B, P, st); break;
-    case 3: adj_stage_dir<0, 1, 1>(mode, g, met, SG, E, A, B, P, st); break;
-    case 4: adj_stage_dir<1, 0, 0>(mode, g, met, SG, E, A, B, P, st); break;
-    case 5: adj_stage_dir<1, 0, 1>(mode, g, met, SG, E, A, B, P, st); break;
-    case 6: adj_stage_dir<1, 1, 0>(mode, g, met, SG, E, A, B, P, st); break;
-    default: adj_stage_dir<1, 1, 1>(mode, g, met, SG, E, A, B, P, st); break;
+    case 0: adj_stage_dir<0, 0, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 1: adj_stage_dir<0, 0, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 2: adj_stage_dir<0, 1, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 3: adj_stage_dir<0, 1, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 4: adj_stage_dir<1, 0, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 5: adj_stage_dir<1, 0, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 6: adj_stage_dir<1, 1, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    default: adj_stage_dir<1, 1, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
   }
 }
 

@@ -169,10 +169,10 @@ int cgfd_core_save_state(cgfd_core *core, void *dst, int dst_on_device);
 int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
 
 /* Discrete adjoint of the time loop, first part: the interior velocity-stress
-   update and RK4 (cores without fault, CFS-PML or free surface; other cores
+   update, CFS-PML and RK4 (cores without fault or free surface; other cores
    get CGFD_ERR_ARG). The reference has no adjoint; this is the transpose of
-   the discrete operator of sv_curv_col_el_iso_gpu.cu:231-388 composed with
-   the RK4 stages of drv_rk_curv_col.cu:329-568.
+   the discrete operators of sv_curv_col_el_iso_gpu.cu:231-388 and :757-1290
+   composed with the RK4 stages of drv_rk_curv_col.cu:329-568.
    With stations set before the forward run (cgfd_core_set_stations) and the
    core advanced over the recorded steps, residual[nsta][nt][9] (the layout
    of cgfd_core_read_stations) defines J = sum_n sum_s residual[s][n] .
