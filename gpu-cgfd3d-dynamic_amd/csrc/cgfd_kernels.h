@@ -125,7 +125,11 @@ struct AdjPml {
   AdjFace f[6];           /* dim-major, side-minor */
 };
 void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
-                     float alpha, const float *E, float beta, float *SG, const AdjPml &pm, hipStream_t st);
+                     float alpha, const float *E, float beta, float *SG, const AdjPml &pm, int isfree, int kdir,
+                     hipStream_t st);
+void launch_adj_top(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *lam,
+                    const float *mu, const float *slw, const float *mx, const float *my, const float *X, float alpha,
+                    const float *E, float beta, float *T, const AdjPml &pm, hipStream_t st);
 void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
                       const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st);
 void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int n, long sicmp, float *P,

@@ -887,8 +887,8 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
   return CGFD_OK;
 }
 
-/* ---- discrete adjoint of the volume update with CFS-PML (no free surface
-   or fault): gradient of J = sum_n sum_s r_n(s) . w_n(s) with respect to the
+/* ---- discrete adjoint of the wave update (interior, CFS-PML, free
+   surface; no fault): gradient of J = sum_n sum_s r_n(s) . w_n(s) with respect to the
    state at the first recorded step, for the residuals r of the recorded
    station traces; the gradient of a least-squares trace misfit is this with
    r = recorded - target. Reverse sweep over the recorded steps, one
@@ -896,9 +896,9 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
 extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
   if (!c || !residual || !grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
-  if (c->nfault > 0 || c->isfree)
+  if (c->nfault > 0)
     return cgfd_set_error(CGFD_ERR_ARG,
-                          "adjoint: this build transposes the interior update and CFS-PML only (no fault or free surface)");
+                          "adjoint: this build transposes the wave update only (interior, CFS-PML, free surface; no fault)");
   if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "adjoint: no stations set");
   if (nt < 1 || nt > c->st_nt) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: nt %d outside 1..%d", nt, c->st_nt);
   HIPCHK(hipSetDevice(c->dev));
@@ -952,8 +952,13 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
         const int idir = (kPairs[ipair][0] + s) % 2, jdir = (kPairs[ipair][1] + s) % 2, kdir = (kPairs[ipair][2] + s) % 2;
         const float cb = kRkB[s] * c->dt, ca = s < 3 ? kRkA[s] * c->dt : 0.0f;
         const float *X = s == 3 ? nullptr : (s == 1 ? B : A);
-        launch_adj_prep(s, g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, pm, c->st);
+        launch_adj_prep(s, g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, pm, c->isfree, kdir, c->st);
         launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, pm, c->st);
+        if (c->isfree) {
+          float *T = (s == 3 || s == 1) ? A : (s == 2 ? B : P);  /* the level the stage wrote L^T into */
+          launch_adj_top(s, idir, jdir, kdir, g, c->met, c->lam, c->mu, c->slw, c->matVx2Vz, c->matVy2Vz, X, ca, E,
+                         cb, T, pm, c->st);
+        }
       }
     }
     launch_adj_inject(c->st_idx, c->st_n, res, nt, n, sc, P, c->st);

@@ -1,6 +1,6 @@
 /*
  * k_adjoint.hip -- discrete adjoint of the RK4 volume update (interior
- * stencil and CFS-PML; no free surface, no fault).
+ * stencil, CFS-PML, free surface; no fault).
  *
  * One forward stage computes, at every physical point p,
  *   h(p) = H(p) . [D_d f(p)],  D_d f(p) = sum_t c_t f(p + (o_d + t) e_d)
@@ -28,6 +28,14 @@
  * face as extra S and G (added by the gather at taps inside the face), and
  * the auxiliary's stage adjoint is -B c - A c_a, pointwise, with the same
  * reverse RK structure as the wavefield (c_a: the auxiliary rhs adjoint).
+ *
+ * Free surface (sv_curv_col_el_iso_gpu.cu:398-747, top four layers): the
+ * forward replaces the velocity rhs by the traction-image form (timg) and
+ * the stress rhs by Hooke's law on the reduced-order velocity gradient
+ * (vlow), and the x/y PML faces add a term through the free-surface
+ * matrices at the surface. Transposed: the prep leaves out the base S / G of
+ * what was replaced, and k_adj_top scatters the adjoint of timg, vlow and
+ * the surface PML term onto the taps (atomic adds; ghosts are not state).
  *
  * RK4 (drv_rk_curv_col.cu:329-568) with a = {1/2, 1/2, 1}, b = {1/6, 1/3,
  * 1/3, 1/6}, ca = a dt, cb = b dt, reversed (E = adjoint of the step's
@@ -74,7 +82,7 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restrict__ lam, const float *__restrict__ mu,
                                                   const float *__restrict__ slw, const float *__restrict__ X,
                                                   float alpha, const float *__restrict__ E, float beta,
-                                                  float *__restrict__ SG, AdjPml pm) {
+                                                  float *__restrict__ SG, AdjPml pm, int isfree, int kdir) {
   const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
   const int k = (int)blockIdx.z + 3;
   if (i >= g.ni + 3 || j >= g.nj + 3) return;
@@ -86,6 +94,13 @@ __global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restric
   const float l = lam[p], m = mu[p], s = slw[p];
   float o[12];
   sg_of(c, l, m, s, o);
+  if (isfree) {
+    /* what timg (velocity rhs) and vlow (stress rhs) replace at the top */
+    if (k >= g.nk2 - (kdir ? 3 : 1)) o[0] = o[1] = o[2] = 0.0f;
+    if (k >= g.nk2 - 2)
+#pragma unroll
+      for (int q = 3; q < 12; q++) o[q] = 0.0f;
+  }
 #pragma unroll
   for (int q = 0; q < 12; q++) SG[q * sc + p] = o[q];
   if (!pm.on) return;
@@ -199,6 +214,217 @@ __global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restri
   }
 }
 
+/* ------------------------------------------------------------------------
+ * Adjoint of the free-surface replacements at the top four layers, added to
+ * the stage's L^T output T (the level the stage kernel just wrote: A for
+ * stages 3 and 1, B for stage 2, P for stage 0). One thread per point.
+ * ---------------------------------------------------------------------- */
+DEVI bool phys(const Grid &g, int i, int j, int k) {
+  return i >= 3 && i < g.ni + 3 && j >= 3 && j < g.nj + 3 && k >= 3 && k <= g.nk2;
+}
+
+template <int ID, int JD, int KD, int MODE>
+__global__ void __launch_bounds__(256) k_adj_top(Grid g, const float *__restrict__ met, const float *__restrict__ lam,
+                                                 const float *__restrict__ mu, const float *__restrict__ slw,
+                                                 const float *__restrict__ mx, const float *__restrict__ my,
+                                                 const float *__restrict__ X, float alpha,
+                                                 const float *__restrict__ E, float beta, float *T, AdjPml pm) {
+  const int i = blockIdx.x * 64 + (int)threadIdx.x + 3, j = blockIdx.y * 4 + (int)threadIdx.y + 3;
+  const int nk2 = g.nk2;
+  const int k = nk2 - (int)blockIdx.z;  /* planes nk2, nk2-1, nk2-2, nk2-3 */
+  if (i >= g.ni + 3 || j >= g.nj + 3 || k < 3) return;
+  const long sc = g.sicmp, sy = g.siy, sz = g.siz;
+  const long p = i + j * sy + (long)k * sz;
+  float c[9];
+#pragma unroll
+  for (int f = 0; f < 9; f++) c[f] = beta * E[f * sc + p] + (X ? alpha * X[f * sc + p] : 0.0f);
+  const float l = lam[p], m = mu[p], s = slw[p];
+  constexpr int fxo = ID ? -1 : -3, fyo = JD ? -1 : -3, fzo = KD ? -1 : -3;
+  const int tf[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
+
+  if (k >= nk2 - (fzo + 4)) {
+    /* timg: h_c = s/J(p) sum_taps coef J(q) (xi_d(q) . T_c(q)), odd image above the surface */
+    const int n_free = nk2 - k - fzo;
+    const float sj = s / met[p];
+#pragma unroll 1
+    for (int cc = 0; cc < 3; cc++) {
+      const float w = c[cc] * sj;
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        {
+          const int ti = i + fxo + n;
+          if (phys(g, ti, j, k)) {
+            const long q = p + fxo + n;
+            const float u = w * tap_coef<ID>(n) * met[q];
+#pragma unroll
+            for (int a = 0; a < 3; a++) atomicAdd(T + tf[cc][a] * sc + q, u * met[(1 + a) * sc + q]);
+          }
+        }
+        {
+          const int tj = j + fyo + n;
+          if (phys(g, i, tj, k)) {
+            const long q = p + (long)(fyo + n) * sy;
+            const float u = w * tap_coef<JD>(n) * met[q];
+#pragma unroll
+            for (int a = 0; a < 3; a++) atomicAdd(T + tf[cc][a] * sc + q, u * met[(4 + a) * sc + q]);
+          }
+        }
+        if (n == n_free) continue;
+        const int dz = n < n_free ? fzo + n : fzo + n - 2 * (n - n_free);
+        const float sign = n < n_free ? 1.0f : -1.0f;
+        if (phys(g, i, j, k + dz)) {
+          const long q = p + (long)dz * sz;
+          const float u = sign * w * tap_coef<KD>(n) * met[q];
+#pragma unroll
+          for (int a = 0; a < 3; a++) atomicAdd(T + tf[cc][a] * sc + q, u * met[(7 + a) * sc + q]);
+        }
+      }
+    }
+  }
+
+  if (k >= nk2 - 2) {
+    /* vlow: Hooke's law on dV with x/y one-sided operators and, in z, the
+       free-surface matrices (k = nk2), mac22 (nk2-1) or mac24 (nk2-2) */
+    const float l2m = (float)(l + 2.0 * m);
+    float gm[3][3];  /* adjoint of the physical gradient dV_c/dx_a */
+    gm[0][0] = l2m * c[TXX] + l * c[TYY] + l * c[TZZ];
+    gm[0][1] = m * c[TXY];
+    gm[0][2] = m * c[TXZ];
+    gm[1][0] = m * c[TXY];
+    gm[1][1] = l * c[TXX] + l2m * c[TYY] + l * c[TZZ];
+    gm[1][2] = m * c[TYZ];
+    gm[2][0] = m * c[TXZ];
+    gm[2][1] = m * c[TYZ];
+    gm[2][2] = l * c[TXX] + l * c[TYY] + l2m * c[TZZ];
+    float dl[3][3];  /* adjoint of D_d V_c */
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++)
+        dl[d][cc] = met[(1 + 3 * d) * sc + p] * gm[cc][0] + met[(2 + 3 * d) * sc + p] * gm[cc][1] +
+                    met[(3 + 3 * d) * sc + p] * gm[cc][2];
+    if (k == nk2) {
+      const long ij = ((long)i + (long)j * sy) * 9;
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) {
+        float ax = 0.0f, ay = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+          ax += mx[ij + 3 * r + cc] * dl[2][r];
+          ay += my[ij + 3 * r + cc] * dl[2][r];
+        }
+        dl[0][cc] += ax;
+        dl[1][cc] += ay;
+      }
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) {
+        float *Tc = T + cc * sc;
+        const float w = dl[2][cc];
+        if (k == nk2 - 1) {  /* mac22 */
+          if (KD) { atomicAdd(Tc + p + sz, w); atomicAdd(Tc + p, -w); }
+          else { atomicAdd(Tc + p, w); atomicAdd(Tc + p - sz, -w); }
+        } else {             /* mac24 */
+          const float b1 = (float)kB1, b2 = (float)kB2, b3 = (float)kB3;
+          if (KD) { atomicAdd(Tc + p, b1 * w); atomicAdd(Tc + p + sz, b2 * w); atomicAdd(Tc + p + 2 * sz, b3 * w); }
+          else { atomicAdd(Tc + p, -b1 * w); atomicAdd(Tc + p - sz, -b2 * w); atomicAdd(Tc + p - 2 * sz, -b3 * w); }
+        }
+      }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+      for (int t = 0; t < 5; t++) {
+        if (phys(g, i + fxo + t, j, k)) atomicAdd(T + cc * sc + p + fxo + t, tap_coef<ID>(t) * dl[0][cc]);
+        if (phys(g, i, j + fyo + t, k)) atomicAdd(T + cc * sc + p + (long)(fyo + t) * sy, tap_coef<JD>(t) * dl[1][cc]);
+      }
+  }
+
+  if (k == nk2 && pm.on) {
+    /* the free-surface term of the x/y PML faces: qT from D_dim V through
+       the matrices, added to the stress rhs with (B-1) and to the auxiliary
+       rhs with D (sv_curv_col_el_iso_gpu.cu:1000-1060, :1146-1205) */
+    const float l2m = l + 2.0f * m;
+    const float zx = met[7 * sc + p], zy = met[8 * sc + p], zz = met[9 * sc + p];
+#pragma unroll 1
+    for (int fc = 0; fc < 4; fc++) {
+      const AdjFace &F = pm.f[fc];
+      if (!in_face(F, i, j, k)) continue;
+      const long ia = face_idx(F, i, j, k), n = F.n;
+      const int ic = F.dim == 0 ? i - F.i1 : j - F.j1;
+      const float cB = F.B[ic], cD = F.D[ic];
+      const float *Ea = F.lev + (long)pm.le * 9 * n;
+      const float *Xa = MODE == 3 ? nullptr : (MODE == 1 ? F.lev + 18 * n : F.lev + 9 * n);
+      float w[9];
+#pragma unroll
+      for (int f = 3; f < 9; f++) {
+        const long o2 = f * n + ia;
+        const float ca_ = beta * Ea[o2] + (Xa ? alpha * Xa[o2] : 0.0f);
+        w[f] = (float)(cB - 1.0) * c[f] + cD * ca_;
+      }
+      /* qT -> q */
+      const float q0 = l2m * zx * w[TXX] + l * zx * w[TYY] + l * zx * w[TZZ] + m * zy * w[TXY] + m * zz * w[TXZ];
+      const float q1 = l * zy * w[TXX] + l2m * zy * w[TYY] + l * zy * w[TZZ] + m * zx * w[TXY] + m * zz * w[TYZ];
+      const float q2 = l * zz * w[TXX] + l * zz * w[TYY] + l2m * zz * w[TZZ] + m * zx * w[TXZ] + m * zy * w[TYZ];
+      const long ij = ((long)i + (long)j * sy) * 9;
+      const float *Xm = (F.dim == 0 ? mx : my) + ij;
+      float dv[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) dv[cc] = Xm[cc] * q0 + Xm[3 + cc] * q1 + Xm[6 + cc] * q2;
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++)
+#pragma unroll
+        for (int t = 0; t < 5; t++) {
+          if (F.dim == 0) {
+            if (phys(g, i + fxo + t, j, k)) atomicAdd(T + cc * sc + p + fxo + t, tap_coef<ID>(t) * dv[cc]);
+          } else {
+            if (phys(g, i, j + fyo + t, k)) atomicAdd(T + cc * sc + p + (long)(fyo + t) * sy, tap_coef<JD>(t) * dv[cc]);
+          }
+        }
+    }
+  }
+}
+
+template <int ID, int JD, int KD>
+static void adj_top_dir(int mode, const Grid &g, const float *met, const float *lam, const float *mu,
+                        const float *slw, const float *mx, const float *my, const float *X, float alpha,
+                        const float *E, float beta, float *T, const AdjPml &pm, hipStream_t st) {
+  dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk < 4 ? g.nk : 4);
+  switch (mode) {
+    case 3:
+      hipLaunchKernelGGL((k_adj_top<ID, JD, KD, 3>), grd, blk, 0, st, g, met, lam, mu, slw, mx, my, X, alpha, E,
+                         beta, T, pm);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_adj_top<ID, JD, KD, 2>), grd, blk, 0, st, g, met, lam, mu, slw, mx, my, X, alpha, E,
+                         beta, T, pm);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_adj_top<ID, JD, KD, 1>), grd, blk, 0, st, g, met, lam, mu, slw, mx, my, X, alpha, E,
+                         beta, T, pm);
+      break;
+    default:
+      hipLaunchKernelGGL((k_adj_top<ID, JD, KD, 0>), grd, blk, 0, st, g, met, lam, mu, slw, mx, my, X, alpha, E,
+                         beta, T, pm);
+      break;
+  }
+}
+
+void launch_adj_top(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *lam,
+                    const float *mu, const float *slw, const float *mx, const float *my, const float *X, float alpha,
+                    const float *E, float beta, float *T, const AdjPml &pm, hipStream_t st) {
+  switch (idir * 4 + jdir * 2 + kdir) {
+    case 0: adj_top_dir<0, 0, 0>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    case 1: adj_top_dir<0, 0, 1>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    case 2: adj_top_dir<0, 1, 0>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    case 3: adj_top_dir<0, 1, 1>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    case 4: adj_top_dir<1, 0, 0>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    case 5: adj_top_dir<1, 0, 1>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    case 6: adj_top_dir<1, 1, 0>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+    default: adj_top_dir<1, 1, 1>(mode, g, met, lam, mu, slw, mx, my, X, alpha, E, beta, T, pm, st); break;
+  }
+}
+
 /* P += residual row n at the stations (all nine components) */
 __global__ void k_adj_inject(const long *__restrict__ idx, int nsta, const float *__restrict__ res, int nt, int n,
                              long sicmp, float *P) {
@@ -209,13 +435,22 @@ __global__ void k_adj_inject(const long *__restrict__ idx, int nsta, const float
 }
 
 void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
-                     float alpha, const float *E, float beta, float *SG, const AdjPml &pm, hipStream_t st) {
+                     float alpha, const float *E, float beta, float *SG, const AdjPml &pm, int isfree, int kdir,
+                     hipStream_t st) {
   dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk);
   switch (mode) {
-    case 3: hipLaunchKernelGGL(k_adj_prep<3>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
-    case 2: hipLaunchKernelGGL(k_adj_prep<2>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
-    case 1: hipLaunchKernelGGL(k_adj_prep<1>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
-    default: hipLaunchKernelGGL(k_adj_prep<0>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm); break;
+    case 3:
+      hipLaunchKernelGGL(k_adj_prep<3>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm, isfree, kdir);
+      break;
+    case 2:
+      hipLaunchKernelGGL(k_adj_prep<2>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm, isfree, kdir);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_adj_prep<1>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm, isfree, kdir);
+      break;
+    default:
+      hipLaunchKernelGGL(k_adj_prep<0>, grd, blk, 0, st, g, lam, mu, slw, X, alpha, E, beta, SG, pm, isfree, kdir);
+      break;
   }
 }
 

@@ -168,11 +168,12 @@ int cgfd_core_state_bytes(cgfd_core *core, size_t *bytes);
 int cgfd_core_save_state(cgfd_core *core, void *dst, int dst_on_device);
 int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
 
-/* Discrete adjoint of the time loop, first part: the interior velocity-stress
-   update, CFS-PML and RK4 (cores without fault or free surface; other cores
-   get CGFD_ERR_ARG). The reference has no adjoint; this is the transpose of
-   the discrete operators of sv_curv_col_el_iso_gpu.cu:231-388 and :757-1290
-   composed with the RK4 stages of drv_rk_curv_col.cu:329-568.
+/* Discrete adjoint of the time loop, first part: the wave update -- the
+   interior velocity-stress stencil, CFS-PML, the free-surface traction image
+   and reduced-order velocity gradient, and RK4 (cores without a fault;
+   cores with one get CGFD_ERR_ARG). The reference has no adjoint; this is
+   the transpose of the discrete operators of sv_curv_col_el_iso_gpu.cu:
+   231-1290 composed with the RK4 stages of drv_rk_curv_col.cu:329-568.
    With stations set before the forward run (cgfd_core_set_stations) and the
    core advanced over the recorded steps, residual[nsta][nt][9] (the layout
    of cgfd_core_read_stations) defines J = sum_n sum_s residual[s][n] .

@@ -1,6 +1,7 @@
-"""Discrete adjoint of the interior velocity-stress update, CFS-PML and RK4
-(cores without fault or free surface), through the C ABI, with and without
-CFS-PML on all six faces (edges and corners included).
+"""Discrete adjoint of the wave update (interior velocity-stress stencil,
+CFS-PML, free surface, RK4; cores without a fault) through the C ABI: no
+boundary layer, CFS-PML on all six faces (edges and corners included), and
+a free surface on top of CFS-PML on the other five faces.
 
 * Linear identity: the station traces are a linear map T of the initial
   wavefield, so for any residual r the adjoint g = T^T r satisfies
@@ -21,12 +22,15 @@ pytestmark = pytest.mark.gpu
 NT = 24
 
 
-def _scenario(pml=0):
+CASES = [(0, False), (5, False), (5, True)]  # (PML layers, free surface)
+
+
+def _scenario(pml=0, free=False):
     from cgfd import configs
-    sc = configs.small(imethod=2)
+    sc = configs.small(imethod=2, curvilinear=free)
     sc.faults = []
-    sc.pml_layers = [[pml, pml], [pml, pml], [pml, pml]]
-    sc.free_top = False
+    sc.pml_layers = [[pml, pml], [pml, pml], [pml, 0 if free else pml]]
+    sc.free_top = free
     rng = np.random.default_rng(41)
     extra = np.stack([rng.integers(2, sc.ni - 2, 6), rng.integers(2, sc.nj - 2, 6), rng.integers(2, sc.nk - 2, 6)], 1)
     sc.stations = np.concatenate([sc.stations, extra]).astype(np.int32)
@@ -57,10 +61,10 @@ def _traces_gpu(st, sc, w0):
     return core, tr
 
 
-@pytest.mark.parametrize("pml", [0, 5])
-def test_adjoint_linear_identity(built_libs, pml):
+@pytest.mark.parametrize("pml,free", CASES)
+def test_adjoint_linear_identity(built_libs, pml, free):
     from cgfd import model
-    sc = _scenario(pml)
+    sc = _scenario(pml, free)
     st = model.Setup(sc)
     w0 = _smooth_field(sc, 5)
     core, tr = _traces_gpu(st, sc, w0)
@@ -70,7 +74,7 @@ def test_adjoint_linear_identity(built_libs, pml):
     g = core.adjoint_initial(r)
     lhs = float(np.sum(g.astype(np.float64) * w0.astype(np.float64)))
     rhs = float(np.sum(r.astype(np.float64) * tr.astype(np.float64)))
-    print(f"[adjoint pml={pml}] <g, w0> = {lhs:.9e}, sum r.T(w0) = {rhs:.9e}", flush=True)
+    print(f"[adjoint pml={pml} free={free}] <g, w0> = {lhs:.9e}, sum r.T(w0) = {rhs:.9e}", flush=True)
     assert np.isfinite(g).all()
     assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
     # ghosts carry no gradient
@@ -79,11 +83,11 @@ def test_adjoint_linear_identity(built_libs, pml):
     st.close()
 
 
-@pytest.mark.parametrize("pml", [0, 5])
-def test_adjoint_least_squares_vs_fp64_central_differences(built_libs, pml):
+@pytest.mark.parametrize("pml,free", CASES)
+def test_adjoint_least_squares_vs_fp64_central_differences(built_libs, pml, free):
     from cgfd import model
     import oracle
-    sc = _scenario(pml)
+    sc = _scenario(pml, free)
     st = model.Setup(sc)
     w0 = _smooth_field(sc, 6)
     target = _smooth_field(sc, 7, amp=0.5)
@@ -112,7 +116,7 @@ def test_adjoint_least_squares_vs_fp64_central_differences(built_libs, pml):
         ad = float(np.sum(g * d))
         rel = abs(ad - fd) / abs(fd)
         worst = max(worst, rel)
-        print(f"[adjoint pml={pml}] direction {d_seed}: adjoint {ad:.8e}, FP64 central difference {fd:.8e}, rel {rel:.2e}",
+        print(f"[adjoint pml={pml} free={free}] direction {d_seed}: adjoint {ad:.8e}, FP64 central difference {fd:.8e}, rel {rel:.2e}",
               flush=True)
     assert worst <= 1e-3
     core.close()
@@ -121,7 +125,7 @@ def test_adjoint_least_squares_vs_fp64_central_differences(built_libs, pml):
 
 def test_adjoint_refuses_unsupported_cores(built_libs):
     from cgfd import configs, model
-    sc = configs.small(imethod=2)  # fault + free surface
+    sc = configs.small(imethod=2)  # with a fault
     st = model.Setup(sc)
     core = model.Core(st)
     core.set_stations(sc.stations + 3, 4)
@@ -131,6 +135,6 @@ def test_adjoint_refuses_unsupported_cores(built_libs):
     out = np.zeros((9, nz, ny, nx), np.float32)
     rc = core.lib.cgfd_adj_backward(core.handle, r.ctypes.data_as(C.c_void_p), 4, out.ctypes.data_as(C.c_void_p))
     assert rc == -1
-    assert b"no fault or free surface" in core.lib.cgfd_last_error()
+    assert b"no fault" in core.lib.cgfd_last_error()
     core.close()
     st.close()
