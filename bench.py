@@ -129,6 +129,44 @@ def batched_leg(steps: int, nscen: int, variant: int, overlap: bool = False):
             "note": "all scenarios active in the timed steps; retirement is exercised by the GPU tests"}
 
 
+def adjoint_leg(steps: int):
+    """Adjoint/forward time of the wave update on the differentiation grid
+    (160 x 240 x 100 nodes at 100 m, homogeneous half-space with 3% noise,
+    free surface, PML 12, 32 surface stations) without the fault: the
+    adjoint in this build transposes the stencil, CFS-PML and free surface,
+    not yet the fault. Forward: `steps` steps recording the stations; adjoint:
+    the gradient of 1/2 |traces|^2 with respect to the initial wavefield over
+    the same steps."""
+    import numpy as np
+    from cgfd import configs, model
+    sc = configs.tpv_like(ni=160, nj=240, nk=100, dh=100.0, pml=12, fault_len_km=12.0, fault_dep_km=6.0,
+                          nstations=32, seed=4004, name="config4-wave")
+    sc.faults = []
+    rng = np.random.default_rng(4004)
+    sc.mu = (sc.mu * (1.0 + 0.03 * rng.standard_normal(sc.mu.shape))).astype(np.float32)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    core.set_stations(sc.stations + 3, steps)
+    nz, ny, nx = sc.shape_ext
+    w0 = np.zeros((9, nz, ny, nx), np.float32)
+    w0[:3, 3:-3, 3:-3, 3:-3] = rng.standard_normal((3, sc.nk, sc.nj, sc.ni)).astype(np.float32)
+    core.write_wavefield(w0)
+    core.sync()
+    t = time.perf_counter()
+    core.run(steps)
+    core.sync()
+    t_fwd = time.perf_counter() - t
+    tr = core.read_stations()
+    t = time.perf_counter()
+    g = core.adjoint_initial(tr)
+    t_adj = time.perf_counter() - t
+    core.close()
+    st.close()
+    return {"value": round(t_adj / t_fwd, 3), "scope": "wave update (interior, CFS-PML, free surface); no fault",
+            "grid": [sc.ni, sc.nj, sc.nk], "steps": steps, "forward_ms_per_step": round(t_fwd / steps * 1e3, 3),
+            "adjoint_ms_per_step": round(t_adj / steps * 1e3, 3), "finite": bool(np.isfinite(g).all())}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +182,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (profiling)")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--batched-steps", type=int, default=16)
     ap.add_argument("--nscen", type=int, default=32)
     ap.add_argument("--batched-variant", type=int, default=19)
@@ -238,6 +278,10 @@ def main() -> int:
         log("[bench] batched ensemble ...")
         out["batched"] = batched_leg(args.batched_steps, args.nscen, args.batched_variant, args.overlap)
     out["adjoint_vs_forward"] = None
+    if not args.no_adjoint:
+        log("[bench] adjoint ...")
+        out["adjoint"] = adjoint_leg(args.adjoint_steps)
+        out["adjoint_vs_forward"] = out["adjoint"]["value"]
     if rank == 0 and not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
