@@ -866,19 +866,17 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
   }
   if (b.nact > 1 && variant >= 16 && variant != 18) {  /* batches, split kernels */
     long items, nb;
-    if (variant == 17) {
+    if (variant != 16) {  /* scenarios of a tile back to back, lean PML kernel */
       items = (long)agrd.x * agrd.y * a.nk_run * b.nact;
       nb = ((items + 7) / 8) * 8;
       hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 0>), dim3((unsigned)nb), blk, 0, st, a, b, items);
       if (a.pml_on)
-        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0, false>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0, true>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
     } else {
       const long nslab = ((long)agrd.y + a.zchunk - 1) / a.zchunk;
       nb = items = (long)agrd.x * a.zchunk * a.nk_run * nslab * b.nact;
       hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 1>), dim3((unsigned)nb), blk, 0, st, a, b, items);
-      if (a.pml_on && variant == 19)
-        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1, true>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
-      else if (a.pml_on)
+      if (a.pml_on)
         hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1, false>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
     }
     return;

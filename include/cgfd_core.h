@@ -248,9 +248,9 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    blocks, 18 = 16 with the interior x taps moved between lanes by DPP
    wave shifts instead of loaded, 19 = 16 with a lean PML-tile kernel that
    accumulates the directional rhs parts field by field). Batches with more than one active scenario run split interior /
-   PML kernels for variants 16 and 19 (one slab-ordered sweep per scenario;
-   19 with the lean PML kernel) and 17 (the scenarios of a tile back to
-   back, XCD-contiguous), and one direct
+   PML kernels for variant 16 (one slab-ordered sweep per scenario) and
+   variants 17 and 19 (the scenarios of a tile back to back, XCD-contiguous,
+   lean PML kernel), and one direct
    tile x scenario kernel for every other variant.
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
