@@ -20,14 +20,17 @@
  * rhs, FD operators and RK4 composition by the closed-form step of cubic
  * fields; the metric by affine grids; friction by closed-form values at its
  * kinks; the free surface by sigma.n = 0 after surface rupture; the CFS-PML
- * by absorbed energy; the fault frame by orthonormality; FP32 vs FP64.
+ * by absorbed energy; the fault frame by orthonormality; FP32 vs FP64; the
+ * trial traction and split-node velocity update by a locked fault that
+ * transmits a pulse without slip.
  * Parity unpinned (no golden vectors exist and the reference cannot be
  * built here; these rest on line-by-line review and on agreement with the
- * independently written GPU kernels): the split-node fault velocity/stress
- * formulas and their free-surface rows (fault_velo/stress_F/B), the
- * method-1/2 split-node matrices (fault_coef_cal), the trial traction
- * assembly before friction, wave2fault/fault2wave including the Txz/Tyz
- * swap, the one-step lag of hT1, and the exact stage ordering.
+ * independently written GPU kernels): the split-node stress formulas and
+ * the free-surface rows of the fault kernels (fault_velo/stress_F/B), the
+ * method-1/2 split-node matrices (fault_coef_cal), the trial traction's
+ * dependence on the fault-plane derivatives under slip, wave2fault/
+ * fault2wave including the Txz/Tyz swap, the one-step lag of hT1, and the
+ * exact stage ordering.
  */
 #include "../include/cgfd_core.h"
 

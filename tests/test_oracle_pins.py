@@ -6,6 +6,9 @@ reference ships no golden vectors, and it cannot be built here).
 * linear slip-weakening friction at and around every kink;
 * free-surface traction imaging keeps the surface traction at zero;
 * CFS-PML absorbs an outgoing pulse that plain zero ghosts reflect;
+* a locked split-node fault transmits a pulse without slip (the trial
+  traction enforces velocity continuity) and close to a medium without a
+  fault;
 * FP32 vs FP64 restatements on a rupture.
 """
 import numpy as np
@@ -220,6 +223,65 @@ def test_cfspml_absorbs_outgoing_pulse(built_libs):
     assert np.isfinite(e_pml)
     assert e_pml < 0.01 * e0, f"PML leaves {e_pml / e0:.3e} of the kinetic energy"
     assert e_none > 20 * e_pml, "zero ghosts must reflect far more than the PML"
+
+
+# ---------------------------------------------------------------- locked fault
+def _locked_fault_run(imethod, with_fault, sigma, steps=60):
+    """FP64 oracle: a Gaussian velocity pulse 8 nodes in front of a fault
+    whose static friction is 1000 everywhere (never reaches strength)."""
+    from cgfd import configs
+    sc = configs.small(imethod=imethod)
+    sc.faults[0].fields["mu_s"][:] = 1000.0
+    i0 = sc.faults[0].x_index + 3
+    if not with_fault:
+        sc.faults = []
+    st, orc = _oracle(sc, True)
+    nz, ny, nx = sc.shape_ext
+    kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    g = np.exp(-(((ii - (i0 - 8)) / sigma) ** 2 + ((jj - ny // 2) / sigma) ** 2 + ((kk - nz // 2) / sigma) ** 2))
+    w0 = np.zeros((9, nz, ny, nx))
+    w0[0], w0[1], w0[2] = g, 0.5 * g, -0.3 * g
+    w0[:, :3] = w0[:, -3:] = 0
+    w0[:, :, :3] = w0[:, :, -3:] = 0
+    w0[:, :, :, :3] = w0[:, :, :, -3:] = 0
+    orc.write_wavefield(w0)
+    vmax = vsmax = 0.0
+    for _ in range(steps // 5):
+        orc.run(5)
+        if with_fault:
+            w = orc.read_wavefield()
+            vmax = max(vmax, float(np.abs(w[:3, :, :, i0 - 3:i0 + 4]).max()))
+            vsmax = max(vsmax, float(np.abs(orc.read_fault(0)[3]).max()))
+    w = orc.read_wavefield()
+    orc.close()
+    st.close()
+    return w, vmax, vsmax
+
+
+@pytest.mark.parametrize("imethod", [1, 2])
+def test_locked_fault_transmits_without_slip(built_libs, imethod):
+    """The trial traction (trial_slipweakening.cu:91-275) is the traction that
+    keeps the two sides' velocities equal after the stage; below strength it
+    is applied as is, so the split-node update (fault_gpu.cu:326-425,
+    fault_wav_t.cu:318-366) must keep the slip rate at rounding level while
+    the pulse crosses the plane. A wrong sign, mass or image coefficient in
+    the trial assembly or in the split-node velocity rhs breaks the
+    continuity and opens O(1) slip."""
+    w, vmax, vsmax = _locked_fault_run(imethod, True, 2.5)
+    assert vmax > 0.05, "the pulse must reach the fault"
+    assert vsmax <= 1e-12 * vmax, f"slip rate {vsmax:.3e} on a locked fault (particle velocity {vmax:.3e})"
+
+
+def test_locked_fault_close_to_no_fault(built_libs):
+    """The split-node columns use lower-order operators next to the plane,
+    so a locked fault differs from the unfaulted medium by truncation error
+    only, shrinking as the pulse widens: 14% relative L2 in velocity at a
+    width of 2.5 nodes, 8.4% at 4, 6.7% at 6 on this grid, against O(1) for
+    a sign or stencil error in the column kernels (fault_gpu.cu:191-1155)."""
+    wf, _, _ = _locked_fault_run(2, True, 4.0)
+    wn, _, _ = _locked_fault_run(2, False, 4.0)
+    r = rel_l2(interior(wf)[:3], interior(wn)[:3])
+    assert r < 0.12, f"locked fault vs no fault: velocity rel L2 {r:.3f}"
 
 
 # ---------------------------------------------------------------- FP32 vs FP64
