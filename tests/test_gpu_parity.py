@@ -248,3 +248,35 @@ def test_state_save_load_restarts_bit_identically(built_libs):
     core.close()
     assert hip.hipFree(dptr) == 0
     st.close()
+
+
+@pytest.mark.parametrize("imethod", [1, 2])
+def test_locked_fault_transmits_without_slip_gpu(built_libs, imethod):
+    """The GPU fault kernels on their own: a locked fault (static friction
+    1000) crossed by a velocity pulse keeps the slip rate at FP32 rounding
+    level, as the trial traction enforces velocity continuity."""
+    from cgfd import configs, model
+    sc = configs.small(imethod=imethod)
+    sc.faults[0].fields["mu_s"][:] = 1000.0
+    i0 = sc.faults[0].x_index + 3
+    st = model.Setup(sc)
+    core = model.Core(st)
+    nz, ny, nx = sc.shape_ext
+    kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    g = np.exp(-(((ii - (i0 - 8)) / 2.5) ** 2 + ((jj - ny // 2) / 2.5) ** 2 + ((kk - nz // 2) / 2.5) ** 2))
+    w0 = np.zeros((9, nz, ny, nx), np.float32)
+    w0[0], w0[1], w0[2] = g, 0.5 * g, -0.3 * g
+    w0[:, :3] = w0[:, -3:] = 0
+    w0[:, :, :3] = w0[:, :, -3:] = 0
+    w0[:, :, :, :3] = w0[:, :, :, -3:] = 0
+    core.write_wavefield(w0)
+    vmax = vsmax = 0.0
+    for _ in range(12):
+        core.run(5)
+        w = core.read_wavefield()
+        vmax = max(vmax, float(np.abs(w[:3, :, :, i0 - 3:i0 + 4]).max()))
+        vsmax = max(vsmax, float(np.abs(core.read_fault(0)[3]).max()))
+    assert vmax > 0.05
+    assert vsmax <= 1e-5 * vmax, f"slip rate {vsmax:.3e} on a locked fault (particle velocity {vmax:.3e})"
+    core.close()
+    st.close()
