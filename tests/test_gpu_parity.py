@@ -280,3 +280,28 @@ def test_locked_fault_transmits_without_slip_gpu(built_libs, imethod):
     assert vsmax <= 1e-5 * vmax, f"slip rate {vsmax:.3e} on a locked fault (particle velocity {vmax:.3e})"
     core.close()
     st.close()
+
+
+@pytest.mark.parametrize("variant", [19, 16, 1, 7])
+def test_ragged_grid_matches_oracle(built_libs, variant):
+    """A grid whose sizes are multiples of none of the tile sizes (37 x 45 x
+    27: partial 64-wide x tiles, a partial last row of 4-row y tiles, z
+    chunks that do not divide the planes) with a fault, PML and a free
+    surface: every point is computed once, against the FP32 oracle."""
+    from cgfd import configs, model
+    import oracle
+    sc = configs.small(ni=37, nj=45, nk=27, pml=5, imethod=2)
+    st = model.Setup(sc)
+    core = model.Core(st, variant=variant)
+    core.set_zchunk(4)
+    orc = oracle.Oracle(st.desc, sc.shape_ext, 1)
+    core.run(30)
+    orc.run(30)
+    wg = interior(core.read_wavefield())
+    wo = interior(orc.read_wavefield())
+    assert rel_l2(wg, wo) <= TOL
+    fg, fo = fault_phys(core.read_fault(0)), fault_phys(orc.read_fault(0))
+    assert rel_l2(fg[6], fo[6]) <= TOL
+    core.close()
+    orc.close()
+    st.close()
