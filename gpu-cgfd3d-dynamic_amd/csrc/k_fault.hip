@@ -317,20 +317,76 @@ DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
    loads of a node's x taps are adjacent addresses. */
 constexpr int kColNodes = 10;  /* nodes per wavefront in the column kernel */
 
+/* Traction vectors J (grad xi_d . T) a block of kColNodes nodes x 4 rows
+   needs, computed once each into LDS: along x at the 13 columns i0-6..i0+6,
+   along y at the 6 columns over rows j-3..j+12, along z at the 6 columns
+   over planes k-3..k+6 (the union of the 7-point stencils of the block's
+   lanes, each of which would otherwise evaluate 21 of them). */
+constexpr int kColRows = 4;                                       /* blockDim.y */
+constexpr int kTvX = 13 * kColNodes * kColRows;                  /* 520 */
+constexpr int kTvY = 6 * (kColNodes + 6) * kColRows;             /* 384 */
+constexpr int kTvZ = 6 * kColNodes * (kColRows + 6);             /* 600 */
+constexpr int kTv = kTvX + kTvY + kTvZ;
+
 template <int IDIR>
 DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
-  const int lane = threadIdx.x;
-  if (lane >= 6 * kColNodes) return;
-  const int iy = blockIdx.x * kColNodes + lane / 6;
-  const int cl = lane % 6;
-  const int iz = blockIdx.y * blockDim.y + threadIdx.y;
-  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  __shared__ float tv[3][kTv];
   const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
+  const int i0 = F.i0;
+  const int j0 = blockIdx.x * kColNodes, k0 = blockIdx.y * kColRows;  /* first node of the block */
+  const float *met = a.met;
+  const float *W = a.w_cur;
+  const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
+  const float *Tyz = W + TYZ * sc, *Txz = W + TXZ * sc, *Txy = W + TXY * sc;
+  {
+    /* blocks whose nodes are all welded (or past the grid) have no work */
+    const int ln = threadIdx.x, jb = j0 + ln / 6, kb = k0 + (int)threadIdx.y;
+    const bool mine = ln < 6 * kColNodes && jb < a.g.nj && kb < a.g.nk &&
+                      F.united[(jb + 3) + (long)(kb + 3) * ny] == 0;
+    if (!__syncthreads_or(mine)) return;
+  }
+  {
+    const int tid = threadIdx.x + 64 * threadIdx.y;
+    for (int v = tid; v < kTv; v += 64 * kColRows) {
+      int x, jg, kg, d;  /* ghost-inclusive point and metric row */
+      if (v < kTvX) {
+        const int xi = v % 13, r = v / 13;
+        x = i0 - 6 + xi; jg = j0 + r % kColNodes + 3; kg = k0 + r / kColNodes + 3; d = 0;
+      } else if (v < kTvX + kTvY) {
+        const int u = v - kTvX, col = u % 6, r = u / 6;
+        x = i0 + (col < 3 ? col - 3 : col - 2); jg = j0 - 3 + r % (kColNodes + 6) + 3;
+        kg = k0 + r / (kColNodes + 6) + 3; d = 1;
+      } else {
+        const int u = v - kTvX - kTvY, col = u % 6, r = u / 6;
+        x = i0 + (col < 3 ? col - 3 : col - 2); jg = j0 + r % kColNodes + 3;
+        kg = k0 - 3 + r / kColNodes + 3; d = 2;
+      }
+      float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
+      if (x >= 0 && x < a.g.nx && jg >= 0 && jg < a.g.ny && kg >= 0 && kg < a.g.nz) {
+        const long q = x + (long)jg * sy + (long)kg * sz;
+        const float jq = met[q], a1 = met[(1 + 3 * d) * sc + q], a2 = met[(2 + 3 * d) * sc + q],
+                    a3 = met[(3 + 3 * d) * sc + q];
+        o0 = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+        o1 = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+        o2 = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+      }
+      tv[0][v] = o0;
+      tv[1][v] = o1;
+      tv[2][v] = o2;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x;
+  if (lane >= 6 * kColNodes) return;
+  const int jn = lane / 6, kr = threadIdx.y;
+  const int iy = j0 + jn;
+  const int cl = lane % 6;
+  const int iz = k0 + kr;
+  if (iy >= a.g.nj || iz >= a.g.nk) return;
   const long pf = (iy + 3) + (long)(iz + 3) * ny;
   if (F.united[pf] != 0) return;
   const int jdir = a.jdir, kdir = a.kdir;
-  const int i0 = F.i0;
   const int mm = cl < 3 ? cl - 3 : cl - 2;  /* column offset from the plane */
   const int i = i0 + mm;
   const int n = -mm;
@@ -339,25 +395,16 @@ DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
   const int km = a.g.nk - (iz + 1);
   const int n_free = km + 3;
   const bool surf = (a.isfree == 1);
-  const float *met = a.met;
-  const float *W = a.w_cur;
 
   /* ---- velocity rhs ---- */
-  const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
-  const float *Tyz = W + TYZ * sc, *Txz = W + TXZ * sc, *Txy = W + TXY * sc;
-  /* J (grad xi_d . T) at the seven points along direction d, one direction
-     at a time and reduced to its derivative before the next direction's
-     loads (a register fence keeps the three groups apart) */
   float dvx[3], dvy[3], dvz[3];
   {
     float v[3][7];
 #pragma unroll
     for (int l = -3; l <= 3; l++) {
-      const long q = (i + l) + row;
-      const float jq = met[q], a1 = met[sc + q], a2 = met[2 * sc + q], a3 = met[3 * sc + q];
-      v[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-      v[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-      v[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+      const int e = mm + l + 6 + 13 * (jn + kColNodes * kr);
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c][l + 3] = tv[c][e];
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -374,31 +421,25 @@ DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
       if (n == -2) { u[0] = (float)(2.0 * u[1] - u[2]); }
       dvx[c] = vecc(u + 3, IDIR);
     }
-    __asm__ volatile("" : "+v"(dvx[0]), "+v"(dvx[1]), "+v"(dvx[2]) : : "memory");
   }
   {
     float v[3][7];
 #pragma unroll
     for (int l = -3; l <= 3; l++) {
-      const long q = p + l * sy;
-      const float jq = met[q], a1 = met[4 * sc + q], a2 = met[5 * sc + q], a3 = met[6 * sc + q];
-      v[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-      v[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-      v[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+      const int e = kTvX + cl + 6 * (jn + 3 + l + (kColNodes + 6) * kr);
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c][l + 3] = tv[c][e];
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) dvy[c] = vecc(v[c] + 3, jdir);
-    __asm__ volatile("" : "+v"(dvy[0]), "+v"(dvy[1]), "+v"(dvy[2]) : : "memory");
   }
   {
     float v[3][7];
 #pragma unroll
     for (int l = -3; l <= 3; l++) {
-      const long q = p + l * sz;
-      const float jq = met[q], a1 = met[7 * sc + q], a2 = met[8 * sc + q], a3 = met[9 * sc + q];
-      v[0][l + 3] = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-      v[1][l + 3] = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-      v[2][l + 3] = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+      const int e = kTvX + kTvY + cl + 6 * (jn + kColNodes * (kr + 3 + l));
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[c][l + 3] = tv[c][e];
     }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
