@@ -128,6 +128,12 @@ DEVI float dfd(const float *__restrict__ p, long s) {
    create). */
 DEVI float ld32(const float *base, unsigned off) { return *(const float *)((const char *)base + off); }
 DEVI void st32(float *base, unsigned off, float v) { *(float *)((char *)base + off) = v; }
+/* the same for data a stage touches once (RK levels, metric, media): marked
+   non-temporal so they do not displace the stencil's working set in L2 */
+DEVI float ld32nt(const float *base, unsigned off) {
+  return __builtin_nontemporal_load((const float *)((const char *)base + off));
+}
+DEVI void st32nt(float *base, unsigned off, float v) { __builtin_nontemporal_store(v, (float *)((char *)base + off)); }
 
 template <int DIR>
 DEVI float dfd32(const float *__restrict__ b, unsigned o, unsigned s) {

@@ -151,19 +151,19 @@ DEVI void rk_point32(const VolArgs &a, unsigned o, const float *h, long woff = 0
   if (a.istage == 0) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      float pv = ld32(pre + c * sc, o);
-      st32(tnew + c * sc, o, pv + a.ca * h[c]);
-      st32(end + c * sc, o, pv + a.cb * h[c]);
+      float pv = ld32nt(pre + c * sc, o);
+      st32nt(tnew + c * sc, o, pv + a.ca * h[c]);
+      st32nt(end + c * sc, o, pv + a.cb * h[c]);
     }
   } else if (a.istage < 3) {
 #pragma unroll
     for (int c = 0; c < 9; c++) {
-      st32(tnew + c * sc, o, ld32(pre + c * sc, o) + a.ca * h[c]);
-      st32(end + c * sc, o, ld32(end + c * sc, o) + a.cb * h[c]);
+      st32nt(tnew + c * sc, o, ld32nt(pre + c * sc, o) + a.ca * h[c]);
+      st32nt(end + c * sc, o, ld32nt(end + c * sc, o) + a.cb * h[c]);
     }
   } else {
 #pragma unroll
-    for (int c = 0; c < 9; c++) st32(end + c * sc, o, ld32(end + c * sc, o) + a.cb * h[c]);
+    for (int c = 0; c < 9; c++) st32nt(end + c * sc, o, ld32nt(end + c * sc, o) + a.cb * h[c]);
   }
 }
 

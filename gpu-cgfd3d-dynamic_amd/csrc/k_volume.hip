@@ -135,9 +135,9 @@ DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int
   const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
   const float *M = a.met;
   const long woff = (long)sn * b.wstride;
-  const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
-  const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
-  const float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
+  const float xx = ld32nt(M + sc, o), xy = ld32nt(M + 2 * sc, o), xz = ld32nt(M + 3 * sc, o);
+  const float ex = ld32nt(M + 4 * sc, o), ey = ld32nt(M + 5 * sc, o), ez = ld32nt(M + 6 * sc, o);
+  const float zx = ld32nt(M + 7 * sc, o), zy = ld32nt(M + 8 * sc, o), zz = ld32nt(M + 9 * sc, o);
   float gvx[3], gvy[3], gvz[3];
   float hv0 = 0.f, hv1 = 0.f, hv2 = 0.f;
 #pragma unroll
@@ -165,7 +165,7 @@ DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int
     }
   }
   if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
-  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+  const float l = ld32nt(a.lam, o), m = ld32nt(a.mu, o), s = ld32nt(a.slw, o);
   const float l2m = l + 2.0f * m;
   float h[9];
   h[VX] = s * hv0;
@@ -369,8 +369,8 @@ DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k,
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) e[d][c] = ld32(M + (1 + 3 * d + c) * sc, o);
-  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
+    for (int c = 0; c < 3; c++) e[d][c] = ld32nt(M + (1 + 3 * d + c) * sc, o);
+  const float l = ld32nt(a.lam, o), m = ld32nt(a.mu, o), s = ld32nt(a.slw, o);
   const float l2m = l + 2.0f * m;
   float r[3][9];
 #pragma unroll
