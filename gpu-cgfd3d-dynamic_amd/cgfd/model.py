@@ -164,6 +164,26 @@ class Core:
         L.check(self.lib, self.lib.cgfd_adj_backward(self.handle, L.ptr(residual), nt, L.ptr(out)))
         return out
 
+    def adjoint_forward(self, nt: int):
+        """Arms the friction tape of the adjoint for the next nt steps (call it
+        at the step the stations were set, before running them)."""
+        L.check(self.lib, self.lib.cgfd_adj_forward(self.handle, int(nt)))
+
+    def adjoint_fault(self, residual: np.ndarray):
+        """(dJ/dw at the first recorded step, dJ/d(fault parameters)) for
+        J = sum residual . traces; the second is [nfault][7][nk][nj] over
+        (mu_s, mu_d, Dc, C0, T0x, T0y, T0z)."""
+        residual = np.ascontiguousarray(residual, np.float32)
+        nsta, nt, nc = residual.shape
+        if nc != 9:
+            raise ValueError("residual must be [nsta][nt][9]")
+        nz, ny, nx = self.shape_ext
+        out = np.zeros((9, nz, ny, nx), np.float32)
+        sc = self.setup.sc
+        gf = np.zeros((len(sc.faults), 7, sc.nk, sc.nj), np.float32)
+        L.check(self.lib, self.lib.cgfd_adj_backward_fault(self.handle, L.ptr(residual), nt, L.ptr(out), L.ptr(gf)))
+        return out, gf
+
     def state_bytes(self) -> int:
         n = C.c_size_t()
         L.check(self.lib, self.lib.cgfd_core_state_bytes(self.handle, C.byref(n)))

@@ -41,7 +41,13 @@ struct FaultArgs {
   int isfree, imethod, fix_q1;
   float dt, ca, cb, rkb;
   const int *it_dev;                        /* completed steps (device counter) */
+  /* friction tape of the adjoint (null: off): per fault [tape_nt][16][sslice],
+     per step the trial traction of each stage (12), Slip at the step start
+     (1) and the split-node velocity jump of the last stage (3) */
+  float *tape;
+  int tape_it0, tape_nt;
 };
+constexpr int kTapeFloats = 16;
 
 /* the fault arguments of scenario s: its wavefield and fault-state levels,
    and the offsets of its per-node parameters, outputs and traction columns
@@ -123,7 +129,30 @@ struct AdjPml {
   int on;
   int le, lp;             /* level indices of E and P in every face's lev (A = 1, B = 2) */
   AdjFace f[6];           /* dim-major, side-minor */
+  FaultSkip fs;           /* fault columns: their rhs belongs to the fault adjoint */
 };
+/* fault part of the adjoint (k_adj_fault.hip): per fault a block of
+   kAdjFaultFloats x sslice floats: split-node adjoint levels [4][18] (E, A,
+   B, P by index), T1 [21], hT1 [3], mT1 [3], Slip [1], parameter gradients
+   [7] (mu_s, mu_d, Dc, C0, T0x, T0y, T0z) */
+constexpr int kAdjFaultFloats = 107;
+struct AdjFaultArgs {
+  Grid g;
+  const float *met, *lam, *mu, *slw, *matVx2Vz, *matVy2Vz;
+  const FaultDev *ftab;
+  int nfault;
+  float *adjf;
+  const float *tape;
+  int tape_nt, n;          /* tape steps; slot of the step being reversed */
+  int istage, idir, jdir, kdir, isfree, imethod, fix_q1;
+  float dt, ca, cb;
+  int le, lp, lx, lc;      /* split-node levels: E, P, the stage output (-1 at stage 3), the stage input */
+  float *wT;               /* wavefield adjoint of the stage input */
+  float *wX, *wE;          /* of the stage output (null at stage 3) and of the end level */
+};
+void launch_adjf_begin(const AdjFaultArgs &a, hipStream_t st);
+void launch_adjf_stage_pre(const AdjFaultArgs &a, hipStream_t st);
+void launch_adjf_stage_post(const AdjFaultArgs &a, hipStream_t st);
 void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
                      float alpha, const float *E, float beta, float *SG, const AdjPml &pm, int isfree, int kdir,
                      hipStream_t st);

@@ -144,6 +144,13 @@ struct cgfd_core {
      of the adjoint gradient terms; allocated by the first backward call */
   float *adj = nullptr, *adj_sg = nullptr;
   float *adj_pml = nullptr;  /* per CFS-PML face: 4 adjoint levels [9][n] + extra S, G [12][n] */
+  /* fault part of the adjoint: the friction tape written by the forward
+     kernels over the recorded window, and per fault the adjoint blocks of
+     kAdjFaultFloats per node (fault state levels E, A, B, P; T1; hT1; mT1;
+     Slip; parameter gradients) */
+  float *tape = nullptr;
+  int tape_it0 = 0, tape_nt = 0;
+  float *adjf = nullptr;
 
   float *wlev(int l) { return wave + (long)l * 9 * g.sicmp; }
   float *flev(int l) { return fwav + (long)l * nfault * 18 * g.sslice; }
@@ -641,6 +648,9 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.cb = kRkB[istage] * c->dt;
   f.rkb = kRkB[istage];
   f.it_dev = c->it_dev;
+  f.tape = c->tape;
+  f.tape_it0 = c->tape_it0;
+  f.tape_nt = c->tape_nt;
 }
 
 /* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
@@ -887,25 +897,70 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
   return CGFD_OK;
 }
 
-/* ---- discrete adjoint of the wave update (interior, CFS-PML, free
-   surface; no fault): gradient of J = sum_n sum_s r_n(s) . w_n(s) with respect to the
-   state at the first recorded step, for the residuals r of the recorded
-   station traces; the gradient of a least-squares trace misfit is this with
-   r = recorded - target. Reverse sweep over the recorded steps, one
-   reverse RK4 step per recorded step (kernels in k_adjoint.hip). ---- */
-extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
-  if (!c || !residual || !grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+/* ---- discrete adjoint: gradient of J = sum_n sum_s r_n(s) . w_n(s) with
+   respect to the state at the first recorded step and, with faults, to the
+   friction parameters and initial traction of every split node, for the
+   residuals r of the recorded station traces; the gradient of a
+   least-squares trace misfit is this with r = recorded - target. Reverse
+   sweep over the recorded steps, one reverse RK4 step per recorded step
+   (k_adjoint.hip: wave update; k_adj_fault.hip: fault and friction, from the
+   tape the forward run wrote). ---- */
+extern "C" int cgfd_adj_forward(cgfd_core *c, int nt) {
+  if (!c || nt <= 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
   if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
-  if (c->nfault > 0)
-    return cgfd_set_error(CGFD_ERR_ARG,
-                          "adjoint: this build transposes the wave update only (interior, CFS-PML, free surface; no fault)");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->drop_graph();
+  c->tape = nullptr;
+  c->tape_nt = 0;
+  c->tape_it0 = c->it_host;
+  if (c->nfault == 0) return CGFD_OK;  /* the wave update needs no tape */
+  const size_t n = (size_t)c->nfault * nt * kTapeFloats * c->g.sslice;
+  if (int rc = c->alloc(&c->tape, n)) return rc;
+  HIPCHK(hipMemset(c->tape, 0, n * 4));
+  c->tape_nt = nt;
+  return CGFD_OK;
+}
+
+namespace {
+/* split nodes whose fault columns reach a CFS-PML slab are not covered */
+int check_fault_columns(cgfd_core *c) {
+  const Grid &g = c->g;
+  std::vector<int> un((size_t)g.sslice);
+  for (int id = 0; id < c->nfault; id++) {
+    HIPCHK(hipMemcpy(un.data(), c->flt[id].united, un.size() * 4, hipMemcpyDeviceToHost));
+    const int i0 = c->flt[id].i0;
+    for (int k = 3; k < g.nk + 3; k++)
+      for (int j = 3; j < g.nj + 3; j++) {
+        if (un[(size_t)j + (size_t)k * g.ny] != 0) continue;
+        for (int dim = 0; dim < 3; dim++)
+          for (int side = 0; side < 2; side++) {
+            const PmlFace &f = c->pml[dim][side];
+            if (f.on && j >= f.j1 && j <= f.j2 && k >= f.k1 && k <= f.k2 && i0 + 3 >= f.i1 && i0 - 3 <= f.i2)
+              return cgfd_set_error(CGFD_ERR_ARG,
+                                    "adjoint: split node (j=%d, k=%d) of fault %d has fault columns in a CFS-PML slab", j,
+                                    k, id);
+          }
+      }
+  }
+  return CGFD_OK;
+}
+
+int adj_sweep(cgfd_core *c, const float *residual, int nt, float *grad, float *grad_fault) {
+  if (!c || !residual) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
   if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "adjoint: no stations set");
   if (nt < 1 || nt > c->st_nt) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: nt %d outside 1..%d", nt, c->st_nt);
+  if (c->nfault > 0 && (!c->tape || c->tape_it0 != c->st_it0 || c->tape_nt < nt))
+    return cgfd_set_error(CGFD_ERR_STATE,
+                          "adjoint: a core with a fault needs the friction tape: call cgfd_adj_forward with at least "
+                          "%d steps at the step the stations were set, before running", nt);
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   const Grid &g = c->g;
-  const long sc = g.sicmp, lv = 9 * sc;
+  const long sc = g.sicmp, lv = 9 * sc, ss = g.sslice;
   int rc;
+  if (c->nfault > 0 && (rc = check_fault_columns(c))) return rc;
   long npml = 0;  /* floats of the PML adjoint block */
   for (int dim = 0; dim < 3; dim++)
     for (int side = 0; side < 2; side++)
@@ -916,11 +971,16 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
     HIPCHK(hipMemset(c->adj_sg, 0, (size_t)12 * sc * 4));  /* ghost entries stay zero */
     if (npml > 0 && (rc = c->alloc(&c->adj_pml, (size_t)npml))) return rc;
   }
+  const size_t nadjf = (size_t)c->nfault * kAdjFaultFloats * ss;
+  if (c->nfault > 0 && !c->adjf && (rc = c->alloc(&c->adjf, nadjf))) return rc;
   HIPCHK(hipMemsetAsync(c->adj, 0, (size_t)4 * lv * 4, c->st));
+  if (c->nfault > 0) HIPCHK(hipMemsetAsync(c->adjf, 0, nadjf * 4, c->st));
   AdjPml pm{};
   pm.on = c->pml_on;
   pm.le = 0;
   pm.lp = 3;
+  pm.fs.nfault = c->nfault;
+  for (int q = 0; q < c->nfault; q++) { pm.fs.i0[q] = c->flt[q].i0; pm.fs.united[q] = c->flt[q].united; }
   {
     long off = 0;
     for (int dim = 0; dim < 3; dim++)
@@ -939,6 +999,19 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
         off += 48 * f.n;
       }
   }
+  AdjFaultArgs fa{};
+  fa.g = g;
+  fa.met = c->met; fa.lam = c->lam; fa.mu = c->mu; fa.slw = c->slw;
+  fa.matVx2Vz = c->matVx2Vz; fa.matVy2Vz = c->matVy2Vz;
+  fa.ftab = c->d_ftab;
+  fa.nfault = c->nfault;
+  fa.adjf = c->adjf;
+  fa.tape = c->tape;
+  fa.tape_nt = c->tape_nt;
+  fa.isfree = c->isfree; fa.imethod = c->imethod; fa.fix_q1 = c->fix_q1;
+  fa.dt = c->dt;
+  fa.le = 0;
+  fa.lp = 3;
   float *res = nullptr;
   const size_t nres = (size_t)c->st_n * nt * 9;
   HIPCHK(hipMalloc(&res, nres * 4));
@@ -948,22 +1021,32 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
     if (n < nt - 1) {
       const int it = c->st_it0 + n;  /* the forward step that took w_n to w_{n+1} */
       const int ipair = it % 8;
+      fa.n = n;
+      fa.wE = E;
+      launch_adjf_begin(fa, c->st);
       for (int s = 3; s >= 0; s--) {
         const int idir = (kPairs[ipair][0] + s) % 2, jdir = (kPairs[ipair][1] + s) % 2, kdir = (kPairs[ipair][2] + s) % 2;
         const float cb = kRkB[s] * c->dt, ca = s < 3 ? kRkA[s] * c->dt : 0.0f;
-        const float *X = s == 3 ? nullptr : (s == 1 ? B : A);
+        float *X = s == 3 ? nullptr : (s == 1 ? B : A);
+        float *T = (s == 3 || s == 1) ? A : (s == 2 ? B : P);  /* the level the stage writes L^T into */
+        fa.istage = s; fa.idir = idir; fa.jdir = jdir; fa.kdir = kdir;
+        fa.ca = ca; fa.cb = cb;
+        fa.lx = s == 3 ? -1 : (s == 1 ? 2 : 1);
+        fa.lc = s == 0 ? fa.lp : (s == 2 ? 2 : 1);
+        fa.wT = T; fa.wX = X; fa.wE = E;
+        launch_adjf_stage_pre(fa, c->st);
         launch_adj_prep(s, g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, pm, c->isfree, kdir, c->st);
         launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, pm, c->st);
-        if (c->isfree) {
-          float *T = (s == 3 || s == 1) ? A : (s == 2 ? B : P);  /* the level the stage wrote L^T into */
+        if (c->isfree)
           launch_adj_top(s, idir, jdir, kdir, g, c->met, c->lam, c->mu, c->slw, c->matVx2Vz, c->matVy2Vz, X, ca, E,
                          cb, T, pm, c->st);
-        }
+        launch_adjf_stage_post(fa, c->st);
       }
     }
     launch_adj_inject(c->st_idx, c->st_n, res, nt, n, sc, P, c->st);
     std::swap(E, P);
     std::swap(pm.le, pm.lp);
+    std::swap(fa.le, fa.lp);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -971,9 +1054,28 @@ extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, fl
   if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "adjoint sweep: %s", hipGetErrorString(e));
   /* E holds the adjoint of the first recorded state */
   const size_t hc = (size_t)g.nx * g.ny * g.nz;
-  for (int f = 0; f < 9; f++)
-    if ((rc = dev_to_host(c, grad + f * hc, E + f * sc, (long)g.ny * g.nz))) return rc;
+  if (grad)
+    for (int f = 0; f < 9; f++)
+      if ((rc = dev_to_host(c, grad + f * hc, E + f * sc, (long)g.ny * g.nz))) return rc;
+  if (grad_fault)
+    for (int id = 0; id < c->nfault; id++)
+      for (int q = 0; q < 7; q++) {
+        const float *src = c->adjf + ((size_t)id * kAdjFaultFloats + 100 + q) * ss + 3 + 3 * (size_t)g.ny;
+        float *dst = grad_fault + ((size_t)id * 7 + q) * g.nj * g.nk;
+        HIPCHK(hipMemcpy2D(dst, (size_t)g.nj * 4, src, (size_t)g.ny * 4, (size_t)g.nj * 4, g.nk, hipMemcpyDeviceToHost));
+      }
   return CGFD_OK;
+}
+}  // namespace
+
+extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
+  if (!grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  return adj_sweep(c, residual, nt, grad, nullptr);
+}
+
+extern "C" int cgfd_adj_backward_fault(cgfd_core *c, const float *residual, int nt, float *grad, float *grad_fault) {
+  if (!grad_fault) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  return adj_sweep(c, residual, nt, grad, grad_fault);
 }
 
 extern "C" int cgfd_core_set_overlap(cgfd_core *c, int on) {

@@ -1,6 +1,6 @@
 /*
  * k_adjoint.hip -- discrete adjoint of the RK4 volume update (interior
- * stencil, CFS-PML, free surface; no fault).
+ * stencil, CFS-PML, free surface); the fault part is in k_adj_fault.hip.
  *
  * One forward stage computes, at every physical point p,
  *   h(p) = H(p) . [D_d f(p)],  D_d f(p) = sum_t c_t f(p + (o_d + t) e_d)
@@ -36,6 +36,9 @@
  * matrices at the surface. Transposed: the prep leaves out the base S / G of
  * what was replaced, and k_adj_top scatters the adjoint of timg, vlow and
  * the surface PML term onto the taps (atomic adds; ghosts are not state).
+ *
+ * Fault columns of split nodes (their rhs comes from the fault kernels) get
+ * no S and G here and no top-layer scatter; k_adj_fault.hip transposes them.
  *
  * RK4 (drv_rk_curv_col.cu:329-568) with a = {1/2, 1/2, 1}, b = {1/6, 1/3,
  * 1/3, 1/6}, ca = a dt, cb = b dt, reversed (E = adjoint of the step's
@@ -88,6 +91,11 @@ __global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restric
   if (i >= g.ni + 3 || j >= g.nj + 3) return;
   const long sc = g.sicmp;
   const long p = i + j * g.siy + (long)k * g.siz;
+  if (pm.fs.nfault > 0 && fault_column(pm.fs, i, j + (long)k * g.ny)) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) SG[q * sc + p] = 0.0f;
+    return;
+  }
   float c[9];
 #pragma unroll
   for (int f = 0; f < 9; f++) c[f] = beta * E[f * sc + p] + (X ? alpha * X[f * sc + p] : 0.0f);
@@ -233,6 +241,7 @@ __global__ void __launch_bounds__(256) k_adj_top(Grid g, const float *__restrict
   const int nk2 = g.nk2;
   const int k = nk2 - (int)blockIdx.z;  /* planes nk2, nk2-1, nk2-2, nk2-3 */
   if (i >= g.ni + 3 || j >= g.nj + 3 || k < 3) return;
+  if (pm.fs.nfault > 0 && fault_column(pm.fs, i, j + (long)k * g.ny)) return;
   const long sc = g.sicmp, sy = g.siy, sz = g.siz;
   const long p = i + j * sy + (long)k * sz;
   float c[9];

@@ -218,6 +218,15 @@ DEVI void k_trial_body(const FaultArgs &a, int id) {
     Trial[c] = (float)(2.0 * (Mrho[0] * Mrho[1] * dV / a.dt + Mrho[0] * R[1][c] - Mrho[1] * R[0][c]) /
                        (kA0 * (Mrho[0] + Mrho[1])));
   }
+  if (a.tape) {
+    const int slot = *a.it_dev - a.tape_it0;
+    if (slot >= 0 && slot < a.tape_nt) {
+      float *tp = a.tape + ((long)id * a.tape_nt + slot) * kTapeFloats * ss + pf;
+#pragma unroll
+      for (int c = 0; c < 3; c++) tp[(3 * a.istage + c) * ss] = Trial[c];
+      if (a.istage == 0) tp[12 * ss] = F.out[6 * ss + pf];
+    }
+  }
   float vs1[3] = {F.vs[pf], F.vs[ss + pf], F.vs[2 * ss + pf]};
   float vs2[3] = {F.vs[3 * ss + pf], F.vs[4 * ss + pf], F.vs[5 * ss + pf]};
   float vn[3] = {met[sc + p0], met[2 * sc + p0], met[3 * sc + p0]};
@@ -713,6 +722,14 @@ DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
     float dV[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) dV[c] = fe[1][c] - fe[0][c];
+    if (a.tape) {
+      const int slot = *a.it_dev - a.tape_it0;
+      if (slot >= 0 && slot < a.tape_nt) {
+        float *tp = a.tape + ((long)id * a.tape_nt + slot) * kTapeFloats * ss + pf;
+#pragma unroll
+        for (int c = 0; c < 3; c++) tp[(13 + c) * ss] = dV[c];
+      }
+    }
     float Vs1 = dV[0] * F.vs[pf] + dV[1] * F.vs[ss + pf] + dV[2] * F.vs[2 * ss + pf];
     float Vs2 = dV[0] * F.vs[3 * ss + pf] + dV[1] * F.vs[4 * ss + pf] + dV[2] * F.vs[5 * ss + pf];
     float Vs = sqrtf(Vs2 * Vs2 + Vs1 * Vs1);

@@ -168,19 +168,37 @@ int cgfd_core_state_bytes(cgfd_core *core, size_t *bytes);
 int cgfd_core_save_state(cgfd_core *core, void *dst, int dst_on_device);
 int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
 
-/* Discrete adjoint of the time loop, first part: the wave update -- the
-   interior velocity-stress stencil, CFS-PML, the free-surface traction image
-   and reduced-order velocity gradient, and RK4 (cores without a fault;
-   cores with one get CGFD_ERR_ARG). The reference has no adjoint; this is
-   the transpose of the discrete operators of sv_curv_col_el_iso_gpu.cu:
-   231-1290 composed with the RK4 stages of drv_rk_curv_col.cu:329-568.
-   With stations set before the forward run (cgfd_core_set_stations) and the
-   core advanced over the recorded steps, residual[nsta][nt][9] (the layout
-   of cgfd_core_read_stations) defines J = sum_n sum_s residual[s][n] .
-   w_n(s); grad[9][nz][ny][nx] (host layout, ghosts zero) receives dJ/dw at
-   the first recorded step. For a least-squares trace misfit pass
-   residual = recorded - target. The core's forward state is not touched. */
+/* Discrete adjoint of the time loop: the interior velocity-stress stencil,
+   CFS-PML, the free-surface traction image and reduced-order velocity
+   gradient, RK4, and the split-node fault with slip-weakening friction. The
+   reference has no adjoint; this is the transpose of the discrete operators
+   of sv_curv_col_el_iso_gpu.cu:231-1290, sv_curv_col_el_iso_fault_gpu.cu:
+   191-1879, trial_slipweakening.cu:91-388, transform.cu:100-376 and
+   fault_wav_t.cu:197-366 composed with the RK4 stages of
+   drv_rk_curv_col.cu:329-568.
+
+   Usage: set the stations (cgfd_core_set_stations) and, on a core with a
+   fault, call cgfd_adj_forward(core, nt) at the same step; run the nt
+   recorded steps; then call cgfd_adj_backward or cgfd_adj_backward_fault.
+   residual[nsta][nt][9] (the layout of cgfd_core_read_stations) defines
+   J = sum_n sum_s residual[s][n] . w_n(s); for a least-squares trace misfit
+   pass residual = recorded - target.
+
+   cgfd_adj_forward: arms the friction tape for the next nt steps (16 floats
+   per fault node and step: the trial traction of each stage, Slip at the
+   step start, the velocity jump of the last stage), written by the forward
+   fault kernels; the friction law is differentiated on the branch the
+   forward run took. No-op on a core without a fault. Single-scenario cores.
+   cgfd_adj_backward: grad[9][nz][ny][nx] (host layout, ghosts zero)
+   receives dJ/dw at the first recorded step.
+   cgfd_adj_backward_fault: the same (grad may be null) and
+   grad_fault[nfault][7][nk][nj] receives dJ/d(mu_s, mu_d, Dc, C0, T0x, T0y,
+   T0z) at every fault node (zero at welded nodes).
+   Faults whose split nodes have fault columns inside a CFS-PML slab are
+   refused (CGFD_ERR_ARG). The core's forward state is not touched. */
+int cgfd_adj_forward(cgfd_core *core, int nt);
 int cgfd_adj_backward(cgfd_core *core, const float *residual, int nt, float *grad);
+int cgfd_adj_backward_fault(cgfd_core *core, const float *residual, int nt, float *grad, float *grad_fault);
 
 /* Stage fork (default off): within each RK stage, after the trial traction,
    the interior volume blocks, the CFS-PML blocks with the top layers, and

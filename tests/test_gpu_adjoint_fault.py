@@ -1,0 +1,179 @@
+"""Discrete adjoint of the time loop with a split-node slip-weakening fault,
+through the C ABI (cgfd_adj_forward / cgfd_adj_backward_fault).
+
+* Locked fault (static friction 1000, compressive normal traction): the
+  friction never yields, so the station traces are a linear map T of the
+  initial wavefield through every fault path (wave2fault, traction columns,
+  trial, fault2wave, fault columns, split nodes, hT1 recurrence, free-surface
+  rows). The adjoint g = T^T r must satisfy <g, w0> = sum r . T(w0), for both
+  split-node methods. Stations sit next to the plane and on it.
+* Rupturing fault: J = 1/2 sum (T(theta) - target)^2 over the friction
+  parameters theta; the adjoint gradient along seeded smooth directions of
+  mu_s, mu_d, Dc and the initial shear traction against FP64 central
+  differences of J computed by the FP64 oracle. The friction law is
+  differentiated on the branch the FP32 forward run took; the difference
+  quotient is of the FP64 discrete map.
+* A core with a fault and no tape is refused.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = ["mu_s", "mu_d", "Dc", "C0", "Tx", "Ty", "Tz"]
+
+
+def _scenario(imethod, locked):
+    from cgfd import configs
+    sc = configs.small(imethod=imethod)
+    f = sc.faults[0]
+    if locked:
+        f.fields["mu_s"][:] = 1000.0
+    i0 = f.x_index
+    j1, j2, k1, k2 = f.grid
+    jc = (j1 + j2) // 2
+    extra = [[i0 - 1, jc, sc.nk - 3], [i0 + 1, jc + 2, sc.nk - 5], [i0 - 3, jc - 3, sc.nk - 2],
+             [i0 + 2, jc + 1, sc.nk - 1], [i0, jc - 1, sc.nk - 4], [i0, jc + 3, sc.nk - 1],
+             [i0 + 4, jc - 2, sc.nk - 6], [i0 - 5, jc + 4, sc.nk - 1]]
+    sc.stations = np.concatenate([sc.stations, np.array(extra)]).astype(np.int32)
+    return sc
+
+
+def _smooth_field(sc, seed, amp):
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = sc.shape_ext
+    w = np.zeros((9, nz, ny, nx), np.float32)
+    core = rng.standard_normal((9, sc.nk, sc.nj, sc.ni))
+    for ax in (1, 2, 3):
+        core = (np.roll(core, 1, ax) + 2 * core + np.roll(core, -1, ax)) / 4
+    scale = np.array([1.0] * 3 + [1e6] * 6)[:, None, None, None]
+    w[:, 3:-3, 3:-3, 3:-3] = (amp * scale * core).astype(np.float32)
+    return w
+
+
+@pytest.mark.parametrize("imethod", [1, 2])
+def test_fault_adjoint_linear_identity_locked(built_libs, imethod):
+    from cgfd import model
+    nt = 30
+    sc = _scenario(imethod, locked=True)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    core.set_stations(sc.stations + 3, nt)
+    core.adjoint_forward(nt)
+    w0 = _smooth_field(sc, 5, 1e-3)
+    core.write_wavefield(w0)
+    core.run(nt)
+    tr = core.read_stations()
+    rng = np.random.default_rng(9)
+    r = rng.standard_normal(tr.shape).astype(np.float32)
+    r[:, :, 3:] *= 1e-6
+    g, gf = core.adjoint_fault(r)
+    lhs = float(np.sum(g.astype(np.float64) * w0.astype(np.float64)))
+    rhs = float(np.sum(r.astype(np.float64) * tr.astype(np.float64)))
+    print(f"[fault adjoint locked, method {imethod}] <g, w0> = {lhs:.9e}, sum r.T(w0) = {rhs:.9e}, "
+          f"rel {abs(lhs - rhs) / abs(rhs):.2e}", flush=True)
+    assert np.isfinite(g).all() and np.isfinite(gf).all()
+    assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
+    core.close()
+    st.close()
+
+
+def _traces_fp64(sc, nt):
+    from cgfd import model
+    import oracle
+    st = model.Setup(sc)
+    orc = oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=True)
+    ijk = sc.stations + 3
+    rec = []
+    for _ in range(nt):
+        x = orc.read_wavefield()
+        rec.append([x[:, k, j, i] for (i, j, k) in ijk])
+        orc.run(1)
+    orc.close()
+    st.close()
+    return np.transpose(np.array(rec, np.float64), (1, 0, 2))
+
+
+def _direction(sc, name, seed):
+    """Smooth seeded field on the fault's own grid (zero outside it)."""
+    rng = np.random.default_rng(seed)
+    j1, j2, k1, k2 = sc.faults[0].grid
+    d = np.zeros((sc.nk, sc.nj))
+    core = rng.standard_normal((k2 - k1 + 1, j2 - j1 + 1))
+    for ax in (0, 1):
+        core = (np.roll(core, 1, ax) + 2 * core + np.roll(core, -1, ax)) / 4
+    d[k1:k2 + 1, j1:j2 + 1] = core
+    return d
+
+
+# parameter, step size of the difference quotient (units of the parameter)
+STEPS = [("mu_s", 2e-4), ("mu_d", 2e-4), ("Dc", 2e-5), ("Ty", 2e3)]
+
+
+@pytest.mark.parametrize("imethod", [2, 1])
+def test_fault_adjoint_parameter_gradient_vs_fp64(built_libs, imethod):
+    from cgfd import model
+    import copy
+    nt = 40
+    sc = _scenario(imethod, locked=False)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    core.set_stations(sc.stations + 3, nt)
+    core.adjoint_forward(nt)
+    core.run(nt)
+    tr = core.read_stations()
+    # target: the traces of a seeded perturbation of Dc and the shear traction
+    sct = copy.deepcopy(sc)
+    sct.faults[0].fields["Dc"] += (0.05 * sc.faults[0].fields["Dc"] * _direction(sc, "Dc", 77)).astype(np.float32)
+    sct.faults[0].fields["Ty"] += (2e5 * _direction(sc, "Ty", 78)).astype(np.float32)
+    stt = model.Setup(sct)
+    ct = model.Core(stt)
+    ct.set_stations(sct.stations + 3, nt)
+    ct.run(nt)
+    target = ct.read_stations().astype(np.float64)
+    ct.close()
+    stt.close()
+    r = (tr - target).astype(np.float32)
+    _, gf = core.adjoint_fault(r)
+    assert np.isfinite(gf).all()
+
+    def j_fp64(name, d, eps):
+        s2 = copy.deepcopy(sc)
+        s2.faults[0].fields[name] = (s2.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32)
+        rec = _traces_fp64(s2, nt)
+        return 0.5 * float(np.sum((rec - target) ** 2))
+
+    worst = 0.0
+    for q, (name, eps) in enumerate(STEPS):
+        d = _direction(sc, name, 200 + q)
+        # the step actually taken in FP32 parameters
+        plus = (sc.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32).astype(np.float64)
+        minus = (sc.faults[0].fields[name].astype(np.float64) - eps * d).astype(np.float32).astype(np.float64)
+        fd = (j_fp64(name, d, eps) - j_fp64(name, d, -eps))
+        ad = float(np.sum(gf[0, PARAMS.index(name)].astype(np.float64) * (plus - minus)))
+        rel = abs(ad - fd) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        print(f"[fault adjoint method {imethod}] d{name}: adjoint {ad:.8e}, FP64 central difference {fd:.8e}, "
+              f"rel {rel:.2e}", flush=True)
+    assert worst <= 2e-2
+    core.close()
+    st.close()
+
+
+def test_fault_adjoint_needs_tape(built_libs):
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    core.set_stations(sc.stations + 3, 4)
+    core.run(4)
+    r = np.zeros((len(sc.stations), 4, 9), np.float32)
+    nz, ny, nx = sc.shape_ext
+    out = np.zeros((9, nz, ny, nx), np.float32)
+    rc = core.lib.cgfd_adj_backward(core.handle, r.ctypes.data_as(C.c_void_p), 4, out.ctypes.data_as(C.c_void_p))
+    assert rc == -4
+    assert b"friction tape" in core.lib.cgfd_last_error()
+    core.close()
+    st.close()
