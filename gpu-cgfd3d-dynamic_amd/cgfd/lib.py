@@ -106,7 +106,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_state_bytes": (I32, [VP, C.POINTER(C.c_size_t)]),
         "cgfd_adj_backward": (I32, [VP, VP, I32, VP]),
         "cgfd_adj_forward": (I32, [VP, I32]),
-        "cgfd_adj_backward_fault": (I32, [VP, VP, I32, VP, VP]),
+        "cgfd_adj_backward_fault": (I32, [VP, VP, VP, I32, VP, VP]),
+        "cgfd_core_set_fault_receivers": (I32, [VP, I32, VP, I32]),
+        "cgfd_core_read_fault_receivers": (I32, [VP, VP, VP]),
         "cgfd_core_save_state": (I32, [VP, VP, I32]),
         "cgfd_core_load_state": (I32, [VP, VP, I32]),
         "cgfd_core_autotune": (I32, [VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(F32)]),

@@ -169,20 +169,42 @@ class Core:
         at the step the stations were set, before running them)."""
         L.check(self.lib, self.lib.cgfd_adj_forward(self.handle, int(nt)))
 
-    def adjoint_fault(self, residual: np.ndarray):
+    def adjoint_fault(self, residual: np.ndarray, residual_fault: np.ndarray | None = None):
         """(dJ/dw at the first recorded step, dJ/d(fault parameters)) for
-        J = sum residual . traces; the second is [nfault][7][nk][nj] over
-        (mu_s, mu_d, Dc, C0, T0x, T0y, T0z)."""
+        J = sum residual . traces + sum residual_fault . fault-receiver
+        records; the second is [nfault][7][nk][nj] over (mu_s, mu_d, Dc, C0,
+        T0x, T0y, T0z)."""
         residual = np.ascontiguousarray(residual, np.float32)
         nsta, nt, nc = residual.shape
         if nc != 9:
             raise ValueError("residual must be [nsta][nt][9]")
+        rf = None
+        if residual_fault is not None:
+            rf = np.ascontiguousarray(residual_fault, np.float32)
+            if rf.ndim != 3 or rf.shape[1] != nt or rf.shape[2] != 9:
+                raise ValueError("residual_fault must be [nrec][nt][9]")
         nz, ny, nx = self.shape_ext
         out = np.zeros((9, nz, ny, nx), np.float32)
         sc = self.setup.sc
         gf = np.zeros((len(sc.faults), 7, sc.nk, sc.nj), np.float32)
-        L.check(self.lib, self.lib.cgfd_adj_backward_fault(self.handle, L.ptr(residual), nt, L.ptr(out), L.ptr(gf)))
+        L.check(self.lib, self.lib.cgfd_adj_backward_fault(self.handle, L.ptr(residual), L.ptr(rf) if rf is not None
+                                                           else None, nt, L.ptr(out), L.ptr(gf)))
         return out, gf
+
+    def set_fault_receivers(self, fjk_ext: np.ndarray, nt_max: int):
+        """Receivers at (fault id, ghost-inclusive j, k) nodes of the fault
+        planes, recording Tn..Slip2 at every step start."""
+        a = np.ascontiguousarray(fjk_ext, np.int32).reshape(-1, 3)
+        L.check(self.lib, self.lib.cgfd_core_set_fault_receivers(self.handle, a.shape[0], L.ptr(a), nt_max))
+        self._nfr = a.shape[0]
+        self._nfr_nt = nt_max
+
+    def read_fault_receivers(self) -> np.ndarray:
+        out = np.zeros((self._nfr, self._nfr_nt, 9), np.float32)
+        nt = C.c_int(0)
+        L.check(self.lib, self.lib.cgfd_core_read_fault_receivers(self.handle, L.ptr(out), C.byref(nt)))
+        flat = out.reshape(-1)[: self._nfr * nt.value * 9]
+        return flat.reshape(self._nfr, nt.value, 9).copy()
 
     def state_bytes(self) -> int:
         n = C.c_size_t()

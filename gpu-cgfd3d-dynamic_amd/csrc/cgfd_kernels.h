@@ -134,8 +134,9 @@ struct AdjPml {
 /* fault part of the adjoint (k_adj_fault.hip): per fault a block of
    kAdjFaultFloats x sslice floats: split-node adjoint levels [4][18] (E, A,
    B, P by index), T1 [21], hT1 [3], mT1 [3], Slip [1], parameter gradients
-   [7] (mu_s, mu_d, Dc, C0, T0x, T0y, T0z) */
-constexpr int kAdjFaultFloats = 107;
+   [7] (mu_s, mu_d, Dc, C0, T0x, T0y, T0z), Slip1 and Slip2 [2], the
+   receiver residuals of Vs, Vs1, Vs2 for the step being reversed [3] */
+constexpr int kAdjFaultFloats = 112;
 struct AdjFaultArgs {
   Grid g;
   const float *met, *lam, *mu, *slw, *matVx2Vz, *matVy2Vz;
@@ -151,6 +152,10 @@ struct AdjFaultArgs {
   float *wX, *wE;          /* of the stage output (null at stage 3) and of the end level */
 };
 void launch_adjf_begin(const AdjFaultArgs &a, hipStream_t st);
+/* residuals of fault-receiver sample `sample` ([nrec][nt][9], Tn..Slip2) into
+   the adjoint of the outputs the step before it produced */
+void launch_adjf_receivers(const AdjFaultArgs &a, const int *rec, int nrec, const float *res, int nt, int sample,
+                           hipStream_t st);
 void launch_adjf_stage_pre(const AdjFaultArgs &a, hipStream_t st);
 void launch_adjf_stage_post(const AdjFaultArgs &a, hipStream_t st);
 void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
@@ -165,6 +170,8 @@ void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int 
                        hipStream_t st);
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
                             int nt_max, const int *it_dev, int it0, hipStream_t st);
+void launch_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
+                         int it0, long ss, hipStream_t st);
 
 }  // namespace cgfd
 

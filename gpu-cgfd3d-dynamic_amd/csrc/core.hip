@@ -102,6 +102,10 @@ struct cgfd_core {
   long *st_idx = nullptr;
   float *st_buf = nullptr;
   int st_n = 0, st_nt = 0, st_it0 = 0;
+  /* fault receivers: (fault id, slice index) pairs, [n][nt][9] records */
+  int *fr_idx = nullptr;
+  float *fr_buf = nullptr;
+  int fr_n = 0, fr_nt = 0, fr_it0 = 0;
 
   /* eight steps (one cycle of the operator pairs; an even number of level
      swaps) captured once and replayed */
@@ -688,6 +692,8 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
   if (c->st_n > 0)
     launch_record_stations(c->bat, c->wlev(c->lv_pre), c->g.sicmp, c->st_idx, c->st_n, c->st_buf, c->st_nt,
                            c->it_dev, c->st_it0, c->st);
+  if (c->fr_n > 0)
+    launch_record_fault(c->d_ftab, c->fr_idx, c->fr_n, c->fr_buf, c->fr_nt, c->it_dev, c->fr_it0, c->g.sslice, c->st);
   for (int s = 0; s < 4; s++) {
     const int idir = (kPairs[ipair][0] + s) % 2;
     const int jdir = (kPairs[ipair][1] + s) % 2;
@@ -946,7 +952,8 @@ int check_fault_columns(cgfd_core *c) {
   return CGFD_OK;
 }
 
-int adj_sweep(cgfd_core *c, const float *residual, int nt, float *grad, float *grad_fault) {
+int adj_sweep(cgfd_core *c, const float *residual, const float *residual_fault, int nt, float *grad,
+              float *grad_fault) {
   if (!c || !residual) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
   if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "adjoint: no stations set");
@@ -955,6 +962,17 @@ int adj_sweep(cgfd_core *c, const float *residual, int nt, float *grad, float *g
     return cgfd_set_error(CGFD_ERR_STATE,
                           "adjoint: a core with a fault needs the friction tape: call cgfd_adj_forward with at least "
                           "%d steps at the step the stations were set, before running", nt);
+  if (residual_fault) {
+    if (c->fr_n <= 0 || c->fr_it0 != c->st_it0 || c->fr_nt < nt)
+      return cgfd_set_error(CGFD_ERR_STATE,
+                            "adjoint: fault-receiver residuals need receivers set at the step the stations were set, "
+                            "with at least %d samples", nt);
+    for (size_t q = 0; q < (size_t)c->fr_n * nt; q++)
+      for (int v = 0; v < 3; v++)
+        if (residual_fault[9 * q + v] != 0.0f)
+          return cgfd_set_error(CGFD_ERR_ARG,
+                                "adjoint: fault-receiver residuals on Tn, Ts1, Ts2 are not supported (must be zero)");
+  }
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   const Grid &g = c->g;
@@ -1012,10 +1030,14 @@ int adj_sweep(cgfd_core *c, const float *residual, int nt, float *grad, float *g
   fa.dt = c->dt;
   fa.le = 0;
   fa.lp = 3;
-  float *res = nullptr;
-  const size_t nres = (size_t)c->st_n * nt * 9;
-  HIPCHK(hipMalloc(&res, nres * 4));
+  float *res = nullptr, *resf = nullptr;
+  const size_t nres = (size_t)c->st_n * nt * 9, nresf = residual_fault ? (size_t)c->fr_n * nt * 9 : 0;
+  HIPCHK(hipMalloc(&res, (nres + nresf) * 4));
   hipError_t e = hipMemcpyAsync(res, residual, nres * 4, hipMemcpyHostToDevice, c->st);
+  if (residual_fault && e == hipSuccess) {
+    resf = res + nres;
+    e = hipMemcpyAsync(resf, residual_fault, nresf * 4, hipMemcpyHostToDevice, c->st);
+  }
   float *E = c->adj, *A = c->adj + lv, *B = c->adj + 2 * lv, *P = c->adj + 3 * lv;
   for (int n = nt - 1; n >= 0 && e == hipSuccess; n--) {
     if (n < nt - 1) {
@@ -1023,6 +1045,7 @@ int adj_sweep(cgfd_core *c, const float *residual, int nt, float *grad, float *g
       const int ipair = it % 8;
       fa.n = n;
       fa.wE = E;
+      launch_adjf_receivers(fa, c->fr_idx, c->fr_n, resf, nt, n + 1, c->st);
       launch_adjf_begin(fa, c->st);
       for (int s = 3; s >= 0; s--) {
         const int idir = (kPairs[ipair][0] + s) % 2, jdir = (kPairs[ipair][1] + s) % 2, kdir = (kPairs[ipair][2] + s) % 2;
@@ -1070,12 +1093,57 @@ int adj_sweep(cgfd_core *c, const float *residual, int nt, float *grad, float *g
 
 extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
   if (!grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
-  return adj_sweep(c, residual, nt, grad, nullptr);
+  return adj_sweep(c, residual, nullptr, nt, grad, nullptr);
 }
 
-extern "C" int cgfd_adj_backward_fault(cgfd_core *c, const float *residual, int nt, float *grad, float *grad_fault) {
+extern "C" int cgfd_adj_backward_fault(cgfd_core *c, const float *residual, const float *residual_fault, int nt,
+                                       float *grad, float *grad_fault) {
   if (!grad_fault) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
-  return adj_sweep(c, residual, nt, grad, grad_fault);
+  return adj_sweep(c, residual, residual_fault, nt, grad, grad_fault);
+}
+
+extern "C" int cgfd_core_set_fault_receivers(cgfd_core *c, int n, const int *fjk, int nt_max) {
+  if (!c || n < 0 || (n > 0 && (!fjk || nt_max <= 0))) return cgfd_set_error(CGFD_ERR_ARG, "bad receiver arguments");
+  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "fault receivers: single-scenario cores only");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  std::vector<int> idx(2 * (size_t)n);
+  for (int r = 0; r < n; r++) {
+    const int id = fjk[3 * r], j = fjk[3 * r + 1], k = fjk[3 * r + 2];
+    if (id < 0 || id >= c->nfault || j < 3 || j > c->g.nj + 2 || k < 3 || k > c->g.nk + 2)
+      return cgfd_set_error(CGFD_ERR_ARG, "fault receiver %d (fault %d, j=%d, k=%d) is outside the fault planes", r, id,
+                            j, k);
+    idx[2 * r] = id;
+    idx[2 * r + 1] = j + k * c->g.ny;
+  }
+  int rc;
+  if (n > 0) {
+    if ((rc = c->alloc(&c->fr_idx, idx.size()))) return rc;
+    HIPCHK(hipMemcpy(c->fr_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = c->alloc(&c->fr_buf, (size_t)n * nt_max * 9))) return rc;
+    HIPCHK(hipMemset(c->fr_buf, 0, (size_t)n * nt_max * 9 * 4));
+  }
+  c->fr_n = n;
+  c->fr_nt = nt_max;
+  c->fr_it0 = c->it_host;
+  c->drop_graph();
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_fault_receivers(cgfd_core *c, float *dst, int *nt) {
+  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  int rec = c->it_host - c->fr_it0;
+  if (rec > c->fr_nt) rec = c->fr_nt;
+  if (rec < 0) rec = 0;
+  if (nt) *nt = rec;
+  if (c->fr_n == 0) return CGFD_OK;
+  std::vector<float> h((size_t)c->fr_n * c->fr_nt * 9);
+  HIPCHK(hipMemcpy(h.data(), c->fr_buf, h.size() * 4, hipMemcpyDeviceToHost));
+  for (int q = 0; q < c->fr_n; q++)
+    std::memcpy(dst + (size_t)q * rec * 9, &h[(size_t)q * c->fr_nt * 9], (size_t)rec * 9 * 4);
+  return CGFD_OK;
 }
 
 extern "C" int cgfd_core_set_overlap(cgfd_core *c, int on) {

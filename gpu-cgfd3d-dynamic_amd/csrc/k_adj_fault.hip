@@ -47,7 +47,8 @@ enum { VX = 0, VY, VZ, TXX, TYY, TZZ, TYZ, TXZ, TXY };
 DEVI long fidx(int c, int m, long ss) { return (long)(2 * c + m) * ss; }
 
 /* per-fault adjoint block, offsets in units of sslice */
-constexpr int kOffT1 = 72, kOffHT1 = 93, kOffMT1 = 96, kOffSlip = 99, kOffGrad = 100;
+constexpr int kOffT1 = 72, kOffHT1 = 93, kOffMT1 = 96, kOffSlip = 99, kOffGrad = 100, kOffSlip12 = 107,
+              kOffVsr = 109;
 
 DEVI void aadd(float *p, float v) { atomicAdd(p, v); }
 
@@ -168,7 +169,9 @@ __global__ void __launch_bounds__(256) k_adjf_begin(AdjFaultArgs a) {
     aM[c * ss + pf] = -h;
     aH[c * ss + pf] = 0.0f;
   }
-  /* Slip += |V_s| dt with V_s the projections of the velocity jump */
+  /* Vs1, Vs2 = projections of the velocity jump, Vs = |(Vs1, Vs2)|;
+     Slip += Vs dt, Slip1 += Vs1 dt, Slip2 += Vs2 dt; plus the receiver
+     residuals of Vs, Vs1, Vs2 of this step */
   const float *tp = a.tape + ((long)id * a.tape_nt + a.n) * kTapeFloats * ss + pf;
   float dV[3] = {tp[13 * ss], tp[14 * ss], tp[15 * ss]};
   const float s1[3] = {F.vs[pf], F.vs[ss + pf], F.vs[2 * ss + pf]};
@@ -176,14 +179,20 @@ __global__ void __launch_bounds__(256) k_adjf_begin(AdjFaultArgs a) {
   const float Vs1 = dV[0] * s1[0] + dV[1] * s1[1] + dV[2] * s1[2];
   const float Vs2 = dV[0] * s2[0] + dV[1] * s2[1] + dV[2] * s2[2];
   const float Vs = sqrtf(Vs2 * Vs2 + Vs1 * Vs1);
+  float *vr = blk + kOffVsr * ss + pf;
+  const float aVs = vr[0] + blk[kOffSlip * ss + pf] * dt;
+  float aV1 = vr[ss] + blk[kOffSlip12 * ss + pf] * dt;
+  float aV2 = vr[2 * ss] + blk[(kOffSlip12 + 1) * ss + pf] * dt;
+  vr[0] = vr[ss] = vr[2 * ss] = 0.0f;
   if (Vs > 0.0f) {
-    const float aVs = blk[kOffSlip * ss + pf] * dt / Vs;
+    aV1 += aVs * Vs1 / Vs;
+    aV2 += aVs * Vs2 / Vs;
+  }
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-      const float g = aVs * (Vs1 * s1[c] + Vs2 * s2[c]);
-      aadd(fE + fidx(c, 1, ss) + pf, g);
-      aadd(fE + fidx(c, 0, ss) + pf, -g);
-    }
+  for (int c = 0; c < 3; c++) {
+    const float g = aV1 * s1[c] + aV2 * s2[c];
+    aadd(fE + fidx(c, 1, ss) + pf, g);
+    aadd(fE + fidx(c, 0, ss) + pf, -g);
   }
   /* fault2wave onto the end level */
   const long p0 = F.i0 + (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
@@ -194,6 +203,25 @@ __global__ void __launch_bounds__(256) k_adjf_begin(AdjFaultArgs a) {
     a.wE[f * sc + p0] = 0.0f;
   }
   t_fault2wave(a, pf, p0, aw, fE, aT1);
+}
+
+/* fault-receiver residuals of one sample (the outputs after the step being
+   reversed): Slip, Slip1, Slip2 into their running adjoints, Vs, Vs1, Vs2
+   into this step's; welded nodes output nothing */
+__global__ void k_adjf_receivers(AdjFaultArgs a, const int *rec, int nrec, const float *res, int nt, int sample) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  const int id = rec[2 * r];
+  const long pf = rec[2 * r + 1], ss = a.g.sslice;
+  if (a.ftab[id].united[pf] != 0) return;
+  float *blk = adj_block(a, id);
+  const float *q = res + ((long)r * nt + sample) * 9;
+  aadd(blk + kOffVsr * ss + pf, q[3]);
+  aadd(blk + (kOffVsr + 1) * ss + pf, q[4]);
+  aadd(blk + (kOffVsr + 2) * ss + pf, q[5]);
+  aadd(blk + kOffSlip * ss + pf, q[6]);
+  aadd(blk + kOffSlip12 * ss + pf, q[7]);
+  aadd(blk + (kOffSlip12 + 1) * ss + pf, q[8]);
 }
 
 /* RK structure of the split-node levels at reverse stage MODE (the
@@ -734,6 +762,12 @@ static dim3 node_grid(const AdjFaultArgs &a) { return dim3((a.g.nj + 63) / 64, (
 void launch_adjf_begin(const AdjFaultArgs &a, hipStream_t st) {
   if (a.nfault <= 0) return;
   hipLaunchKernelGGL(k_adjf_begin, node_grid(a), dim3(64, 4), 0, st, a);
+}
+
+void launch_adjf_receivers(const AdjFaultArgs &a, const int *rec, int nrec, const float *res, int nt, int sample,
+                           hipStream_t st) {
+  if (a.nfault <= 0 || nrec <= 0 || !res) return;
+  hipLaunchKernelGGL(k_adjf_receivers, dim3((nrec + 63) / 64), dim3(64), 0, st, a, rec, nrec, res, nt, sample);
 }
 
 void launch_adjf_stage_pre(const AdjFaultArgs &a, hipStream_t st) {

@@ -850,6 +850,24 @@ void launch_nan_check(const float *p, long n, int *flag, hipStream_t st) {
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_nan_check, dim3((unsigned)blocks), dim3(256), 0, st, p, n, flag);
 }
+/* fault receivers: the nine outputs Tn, Ts1, Ts2, Vs, Vs1, Vs2, Slip, Slip1,
+   Slip2 at each receiver's node, recorded at the step start
+   (io_fault_recv_keep, io_funcs.cu:2313-2351, at grid nodes) */
+__global__ void k_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
+                               int it0, long ss) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * 9) return;
+  const int r = t / 9, v = t % 9;
+  const int it = *it_dev - it0;
+  if (it < 0 || it >= nt_max) return;
+  buf[((long)r * nt_max + it) * 9 + v] = ftab[rec[2 * r]].out[v * ss + rec[2 * r + 1]];
+}
+void launch_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
+                         int it0, long ss, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_record_fault, dim3((n * 9 + 255) / 256), dim3(256), 0, st, ftab, rec, n, buf, nt_max, it_dev,
+                     it0, ss);
+}
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
                             int nt_max, const int *it_dev, int it0, hipStream_t st) {
   if (n <= 0) return;

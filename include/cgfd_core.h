@@ -193,12 +193,19 @@ int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
    receives dJ/dw at the first recorded step.
    cgfd_adj_backward_fault: the same (grad may be null) and
    grad_fault[nfault][7][nk][nj] receives dJ/d(mu_s, mu_d, Dc, C0, T0x, T0y,
-   T0z) at every fault node (zero at welded nodes).
+   T0z) at every fault node (zero at welded nodes). With fault receivers set
+   at the same step (cgfd_core_set_fault_receivers), residual_fault
+   [nrec][nt][9] (the layout of cgfd_core_read_fault_receivers, may be null)
+   adds sum_n sum_r residual_fault[r][n] . out_n(r) to J over the slip-rate
+   and slip outputs Vs, Vs1, Vs2, Slip, Slip1, Slip2; its traction entries
+   (Tn, Ts1, Ts2) must be zero (CGFD_ERR_ARG otherwise), and sample 0, which
+   precedes the recorded steps, carries no gradient.
    Faults whose split nodes have fault columns inside a CFS-PML slab are
    refused (CGFD_ERR_ARG). The core's forward state is not touched. */
 int cgfd_adj_forward(cgfd_core *core, int nt);
 int cgfd_adj_backward(cgfd_core *core, const float *residual, int nt, float *grad);
-int cgfd_adj_backward_fault(cgfd_core *core, const float *residual, int nt, float *grad, float *grad_fault);
+int cgfd_adj_backward_fault(cgfd_core *core, const float *residual, const float *residual_fault, int nt, float *grad,
+                            float *grad_fault);
 
 /* Stage fork (default off): within each RK stage, after the trial traction,
    the interior volume blocks, the CFS-PML blocks with the top layers, and
@@ -243,6 +250,17 @@ int cgfd_core_read_pg(cgfd_core *core, float *host_dst);
 int cgfd_core_set_stations(cgfd_core *core, int n, const int *ijk, int nt_max);
 /* host_dst[n][nt_recorded][9]; returns the number of recorded samples in *nt */
 int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
+
+/* Fault receivers (io_fault_recv_keep, io_funcs.cu:2313-2351): n receivers
+   fjk[n][3] = (fault id, ghost-inclusive j, k) on grid nodes of the fault
+   planes. The core records the nine fault outputs Tn, Ts1, Ts2, Vs, Vs1,
+   Vs2, Slip, Slip1, Slip2 at every step before the step is taken (as the
+   reference's call at drv_rk_curv_col.cu:217) into nt_max samples; reading
+   gives host_dst[n][nt_recorded][9]. The reference interpolates between
+   nodes (dj, dk); receivers here sit on nodes, its dj = dk = 0 case.
+   Single-scenario cores. */
+int cgfd_core_set_fault_receivers(cgfd_core *core, int n, const int *fjk, int nt_max);
+int cgfd_core_read_fault_receivers(cgfd_core *core, float *host_dst, int *nt);
 
 /* Timing support: runs nsteps and returns the total elapsed device time of
    the stream (HIP events) in *ms_total and the summed duration of the volume

@@ -7,8 +7,10 @@ through the C ABI (cgfd_adj_forward / cgfd_adj_backward_fault).
   trial, fault2wave, fault columns, split nodes, hT1 recurrence, free-surface
   rows). The adjoint g = T^T r must satisfy <g, w0> = sum r . T(w0), for both
   split-node methods. Stations sit next to the plane and on it.
-* Rupturing fault: J = 1/2 sum (T(theta) - target)^2 over the friction
-  parameters theta; the adjoint gradient along seeded smooth directions of
+* Rupturing fault: J_tr = 1/2 sum (T(theta) - target)^2 over the station
+  traces and J_vs = 1/2 sum (V_s(theta) - V_s target)^2 over the slip rate
+  at fault receivers, as functions of the friction parameters theta; the
+  adjoint gradients along seeded smooth directions of
   mu_s, mu_d, Dc and the initial shear traction against FP64 central
   differences of J computed by the FP64 oracle. The friction law is
   differentiated on the branch the FP32 forward run took; the difference
@@ -80,20 +82,30 @@ def test_fault_adjoint_linear_identity_locked(built_libs, imethod):
     st.close()
 
 
-def _traces_fp64(sc, nt):
+def _traces_fp64(sc, nt, frec):
+    """Station traces and fault-receiver slip rates of the FP64 oracle."""
     from cgfd import model
     import oracle
     st = model.Setup(sc)
     orc = oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=True)
     ijk = sc.stations + 3
-    rec = []
+    rec, vs = [], []
     for _ in range(nt):
         x = orc.read_wavefield()
         rec.append([x[:, k, j, i] for (i, j, k) in ijk])
+        f = orc.read_fault(0)
+        vs.append([f[3, k, j] for (_, j, k) in frec])
         orc.run(1)
     orc.close()
     st.close()
-    return np.transpose(np.array(rec, np.float64), (1, 0, 2))
+    return np.transpose(np.array(rec, np.float64), (1, 0, 2)), np.array(vs, np.float64).T
+
+
+def _fault_receivers(sc):
+    """(fault id, ghost-inclusive j, k) around the nucleation patch"""
+    j1, j2, k1, k2 = sc.faults[0].grid
+    jc, kc = (j1 + j2) // 2, sc.nk - 1 - 9
+    return np.array([[0, jc + dj + 3, kc + dk + 3] for dj in (-6, -3, 0, 2, 5) for dk in (-4, 0, 3)], np.int32)
 
 
 def _direction(sc, name, seed):
@@ -118,12 +130,15 @@ def test_fault_adjoint_parameter_gradient_vs_fp64(built_libs, imethod):
     import copy
     nt = 40
     sc = _scenario(imethod, locked=False)
+    frec = _fault_receivers(sc)
     st = model.Setup(sc)
     core = model.Core(st)
     core.set_stations(sc.stations + 3, nt)
+    core.set_fault_receivers(frec, nt)
     core.adjoint_forward(nt)
     core.run(nt)
     tr = core.read_stations()
+    fr = core.read_fault_receivers()
     # target: the traces of a seeded perturbation of Dc and the shear traction
     sct = copy.deepcopy(sc)
     sct.faults[0].fields["Dc"] += (0.05 * sc.faults[0].fields["Dc"] * _direction(sc, "Dc", 77)).astype(np.float32)
@@ -131,19 +146,25 @@ def test_fault_adjoint_parameter_gradient_vs_fp64(built_libs, imethod):
     stt = model.Setup(sct)
     ct = model.Core(stt)
     ct.set_stations(sct.stations + 3, nt)
+    ct.set_fault_receivers(frec, nt)
     ct.run(nt)
     target = ct.read_stations().astype(np.float64)
+    vs_target = ct.read_fault_receivers()[:, :, 3].astype(np.float64)
     ct.close()
     stt.close()
+    assert float(np.abs(fr[:, :, 3]).max()) > 0.05, "the receivers should see the rupture"
     r = (tr - target).astype(np.float32)
-    _, gf = core.adjoint_fault(r)
-    assert np.isfinite(gf).all()
+    rf = np.zeros_like(fr)
+    rf[:, :, 3] = fr[:, :, 3] - vs_target
+    _, gf_tr = core.adjoint_fault(r)
+    _, gf_vs = core.adjoint_fault(np.zeros_like(r), rf)
+    assert np.isfinite(gf_tr).all() and np.isfinite(gf_vs).all()
 
     def j_fp64(name, d, eps):
         s2 = copy.deepcopy(sc)
         s2.faults[0].fields[name] = (s2.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32)
-        rec = _traces_fp64(s2, nt)
-        return 0.5 * float(np.sum((rec - target) ** 2))
+        rec, vs = _traces_fp64(s2, nt, frec)
+        return np.array([0.5 * float(np.sum((rec - target) ** 2)), 0.5 * float(np.sum((vs[:, 1:] - vs_target[:, 1:]) ** 2))])
 
     worst = 0.0
     for q, (name, eps) in enumerate(STEPS):
@@ -151,12 +172,13 @@ def test_fault_adjoint_parameter_gradient_vs_fp64(built_libs, imethod):
         # the step actually taken in FP32 parameters
         plus = (sc.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32).astype(np.float64)
         minus = (sc.faults[0].fields[name].astype(np.float64) - eps * d).astype(np.float32).astype(np.float64)
-        fd = (j_fp64(name, d, eps) - j_fp64(name, d, -eps))
-        ad = float(np.sum(gf[0, PARAMS.index(name)].astype(np.float64) * (plus - minus)))
-        rel = abs(ad - fd) / max(abs(fd), 1e-300)
-        worst = max(worst, rel)
-        print(f"[fault adjoint method {imethod}] d{name}: adjoint {ad:.8e}, FP64 central difference {fd:.8e}, "
-              f"rel {rel:.2e}", flush=True)
+        fd = j_fp64(name, d, eps) - j_fp64(name, d, -eps)
+        for t, (lab, gf) in enumerate((("traces", gf_tr), ("slip rate", gf_vs))):
+            ad = float(np.sum(gf[0, PARAMS.index(name)].astype(np.float64) * (plus - minus)))
+            rel = abs(ad - fd[t]) / max(abs(fd[t]), 1e-300)
+            worst = max(worst, rel)
+            print(f"[fault adjoint method {imethod}] {lab} d{name}: adjoint {ad:.8e}, FP64 central difference "
+                  f"{fd[t]:.8e}, rel {rel:.2e}", flush=True)
     assert worst <= 2e-2
     core.close()
     st.close()
