@@ -108,6 +108,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_adj_forward": (I32, [VP, I32]),
         "cgfd_adj_backward_fault": (I32, [VP, VP, VP, I32, VP, VP]),
         "cgfd_core_set_fault_receivers": (I32, [VP, I32, VP, I32]),
+        "cgfd_core_read_slice": (I32, [VP, I32, I32, VP]),
+        "cgfd_core_read_snapshot": (I32, [VP, VP, I32, VP]),
+        "cgfd_batch_read_snapshot": (I32, [VP, I32, VP, I32, VP]),
         "cgfd_core_read_fault_receivers": (I32, [VP, VP, VP]),
         "cgfd_core_save_state": (I32, [VP, VP, I32]),
         "cgfd_core_load_state": (I32, [VP, VP, I32]),

@@ -191,6 +191,24 @@ class Core:
                                                            else None, nt, L.ptr(out), L.ptr(gf)))
         return out, gf
 
+    def read_slice(self, axis: int, index_ext: int) -> np.ndarray:
+        """The nine components on the plane at ghost-inclusive `index_ext`
+        along axis 0 (x: [9][nk][nj]), 1 (y: [9][nk][ni]) or 2 (z: [9][nj][ni])."""
+        sc = self.setup.sc
+        shape = {0: (9, sc.nk, sc.nj), 1: (9, sc.nk, sc.ni), 2: (9, sc.nj, sc.ni)}[axis]
+        out = np.empty(shape, np.float32)
+        L.check(self.lib, self.lib.cgfd_core_read_slice(self.handle, axis, index_ext, L.ptr(out)))
+        return out
+
+    def read_snapshot(self, box, flags: int = 3) -> np.ndarray:
+        """box = (i1, ni, di, j1, nj, dj, k1, nk, dk), ghost-inclusive starts;
+        flags: 1 velocity, 2 stress, 4 strain; [nfield][nk][nj][ni]."""
+        b = np.ascontiguousarray(box, np.int32)
+        nf = (3 if flags & 1 else 0) + (6 if flags & 2 else 0) + (6 if flags & 4 else 0)
+        out = np.empty((nf, int(b[7]), int(b[4]), int(b[1])), np.float32)
+        L.check(self.lib, self.lib.cgfd_core_read_snapshot(self.handle, L.ptr(b), flags, L.ptr(out)))
+        return out
+
     def set_fault_receivers(self, fjk_ext: np.ndarray, nt_max: int):
         """Receivers at (fault id, ghost-inclusive j, k) nodes of the fault
         planes, recording Tn..Slip2 at every step start."""

@@ -172,6 +172,12 @@ void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const 
                             int nt_max, const int *it_dev, int it0, hipStream_t st);
 void launch_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
                          int it0, long ss, hipStream_t st);
+/* output boxes (k_io.hip): start, count, stride per axis, ghost-inclusive */
+struct IoBox {
+  int s[3], n[3], d[3];
+};
+void launch_pack_box(const float *w, const Grid &g, const float *lam, const float *mu, const IoBox &b, int flags,
+                     float *out, hipStream_t st);
 
 }  // namespace cgfd
 

@@ -1329,6 +1329,49 @@ extern "C" int cgfd_batch_read_stations(cgfd_core *c, int s, float *dst, int *nt
     std::memcpy(dst + (size_t)q * rec * 9, &h[(size_t)q * c->st_nt * 9], (size_t)rec * 9 * 4);
   return CGFD_OK;
 }
+extern "C" int cgfd_batch_read_snapshot(cgfd_core *c, int s, const int *box, int flags, float *dst) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!box || !dst || flags <= 0 || flags > 7) return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot arguments");
+  const Grid &g = c->g;
+  IoBox b;
+  const int lo[3] = {3, 3, 3}, hi[3] = {g.ni + 2, g.nj + 2, g.nk + 2};
+  for (int a = 0; a < 3; a++) {
+    b.s[a] = box[3 * a];
+    b.n[a] = box[3 * a + 1];
+    b.d[a] = box[3 * a + 2];
+    if (b.n[a] < 1 || b.d[a] < 1 || b.s[a] < lo[a] || b.s[a] + (long)(b.n[a] - 1) * b.d[a] > hi[a])
+      return cgfd_set_error(CGFD_ERR_ARG, "snapshot box axis %d (start %d, count %d, stride %d) leaves the physical grid",
+                            a, b.s[a], b.n[a], b.d[a]);
+  }
+  const int groups = (flags & 1 ? 3 : 0) + (flags & 2 ? 6 : 0) + (flags & 4 ? 6 : 0);
+  const size_t n = (size_t)b.n[0] * b.n[1] * b.n[2] * groups;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  float *buf = nullptr;
+  HIPCHK(hipMalloc(&buf, n * 4));
+  launch_pack_box(c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, g, c->lam, c->mu, b, flags, buf, c->st);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, buf, n * 4, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "snapshot: %s", hipGetErrorString(e));
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_snapshot(cgfd_core *c, const int *box, int flags, float *dst) {
+  return cgfd_batch_read_snapshot(c, 0, box, flags, dst);
+}
+
+extern "C" int cgfd_core_read_slice(cgfd_core *c, int axis, int index, float *dst) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  const Grid &g = c->g;
+  int box[9] = {3, g.ni, 1, 3, g.nj, 1, 3, g.nk, 1};
+  if (axis < 0 || axis > 2) return cgfd_set_error(CGFD_ERR_ARG, "slice axis %d (0 = x, 1 = y, 2 = z)", axis);
+  box[3 * axis] = index;
+  box[3 * axis + 1] = 1;
+  return cgfd_batch_read_snapshot(c, 0, box, 3, dst);
+}
+
 extern "C" int cgfd_core_read_stations(cgfd_core *c, float *dst, int *nt) {
   return cgfd_batch_read_stations(c, 0, dst, nt);
 }

@@ -251,6 +251,23 @@ int cgfd_core_set_stations(cgfd_core *core, int n, const int *ijk, int nt_max);
 /* host_dst[n][nt_recorded][9]; returns the number of recorded samples in *nt */
 int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 
+/* Slices and snapshots of level "pre" (the state at the current step), packed
+   on the device so only the requested values are copied to the host.
+   cgfd_core_read_slice (io_slice_nc_put, io_funcs.cu:1013-1131, pack kernels
+   :1653-1689): the nine components Vx..Txy on the plane at ghost-inclusive
+   `index` along `axis` (0 = x: host_dst[9][nk][nj]; 1 = y: [9][nk][ni];
+   2 = z: [9][nj][ni]).
+   cgfd_core_read_snapshot (io_snap_nc_put, io_funcs.cu:1380-1583):
+   box = {i1, ni, di, j1, nj, dj, k1, nk, dk} (ghost-inclusive starts, counts,
+   strides, inside the physical grid); flags bit 0 velocity (3 fields), bit 1
+   stress (6), bit 2 strain (6, isotropic compliance of
+   io_snap_stress_to_strain_eliso, io_funcs.cu:1585-1651); host_dst holds the
+   selected groups in that order, each field [nk][nj][ni]. The batch form
+   reads scenario s. */
+int cgfd_core_read_slice(cgfd_core *core, int axis, int index, float *host_dst);
+int cgfd_core_read_snapshot(cgfd_core *core, const int *box, int flags, float *host_dst);
+int cgfd_batch_read_snapshot(cgfd_core *core, int s, const int *box, int flags, float *host_dst);
+
 /* Fault receivers (io_fault_recv_keep, io_funcs.cu:2313-2351): n receivers
    fjk[n][3] = (fault id, ghost-inclusive j, k) on grid nodes of the fault
    planes. The core records the nine fault outputs Tn, Ts1, Ts2, Vs, Vs1,

@@ -305,3 +305,42 @@ def test_ragged_grid_matches_oracle(built_libs, variant):
     core.close()
     orc.close()
     st.close()
+
+
+def test_slices_and_snapshots_match_the_wavefield(built_libs):
+    """Device-packed slices and strided snapshots (io_slice_nc_put /
+    io_snap_nc_put) against the full wavefield read back after a short
+    rupture: bit-identical velocity and stress; strain against the isotropic
+    compliance of io_snap_stress_to_strain_eliso (io_funcs.cu:1585-1651)
+    evaluated in FP64 from the same stress and media."""
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    core.run(25)
+    w = core.read_wavefield()
+    i0 = sc.faults[0].x_index + 3
+    assert np.array_equal(core.read_slice(0, i0 - 1), w[:, 3:-3, 3:-3, i0 - 1])
+    assert np.array_equal(core.read_slice(1, 20), w[:, 3:-3, 20, 3:-3])
+    assert np.array_equal(core.read_slice(2, sc.nk + 2), w[:, sc.nk + 2, 3:-3, 3:-3])
+    box = (4, 9, 3, 5, 7, 5, 6, 8, 2)
+    snap = core.read_snapshot(box, 7)
+    sl = (slice(box[6], box[6] + box[7] * box[8], box[8]), slice(box[3], box[3] + box[4] * box[5], box[5]),
+          slice(box[0], box[0] + box[1] * box[2], box[2]))
+    wb = w[(slice(None),) + sl]
+    assert np.array_equal(snap[:9], wb)
+    nz, ny, nx = sc.shape_ext
+    pad = lambda a: np.pad(a, 3, mode="edge")
+    lam, mu = pad(sc.lam.astype(np.float64))[sl], pad(sc.mu.astype(np.float64))[sl]
+    e1 = (lam + mu) / (mu * (3 * lam + 2 * mu))
+    e2 = -lam / (2 * mu * (3 * lam + 2 * mu))
+    t = wb[3:].astype(np.float64)
+    e0 = e2 * (t[0] + t[1] + t[2])
+    ref = np.stack([e0 - (e2 - e1) * t[0], e0 - (e2 - e1) * t[1], e0 - (e2 - e1) * t[2],
+                    0.5 / mu * t[3], 0.5 / mu * t[4], 0.5 / mu * t[5]])
+    err = np.abs(snap[9:] - ref).max() / np.abs(ref).max()
+    print(f"[snapshot] strain max rel err {err:.2e}", flush=True)
+    assert err <= 1e-6
+    assert np.array_equal(core.read_snapshot(box, 4), snap[9:])
+    core.close()
+    st.close()
