@@ -109,6 +109,13 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_adj_backward_fault": (I32, [VP, VP, VP, I32, VP, VP]),
         "cgfd_core_set_fault_receivers": (I32, [VP, I32, VP, I32]),
         "cgfd_core_read_slice": (I32, [VP, I32, I32, VP]),
+        "cgfd_io_fault_create": (I32, [VP, I32, C.c_char_p, C.c_char_p, VP]),
+        "cgfd_io_fault_put": (I32, [VP, VP, I32, C.c_float]),
+        "cgfd_io_fault_end": (I32, [VP, VP]),
+        "cgfd_io_slice_create": (I32, [VP, I32, I32, C.c_char_p, C.c_char_p, VP]),
+        "cgfd_io_slice_put": (I32, [VP, VP, I32, C.c_float]),
+        "cgfd_io_close": (I32, [VP]),
+        "cgfd_io_pg_write": (I32, [VP, C.c_char_p, C.c_char_p]),
         "cgfd_core_read_snapshot": (I32, [VP, VP, I32, VP]),
         "cgfd_batch_read_snapshot": (I32, [VP, I32, VP, I32, VP]),
         "cgfd_core_read_fault_receivers": (I32, [VP, VP, VP]),

@@ -24,6 +24,9 @@
 
 #include "../../include/cgfd_core.h"
 #include "../host/cgfd_errors.h"
+#include "../host/cgfd_nc.h"
+#include "../../include/cgfd_io.h"
+#include <string>
 #include "cgfd_kernels.h"
 
 using namespace cgfd;
@@ -1486,4 +1489,159 @@ extern "C" int cgfd_drv_rk_curv_col_allstep(const cgfd_desc *d, float *wave_out,
   if (!rc && pg_out) rc = cgfd_core_read_pg(c, pg_out);
   cgfd_core_destroy(c);
   return rc;
+}
+
+/* ---- output files in the reference's NetCDF layouts (io_funcs.cu:900-1131,
+   1827-1879, 1952-2139), from device-packed data ---- */
+struct cgfd_out {
+  cgfd::NcWriter w;
+  int kind = 0;             /* 0 fault, 1 slice */
+  int id = 0, axis = 0, index = 0;
+  int vtime = -1, vfirst = -1, vpeak = -1, vinit = -1;
+};
+
+namespace {
+std::string out_name(const char *dir, const char *stem, const char *frame) {
+  std::string p = dir && *dir ? std::string(dir) + "/" : std::string();
+  return p + stem + "_" + (frame && *frame ? frame : "px0_py0_pz0") + ".nc";
+}
+}  // namespace
+
+extern "C" int cgfd_io_fault_create(cgfd_core *c, int id, const char *out_dir, const char *frame, cgfd_out **out) {
+  if (!c || !out || id < 0 || id >= c->nfault) return cgfd_set_error(CGFD_ERR_ARG, "bad fault output arguments");
+  const Grid &g = c->g;
+  const int i0 = c->flt[id].i0;
+  char stem[64];
+  std::snprintf(stem, sizeof stem, "fault_i%d", i0 - 3 + 1);
+  cgfd_out *o = new (std::nothrow) cgfd_out;
+  if (!o) return cgfd_set_error(CGFD_ERR_NOMEM, "out of host memory");
+  o->kind = 0;
+  o->id = id;
+  if (int rc = o->w.create(out_name(out_dir, stem, frame))) { delete o; return rc; }
+  const int dt = o->w.def_dim("time", 0), dk = o->w.def_dim("k", g.nk), dj = o->w.def_dim("j", g.nj);
+  o->vtime = o->w.def_var("time", {dt});
+  static const char *names[9] = {"Tn", "Ts1", "Ts2", "Vs", "Vs1", "Vs2", "Slip", "Slip1", "Slip2"};
+  for (int q = 0; q < 9; q++) {
+    const int v = o->w.def_var(names[q], {dt, dk, dj});
+    if (q == 0) o->vfirst = v;
+  }
+  o->vpeak = o->w.def_var("Peak_vs", {dk, dj});
+  o->vinit = o->w.def_var("Init_t0", {dk, dj});
+  o->w.att_int("i_index_with_ghosts_in_this_thread", {i0});
+  o->w.att_int("coords_of_mpi_topo", {0, 0, 0});
+  if (int rc = o->w.enddef()) { delete o; return rc; }
+  *out = o;
+  return CGFD_OK;
+}
+
+namespace {
+/* physical [nk][nj] of field q of a ghost-inclusive [..][nz][ny] array */
+void phys_plane(const Grid &g, const float *src, float *dst) {
+  for (int k = 0; k < g.nk; k++)
+    std::memcpy(dst + (size_t)k * g.nj, src + (size_t)(k + 3) * g.ny + 3, (size_t)g.nj * 4);
+}
+}  // namespace
+
+extern "C" int cgfd_io_fault_put(cgfd_out *o, cgfd_core *c, int it_skip, float time) {
+  if (!o || !c || o->kind != 0 || it_skip < 0) return cgfd_set_error(CGFD_ERR_ARG, "bad fault output arguments");
+  const Grid &g = c->g;
+  std::vector<float> f((size_t)11 * g.sslice), plane((size_t)g.nj * g.nk);
+  if (int rc = cgfd_core_read_fault(c, o->id, f.data())) return rc;
+  if (int rc = o->w.put_rec(o->vtime, it_skip, &time)) return rc;
+  for (int q = 0; q < 9; q++) {
+    phys_plane(g, f.data() + (size_t)q * g.sslice, plane.data());
+    if (int rc = o->w.put_rec(o->vfirst + q, it_skip, plane.data())) return rc;
+  }
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_io_fault_end(cgfd_out *o, cgfd_core *c) {
+  if (!o || !c || o->kind != 0) return cgfd_set_error(CGFD_ERR_ARG, "bad fault output arguments");
+  const Grid &g = c->g;
+  std::vector<float> f((size_t)11 * g.sslice), plane((size_t)g.nj * g.nk);
+  int rc = cgfd_core_read_fault(c, o->id, f.data());
+  if (!rc) { phys_plane(g, f.data() + 9 * (size_t)g.sslice, plane.data()); rc = o->w.put_var(o->vpeak, plane.data()); }
+  if (!rc) { phys_plane(g, f.data() + 10 * (size_t)g.sslice, plane.data()); rc = o->w.put_var(o->vinit, plane.data()); }
+  const int rc2 = o->w.close();
+  delete o;
+  return rc ? rc : rc2;
+}
+
+extern "C" int cgfd_io_slice_create(cgfd_core *c, int axis, int index, const char *out_dir, const char *frame,
+                                    cgfd_out **out) {
+  if (!c || !out || axis < 0 || axis > 2) return cgfd_set_error(CGFD_ERR_ARG, "bad slice output arguments");
+  const Grid &g = c->g;
+  const int n[3] = {g.ni, g.nj, g.nk};
+  if (index < 3 || index > n[axis] + 2) return cgfd_set_error(CGFD_ERR_ARG, "slice index %d outside the grid", index);
+  static const char *ax = "xyz", *ix = "ijk";
+  char stem[64];
+  std::snprintf(stem, sizeof stem, "slice%c_%c%d", ax[axis], ix[axis], index - 3 + 1);
+  cgfd_out *o = new (std::nothrow) cgfd_out;
+  if (!o) return cgfd_set_error(CGFD_ERR_NOMEM, "out of host memory");
+  o->kind = 1;
+  o->axis = axis;
+  o->index = index;
+  if (int rc = o->w.create(out_name(out_dir, stem, frame))) { delete o; return rc; }
+  const int dt = o->w.def_dim("time", 0);
+  int d1, d2;
+  if (axis == 0) { d1 = o->w.def_dim("k", g.nk); d2 = o->w.def_dim("j", g.nj); }
+  else if (axis == 1) { d1 = o->w.def_dim("k", g.nk); d2 = o->w.def_dim("i", g.ni); }
+  else { d1 = o->w.def_dim("j", g.nj); d2 = o->w.def_dim("i", g.ni); }
+  o->vtime = o->w.def_var("time", {dt});
+  static const char *names[9] = {"Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"};
+  for (int q = 0; q < 9; q++) {
+    const int v = o->w.def_var(names[q], {dt, d1, d2});
+    if (q == 0) o->vfirst = v;
+  }
+  static const char *att[3] = {"i_index_with_ghosts_in_this_thread", "j_index_with_ghosts_in_this_thread",
+                               "k_index_with_ghosts_in_this_thread"};
+  o->w.att_int(att[axis], {index});
+  o->w.att_int("coords_of_mpi_topo", {0, 0, 0});
+  if (int rc = o->w.enddef()) { delete o; return rc; }
+  *out = o;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_io_slice_put(cgfd_out *o, cgfd_core *c, int it_skip, float time) {
+  if (!o || !c || o->kind != 1 || it_skip < 0) return cgfd_set_error(CGFD_ERR_ARG, "bad slice output arguments");
+  const Grid &g = c->g;
+  const size_t plane = o->axis == 0 ? (size_t)g.nk * g.nj : (o->axis == 1 ? (size_t)g.nk * g.ni : (size_t)g.nj * g.ni);
+  std::vector<float> buf(9 * plane);
+  if (int rc = cgfd_core_read_slice(c, o->axis, o->index, buf.data())) return rc;
+  if (int rc = o->w.put_rec(o->vtime, it_skip, &time)) return rc;
+  for (int q = 0; q < 9; q++)
+    if (int rc = o->w.put_rec(o->vfirst + q, it_skip, buf.data() + q * plane)) return rc;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_io_close(cgfd_out *o) {
+  if (!o) return CGFD_OK;
+  const int rc = o->w.close();
+  delete o;
+  return rc;
+}
+
+extern "C" int cgfd_io_pg_write(cgfd_core *c, const char *out_dir, const char *frame) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (!c->PG) return cgfd_set_error(CGFD_ERR_STATE, "PG output needs a free surface");
+  const Grid &g = c->g;
+  std::vector<float> pg((size_t)15 * g.nx * g.ny);
+  if (int rc = cgfd_core_read_pg(c, pg.data())) return rc;
+  cgfd::NcWriter w;
+  if (int rc = w.create(out_name(out_dir, "PG_V_A_D", frame))) return rc;
+  const int dj = w.def_dim("j", g.ny), di = w.def_dim("i", g.nx);
+  static const char *names[15] = {"PGV", "PGVh", "PGVx", "PGVy", "PGVz", "PGA", "PGAh", "PGAx",
+                                  "PGAy", "PGAz", "PGD", "PGDh", "PGDx", "PGDy", "PGDz"};
+  int v0 = -1;
+  for (int q = 0; q < 15; q++) {
+    const int v = w.def_var(names[q], {dj, di});
+    if (q == 0) v0 = v;
+  }
+  w.att_int("global_index_of_first_physical_points", {0, 0});
+  w.att_int("count_index_of_physical_points", {g.ni, g.nj});
+  w.att_int("coords_of_mpi_topo", {0, 0, 0});
+  int rc = w.enddef();
+  for (int q = 0; q < 15 && !rc; q++) rc = w.put_var(v0 + q, pg.data() + (size_t)q * g.nx * g.ny);
+  const int rc2 = w.close();
+  return rc ? rc : rc2;
 }

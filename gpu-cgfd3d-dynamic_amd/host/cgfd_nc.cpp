@@ -1,0 +1,162 @@
+/*
+ * cgfd_nc.cpp -- NetCDF classic (64-bit offset) writer; see cgfd_nc.h.
+ */
+#include "cgfd_nc.h"
+
+#include <cstring>
+
+#include "cgfd_core.h"
+#include "cgfd_errors.h"
+
+namespace cgfd {
+
+namespace {
+constexpr uint32_t kNcDimension = 10, kNcVariable = 11, kNcAttribute = 12;
+constexpr uint32_t kNcInt = 4, kNcFloat = 5;
+
+void put_u32(std::vector<unsigned char> &b, uint32_t v) {
+  for (int s = 24; s >= 0; s -= 8) b.push_back((unsigned char)(v >> s));
+}
+void put_u64(std::vector<unsigned char> &b, uint64_t v) {
+  for (int s = 56; s >= 0; s -= 8) b.push_back((unsigned char)(v >> s));
+}
+void put_name(std::vector<unsigned char> &b, const std::string &s) {
+  put_u32(b, (uint32_t)s.size());
+  b.insert(b.end(), s.begin(), s.end());
+  while (b.size() % 4) b.push_back(0);
+}
+/* big-endian copy of n floats */
+void be_floats(const float *src, size_t n, std::vector<unsigned char> &dst) {
+  dst.resize(n * 4);
+  for (size_t i = 0; i < n; i++) {
+    uint32_t u;
+    std::memcpy(&u, src + i, 4);
+    dst[4 * i] = (unsigned char)(u >> 24);
+    dst[4 * i + 1] = (unsigned char)(u >> 16);
+    dst[4 * i + 2] = (unsigned char)(u >> 8);
+    dst[4 * i + 3] = (unsigned char)u;
+  }
+}
+}  // namespace
+
+NcWriter::~NcWriter() {
+  if (fp_) std::fclose(fp_);
+}
+
+int NcWriter::create(const std::string &path) {
+  fp_ = std::fopen(path.c_str(), "wb+");
+  if (!fp_) return cgfd_set_error(CGFD_ERR_ARG, "cannot create %s", path.c_str());
+  path_ = path;
+  return CGFD_OK;
+}
+
+int NcWriter::def_dim(const std::string &name, long len) {
+  dims_.push_back({name, len});
+  return (int)dims_.size() - 1;
+}
+
+int NcWriter::def_var(const std::string &name, const std::vector<int> &dims) {
+  Var v;
+  v.name = name;
+  v.dims = dims;
+  v.rec = !dims.empty() && dims_[dims[0]].len == 0;
+  int64_t n = 1;
+  for (size_t q = v.rec ? 1 : 0; q < dims.size(); q++) n *= dims_[dims[q]].len;
+  v.vsize = n * 4;
+  vars_.push_back(v);
+  return (int)vars_.size() - 1;
+}
+
+void NcWriter::att_int(const std::string &name, const std::vector<int> &vals) { gatts_.push_back({name, vals}); }
+
+int NcWriter::enddef() {
+  /* header with placeholder offsets, sized first */
+  auto header = [&](std::vector<unsigned char> &b) {
+    b.clear();
+    const char magic[4] = {'C', 'D', 'F', 2};
+    b.insert(b.end(), magic, magic + 4);
+    put_u32(b, (uint32_t)numrecs_);
+    if (dims_.empty()) { put_u32(b, 0); put_u32(b, 0); }
+    else {
+      put_u32(b, kNcDimension);
+      put_u32(b, (uint32_t)dims_.size());
+      for (const Dim &d : dims_) { put_name(b, d.name); put_u32(b, (uint32_t)d.len); }
+    }
+    if (gatts_.empty()) { put_u32(b, 0); put_u32(b, 0); }
+    else {
+      put_u32(b, kNcAttribute);
+      put_u32(b, (uint32_t)gatts_.size());
+      for (const Att &a : gatts_) {
+        put_name(b, a.name);
+        put_u32(b, kNcInt);
+        put_u32(b, (uint32_t)a.vals.size());
+        for (int v : a.vals) put_u32(b, (uint32_t)v);
+      }
+    }
+    if (vars_.empty()) { put_u32(b, 0); put_u32(b, 0); }
+    else {
+      put_u32(b, kNcVariable);
+      put_u32(b, (uint32_t)vars_.size());
+      for (const Var &v : vars_) {
+        put_name(b, v.name);
+        put_u32(b, (uint32_t)v.dims.size());
+        for (int d : v.dims) put_u32(b, (uint32_t)d);
+        put_u32(b, 0); put_u32(b, 0);  /* no variable attributes */
+        put_u32(b, kNcFloat);
+        put_u32(b, (uint32_t)(v.vsize > 0xFFFFFFFFLL ? 0xFFFFFFFFu : (uint32_t)v.vsize));
+        put_u64(b, (uint64_t)v.begin);
+      }
+    }
+  };
+  std::vector<unsigned char> b;
+  header(b);
+  int64_t off = (int64_t)b.size();
+  for (Var &v : vars_)
+    if (!v.rec) { v.begin = off; off += v.vsize; }
+  recsize_ = 0;
+  for (Var &v : vars_)
+    if (v.rec) { v.begin = off + recsize_; recsize_ += v.vsize; }
+  header(b);
+  defined_ = true;
+  return write_at(0, b.data(), b.size());
+}
+
+int NcWriter::write_at(int64_t off, const void *p, size_t n) {
+  if (!fp_) return cgfd_set_error(CGFD_ERR_STATE, "NetCDF file is not open");
+  if (std::fseek(fp_, (long)off, SEEK_SET) != 0 || std::fwrite(p, 1, n, fp_) != n)
+    return cgfd_set_error(CGFD_ERR_ARG, "write to %s failed", path_.c_str());
+  return CGFD_OK;
+}
+
+int NcWriter::put_var(int var, const float *data) {
+  if (!defined_ || var < 0 || var >= (int)vars_.size() || vars_[var].rec)
+    return cgfd_set_error(CGFD_ERR_ARG, "put_var: bad variable %d", var);
+  std::vector<unsigned char> buf;
+  be_floats(data, (size_t)vars_[var].vsize / 4, buf);
+  return write_at(vars_[var].begin, buf.data(), buf.size());
+}
+
+int NcWriter::put_rec(int var, long rec, const float *data) {
+  if (!defined_ || var < 0 || var >= (int)vars_.size() || !vars_[var].rec || rec < 0)
+    return cgfd_set_error(CGFD_ERR_ARG, "put_rec: bad variable %d or record %ld", var, rec);
+  std::vector<unsigned char> buf;
+  be_floats(data, (size_t)vars_[var].vsize / 4, buf);
+  if (int rc = write_at(vars_[var].begin + (int64_t)rec * recsize_, buf.data(), buf.size())) return rc;
+  if (rec + 1 > numrecs_) {
+    numrecs_ = rec + 1;
+    unsigned char n[4] = {(unsigned char)(numrecs_ >> 24), (unsigned char)(numrecs_ >> 16),
+                          (unsigned char)(numrecs_ >> 8), (unsigned char)numrecs_};
+    if (int rc = write_at(4, n, 4)) return rc;
+  }
+  return CGFD_OK;
+}
+
+int NcWriter::close() {
+  if (!fp_) return CGFD_OK;
+  /* a record file ends at its last record */
+  const int rc = std::fclose(fp_) == 0 ? CGFD_OK : cgfd_set_error(CGFD_ERR_ARG, "close of %s failed", path_.c_str());
+  fp_ = nullptr;
+  return rc;
+}
+
+}  // namespace cgfd

@@ -1,0 +1,59 @@
+/*
+ * cgfd_io.h -- output files of the time loop in the reference's NetCDF
+ * layouts, written without libnetcdf (NetCDF classic, 64-bit offset
+ * variant, readable by any NetCDF reader). The values are packed on the
+ * device (cgfd_core_read_fault / _read_slice / _read_pg); the driver calls
+ * these at the reference's cadence (drv_rk_curv_col.cu:219-235, 641-647):
+ *
+ *   every io_time_skip steps, before the step:   cgfd_io_fault_put(f, core, it / io_time_skip, t)
+ *                                                 cgfd_io_slice_put(s, core, it / io_time_skip, t)
+ *   after the loop:                               cgfd_io_fault_end(f, core); cgfd_io_close(s);
+ *                                                 cgfd_io_pg_write(core, out_dir, frame)
+ *
+ * frame is the reference's output_fname_part ("px0_py0_pz0" on one GPU;
+ * NULL or "" means that). All functions return 0 or a negative CGFD_ERR_*
+ * code with cgfd_last_error() set.
+ */
+#ifndef CGFD_IO_H
+#define CGFD_IO_H
+
+#include "cgfd_core.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cgfd_out cgfd_out;
+
+/* <out_dir>/fault_i<fault x index + 1>_<frame>.nc (io_fault_nc_create,
+   io_funcs.cu:1952-2025): dims time (unlimited), k (nk), j (nj); float time,
+   Tn, Ts1, Ts2, Vs, Vs1, Vs2, Slip, Slip1, Slip2 (time, k, j), Peak_vs,
+   Init_t0 (k, j); global i_index_with_ghosts_in_this_thread,
+   coords_of_mpi_topo. */
+int cgfd_io_fault_create(cgfd_core *core, int fault_id, const char *out_dir, const char *frame, cgfd_out **out);
+/* record it_skip: time and the nine outputs (io_fault_nc_put, :2027-2099) */
+int cgfd_io_fault_put(cgfd_out *f, cgfd_core *core, int it_skip, float time);
+/* Peak_vs and Init_t0 (io_fault_end_t_nc_put, :2101-2139), then closes f */
+int cgfd_io_fault_end(cgfd_out *f, cgfd_core *core);
+
+/* <out_dir>/slice{x,y,z}_{i,j,k}<index + 1>_<frame>.nc at ghost-inclusive
+   `index` along axis 0/1/2 (io_slice_nc_create, :900-1011): dims time and
+   (k, j), (k, i) or (j, i); float time and Vx..Txy; global
+   {i,j,k}_index_with_ghosts_in_this_thread, coords_of_mpi_topo. */
+int cgfd_io_slice_create(cgfd_core *core, int axis, int index, const char *out_dir, const char *frame,
+                         cgfd_out **out);
+/* record it_skip: time and the nine components (io_slice_nc_put, :1013-1131) */
+int cgfd_io_slice_put(cgfd_out *s, cgfd_core *core, int it_skip, float time);
+int cgfd_io_close(cgfd_out *s);
+
+/* <out_dir>/PG_V_A_D_<frame>.nc (PG_slice_output, :1827-1879): dims j (ny),
+   i (nx), ghost-inclusive; PGV, PGVh, PGVx..PGDz; global
+   global_index_of_first_physical_points, count_index_of_physical_points,
+   coords_of_mpi_topo. Needs a free surface. */
+int cgfd_io_pg_write(cgfd_core *core, const char *out_dir, const char *frame);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,79 @@
+"""Output files in the reference's NetCDF layouts (include/cgfd_io.h) from a
+short rupture: fault frames every 5 steps with Peak_vs/Init_t0 at the end,
+x/y/z slices, and the PG file, read back with scipy's NetCDF reader and
+compared bit for bit with what the core returns at the same steps."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _ok(core, rc):
+    assert rc == 0, core.lib.cgfd_last_error()
+
+
+def test_fault_slice_pg_files(built_libs, tmp_path):
+    from cgfd import configs, model
+    from scipy.io import netcdf_file
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    lib, h = core.lib, core.handle
+    i0 = sc.faults[0].x_index + 3
+    d = str(tmp_path).encode()
+    fo = C.c_void_p()
+    _ok(core, lib.cgfd_io_fault_create(h, 0, d, b"px0_py0_pz0", C.byref(fo)))
+    slices = [(0, i0 - 2), (1, 20), (2, sc.nk + 2)]
+    so = []
+    for ax, idx in slices:
+        o = C.c_void_p()
+        _ok(core, lib.cgfd_io_slice_create(h, ax, idx, d, None, C.byref(o)))
+        so.append(o)
+    dt = float(st.desc.dt)
+    faults, slc = [], []
+    for it in range(20):
+        if it % 5 == 0:
+            _ok(core, lib.cgfd_io_fault_put(fo, h, it // 5, C.c_float(it * dt)))
+            for o in so:
+                _ok(core, lib.cgfd_io_slice_put(o, h, it // 5, C.c_float(it * dt)))
+            faults.append(core.read_fault(0)[:9, 3:-3, 3:-3].copy())
+            slc.append([core.read_slice(ax, idx) for ax, idx in slices])
+        core.run(1)
+    _ok(core, lib.cgfd_io_fault_end(fo, h))
+    for o in so:
+        _ok(core, lib.cgfd_io_close(o))
+    _ok(core, lib.cgfd_io_pg_write(h, d, None))
+    fend = core.read_fault(0)
+
+    names = ["Tn", "Ts1", "Ts2", "Vs", "Vs1", "Vs2", "Slip", "Slip1", "Slip2"]
+    with netcdf_file(str(tmp_path / f"fault_i{i0 - 2}_px0_py0_pz0.nc"), "r", mmap=False) as f:
+        assert f.dimensions["k"] == sc.nk and f.dimensions["j"] == sc.nj
+        assert f.variables["time"].shape == (4,)
+        assert np.allclose(f.variables["time"][:], np.arange(0, 20, 5) * dt)
+        for q, n in enumerate(names):
+            for r in range(4):
+                assert np.array_equal(f.variables[n][r], faults[r][q]), (n, r)
+        assert np.array_equal(f.variables["Peak_vs"][:], fend[9, 3:-3, 3:-3])
+        assert np.array_equal(f.variables["Init_t0"][:], fend[10, 3:-3, 3:-3])
+        assert int(f.i_index_with_ghosts_in_this_thread) == i0
+        assert float(np.abs(f.variables["Slip"][3]).max()) > 0.0
+    wn = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]
+    for s_, (ax, idx) in enumerate(slices):
+        fn = ["slicex_i", "slicey_j", "slicez_k"][ax] + f"{idx - 2}_px0_py0_pz0.nc"
+        with netcdf_file(str(tmp_path / fn), "r", mmap=False) as f:
+            for q, n in enumerate(wn):
+                for r in range(4):
+                    assert np.array_equal(f.variables[n][r], slc[r][s_][q]), (fn, n, r)
+    with netcdf_file(str(tmp_path / "PG_V_A_D_px0_py0_pz0.nc"), "r", mmap=False) as f:
+        nz, ny, nx = sc.shape_ext
+        assert f.dimensions["j"] == ny and f.dimensions["i"] == nx
+        assert list(f.variables)[:3] == ["PGV", "PGVh", "PGVx"]
+        buf = np.empty((15, ny, nx), np.float32)
+        _ok(core, lib.cgfd_core_read_pg(h, buf.ctypes.data_as(C.c_void_p)))
+        assert np.array_equal(f.variables["PGV"][:], buf[0])
+        assert np.array_equal(f.variables["PGDz"][:], buf[14])
+        assert float(buf[0].max()) > 0.0
+    core.close()
+    st.close()
