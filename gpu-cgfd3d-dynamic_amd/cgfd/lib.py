@@ -115,6 +115,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_io_slice_create": (I32, [VP, I32, I32, C.c_char_p, C.c_char_p, VP]),
         "cgfd_io_slice_put": (I32, [VP, VP, I32, C.c_float]),
         "cgfd_io_close": (I32, [VP]),
+        "cgfd_io_snap_create": (I32, [VP, C.c_char_p, VP, I32, C.c_char_p, C.c_char_p, VP]),
+        "cgfd_io_snap_put": (I32, [VP, VP, I32, C.c_float]),
         "cgfd_io_pg_write": (I32, [VP, C.c_char_p, C.c_char_p]),
         "cgfd_core_read_snapshot": (I32, [VP, VP, I32, VP]),
         "cgfd_batch_read_snapshot": (I32, [VP, I32, VP, I32, VP]),

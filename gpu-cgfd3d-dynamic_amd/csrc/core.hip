@@ -1495,8 +1495,9 @@ extern "C" int cgfd_drv_rk_curv_col_allstep(const cgfd_desc *d, float *wave_out,
    1827-1879, 1952-2139), from device-packed data ---- */
 struct cgfd_out {
   cgfd::NcWriter w;
-  int kind = 0;             /* 0 fault, 1 slice */
+  int kind = 0;             /* 0 fault, 1 slice, 2 snapshot */
   int id = 0, axis = 0, index = 0;
+  int box[9] = {}, flags = 0, nvar = 0;
   int vtime = -1, vfirst = -1, vpeak = -1, vinit = -1;
 };
 
@@ -1611,6 +1612,62 @@ extern "C" int cgfd_io_slice_put(cgfd_out *o, cgfd_core *c, int it_skip, float t
   if (int rc = o->w.put_rec(o->vtime, it_skip, &time)) return rc;
   for (int q = 0; q < 9; q++)
     if (int rc = o->w.put_rec(o->vfirst + q, it_skip, buf.data() + q * plane)) return rc;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_io_snap_create(cgfd_core *c, const char *name, const int *box, int flags, const char *out_dir,
+                                   const char *frame, cgfd_out **out) {
+  if (!c || !out || !box || !name || !*name || flags <= 0 || flags > 7)
+    return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot output arguments");
+  const Grid &g = c->g;
+  const int hi[3] = {g.ni + 2, g.nj + 2, g.nk + 2};
+  for (int a = 0; a < 3; a++)
+    if (box[3 * a + 1] < 1 || box[3 * a + 2] < 1 || box[3 * a] < 3 ||
+        box[3 * a] + (long)(box[3 * a + 1] - 1) * box[3 * a + 2] > hi[a])
+      return cgfd_set_error(CGFD_ERR_ARG, "snapshot box axis %d leaves the physical grid", a);
+  cgfd_out *o = new (std::nothrow) cgfd_out;
+  if (!o) return cgfd_set_error(CGFD_ERR_NOMEM, "out of host memory");
+  o->kind = 2;
+  o->flags = flags;
+  for (int q = 0; q < 9; q++) o->box[q] = box[q];
+  if (int rc = o->w.create(out_name(out_dir, name, frame))) { delete o; return rc; }
+  const int dt = o->w.def_dim("time", 0), dk = o->w.def_dim("k", box[7]), dj = o->w.def_dim("j", box[4]),
+            di = o->w.def_dim("i", box[1]);
+  o->vtime = o->w.def_var("time", {dt});
+  /* the reference's variable order (io_funcs.cu:1329-1366) */
+  static const char *vn[3] = {"Vx", "Vy", "Vz"}, *tn[6] = {"Txx", "Tyy", "Tzz", "Txz", "Tyz", "Txy"},
+                    *en[6] = {"Exx", "Eyy", "Ezz", "Exz", "Eyz", "Exy"};
+  o->nvar = 0;
+  if (flags & 1) for (int q = 0; q < 3; q++) { int v = o->w.def_var(vn[q], {dt, dk, dj, di}); if (!o->nvar++) o->vfirst = v; }
+  if (flags & 2) for (int q = 0; q < 6; q++) { int v = o->w.def_var(tn[q], {dt, dk, dj, di}); if (!o->nvar++) o->vfirst = v; }
+  if (flags & 4) for (int q = 0; q < 6; q++) { int v = o->w.def_var(en[q], {dt, dk, dj, di}); if (!o->nvar++) o->vfirst = v; }
+  o->w.att_int("first_index_to_snapshot_output", {box[0] - 3, box[3] - 3, box[6] - 3});
+  o->w.att_int("first_index_in_this_thread_with_ghosts", {box[0], box[3], box[6]});
+  o->w.att_int("index_stride_in_this_thread", {box[2], box[5], box[8]});
+  o->w.att_int("coords_of_mpi_topo", {0, 0, 0});
+  if (int rc = o->w.enddef()) { delete o; return rc; }
+  *out = o;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_io_snap_put(cgfd_out *o, cgfd_core *c, int rec, float time) {
+  if (!o || !c || o->kind != 2 || rec < 0) return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot output arguments");
+  const size_t n = (size_t)o->box[1] * o->box[4] * o->box[7];
+  std::vector<float> buf(n * o->nvar);
+  if (int rc = cgfd_core_read_snapshot(c, o->box, o->flags, buf.data())) return rc;
+  if (int rc = o->w.put_rec(o->vtime, rec, &time)) return rc;
+  /* packed groups hold the wavefield order (..., Tyz, Txz, Txy); the file the
+     reference's (..., Txz, Tyz, Txy) */
+  static const int perm[6] = {0, 1, 2, 4, 3, 5};
+  int v = o->vfirst;
+  size_t g0 = 0;
+  for (int grp = 0; grp < 3; grp++) {
+    if (!(o->flags & (1 << grp))) continue;
+    const int nf = grp == 0 ? 3 : 6;
+    for (int q = 0; q < nf; q++)
+      if (int rc = o->w.put_rec(v++, rec, buf.data() + (g0 + (nf == 6 ? perm[q] : q)) * n)) return rc;
+    g0 += nf;
+  }
   return CGFD_OK;
 }
 

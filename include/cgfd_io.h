@@ -46,6 +46,17 @@ int cgfd_io_slice_create(cgfd_core *core, int axis, int index, const char *out_d
 int cgfd_io_slice_put(cgfd_out *s, cgfd_core *core, int it_skip, float time);
 int cgfd_io_close(cgfd_out *s);
 
+/* <out_dir>/<name>_<frame>.nc (io_snap_nc_create, :1260-1378): the
+   strided box of cgfd_core_read_snapshot, dims time, k, j, i; float time and,
+   by flags (1 velocity, 2 stress, 4 strain), Vx Vy Vz, Txx Tyy Tzz Txz Tyz
+   Txy, Exx Eyy Ezz Exz Eyz Exy; global first_index_to_snapshot_output,
+   first_index_in_this_thread_with_ghosts, index_stride_in_this_thread,
+   coords_of_mpi_topo. cgfd_io_snap_put writes record rec (the reference's
+   (it - it1) / dit at steps with (it - it1) % dit == 0, io_funcs.cu:1417-1424). */
+int cgfd_io_snap_create(cgfd_core *core, const char *name, const int *box, int flags, const char *out_dir,
+                        const char *frame, cgfd_out **out);
+int cgfd_io_snap_put(cgfd_out *snap, cgfd_core *core, int rec, float time);
+
 /* <out_dir>/PG_V_A_D_<frame>.nc (PG_slice_output, :1827-1879): dims j (ny),
    i (nx), ghost-inclusive; PGV, PGVh, PGVx..PGDz; global
    global_index_of_first_physical_points, count_index_of_physical_points,

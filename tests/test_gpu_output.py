@@ -31,8 +31,12 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
         o = C.c_void_p()
         _ok(core, lib.cgfd_io_slice_create(h, ax, idx, d, None, C.byref(o)))
         so.append(o)
+    box = (4, 9, 3, 5, 7, 5, 6, 8, 2)
+    bx = (C.c_int * 9)(*box)
+    sn = C.c_void_p()
+    _ok(core, lib.cgfd_io_snap_create(h, b"snap_001", bx, 7, d, None, C.byref(sn)))
     dt = float(st.desc.dt)
-    faults, slc = [], []
+    faults, slc, snaps = [], [], []
     for it in range(20):
         if it % 5 == 0:
             _ok(core, lib.cgfd_io_fault_put(fo, h, it // 5, C.c_float(it * dt)))
@@ -40,9 +44,11 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
                 _ok(core, lib.cgfd_io_slice_put(o, h, it // 5, C.c_float(it * dt)))
             faults.append(core.read_fault(0)[:9, 3:-3, 3:-3].copy())
             slc.append([core.read_slice(ax, idx) for ax, idx in slices])
+            _ok(core, lib.cgfd_io_snap_put(sn, h, it // 5, C.c_float(it * dt)))
+            snaps.append(core.read_snapshot(box, 7))
         core.run(1)
     _ok(core, lib.cgfd_io_fault_end(fo, h))
-    for o in so:
+    for o in so + [sn]:
         _ok(core, lib.cgfd_io_close(o))
     _ok(core, lib.cgfd_io_pg_write(h, d, None))
     fend = core.read_fault(0)
@@ -66,6 +72,14 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
             for q, n in enumerate(wn):
                 for r in range(4):
                     assert np.array_equal(f.variables[n][r], slc[r][s_][q]), (fn, n, r)
+    with netcdf_file(str(tmp_path / "snap_001_px0_py0_pz0.nc"), "r", mmap=False) as f:
+        assert (f.dimensions["k"], f.dimensions["j"], f.dimensions["i"]) == (8, 7, 9)
+        order = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy", "Exx", "Eyy", "Ezz", "Eyz", "Exz", "Exy"]
+        for q, n in enumerate(order):
+            for r in range(4):
+                assert np.array_equal(f.variables[n][r], snaps[r][q]), (n, r)
+        assert list(f.variables)[4:10] == ["Txx", "Tyy", "Tzz", "Txz", "Tyz", "Txy"]
+        assert list(f.index_stride_in_this_thread) == [3, 5, 2]
     with netcdf_file(str(tmp_path / "PG_V_A_D_px0_py0_pz0.nc"), "r", mmap=False) as f:
         nz, ny, nx = sc.shape_ext
         assert f.dimensions["j"] == ny and f.dimensions["i"] == nx
