@@ -24,7 +24,7 @@ ORACLE_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle.so"
 
 HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_vol_lds.hip", "k_vol_zq.hip", "k_vol_pipe.hip",
                "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip"]
-HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp", "cgfd_nc.cpp"]
+HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp", "cgfd_nc.cpp", "cgfd_sac.cpp"]
 
 # Packed FP32 instructions stay disabled: hipcc otherwise forms v_pk_*_f32
 # from scalar float code.

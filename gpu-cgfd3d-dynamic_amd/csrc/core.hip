@@ -25,6 +25,7 @@
 #include "../../include/cgfd_core.h"
 #include "../host/cgfd_errors.h"
 #include "../host/cgfd_nc.h"
+#include "../host/cgfd_sac.h"
 #include "../../include/cgfd_io.h"
 #include <string>
 #include "cgfd_kernels.h"
@@ -1701,4 +1702,30 @@ extern "C" int cgfd_io_pg_write(cgfd_core *c, const char *out_dir, const char *f
   for (int q = 0; q < 15 && !rc; q++) rc = w.put_var(v0 + q, pg.data() + (size_t)q * g.nx * g.ny);
   const int rc2 = w.close();
   return rc ? rc : rc2;
+}
+
+/* station traces as SAC files (io_recv_output_sac, io_funcs.cu:577-618):
+   <out_dir>/<name>.<Vx..Txy>.sac, all nt_max samples of the station window.
+   As the reference's call, the event position is zero and the station's
+   depth field holds dt (its arguments stop one position short of stdp). */
+extern "C" int cgfd_io_stations_sac(cgfd_core *c, const char *out_dir, const char *const *names, const float *xyz) {
+  if (!c || !names || !xyz) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "no stations set");
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const int n = c->st_n, nt = c->st_nt;
+  std::vector<float> h((size_t)n * nt * 9), tr((size_t)nt);
+  HIPCHK(hipMemcpy(h.data(), c->st_buf, h.size() * 4, hipMemcpyDeviceToHost));
+  static const char *cmp[9] = {"Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"};
+  const float ev[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  const std::string dir = out_dir && *out_dir ? std::string(out_dir) + "/" : std::string();
+  for (int s = 0; s < n; s++) {
+    if (!names[s]) return cgfd_set_error(CGFD_ERR_ARG, "station %d has no name", s);
+    const float st4[4] = {xyz[3 * s], xyz[3 * s + 1], xyz[3 * s + 2], c->dt};
+    for (int q = 0; q < 9; q++) {
+      for (int t = 0; t < nt; t++) tr[t] = h[((size_t)s * nt + t) * 9 + q];
+      if (int rc = cgfd::write_sac(dir + names[s] + "." + cmp[q] + ".sac", tr.data(), nt, c->dt, ev, st4)) return rc;
+    }
+  }
+  return CGFD_OK;
 }

@@ -63,6 +63,14 @@ int cgfd_io_snap_put(cgfd_out *snap, cgfd_core *core, int rec, float time);
    coords_of_mpi_topo. Needs a free surface. */
 int cgfd_io_pg_write(cgfd_core *core, const char *out_dir, const char *frame);
 
+/* Station traces as SAC files (io_recv_output_sac, io_funcs.cu:577-618;
+   sacExport1C1R, src/lib/sacLib.cu:8-93): <out_dir>/<names[s]>.<cmp>.sac for
+   cmp = Vx, Vy, Vz, Txx, Tyy, Tzz, Tyz, Txz, Txy, with every sample of the
+   station window (cgfd_core_set_stations), delta = dt, b = 0, station
+   position xyz[s] in stla, stlo, stel and, as in the reference's call, dt in
+   stdp; native byte order. */
+int cgfd_io_stations_sac(cgfd_core *core, const char *out_dir, const char *const *names, const float *xyz);
+
 #ifdef __cplusplus
 }
 #endif

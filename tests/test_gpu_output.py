@@ -91,3 +91,41 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
         assert float(buf[0].max()) > 0.0
     core.close()
     st.close()
+
+
+def _read_sac(path):
+    raw = path.read_bytes()
+    f = np.frombuffer(raw[:280], np.float32)
+    n = np.frombuffer(raw[280:440], np.int32)
+    k = raw[440:632]
+    return f, n, k, np.frombuffer(raw[632:], np.float32)
+
+
+def test_station_sac_files(built_libs, tmp_path):
+    """SAC files of the station traces: header fields as the reference's
+    sacExport1C1R sets them, samples equal to the recorded traces."""
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    nt = 30
+    core.set_stations(sc.stations + 3, nt)
+    core.run(nt)
+    tr = core.read_stations()
+    ns = len(sc.stations)
+    names = (C.c_char_p * ns)(*[f"ST{q:02d}".encode() for q in range(ns)])
+    xyz = np.ascontiguousarray(np.arange(ns * 3, dtype=np.float32).reshape(ns, 3) * 100.0)
+    _ok(core, core.lib.cgfd_io_stations_sac(core.handle, str(tmp_path).encode(), names, xyz.ctypes.data_as(C.c_void_p)))
+    dt = np.float32(st.desc.dt)
+    for q, cmp in enumerate(["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]):
+        for s_ in (0, ns - 1):
+            f, n, k, data = _read_sac(tmp_path / f"ST{s_:02d}.{cmp}.sac")
+            assert np.array_equal(data, tr[s_, :, q])
+            assert f[0] == dt and f[5] == 0 and f[6] == np.float32(dt * np.float32(nt - 1))
+            assert list(f[31:35]) == [xyz[s_, 0], xyz[s_, 1], xyz[s_, 2], dt]
+            assert list(f[35:39]) == [0, 0, 0, 0] and f[57] == 0 and f[58] == 0 and f[1] == -12345.0
+            assert n[6] == 6 and n[9] == nt and n[15] == 1 and n[35] == 1 and n[0] == -12345
+            assert k[:8] == b"-12345\0\0" and k[160:168] == b"-12345\0\0" and k[8:24] == b"-12345  " + bytes(8)
+            assert k[24:32] == b"-12345  "
+    core.close()
+    st.close()
