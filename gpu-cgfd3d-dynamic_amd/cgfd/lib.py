@@ -106,6 +106,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_state_bytes": (I32, [VP, C.POINTER(C.c_size_t)]),
         "cgfd_adj_backward": (I32, [VP, VP, I32, VP]),
         "cgfd_adj_forward": (I32, [VP, I32]),
+        "cgfd_batch_adj_backward": (I32, [VP, I32, VP, I32, VP, VP]),
         "cgfd_adj_backward_fault": (I32, [VP, VP, VP, I32, VP, VP]),
         "cgfd_core_set_fault_receivers": (I32, [VP, I32, VP, I32]),
         "cgfd_core_read_slice": (I32, [VP, I32, I32, VP]),

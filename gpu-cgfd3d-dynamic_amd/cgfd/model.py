@@ -402,6 +402,34 @@ class Batch:
         L.check(self.lib, self.lib.cgfd_batch_read_pg(self.handle, s, L.ptr(out)))
         return out
 
+    def set_stations(self, ijk_ext: np.ndarray, nt_max: int):
+        a = np.ascontiguousarray(ijk_ext, np.int32).reshape(-1, 3)
+        L.check(self.lib, self.lib.cgfd_core_set_stations(self.handle, a.shape[0], L.ptr(a), nt_max))
+        self._nst, self._ntmax = a.shape[0], nt_max
+
+    def read_stations(self, s: int) -> np.ndarray:
+        buf = np.empty((self._nst, self._ntmax, 9), np.float32)
+        nt = C.c_int()
+        L.check(self.lib, self.lib.cgfd_batch_read_stations(self.handle, s, L.ptr(buf), C.byref(nt)))
+        flat = buf.reshape(-1)[: self._nst * nt.value * 9]
+        return flat.reshape(self._nst, nt.value, 9).copy()
+
+    def adjoint_forward(self, nt: int):
+        """Arms every scenario's friction tape for the next nt steps."""
+        L.check(self.lib, self.lib.cgfd_adj_forward(self.handle, int(nt)))
+
+    def adjoint_fault(self, s: int, residual: np.ndarray):
+        """(dJ/dw at the first recorded step, dJ/d(fault parameters)) of
+        scenario s for J = sum residual . its station traces."""
+        residual = np.ascontiguousarray(residual, np.float32)
+        nsta, nt, _ = residual.shape
+        nz, ny, nx = self.shape_ext
+        out = np.zeros((9, nz, ny, nx), np.float32)
+        sc = self.setup.sc
+        gf = np.zeros((len(sc.faults), 7, sc.nk, sc.nj), np.float32)
+        L.check(self.lib, self.lib.cgfd_batch_adj_backward(self.handle, s, L.ptr(residual), nt, L.ptr(out), L.ptr(gf)))
+        return out, gf
+
     def close(self):
         if getattr(self, "handle", None):
             self.lib.cgfd_core_destroy(self.handle)

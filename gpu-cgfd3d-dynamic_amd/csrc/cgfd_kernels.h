@@ -46,6 +46,7 @@ struct FaultArgs {
      (1) and the split-node velocity jump of the last stage (3) */
   float *tape;
   int tape_it0, tape_nt;
+  long tape_sstride;                        /* floats between scenarios' tapes */
 };
 constexpr int kTapeFloats = 16;
 
@@ -65,6 +66,7 @@ DEVI FaultArgs fault_at(const FaultArgs &a, const BatArgs &b, int s) {
   r.f_end += f;
   r.sfp = (long)s * b.fpf;
   r.sfi = (long)s * b.fpi;
+  if (r.tape) r.tape += (long)s * a.tape_sstride;
   return r;
 }
 

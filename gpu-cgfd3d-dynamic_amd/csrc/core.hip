@@ -159,6 +159,7 @@ struct cgfd_core {
   float *tape = nullptr;
   int tape_it0 = 0, tape_nt = 0;
   float *adjf = nullptr;
+  FaultDev *d_ftab_adj = nullptr;  /* the fault table as one scenario of a batch sees it */
 
   float *wlev(int l) { return wave + (long)l * 9 * g.sicmp; }
   float *flev(int l) { return fwav + (long)l * nfault * 18 * g.sslice; }
@@ -659,6 +660,7 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.tape = c->tape;
   f.tape_it0 = c->tape_it0;
   f.tape_nt = c->tape_nt;
+  f.tape_sstride = (long)c->nfault * c->tape_nt * kTapeFloats * c->g.sslice;
 }
 
 /* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
@@ -917,7 +919,6 @@ extern "C" int cgfd_core_load_state(cgfd_core *c, const void *src, int src_on_de
    tape the forward run wrote). ---- */
 extern "C" int cgfd_adj_forward(cgfd_core *c, int nt) {
   if (!c || nt <= 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
-  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   c->drop_graph();
@@ -925,12 +926,14 @@ extern "C" int cgfd_adj_forward(cgfd_core *c, int nt) {
   c->tape_nt = 0;
   c->tape_it0 = c->it_host;
   if (c->nfault == 0) return CGFD_OK;  /* the wave update needs no tape */
-  const size_t n = (size_t)c->nfault * nt * kTapeFloats * c->g.sslice;
+  const size_t n = (size_t)c->nscen * c->nfault * nt * kTapeFloats * c->g.sslice;
   if (int rc = c->alloc(&c->tape, n)) return rc;
   HIPCHK(hipMemset(c->tape, 0, n * 4));
   c->tape_nt = nt;
   return CGFD_OK;
 }
+
+static int scen_steps(const cgfd_core *c, int s);
 
 namespace {
 /* split nodes whose fault columns reach a CFS-PML slab are not covered */
@@ -956,10 +959,15 @@ int check_fault_columns(cgfd_core *c) {
   return CGFD_OK;
 }
 
-int adj_sweep(cgfd_core *c, const float *residual, const float *residual_fault, int nt, float *grad,
+int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residual_fault, int nt, float *grad,
               float *grad_fault) {
   if (!c || !residual) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
-  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: single-scenario cores only");
+  if (scen < 0 || scen >= c->nscen) return cgfd_set_error(CGFD_ERR_ARG, "scenario %d out of range", scen);
+  if (c->nscen > 1 && residual_fault)
+    return cgfd_set_error(CGFD_ERR_ARG, "adjoint: fault-receiver residuals need a single-scenario core");
+  if (scen_steps(c, scen) < c->st_it0 + nt - 1)
+    return cgfd_set_error(CGFD_ERR_STATE, "adjoint: scenario %d ran %d of the %d recorded steps", scen,
+                          scen_steps(c, scen) - c->st_it0, nt - 1);
   if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "adjoint: no stations set");
   if (nt < 1 || nt > c->st_nt) return cgfd_set_error(CGFD_ERR_ARG, "adjoint: nt %d outside 1..%d", nt, c->st_nt);
   if (c->nfault > 0 && (!c->tape || c->tape_it0 != c->st_it0 || c->tape_nt < nt))
@@ -1026,9 +1034,21 @@ int adj_sweep(cgfd_core *c, const float *residual, const float *residual_fault, 
   fa.met = c->met; fa.lam = c->lam; fa.mu = c->mu; fa.slw = c->slw;
   fa.matVx2Vz = c->matVx2Vz; fa.matVy2Vz = c->matVy2Vz;
   fa.ftab = c->d_ftab;
+  if (c->nfault > 0 && scen > 0) {
+    /* the scenario's own parameters, outputs and traction columns */
+    if (!c->d_ftab_adj && (rc = c->alloc(&c->d_ftab_adj, (size_t)c->nfault))) return rc;
+    std::vector<FaultDev> t(c->flt, c->flt + c->nfault);
+    for (FaultDev &F : t) {
+      const long fp = scen * c->bat.fpf, fi = scen * c->bat.fpi;
+      F.T0 += fp; F.fric += fp; F.out += fp; F.T1 += fp; F.hT1 += fp; F.mT1 += fp;
+      F.flag_rup += fi; F.init_t0_flag += fi;
+    }
+    HIPCHK(hipMemcpy(c->d_ftab_adj, t.data(), t.size() * sizeof(FaultDev), hipMemcpyHostToDevice));
+    fa.ftab = c->d_ftab_adj;
+  }
   fa.nfault = c->nfault;
   fa.adjf = c->adjf;
-  fa.tape = c->tape;
+  fa.tape = c->tape ? c->tape + (size_t)scen * c->nfault * c->tape_nt * kTapeFloats * ss : nullptr;
   fa.tape_nt = c->tape_nt;
   fa.isfree = c->isfree; fa.imethod = c->imethod; fa.fix_q1 = c->fix_q1;
   fa.dt = c->dt;
@@ -1097,13 +1117,19 @@ int adj_sweep(cgfd_core *c, const float *residual, const float *residual_fault, 
 
 extern "C" int cgfd_adj_backward(cgfd_core *c, const float *residual, int nt, float *grad) {
   if (!grad) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
-  return adj_sweep(c, residual, nullptr, nt, grad, nullptr);
+  return adj_sweep(c, 0, residual, nullptr, nt, grad, nullptr);
 }
 
 extern "C" int cgfd_adj_backward_fault(cgfd_core *c, const float *residual, const float *residual_fault, int nt,
                                        float *grad, float *grad_fault) {
   if (!grad_fault) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
-  return adj_sweep(c, residual, residual_fault, nt, grad, grad_fault);
+  return adj_sweep(c, 0, residual, residual_fault, nt, grad, grad_fault);
+}
+
+extern "C" int cgfd_batch_adj_backward(cgfd_core *c, int s, const float *residual, int nt, float *grad,
+                                       float *grad_fault) {
+  if (!grad && !grad_fault) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  return adj_sweep(c, s, residual, nullptr, nt, grad, grad_fault);
 }
 
 extern "C" int cgfd_core_set_fault_receivers(cgfd_core *c, int n, const int *fjk, int nt_max) {

@@ -188,7 +188,8 @@ int cgfd_core_load_state(cgfd_core *core, const void *src, int src_on_device);
    per fault node and step: the trial traction of each stage, Slip at the
    step start, the velocity jump of the last stage), written by the forward
    fault kernels; the friction law is differentiated on the branch the
-   forward run took. No-op on a core without a fault. Single-scenario cores.
+   forward run took. No-op on a core without a fault. On a batch every
+   scenario gets its own tape.
    cgfd_adj_backward: grad[9][nz][ny][nx] (host layout, ghosts zero)
    receives dJ/dw at the first recorded step.
    cgfd_adj_backward_fault: the same (grad may be null) and
@@ -206,6 +207,11 @@ int cgfd_adj_forward(cgfd_core *core, int nt);
 int cgfd_adj_backward(cgfd_core *core, const float *residual, int nt, float *grad);
 int cgfd_adj_backward_fault(cgfd_core *core, const float *residual, const float *residual_fault, int nt, float *grad,
                             float *grad_fault);
+/* The same for scenario s of a batch (cgfd_batch_create) with its own
+   residual[nsta][nt][9] (cgfd_batch_read_stations of s); grad and/or
+   grad_fault may be null; no fault-receiver term. The scenario must have run
+   the recorded steps. */
+int cgfd_batch_adj_backward(cgfd_core *core, int s, const float *residual, int nt, float *grad, float *grad_fault);
 
 /* Stage fork (default off): within each RK stage, after the trial traction,
    the interior volume blocks, the CFS-PML blocks with the top layers, and
