@@ -166,6 +166,54 @@ def adjoint_leg(steps: int):
             "gradients": "initial wavefield; mu_s, mu_d, Dc, C0, T0x, T0y, T0z per fault node"}
 
 
+def batched_adjoint_leg(steps: int, nscen: int = 8):
+    """Batched forward (per-scenario friction tapes) and per-scenario
+    adjoints of nscen rupture scenarios on the 320 x 200 x 120 grid (4 hills,
+    PML 20, own friction/stress fields), 32 surface stations, against
+    scenario 0 run and differentiated alone on the same grid."""
+    import numpy as np
+    from cgfd import configs, model
+    base, fields, _ = configs.config5(nscen=nscen)
+    rng = np.random.default_rng(5000)
+    base.stations = np.stack([rng.integers(23, base.ni - 23, 32), rng.integers(23, base.nj - 23, 32),
+                              np.full(32, base.nk - 1)], 1).astype(np.int32)
+    st = model.Setup(base)
+    bat = model.Batch(st, [(None if s == 0 else [fields[s]], steps + 5) for s in range(nscen)])
+    bat.set_stations(base.stations + 3, steps)
+    bat.adjoint_forward(steps)
+    bat.sync()
+    t = time.perf_counter()
+    bat.run(steps)
+    bat.sync()
+    t_bf = time.perf_counter() - t
+    t = time.perf_counter()
+    finite = True
+    for s in range(nscen):
+        g, gf = bat.adjoint_fault(s, bat.read_stations(s))
+        finite = finite and bool(np.isfinite(gf).all())
+    t_ba = time.perf_counter() - t
+    bat.close()
+    core = model.Core(st)
+    core.set_stations(base.stations + 3, steps)
+    core.adjoint_forward(steps)
+    core.sync()
+    t = time.perf_counter()
+    core.run(steps)
+    core.sync()
+    t_f = time.perf_counter() - t
+    t = time.perf_counter()
+    core.adjoint_fault(core.read_stations())
+    t_a = time.perf_counter() - t
+    core.close()
+    st.close()
+    per = (t_bf + t_ba) / nscen
+    return {"nscen": nscen, "grid": [base.ni, base.nj, base.nk], "steps": steps,
+            "batched_forward_s_per_scenario": round(t_bf / nscen, 3), "adjoint_s_per_scenario": round(t_ba / nscen, 3),
+            "adjoint_vs_batched_forward": round(t_ba / t_bf, 3),
+            "unbatched_forward_s": round(t_f, 3), "unbatched_adjoint_s": round(t_a, 3),
+            "per_scenario_forward_plus_adjoint_vs_unbatched": round(per / (t_f + t_a), 3), "finite": finite}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -281,6 +329,8 @@ def main() -> int:
         log("[bench] adjoint ...")
         out["adjoint"] = adjoint_leg(args.adjoint_steps)
         out["adjoint_vs_forward"] = out["adjoint"]["value"]
+        log("[bench] batched adjoint ...")
+        out["batched_adjoint"] = batched_adjoint_leg(args.adjoint_steps)
     if rank == 0 and not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(args.cpu_threads)

@@ -245,7 +245,8 @@ def config2(scale: float = 1.0, steps: int = 4000, seed: int = 2002, imethod: in
                     name=f"config2x{scale:g}" if scale != 1.0 else "config2")
 
 
-def config3(nscen: int = 32, scale: float = 1.0, steps_scale: float = 1.0, seed: int = 3000, imethod: int = 2):
+def config3(nscen: int = 32, scale: float = 1.0, steps_scale: float = 1.0, seed: int = 3000, imethod: int = 2,
+            dims=(200, 400, 150), nhills: int = 3, hill_max: float = 800.0, name: str = "config3"):
     """Batched ensemble: ``nscen`` rupture scenarios on one 400x200x150 grid
     at 100 m (reference axes ni=200 fault-normal, nj=400 strike, nk=150),
     3 seeded Gaussian hills, layered medium as in config2, PML 20, dt 0.008.
@@ -259,12 +260,12 @@ def config3(nscen: int = 32, scale: float = 1.0, steps_scale: float = 1.0, seed:
     fault grid."""
     rng = _rng(seed)
     dh = 100.0 / scale
-    ni, nj, nk = int(200 * scale), int(400 * scale), int(150 * scale)
+    ni, nj, nk = int(dims[0] * scale), int(dims[1] * scale), int(dims[2] * scale)
     pml = max(int(round(20 * scale)), 5)
     i0 = ni // 2
     xs = ((np.arange(ni) - i0) * dh)[None, :]
     ys = ((np.arange(nj) - nj // 2) * dh)[:, None]
-    topo = hills(np.broadcast_to(xs, (nj, ni)), np.broadcast_to(ys, (nj, ni)), rng, 3, 800.0, 2000.0, 10000.0)
+    topo = hills(np.broadcast_to(xs, (nj, ni)), np.broadcast_to(ys, (nj, ni)), rng, nhills, hill_max, 2000.0, 10000.0)
     x, y, z = curvilinear_coords(ni, nj, nk, dh, i0, topo, dh, 1.5 * dh)
     lam, mu, rho = layered_media(z, topo, rng, 0.05)
     nlen_max = min(int(round(30000 / dh)) + 1, nj - 2 * (pml + 3) - 8)
@@ -303,5 +304,14 @@ def config3(nscen: int = 32, scale: float = 1.0, steps_scale: float = 1.0, seed:
                     pml_layers=layers, imethod=imethod,
                     faults=[FaultSpec(x_index=i0, grid=(j1, j2, k1, k2), fields=fields[0])],
                     united_margin=pml + 3, dt=0.008 / scale, nt_total=nts[0], stations=None,
-                    name="config3" if scale == 1.0 else f"config3x{scale:g}")
+                    name=name if scale == 1.0 else f"{name}x{scale:g}")
     return base, fields, nts
+
+
+def config5(nscen: int = 8, steps_scale: float = 1.0, seed: int = 5000, imethod: int = 2):
+    """Batched adjoint ensemble: ``nscen`` scenarios of the config-3 physics
+    (own friction and stress fields, fault lengths, patches) on a 320 x 200 x
+    120 grid at 100 m (reference axes ni=200, nj=320, nk=120) with 4 seeded
+    hills up to 500 m and PML 20."""
+    return config3(nscen=nscen, steps_scale=steps_scale, seed=seed, imethod=imethod, dims=(200, 320, 120), nhills=4,
+                   hill_max=500.0, name="config5")
