@@ -105,10 +105,12 @@ struct cgfd_core {
 
   long *st_idx = nullptr;
   float *st_buf = nullptr;
+  size_t st_idx_cap = 0, st_buf_cap = 0;
   int st_n = 0, st_nt = 0, st_it0 = 0;
   /* fault receivers: (fault id, slice index) pairs, [n][nt][9] records */
   int *fr_idx = nullptr;
   float *fr_buf = nullptr;
+  size_t fr_idx_cap = 0, fr_buf_cap = 0;
   int fr_n = 0, fr_nt = 0, fr_it0 = 0;
 
   /* eight steps (one cycle of the operator pairs; an even number of level
@@ -157,6 +159,7 @@ struct cgfd_core {
      kAdjFaultFloats per node (fault state levels E, A, B, P; T1; hT1; mT1;
      Slip; parameter gradients) */
   float *tape = nullptr;
+  size_t tape_cap = 0;
   int tape_it0 = 0, tape_nt = 0;
   float *adjf = nullptr;
   FaultDev *d_ftab_adj = nullptr;  /* the fault table as one scenario of a batch sees it */
@@ -174,6 +177,22 @@ struct cgfd_core {
       return cgfd_set_error(CGFD_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
     mem.ptrs.push_back(q);
     *p = (T *)q;
+    return CGFD_OK;
+  }
+  /* a buffer that set-up calls may size again: kept while it is large
+     enough, else replaced (the old one freed at once, so repeated calls in a
+     caller's loop do not accumulate device memory) */
+  template <class T>
+  int grow(T **p, size_t *cap, size_t n) {
+    if (*p && n <= *cap) return CGFD_OK;
+    if (*p) {
+      for (void *&q : mem.ptrs)
+        if (q == (void *)*p) { (void)hipFree(q); q = nullptr; }
+      *p = nullptr;
+      *cap = 0;
+    }
+    if (int rc = alloc(p, n)) return rc;
+    *cap = n;
     return CGFD_OK;
   }
 };
@@ -922,12 +941,11 @@ extern "C" int cgfd_adj_forward(cgfd_core *c, int nt) {
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   c->drop_graph();
-  c->tape = nullptr;
   c->tape_nt = 0;
   c->tape_it0 = c->it_host;
   if (c->nfault == 0) return CGFD_OK;  /* the wave update needs no tape */
   const size_t n = (size_t)c->nscen * c->nfault * nt * kTapeFloats * c->g.sslice;
-  if (int rc = c->alloc(&c->tape, n)) return rc;
+  if (int rc = c->grow(&c->tape, &c->tape_cap, n)) return rc;
   HIPCHK(hipMemset(c->tape, 0, n * 4));
   c->tape_nt = nt;
   return CGFD_OK;
@@ -1148,9 +1166,9 @@ extern "C" int cgfd_core_set_fault_receivers(cgfd_core *c, int n, const int *fjk
   }
   int rc;
   if (n > 0) {
-    if ((rc = c->alloc(&c->fr_idx, idx.size()))) return rc;
+    if ((rc = c->grow(&c->fr_idx, &c->fr_idx_cap, idx.size()))) return rc;
     HIPCHK(hipMemcpy(c->fr_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
-    if ((rc = c->alloc(&c->fr_buf, (size_t)n * nt_max * 9))) return rc;
+    if ((rc = c->grow(&c->fr_buf, &c->fr_buf_cap, (size_t)n * nt_max * 9))) return rc;
     HIPCHK(hipMemset(c->fr_buf, 0, (size_t)n * nt_max * 9 * 4));
   }
   c->fr_n = n;
@@ -1331,9 +1349,9 @@ extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int n
   int rc;
   c->bat.ststride = (long)n * nt_max * 9;
   if (n > 0) {
-    if ((rc = c->alloc(&c->st_idx, (size_t)n))) return rc;
+    if ((rc = c->grow(&c->st_idx, &c->st_idx_cap, (size_t)n))) return rc;
     HIPCHK(hipMemcpy(c->st_idx, idx.data(), (size_t)n * sizeof(long), hipMemcpyHostToDevice));
-    if ((rc = c->alloc(&c->st_buf, (size_t)c->nscen * n * nt_max * 9))) return rc;
+    if ((rc = c->grow(&c->st_buf, &c->st_buf_cap, (size_t)c->nscen * n * nt_max * 9))) return rc;
     HIPCHK(hipMemset(c->st_buf, 0, (size_t)c->nscen * n * nt_max * 9 * 4));
   }
   c->st_n = n;

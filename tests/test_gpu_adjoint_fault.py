@@ -199,3 +199,44 @@ def test_fault_adjoint_needs_tape(built_libs):
     assert b"friction tape" in core.lib.cgfd_last_error()
     core.close()
     st.close()
+
+
+def test_adjoint_cycles_reuse_buffers(built_libs):
+    """An inversion loop restarts one core from its saved initial state and
+    re-arms stations and tape every cycle: the second cycle's adjoint is as
+    exact as the first, and re-arming does not grow device memory (the
+    buffers are reused, not reallocated)."""
+    import ctypes as C
+    from cgfd import model
+    nt = 20
+    sc = _scenario(2, locked=True)
+    st = model.Setup(sc)
+    core = model.Core(st)
+    s0 = core.save_state()
+    for cycle in range(2):
+        core.load_state(s0)
+        core.set_stations(sc.stations + 3, nt)
+        core.adjoint_forward(nt)
+        w0 = _smooth_field(sc, 50 + cycle, 1e-3)
+        core.write_wavefield(w0)
+        core.run(nt)
+        tr = core.read_stations()
+        r = np.random.default_rng(60 + cycle).standard_normal(tr.shape).astype(np.float32)
+        r[:, :, 3:] *= 1e-6
+        g, _ = core.adjoint_fault(r)
+        lhs = float(np.sum(g.astype(np.float64) * w0.astype(np.float64)))
+        rhs = float(np.sum(r.astype(np.float64) * tr.astype(np.float64)))
+        print(f"[adjoint cycle {cycle}] rel {abs(lhs - rhs) / abs(rhs):.2e}", flush=True)
+        assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
+    hip = C.CDLL("libamdhip64.so.7")
+    free0, free1, total = C.c_size_t(), C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free0), C.byref(total)) == 0
+    for _ in range(20):
+        core.set_stations(sc.stations + 3, nt)
+        core.adjoint_forward(nt)
+    assert hip.hipMemGetInfo(C.byref(free1), C.byref(total)) == 0
+    grew = free0.value - free1.value
+    print(f"[adjoint cycles] device memory taken by 20 re-arms: {grew / 2**20:.1f} MiB", flush=True)
+    assert grew < 8 << 20
+    core.close()
+    st.close()
