@@ -229,6 +229,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-only", action="store_true", help="run only the batched leg (profiling)")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--batched-steps", type=int, default=16)
@@ -245,6 +246,9 @@ def main() -> int:
 
     if args.calib_copy:
         calib_copy(args.calib_copy)
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_leg(args.adjoint_steps)}), flush=True)
+        return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args.batched_steps, args.nscen, args.batched_variant,
                                                  args.overlap)}), flush=True)

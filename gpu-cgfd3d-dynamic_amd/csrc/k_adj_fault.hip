@@ -249,20 +249,35 @@ __global__ void __launch_bounds__(256) k_adjf_lev(AdjFaultArgs a) {
 /* fault_cols (k_fault.hip) transposed: six lanes per node, one per column */
 constexpr int kColNodes = 10;
 
+/* The block's traction-vector adjoints (the entries the forward kernel
+   computes once into LDS: 13 x-columns, 16 y-rows and 10 z-planes around
+   its kColNodes x kColRows nodes) are accumulated in LDS by the lanes and
+   turned into stress adjoints once per entry. */
+constexpr int kColRows = 4;
+constexpr int kTvX = 13 * kColNodes * kColRows;
+constexpr int kTvY = 6 * (kColNodes + 6) * kColRows;
+constexpr int kTvZ = 6 * kColNodes * (kColRows + 6);
+constexpr int kTv = kTvX + kTvY + kTvZ;
+
 template <int IDIR>
 __global__ void __launch_bounds__(256) k_adjf_cols(AdjFaultArgs a) {
-  const int lane = threadIdx.x;
-  if (lane >= 6 * kColNodes) return;
-  const int iy = blockIdx.x * kColNodes + lane / 6, iz = blockIdx.y * blockDim.y + threadIdx.y;
-  if (iy >= a.g.nj || iz >= a.g.nk) return;
+  __shared__ float atv[3][kTv];
   const int id = blockIdx.z;
   const FaultDev &F = a.ftab[id];
   const long ss = a.g.sslice, sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
+  const int i0 = F.i0;
+  const int j0 = blockIdx.x * kColNodes, k0 = blockIdx.y * kColRows;
+  const int lane = threadIdx.x, jn = lane / 6, kr = threadIdx.y;
+  const int iy = j0 + jn, iz = k0 + kr;
   const long pf = (iy + 3) + (long)(iz + 3) * a.g.ny;
-  if (F.united[pf] != 0) return;
+  const bool mine = lane < 6 * kColNodes && iy < a.g.nj && iz < a.g.nk && F.united[pf] == 0;
+  if (!__syncthreads_or(mine)) return;  /* blocks of welded nodes only */
+  const int tid = threadIdx.x + 64 * threadIdx.y;
+  for (int v = tid; v < 3 * kTv; v += 64 * kColRows) (&atv[0][0])[v] = 0.0f;
+  __syncthreads();
   const int cl = lane % 6;
   const int mm = cl < 3 ? cl - 3 : cl - 2;
-  const int i0 = F.i0, i = i0 + mm, n = -mm;
+  const int i = i0 + mm, n = -mm;
   const long row = (long)(iy + 3) * sy + (long)(iz + 3) * sz;
   const long p = i + row;
   const int km = a.g.nk - (iz + 1), n_free = km + 3;
@@ -272,60 +287,73 @@ __global__ void __launch_bounds__(256) k_adjf_cols(AdjFaultArgs a) {
   float *blk = adj_block(a, id);
   float *aT1 = blk + kOffT1 * ss, *aC = adj_lev(a, blk, a.lc);
   float hh[9];
+  if (mine) {
 #pragma unroll
-  for (int f = 0; f < 9; f++) hh[f] = a.cb * a.wE[f * sc + p] + (a.wX ? a.ca * a.wX[f * sc + p] : 0.0f);
-  const int tf[3][3] = {{TXX, TXY, TXZ}, {TXY, TYY, TYZ}, {TXZ, TYZ, TZZ}};
-
-  /* velocity rhs: (dvx + dvy + dvz) slw / J over the traction vectors
-     J (grad xi_d . sigma) at the taps */
-  const float rrhojac = a.slw[p] / met[p];
+    for (int f = 0; f < 9; f++) hh[f] = a.cb * a.wE[f * sc + p] + (a.wX ? a.ca * a.wX[f * sc + p] : 0.0f);
+    /* velocity rhs: (dvx + dvy + dvz) slw / J over the traction vectors at
+       the taps */
+    const float rrhojac = a.slw[p] / met[p];
 #pragma unroll 1
-  for (int c = 0; c < 3; c++) {
-    const float gc = hh[c] * rrhojac;
-    float au[7];
+    for (int c = 0; c < 3; c++) {
+      const float gc = hh[c] * rrhojac;
+      float au[7];
 #pragma unroll
-    for (int l = 0; l < 7; l++) au[l] = gc * vecc_coef(l - 3, IDIR);
-    /* images beyond the plane, then the plane tap (the fault traction) */
-    if (n == 2) { au[5] += 2.0f * au[6]; au[4] -= au[6]; au[6] = 0.0f; }
-    if (n == 1) {
-      au[4] += 2.0f * au[6]; au[2] -= au[6]; au[6] = 0.0f;
-      au[4] += 2.0f * au[5]; au[3] -= au[5]; au[5] = 0.0f;
-    }
-    if (n == -1) {
-      au[2] += 2.0f * au[1]; au[3] -= au[1]; au[1] = 0.0f;
-      au[2] += 2.0f * au[0]; au[4] -= au[0]; au[0] = 0.0f;
-    }
-    if (n == -2) { au[1] += 2.0f * au[0]; au[2] -= au[0]; au[0] = 0.0f; }
-    aadd(aT1 + (long)(7 * c + 3) * ss + pf, au[n + 3]);
-    au[n + 3] = 0.0f;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      float w[7];
-      long st;
-      if (d == 0) {
-#pragma unroll
-        for (int l = 0; l < 7; l++) w[l] = au[l];
-        st = 1;
-      } else if (d == 1) {
-#pragma unroll
-        for (int l = 0; l < 7; l++) w[l] = gc * vecc_coef(l - 3, a.jdir);
-        st = sy;
-      } else {
-#pragma unroll
-        for (int l = 0; l < 7; l++) w[l] = gc * vecc_coef(l - 3, a.kdir);
-        if (surf && km <= 3) t_free_image(w, n_free);
-        st = sz;
+      for (int l = 0; l < 7; l++) au[l] = gc * vecc_coef(l - 3, IDIR);
+      /* images beyond the plane, then the plane tap (the fault traction) */
+      if (n == 2) { au[5] += 2.0f * au[6]; au[4] -= au[6]; au[6] = 0.0f; }
+      if (n == 1) {
+        au[4] += 2.0f * au[6]; au[2] -= au[6]; au[6] = 0.0f;
+        au[4] += 2.0f * au[5]; au[3] -= au[5]; au[5] = 0.0f;
       }
+      if (n == -1) {
+        au[2] += 2.0f * au[1]; au[3] -= au[1]; au[1] = 0.0f;
+        au[2] += 2.0f * au[0]; au[4] -= au[0]; au[0] = 0.0f;
+      }
+      if (n == -2) { au[1] += 2.0f * au[0]; au[2] -= au[0]; au[0] = 0.0f; }
+      aadd(aT1 + (long)(7 * c + 3) * ss + pf, au[n + 3]);
+      au[n + 3] = 0.0f;
+      float wz[7];
 #pragma unroll
-      for (int l = 0; l < 7; l++) {
-        if (w[l] == 0.0f) continue;
-        const long q = p + (l - 3) * st;
-        const float u = w[l] * met[q];
+      for (int t = 0; t < 7; t++) wz[t] = gc * vecc_coef(t - 3, a.kdir);
+      if (surf && km <= 3) t_free_image(wz, n_free);
 #pragma unroll
-        for (int b = 0; b < 3; b++) aadd(T + tf[c][b] * sc + q, u * met[(1 + 3 * d + b) * sc + q]);
+      for (int t = 0; t < 7; t++) {
+        if (au[t] != 0.0f) atomicAdd(&atv[c][mm + t + 3 + 13 * (jn + kColNodes * kr)], au[t]);
+        const float wy = gc * vecc_coef(t - 3, a.jdir);
+        if (wy != 0.0f) atomicAdd(&atv[c][kTvX + cl + 6 * (jn + t + (kColNodes + 6) * kr)], wy);
+        if (wz[t] != 0.0f) atomicAdd(&atv[c][kTvX + kTvY + cl + 6 * (jn + kColNodes * (kr + t))], wz[t]);
       }
     }
   }
+  __syncthreads();
+  /* each entry's stress adjoints: tv_c = J (a1 s_c0 + a2 s_c1 + a3 s_c2) */
+  for (int v = tid; v < kTv; v += 64 * kColRows) {
+    const float t0 = atv[0][v], t1 = atv[1][v], t2 = atv[2][v];
+    if (t0 == 0.0f && t1 == 0.0f && t2 == 0.0f) continue;
+    int x, jg, kg, d;
+    if (v < kTvX) {
+      const int xi = v % 13, r = v / 13;
+      x = i0 - 6 + xi; jg = j0 + r % kColNodes + 3; kg = k0 + r / kColNodes + 3; d = 0;
+    } else if (v < kTvX + kTvY) {
+      const int u = v - kTvX, col = u % 6, r = u / 6;
+      x = i0 + (col < 3 ? col - 3 : col - 2); jg = j0 + r % (kColNodes + 6); kg = k0 + r / (kColNodes + 6) + 3; d = 1;
+    } else {
+      const int u = v - kTvX - kTvY, col = u % 6, r = u / 6;
+      x = i0 + (col < 3 ? col - 3 : col - 2); jg = j0 + r % kColNodes + 3; kg = k0 + r / kColNodes; d = 2;
+    }
+    if (x < 0 || x >= a.g.nx || jg < 0 || jg >= a.g.ny || kg < 0 || kg >= a.g.nz) continue;
+    const long q = x + (long)jg * sy + (long)kg * sz;
+    const float jq = met[q], a1 = met[(1 + 3 * d) * sc + q], a2 = met[(2 + 3 * d) * sc + q],
+                a3 = met[(3 + 3 * d) * sc + q];
+    const float u0 = jq * t0, u1 = jq * t1, u2 = jq * t2;
+    aadd(T + TXX * sc + q, a1 * u0);
+    aadd(T + TXY * sc + q, a2 * u0 + a1 * u1);
+    aadd(T + TXZ * sc + q, a3 * u0 + a1 * u2);
+    aadd(T + TYY * sc + q, a2 * u1);
+    aadd(T + TYZ * sc + q, a3 * u1 + a2 * u2);
+    aadd(T + TZZ * sc + q, a3 * u2);
+  }
+  if (!mine) return;
 
   /* stress rhs: Hooke's law on the velocity differences */
   const float l = a.lam[p], m = a.mu[p], l2m = (float)(l + 2.0 * m);
