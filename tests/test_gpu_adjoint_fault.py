@@ -27,12 +27,13 @@ pytestmark = pytest.mark.gpu
 PARAMS = ["mu_s", "mu_d", "Dc", "C0", "Tx", "Ty", "Tz"]
 
 
-def _scenario(imethod, locked):
+def _scenario(imethod, locked, nfault=1, free_top=True):
     from cgfd import configs
-    sc = configs.small(imethod=imethod)
+    sc = configs.small(imethod=imethod, nfault=nfault, free_top=free_top)
     f = sc.faults[0]
     if locked:
-        f.fields["mu_s"][:] = 1000.0
+        for ff in sc.faults:
+            ff.fields["mu_s"][:] = 1000.0
     i0 = f.x_index
     j1, j2, k1, k2 = f.grid
     jc = (j1 + j2) // 2
@@ -55,11 +56,12 @@ def _smooth_field(sc, seed, amp):
     return w
 
 
-@pytest.mark.parametrize("imethod", [1, 2])
-def test_fault_adjoint_linear_identity_locked(built_libs, imethod):
+@pytest.mark.parametrize("imethod,nfault,free_top", [(1, 1, True), (2, 1, True), (2, 2, True), (1, 2, True),
+                                                     (2, 1, False), (1, 1, False)])
+def test_fault_adjoint_linear_identity_locked(built_libs, imethod, nfault, free_top):
     from cgfd import model
     nt = 30
-    sc = _scenario(imethod, locked=True)
+    sc = _scenario(imethod, locked=True, nfault=nfault, free_top=free_top)
     st = model.Setup(sc)
     core = model.Core(st)
     core.set_stations(sc.stations + 3, nt)
@@ -74,7 +76,8 @@ def test_fault_adjoint_linear_identity_locked(built_libs, imethod):
     g, gf = core.adjoint_fault(r)
     lhs = float(np.sum(g.astype(np.float64) * w0.astype(np.float64)))
     rhs = float(np.sum(r.astype(np.float64) * tr.astype(np.float64)))
-    print(f"[fault adjoint locked, method {imethod}] <g, w0> = {lhs:.9e}, sum r.T(w0) = {rhs:.9e}, "
+    print(f"[fault adjoint locked, method {imethod}, {nfault} fault(s), free top {free_top}] <g, w0> = {lhs:.9e}, "
+          f"sum r.T(w0) = {rhs:.9e}, "
           f"rel {abs(lhs - rhs) / abs(rhs):.2e}", flush=True)
     assert np.isfinite(g).all() and np.isfinite(gf).all()
     assert abs(lhs - rhs) <= 1e-4 * abs(rhs)
