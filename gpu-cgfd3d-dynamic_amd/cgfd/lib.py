@@ -119,7 +119,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_io_snap_create": (I32, [VP, C.c_char_p, VP, I32, C.c_char_p, C.c_char_p, VP]),
         "cgfd_io_snap_put": (I32, [VP, VP, I32, C.c_float]),
         "cgfd_io_pg_write": (I32, [VP, C.c_char_p, C.c_char_p]),
-        "cgfd_io_stations_sac": (I32, [VP, C.c_char_p, VP, VP]),
+        "cgfd_io_stations_sac": (I32, [VP, C.c_char_p, VP, VP, I32]),
         "cgfd_core_read_snapshot": (I32, [VP, VP, I32, VP]),
         "cgfd_batch_read_snapshot": (I32, [VP, I32, VP, I32, VP]),
         "cgfd_core_read_fault_receivers": (I32, [VP, VP, VP]),

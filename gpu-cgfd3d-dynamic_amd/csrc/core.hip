@@ -1752,7 +1752,8 @@ extern "C" int cgfd_io_pg_write(cgfd_core *c, const char *out_dir, const char *f
    <out_dir>/<name>.<Vx..Txy>.sac, all nt_max samples of the station window.
    As the reference's call, the event position is zero and the station's
    depth field holds dt (its arguments stop one position short of stdp). */
-extern "C" int cgfd_io_stations_sac(cgfd_core *c, const char *out_dir, const char *const *names, const float *xyz) {
+extern "C" int cgfd_io_stations_sac(cgfd_core *c, const char *out_dir, const char *const *names, const float *xyz,
+                                    int with_strain) {
   if (!c || !names || !xyz) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   if (c->st_n <= 0) return cgfd_set_error(CGFD_ERR_STATE, "no stations set");
   HIPCHK(hipSetDevice(c->dev));
@@ -1763,6 +1764,8 @@ extern "C" int cgfd_io_stations_sac(cgfd_core *c, const char *out_dir, const cha
   static const char *cmp[9] = {"Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"};
   const float ev[4] = {0.0f, 0.0f, 0.0f, 0.0f};
   const std::string dir = out_dir && *out_dir ? std::string(out_dir) + "/" : std::string();
+  std::vector<long> idx((size_t)n);
+  if (with_strain) HIPCHK(hipMemcpy(idx.data(), c->st_idx, (size_t)n * sizeof(long), hipMemcpyDeviceToHost));
   for (int s = 0; s < n; s++) {
     if (!names[s]) return cgfd_set_error(CGFD_ERR_ARG, "station %d has no name", s);
     const float st4[4] = {xyz[3 * s], xyz[3 * s + 1], xyz[3 * s + 2], c->dt};
@@ -1770,6 +1773,32 @@ extern "C" int cgfd_io_stations_sac(cgfd_core *c, const char *out_dir, const cha
       for (int t = 0; t < nt; t++) tr[t] = h[((size_t)s * nt + t) * 9 + q];
       if (int rc = cgfd::write_sac(dir + names[s] + "." + cmp[q] + ".sac", tr.data(), nt, c->dt, ev, st4)) return rc;
     }
+    if (!with_strain) continue;
+    /* strain from the stress with the isotropic compliance at the station
+       node (io_recv_output_sac_el_iso_strain, io_funcs.cu:626-700) */
+    float lm[2];
+    HIPCHK(hipMemcpy(&lm[0], c->lam + idx[s], 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&lm[1], c->mu + idx[s], 4, hipMemcpyDeviceToHost));
+    const float lam = lm[0], mu = lm[1];
+    const float E1 = (lam + mu) / (mu * (3.0 * lam + 2.0 * mu));
+    const float E2 = -lam / (2.0 * mu * (3.0 * lam + 2.0 * mu));
+    const float E3 = 1.0 / mu;
+    static const char *ecmp[6] = {"Exx", "Eyy", "Ezz", "Eyz", "Exz", "Exy"};
+    std::vector<float> e((size_t)6 * nt);
+    for (int t = 0; t < nt; t++) {
+      const float *w = &h[((size_t)s * nt + t) * 9];
+      const float E0 = E2 * (w[3] + w[4] + w[5]);
+      e[0 * (size_t)nt + t] = E0 - (E2 - E1) * w[3];
+      e[1 * (size_t)nt + t] = E0 - (E2 - E1) * w[4];
+      e[2 * (size_t)nt + t] = E0 - (E2 - E1) * w[5];
+      e[3 * (size_t)nt + t] = 0.5 * E3 * w[6];
+      e[4 * (size_t)nt + t] = 0.5 * E3 * w[7];
+      e[5 * (size_t)nt + t] = 0.5 * E3 * w[8];
+    }
+    for (int q = 0; q < 6; q++)
+      if (int rc = cgfd::write_sac(dir + names[s] + "." + ecmp[q] + ".sac", e.data() + (size_t)q * nt, nt, c->dt, ev,
+                                   st4))
+        return rc;
   }
   return CGFD_OK;
 }

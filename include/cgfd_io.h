@@ -68,8 +68,11 @@ int cgfd_io_pg_write(cgfd_core *core, const char *out_dir, const char *frame);
    cmp = Vx, Vy, Vz, Txx, Tyy, Tzz, Tyz, Txz, Txy, with every sample of the
    station window (cgfd_core_set_stations), delta = dt, b = 0, station
    position xyz[s] in stla, stlo, stel and, as in the reference's call, dt in
-   stdp; native byte order. */
-int cgfd_io_stations_sac(cgfd_core *core, const char *out_dir, const char *const *names, const float *xyz);
+   stdp; native byte order. With with_strain also <name>.{Exx,Eyy,Ezz,Eyz,
+   Exz,Exy}.sac, the strain from the stress with the isotropic compliance at
+   the station node (io_recv_output_sac_el_iso_strain, io_funcs.cu:626-700). */
+int cgfd_io_stations_sac(cgfd_core *core, const char *out_dir, const char *const *names, const float *xyz,
+                         int with_strain);
 
 #ifdef __cplusplus
 }

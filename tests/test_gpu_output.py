@@ -115,7 +115,8 @@ def test_station_sac_files(built_libs, tmp_path):
     ns = len(sc.stations)
     names = (C.c_char_p * ns)(*[f"ST{q:02d}".encode() for q in range(ns)])
     xyz = np.ascontiguousarray(np.arange(ns * 3, dtype=np.float32).reshape(ns, 3) * 100.0)
-    _ok(core, core.lib.cgfd_io_stations_sac(core.handle, str(tmp_path).encode(), names, xyz.ctypes.data_as(C.c_void_p)))
+    _ok(core, core.lib.cgfd_io_stations_sac(core.handle, str(tmp_path).encode(), names, xyz.ctypes.data_as(C.c_void_p),
+                                            1))
     dt = np.float32(st.desc.dt)
     for q, cmp in enumerate(["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]):
         for s_ in (0, ns - 1):
@@ -127,5 +128,22 @@ def test_station_sac_files(built_libs, tmp_path):
             assert n[6] == 6 and n[9] == nt and n[15] == 1 and n[35] == 1 and n[0] == -12345
             assert k[:8] == b"-12345\0\0" and k[160:168] == b"-12345\0\0" and k[8:24] == b"-12345  " + bytes(8)
             assert k[24:32] == b"-12345  "
+    # strain files: the reference's float/double mix, at the station node's media
+    for s_ in (0, ns - 1):
+        i, j, k_ = sc.stations[s_]
+        lam, mu = np.float32(sc.lam[k_, j, i]), np.float32(sc.mu[k_, j, i])
+        e1 = np.float32((np.float64(lam + mu)) / (np.float64(mu) * (3.0 * np.float64(lam) + 2.0 * np.float64(mu))))
+        e2 = np.float32(-np.float64(lam) / (2.0 * np.float64(mu) * (3.0 * np.float64(lam) + 2.0 * np.float64(mu))))
+        e3 = np.float32(1.0 / np.float64(mu))
+        t = tr[s_].astype(np.float32)
+        e0 = (e2 * (t[:, 3] + t[:, 4] + t[:, 5])).astype(np.float32)
+        ref = {"Exx": e0 - (e2 - e1) * t[:, 3], "Eyy": e0 - (e2 - e1) * t[:, 4], "Ezz": e0 - (e2 - e1) * t[:, 5],
+               "Eyz": (0.5 * np.float64(e3) * t[:, 6].astype(np.float64)).astype(np.float32),
+               "Exz": (0.5 * np.float64(e3) * t[:, 7].astype(np.float64)).astype(np.float32),
+               "Exy": (0.5 * np.float64(e3) * t[:, 8].astype(np.float64)).astype(np.float32)}
+        for n_, r_ in ref.items():
+            _, _, _, data = _read_sac(tmp_path / f"ST{s_:02d}.{n_}.sac")
+            scale = max(float(np.abs(r_).max()), 1e-30)
+            assert float(np.abs(data - r_).max()) <= 1e-6 * scale, n_
     core.close()
     st.close()
