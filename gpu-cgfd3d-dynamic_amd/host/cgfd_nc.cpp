@@ -1,5 +1,6 @@
 /*
- * cgfd_nc.cpp -- NetCDF classic (64-bit offset) writer; see cgfd_nc.h.
+ * cgfd_nc.cpp -- NetCDF classic writer (64-bit offset) and reader; see
+ * cgfd_nc.h. Also the reader of the reference's init_stress input.
  */
 #include "cgfd_nc.h"
 
@@ -7,6 +8,7 @@
 
 #include "cgfd_core.h"
 #include "cgfd_errors.h"
+#include "cgfd_io.h"
 
 namespace cgfd {
 
@@ -159,4 +161,127 @@ int NcWriter::close() {
   return rc;
 }
 
+/* ---------------------------------------------------------------------- */
+namespace {
+struct Cursor {
+  const std::vector<unsigned char> &b;
+  size_t o = 0;
+  bool ok = true;
+  uint32_t u32() {
+    if (o + 4 > b.size()) { ok = false; return 0; }
+    uint32_t v = ((uint32_t)b[o] << 24) | ((uint32_t)b[o + 1] << 16) | ((uint32_t)b[o + 2] << 8) | b[o + 3];
+    o += 4;
+    return v;
+  }
+  uint64_t u64() { uint64_t hi = u32(); return (hi << 32) | u32(); }
+  std::string name() {
+    const uint32_t n = u32();
+    if (!ok || o + n > b.size()) { ok = false; return std::string(); }
+    std::string s((const char *)&b[o], n);
+    o += (n + 3) / 4 * 4;
+    return s;
+  }
+  void skip_values(uint32_t type, uint32_t n) {
+    const size_t sz = type == 1 || type == 2 ? 1 : (type == 3 ? 2 : (type == 6 ? 8 : 4));
+    o += (n * sz + 3) / 4 * 4;
+    if (o > b.size()) ok = false;
+  }
+};
+}  // namespace
+
+NcReader::~NcReader() {
+  if (fp_) std::fclose(fp_);
+}
+
+int NcReader::open(const std::string &path) {
+  fp_ = std::fopen(path.c_str(), "rb");
+  if (!fp_) return cgfd_set_error(CGFD_ERR_ARG, "cannot open %s", path.c_str());
+  path_ = path;
+  std::vector<unsigned char> b(1 << 16);
+  b.resize(std::fread(b.data(), 1, b.size(), fp_));
+  if (b.size() < 8 || b[0] != 'C' || b[1] != 'D' || b[2] != 'F' || (b[3] != 1 && b[3] != 2))
+    return cgfd_set_error(CGFD_ERR_ARG, "%s is not a NetCDF classic file", path.c_str());
+  const bool off64 = b[3] == 2;
+  Cursor c{b};
+  c.o = 4;
+  c.u32();  /* numrecs */
+  uint32_t tag = c.u32(), n = c.u32();
+  if (tag == kNcDimension)
+    for (uint32_t q = 0; q < n; q++) { c.name(); dimlen_.push_back((long)c.u32()); }
+  tag = c.u32(); n = c.u32();
+  if (tag == kNcAttribute)
+    for (uint32_t q = 0; q < n; q++) { c.name(); const uint32_t t = c.u32(), m = c.u32(); c.skip_values(t, m); }
+  tag = c.u32(); n = c.u32();
+  if (tag == kNcVariable)
+    for (uint32_t q = 0; q < n; q++) {
+      Var v;
+      v.name = c.name();
+      const uint32_t nd = c.u32();
+      for (uint32_t d = 0; d < nd; d++) v.dims.push_back((int)c.u32());
+      const uint32_t at = c.u32(), na = c.u32();
+      if (at == kNcAttribute)
+        for (uint32_t a = 0; a < na; a++) { c.name(); const uint32_t t = c.u32(), m = c.u32(); c.skip_values(t, m); }
+      v.type = c.u32();
+      c.u32();  /* vsize */
+      v.begin = off64 ? (int64_t)c.u64() : (int64_t)c.u32();
+      vars_.push_back(v);
+    }
+  if (!c.ok) return cgfd_set_error(CGFD_ERR_ARG, "%s: header larger than 64 KiB or truncated", path.c_str());
+  return CGFD_OK;
+}
+
+int NcReader::shape(const std::string &name, std::vector<long> &dims) const {
+  for (const Var &v : vars_)
+    if (v.name == name) {
+      dims.clear();
+      for (int d : v.dims) dims.push_back(dimlen_[d]);
+      return CGFD_OK;
+    }
+  return cgfd_set_error(CGFD_ERR_ARG, "%s has no variable %s", path_.c_str(), name.c_str());
+}
+
+int NcReader::get_float(const std::string &name, std::vector<float> &out) {
+  for (const Var &v : vars_) {
+    if (v.name != name) continue;
+    size_t n = 1;
+    for (int d : v.dims) {
+      if (dimlen_[d] == 0) return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is a record variable", path_.c_str(), name.c_str());
+      n *= (size_t)dimlen_[d];
+    }
+    if (v.type != kNcFloat && v.type != 6)
+      return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is neither float nor double", path_.c_str(), name.c_str());
+    const size_t sz = v.type == kNcFloat ? 4 : 8;
+    std::vector<unsigned char> raw(n * sz);
+    if (std::fseek(fp_, (long)v.begin, SEEK_SET) != 0 || std::fread(raw.data(), 1, raw.size(), fp_) != raw.size())
+      return cgfd_set_error(CGFD_ERR_ARG, "%s: short read of %s", path_.c_str(), name.c_str());
+    out.resize(n);
+    for (size_t i = 0; i < n; i++) {
+      uint64_t u = 0;
+      for (size_t k = 0; k < sz; k++) u = (u << 8) | raw[i * sz + k];
+      if (sz == 4) { uint32_t w = (uint32_t)u; std::memcpy(&out[i], &w, 4); }
+      else { double d; std::memcpy(&d, &u, 8); out[i] = (float)d; }
+    }
+    return CGFD_OK;
+  }
+  return cgfd_set_error(CGFD_ERR_ARG, "%s has no variable %s", path_.c_str(), name.c_str());
+}
+
 }  // namespace cgfd
+
+/* init_stress_<n>.nc (nc_read_init_stress, fault_info.cu:1076-1187) */
+extern "C" int cgfd_io_read_init_stress(const char *path, int nj, int nk, float *fields) {
+  if (!path || !fields || nj <= 0 || nk <= 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  cgfd::NcReader r;
+  if (int rc = r.open(path)) return rc;
+  static const char *names[7] = {"Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0"};
+  std::vector<float> v;
+  std::vector<long> dims;
+  for (int q = 0; q < 7; q++) {
+    if (int rc = r.shape(names[q], dims)) return rc;
+    if (dims.size() != 2 || dims[0] != nk || dims[1] != nj)
+      return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is not [%d][%d] (nk, nj)", path, names[q], nk, nj);
+    if (int rc = r.get_float(names[q], v)) return rc;
+    std::memcpy(fields + (size_t)q * nj * nk, v.data(), (size_t)nj * nk * 4);
+  }
+  return CGFD_OK;
+}

@@ -1,5 +1,6 @@
 /*
- * cgfd_nc.h -- writer of NetCDF classic files (64-bit offset variant), the
+ * cgfd_nc.h -- writer (and a small reader) of NetCDF classic files; the writer
+ * produces the 64-bit offset variant, the
  * subset the output files of the time loop use: named dimensions (one of
  * them unlimited), float variables, integer global attributes, whole
  * fixed-size variables and one record of a record variable at a time.
@@ -47,6 +48,24 @@ class NcWriter {
   long numrecs_ = 0;
   bool defined_ = false;
   int write_at(int64_t off, const void *p, size_t n);
+};
+
+/* reader of NetCDF classic files (CDF-1 and the 64-bit offset CDF-2): the
+   header, and whole fixed-size NC_FLOAT / NC_DOUBLE variables as float */
+class NcReader {
+ public:
+  ~NcReader();
+  int open(const std::string &path);  /* 0 or a CGFD_ERR_* code */
+  /* the variable's dimension lengths, or an error if it is absent */
+  int shape(const std::string &name, std::vector<long> &dims) const;
+  int get_float(const std::string &name, std::vector<float> &out);
+
+ private:
+  struct Var { std::string name; std::vector<int> dims; uint32_t type = 0; int64_t begin = 0; };
+  std::FILE *fp_ = nullptr;
+  std::string path_;
+  std::vector<long> dimlen_;
+  std::vector<Var> vars_;
 };
 
 }  // namespace cgfd

@@ -74,6 +74,12 @@ int cgfd_io_pg_write(cgfd_core *core, const char *out_dir, const char *frame);
 int cgfd_io_stations_sac(cgfd_core *core, const char *out_dir, const char *const *names, const float *xyz,
                          int with_strain);
 
+/* Reads the reference's fault input init_stress_<n>.nc (nc_read_init_stress,
+   fault_info.cu:1076-1187; written by the examples' prep_fault/conf_fault_stress.m
+   as NetCDF classic): fields[7][nk][nj] = Tx, Ty, Tz, mu_s, mu_d, Dc, C0 (float
+   or double in the file), to fill cgfd_fault_desc. Host only; no GPU call. */
+int cgfd_io_read_init_stress(const char *path, int nj, int nk, float *fields);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,3 +78,31 @@ def test_nc_writer_read_back_by_scipy(nc_file):
         assert np.array_equal(f.variables["Peak_vs"][:], pk)
         assert int(f.i_index_with_ghosts_in_this_thread) == 21
         assert list(f.coords_of_mpi_topo) == [0, 1, 2]
+
+
+def test_init_stress_reader(built_libs, tmp_path):
+    """init_stress_<n>.nc as the reference's conf_fault_stress.m writes it
+    (NetCDF classic, dims nj, nk, float variables over (nk, nj), plus the
+    coordinates and increments it also stores), here one of them double:
+    read back exactly by cgfd_io_read_init_stress."""
+    import ctypes as C
+    from scipy.io import netcdf_file
+    from cgfd import lib as L
+    nj, nk = 17, 9
+    rng = np.random.default_rng(3)
+    vals = {n: rng.standard_normal((nk, nj)).astype(np.float32) for n in ("Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0")}
+    path = tmp_path / "init_stress_1.nc"
+    with netcdf_file(str(path), "w", version=1) as f:
+        f.createDimension("nj", nj)
+        f.createDimension("nk", nk)
+        for n in ("x", "y", "z", "Tx", "Ty", "Tz", "dTx", "dTy", "dTz", "mu_s", "mu_d", "Dc", "C0"):
+            v = f.createVariable(n, "d" if n == "C0" else "f", ("nk", "nj"))
+            v[:] = vals[n] if n in vals else np.zeros((nk, nj), np.float32)
+    lib = L.load_library()
+    out = np.zeros((7, nk, nj), np.float32)
+    rc = lib.cgfd_io_read_init_stress(str(path).encode(), nj, nk, out.ctypes.data_as(C.c_void_p))
+    assert rc == 0, lib.cgfd_last_error()
+    for q, n in enumerate(("Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0")):
+        assert np.array_equal(out[q], vals[n]), n
+    rc = lib.cgfd_io_read_init_stress(str(path).encode(), nj + 1, nk, out.ctypes.data_as(C.c_void_p))
+    assert rc == -1 and b"not [9][18]" in lib.cgfd_last_error()
