@@ -121,6 +121,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_io_pg_write": (I32, [VP, C.c_char_p, C.c_char_p]),
         "cgfd_io_stations_sac": (I32, [VP, C.c_char_p, VP, VP, I32]),
         "cgfd_io_read_init_stress": (I32, [C.c_char_p, I32, I32, VP]),
+        "cgfd_io_read_fault_coord": (I32, [C.c_char_p, I32, I32, VP]),
         "cgfd_core_read_snapshot": (I32, [VP, VP, I32, VP]),
         "cgfd_batch_read_snapshot": (I32, [VP, I32, VP, I32, VP]),
         "cgfd_core_read_fault_receivers": (I32, [VP, VP, VP]),

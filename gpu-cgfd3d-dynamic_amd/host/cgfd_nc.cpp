@@ -268,15 +268,15 @@ int NcReader::get_float(const std::string &name, std::vector<float> &out) {
 
 }  // namespace cgfd
 
-/* init_stress_<n>.nc (nc_read_init_stress, fault_info.cu:1076-1187) */
-extern "C" int cgfd_io_read_init_stress(const char *path, int nj, int nk, float *fields) {
+namespace {
+/* nf float fields of [nk][nj] from a classic file, in order */
+int read_planes(const char *path, int nj, int nk, const char *const *names, int nf, float *fields) {
   if (!path || !fields || nj <= 0 || nk <= 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
   cgfd::NcReader r;
   if (int rc = r.open(path)) return rc;
-  static const char *names[7] = {"Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0"};
   std::vector<float> v;
   std::vector<long> dims;
-  for (int q = 0; q < 7; q++) {
+  for (int q = 0; q < nf; q++) {
     if (int rc = r.shape(names[q], dims)) return rc;
     if (dims.size() != 2 || dims[0] != nk || dims[1] != nj)
       return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is not [%d][%d] (nk, nj)", path, names[q], nk, nj);
@@ -284,4 +284,17 @@ extern "C" int cgfd_io_read_init_stress(const char *path, int nj, int nk, float 
     std::memcpy(fields + (size_t)q * nj * nk, v.data(), (size_t)nj * nk * 4);
   }
   return CGFD_OK;
+}
+}  // namespace
+
+/* init_stress_<n>.nc (nc_read_init_stress, fault_info.cu:1076-1187) */
+extern "C" int cgfd_io_read_init_stress(const char *path, int nj, int nk, float *fields) {
+  static const char *names[7] = {"Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0"};
+  return read_planes(path, nj, nk, names, 7, fields);
+}
+
+/* fault_coord_<n>.nc (nc_read_fault_geometry, gd_t.cu:548-569) */
+extern "C" int cgfd_io_read_fault_coord(const char *path, int nj, int nk, float *xyz) {
+  static const char *names[3] = {"x", "y", "z"};
+  return read_planes(path, nj, nk, names, 3, xyz);
 }

@@ -79,6 +79,10 @@ int cgfd_io_stations_sac(cgfd_core *core, const char *out_dir, const char *const
    as NetCDF classic): fields[7][nk][nj] = Tx, Ty, Tz, mu_s, mu_d, Dc, C0 (float
    or double in the file), to fill cgfd_fault_desc. Host only; no GPU call. */
 int cgfd_io_read_init_stress(const char *path, int nj, int nk, float *fields);
+/* Reads the fault-plane coordinates of fault_coord_<n>.nc (nc_read_fault_geometry,
+   gd_t.cu:548-569): xyz[3][nk][nj] = x, y, z, the plane a grid generator
+   builds the curvilinear grid around. Host only. */
+int cgfd_io_read_fault_coord(const char *path, int nj, int nk, float *xyz);
 
 #ifdef __cplusplus
 }

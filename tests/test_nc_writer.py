@@ -106,3 +106,27 @@ def test_init_stress_reader(built_libs, tmp_path):
         assert np.array_equal(out[q], vals[n]), n
     rc = lib.cgfd_io_read_init_stress(str(path).encode(), nj + 1, nk, out.ctypes.data_as(C.c_void_p))
     assert rc == -1 and b"not [9][18]" in lib.cgfd_last_error()
+
+
+def test_fault_coord_reader(built_libs, tmp_path):
+    """fault_coord_<n>.nc as the reference's conf_fault_grid.m writes it
+    (dims nj, nk and dim; x, y, z over (nk, nj); vectors over dim)."""
+    import ctypes as C
+    from scipy.io import netcdf_file
+    from cgfd import lib as L
+    nj, nk = 12, 7
+    rng = np.random.default_rng(4)
+    xyz = rng.standard_normal((3, nk, nj)).astype(np.float32) * 1e4
+    path = tmp_path / "fault_coord_1.nc"
+    with netcdf_file(str(path), "w", version=1) as f:
+        f.createDimension("nj", nj)
+        f.createDimension("nk", nk)
+        f.createDimension("dim", 3)
+        for q, n in enumerate(("x", "y", "z")):
+            f.createVariable(n, "f", ("nk", "nj"))[:] = xyz[q]
+        f.createVariable("vec_n", "f", ("dim",))[:] = np.array([1, 0, 0], np.float32)
+    lib = L.load_library()
+    out = np.zeros((3, nk, nj), np.float32)
+    rc = lib.cgfd_io_read_fault_coord(str(path).encode(), nj, nk, out.ctypes.data_as(C.c_void_p))
+    assert rc == 0, lib.cgfd_last_error()
+    assert np.array_equal(out, xyz)
