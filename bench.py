@@ -234,7 +234,7 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--batched-steps", type=int, default=16)
     ap.add_argument("--nscen", type=int, default=32)
-    ap.add_argument("--batched-variant", type=int, default=19)
+    ap.add_argument("--batched-variant", type=int, default=20)
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()

@@ -63,7 +63,7 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
-  int variant = 19;
+  int variant = 20;
   int zchunk = 4;
   ZqTiles zq;                       /* tile lists for the current zchunk */
   std::map<int, ZqTiles> zq_cache;  /* by zchunk */
@@ -695,10 +695,13 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
    with XCD-contiguous block ids for both the interior and the PML blocks,
    18 = 16 with the interior x taps taken from neighbouring lanes (DPP),
    19 = 16 with the lean PML-tile kernel (directional rhs parts accumulated
-   per field) */
+   per field); 20 and 21 run 19's kernels on one scenario and differ on
+   batches (20: the PML-tile blocks in per-scenario slab sweeps, the interior
+   blocks with the scenarios of a tile back to back; 21: both in
+   per-scenario sweeps) */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   if (c->bat.nact > 1) {  /* batch kernels: split interior/PML for 16 and 17, else one direct kernel */
-    const int bv = (c->variant == 16 || c->variant == 17 || c->variant == 19) ? c->variant : 1;
+    const int bv = (c->variant == 16 || c->variant == 17 || c->variant >= 19) ? c->variant : 1;
     launch_volume(bv, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
   }
   else if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
@@ -1438,8 +1441,8 @@ extern "C" int cgfd_batch_active(cgfd_core *c) {
 
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
-  if (variant == 0) variant = 19;
-  if (variant < 1 || variant > 19) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
+  if (variant == 0) variant = 20;
+  if (variant < 1 || variant > 21) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
   c->variant = variant;
   c->drop_graph();
   return CGFD_OK;

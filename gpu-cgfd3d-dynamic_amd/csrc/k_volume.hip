@@ -866,12 +866,19 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
   }
   if (b.nact > 1 && variant >= 16 && variant != 18) {  /* batches, split kernels */
     long items, nb;
-    if (variant != 16) {  /* scenarios of a tile back to back, lean PML kernel */
-      items = (long)agrd.x * agrd.y * a.nk_run * b.nact;
-      nb = ((items + 7) / 8) * 8;
-      hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 0>), dim3((unsigned)nb), blk, 0, st, a, b, items);
-      if (a.pml_on)
-        hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0, true>), dim3((unsigned)nb), blk, 0, st_pml, a, b, items);
+    if (variant != 16) {  /* lean PML kernel; 20: its blocks in per-scenario slab sweeps, 21: both kernels */
+      const long items0 = (long)agrd.x * agrd.y * a.nk_run * b.nact, nb0 = ((items0 + 7) / 8) * 8;
+      const long nslab = ((long)agrd.y + a.zchunk - 1) / a.zchunk;
+      const long items1 = (long)agrd.x * a.zchunk * a.nk_run * nslab * b.nact;
+      if (variant == 21) hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 1>), dim3((unsigned)items1), blk, 0, st, a, b, items1);
+      else hipLaunchKernelGGL((k_vol_bat_acc<ID, JD, KD, 0>), dim3((unsigned)nb0), blk, 0, st, a, b, items0);
+      if (a.pml_on) {
+        if (variant >= 20)
+          hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 1, true>), dim3((unsigned)items1), blk, 0, st_pml, a, b, items1);
+        else
+          hipLaunchKernelGGL((k_vol_bat_pml<ID, JD, KD, 0, true>), dim3((unsigned)nb0), blk, 0, st_pml, a, b, items0);
+      }
+      (void)items; (void)nb;
     } else {
       const long nslab = ((long)agrd.y + a.zchunk - 1) / a.zchunk;
       nb = items = (long)agrd.x * a.zchunk * a.nk_run * nslab * b.nact;
@@ -891,7 +898,7 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     launch_vol_lds(a, ID, JD, KD, st);
     return;
   }
-  if (variant == 19) {
+  if (variant >= 19) {
     const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
     const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
     hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, false>), dim3((unsigned)nb), blk, 0, st, a, b);

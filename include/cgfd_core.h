@@ -292,7 +292,7 @@ int cgfd_core_read_fault_receivers(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
                         float *ms_volume, int *n_volume);
 
-/* Kernel variant selection for the volume stencil (0 = default = 19,
+/* Kernel variant selection for the volume stencil (0 = default = 20,
    1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
    4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
    6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
@@ -306,10 +306,13 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    17 = 16 with XCD-contiguous block ids for the interior and the PML
    blocks, 18 = 16 with the interior x taps moved between lanes by DPP
    wave shifts instead of loaded, 19 = 16 with a lean PML-tile kernel that
-   accumulates the directional rhs parts field by field). Batches with more than one active scenario run split interior /
-   PML kernels for variant 16 (one slab-ordered sweep per scenario) and
+   accumulates the directional rhs parts field by field, 20 and 21 = 19 on
+   one scenario). Batches with more than one active scenario run split interior /
+   PML kernels for variant 16 (one slab-ordered sweep per scenario),
    variants 17 and 19 (the scenarios of a tile back to back, XCD-contiguous,
-   lean PML kernel), and one direct
+   lean PML kernel), variant 20 (interior as 19, lean PML blocks in one
+   slab-ordered sweep per scenario: the fastest), variant 21 (both kernels
+   in per-scenario sweeps), and one direct
    tile x scenario kernel for every other variant.
    Returns CGFD_ERR_ARG for an unknown variant. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);

@@ -29,7 +29,7 @@ def _plain_with_fields(sc, fields):
     return model.Setup(sc2)
 
 
-@pytest.mark.parametrize("variant", [1, 7, 10, 16, 17, 19])
+@pytest.mark.parametrize("variant", [1, 7, 10, 16, 17, 19, 20, 21])
 def test_batch_matches_plain_runs(built_libs, variant):
     from cgfd import configs, model
     import oracle
