@@ -22,8 +22,7 @@ LIB_PATH = BUILD_DIR / "libcgfd.so"
 ORACLE_DIR = REPO / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle.so"
 
-HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_vol_lds.hip", "k_vol_zq.hip", "k_vol_pipe.hip",
-               "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip"]
+HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip"]
 HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp", "cgfd_nc.cpp", "cgfd_sac.cpp"]
 
 # Packed FP32 instructions stay disabled: hipcc otherwise forms v_pk_*_f32
@@ -49,17 +48,70 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
         raise RuntimeError(f"build step failed ({proc.returncode}): {' '.join(cmd)}")
 
 
-def build_core(verbose: bool = False) -> Path:
-    """Compile the HIP core + host setup into one shared library."""
+def _sources() -> list[Path]:
+    return [CSRC / s for s in HIP_SOURCES if (CSRC / s).exists()] + [HOST / s for s in HOST_SOURCES]
+
+
+def _digest(paths) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for p in sorted(paths):
+        h.update(p.name.encode())
+        h.update(p.read_bytes())
+    return h.hexdigest()
+
+
+def _core_deps() -> list[Path]:
+    return _sources() + list(CSRC.glob("*.h")) + list(HOST.glob("*.h")) + list(INCLUDE.glob("*.h"))
+
+
+def _oracle_deps() -> list[Path]:
+    return list(ORACLE_DIR.glob("*.cpp")) + list(INCLUDE.glob("*.h"))
+
+
+def _stale(lib: Path, deps) -> bool:
+    """A library is stale when it is missing or when the digest of the
+    sources it was built from (stamp file beside it) differs from the tree's:
+    content, not file times, so a copied tree with new mtimes is not stale."""
+    stamp = lib.with_suffix(".stamp")
+    if not lib.exists() or not stamp.exists():
+        return True
+    return stamp.read_text().strip() != _digest(deps)
+
+
+def core_is_stale() -> bool:
+    return _stale(LIB_PATH, _core_deps())
+
+
+def oracle_is_stale() -> bool:
+    return _stale(ORACLE_LIB, _oracle_deps())
+
+
+def build_core(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile the HIP core + host setup into one shared library: one object
+    per source, compiled in parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
-    srcs = [str(CSRC / s) for s in HIP_SOURCES if (CSRC / s).exists()]
-    srcs += [str(HOST / s) for s in HOST_SOURCES]
-    tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [_hipcc(), *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", f"-I{HOST}", *srcs, "-o", str(tmp)]
+    srcs = _sources()
+    flags = [f for f in HIP_FLAGS if f != "-shared"]
+    inc = [f"-I{INCLUDE}", f"-I{CSRC}", f"-I{HOST}"]
+    objs = [BUILD_DIR / (s.name + ".o") for s in srcs]
+    cmds = [[_hipcc(), *flags, *inc, "-c", str(s), "-o", str(o)] for s, o in zip(srcs, objs)]
     if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
+        for c in cmds:
+            print(" ".join(c))
+    n = jobs or min(len(cmds), max(1, min(os.cpu_count() or 1, 8)))
+    with ThreadPoolExecutor(max_workers=n) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    link = [_hipcc(), *HIP_FLAGS, *map(str, objs), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(link))
+    digest = _digest(_core_deps())
+    _run(link)
     os.replace(tmp, LIB_PATH)
+    LIB_PATH.with_suffix(".stamp").write_text(digest + "\n")
     return LIB_PATH
 
 
@@ -72,8 +124,10 @@ def build_oracle(verbose: bool = False) -> Path:
            "-ffp-contract=off", f"-I{INCLUDE}", str(ORACLE_DIR / "cgfd_oracle.cpp"), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd))
+    digest = _digest(_oracle_deps())
     _run(cmd)
     os.replace(tmp, out)
+    out.with_suffix(".stamp").write_text(digest + "\n")
     return out
 
 

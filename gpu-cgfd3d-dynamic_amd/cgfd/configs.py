@@ -315,3 +315,38 @@ def config5(nscen: int = 8, steps_scale: float = 1.0, seed: int = 5000, imethod:
     hills up to 500 m and PML 20."""
     return config3(nscen=nscen, steps_scale=steps_scale, seed=seed, imethod=imethod, dims=(200, 320, 120), nhills=4,
                    hill_max=500.0, name="config5")
+
+
+def stress_drop(mode: str = "strike", imethod: int = 2, ni: int = 32, nj: int = 100, nk: int = 60, dh: float = 100.0,
+                pml: int = 5, sigma_n: float = 100.0 * MPA, mu_d: float = 0.5, dtau: float = 10.0 * MPA,
+                dt: float = 0.008, steps: int = 80) -> Scenario:
+    """Uniform stress drop on a fault covering the plane: a homogeneous
+    Cartesian medium, mu_s = mu_d, and an initial shear traction dtau above
+    the dynamic strength everywhere, so every unwelded node slips from the
+    first step. Until waves from the welded border arrive, the solution is
+    the 1-D one: slip rate 2 dtau / (rho Vs), side velocities +-dtau / (rho
+    Vs), shear traction mu_d sigma_n, slip = rate * t.
+
+    mode "strike": shear along y (strike) with a free surface on top (that
+    solution has sigma_yz = sigma_zz = 0, so it is traction-free at the
+    surface); mode "dip": shear along z with the fault buried (PML on all six
+    faces)."""
+    if mode not in ("strike", "dip"):
+        raise ValueError("mode must be 'strike' or 'dip'")
+    free_top = mode == "strike"
+    i0 = ni // 2
+    x, y, z = cartesian_coords(ni, nj, nk, dh, i0)
+    lam, mu, rho = homogeneous_media((nk, nj, ni))
+    f = _fault_fields(nj, nk)
+    f["mu_s"][:] = mu_d
+    f["mu_d"][:] = mu_d
+    f["Dc"][:] = 0.4
+    f["C0"][:] = 0.0
+    f["Tx"][:] = -sigma_n
+    f[("Ty" if mode == "strike" else "Tz")][:] = mu_d * sigma_n + dtau
+    layers = [[pml, pml], [pml, pml], [pml, 0 if free_top else pml]]
+    return Scenario(ni=ni, nj=nj, nk=nk, x=x, y=y, z=z, lam=lam, mu=mu, rho=rho, free_top=free_top,
+                    pml_layers=layers, imethod=imethod,
+                    faults=[FaultSpec(x_index=i0, grid=(0, nj - 1, 0, nk - 1), fields=f)],
+                    united_margin=pml + 3, dt=dt, nt_total=steps + 1, stations=None,
+                    name=f"stress_drop_{mode}")

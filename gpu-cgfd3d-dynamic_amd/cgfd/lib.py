@@ -95,6 +95,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_write_wavefield": (I32, [VP, VP]),
         "cgfd_core_read_fault": (I32, [VP, I32, VP]),
         "cgfd_core_read_fault_state": (I32, [VP, I32, VP]),
+        "cgfd_core_write_fault_state": (I32, [VP, I32, VP, VP]),
         "cgfd_core_read_pg": (I32, [VP, VP]),
         "cgfd_core_set_stations": (I32, [VP, I32, VP, I32]),
         "cgfd_core_read_stations": (I32, [VP, VP, C.POINTER(I32)]),
@@ -128,6 +129,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_save_state": (I32, [VP, VP, I32]),
         "cgfd_core_load_state": (I32, [VP, VP, I32]),
         "cgfd_core_autotune": (I32, [VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(F32)]),
+        "cgfd_core_get_variant": (I32, [VP, C.POINTER(I32), C.POINTER(I32)]),
         "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),
         "cgfd_batch_create": (I32, [C.POINTER(Desc), I32, C.POINTER(VP)]),
         "cgfd_batch_active": (I32, [VP]),
@@ -136,6 +138,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_batch_write_wavefield": (I32, [VP, I32, VP]),
         "cgfd_batch_read_fault": (I32, [VP, I32, I32, VP]),
         "cgfd_batch_read_fault_state": (I32, [VP, I32, I32, VP]),
+        "cgfd_batch_write_fault_state": (I32, [VP, I32, I32, VP, VP]),
         "cgfd_batch_read_pg": (I32, [VP, I32, VP]),
         "cgfd_batch_read_stations": (I32, [VP, I32, VP, C.POINTER(I32)]),
         "cgfd_setup_create": (I32, [C.POINTER(SetupParams), C.POINTER(VP)]),

@@ -124,6 +124,18 @@ class Setup:
             pass
 
 
+def _autotune(obj, reps):
+    v, z, ms = C.c_int(), C.c_int(), C.c_float()
+    L.check(obj.lib, obj.lib.cgfd_core_autotune(obj.handle, reps, C.byref(v), C.byref(z), C.byref(ms)))
+    return v.value, z.value, ms.value
+
+
+def _variant_info(obj):
+    v, z = C.c_int(), C.c_int()
+    L.check(obj.lib, obj.lib.cgfd_core_get_variant(obj.handle, C.byref(v), C.byref(z)))
+    return v.value, z.value
+
+
 class Core:
     """The GPU time-stepping core (one HIP stream, one device)."""
 
@@ -252,10 +264,12 @@ class Core:
         L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
 
     def autotune(self, reps: int = 3):
-        """Pick the fastest volume-kernel variant and z chunk for this grid."""
-        v, z, ms = C.c_int(), C.c_int(), C.c_float()
-        L.check(self.lib, self.lib.cgfd_core_autotune(self.handle, reps, C.byref(v), C.byref(z), C.byref(ms)))
-        return v.value, z.value, ms.value
+        """Pick the fastest volume-kernel variant and z chunk for this grid
+        (recorded for every later core of the same shape)."""
+        return _autotune(self, reps)
+
+    def variant_info(self):
+        return _variant_info(self)
 
     def run(self, nsteps: int):
         L.check(self.lib, self.lib.cgfd_core_run(self.handle, nsteps))
@@ -293,6 +307,20 @@ class Core:
         out = np.empty((9, 2, nz, ny), np.float32)
         L.check(self.lib, self.lib.cgfd_core_read_fault_state(self.handle, fid, L.ptr(out)))
         return out
+
+    def write_fault_state(self, state: np.ndarray, fid: int = 0, t1rec: np.ndarray | None = None):
+        """Overwrites the split-node state [9][2][nz][ny]; t1rec [6][nz][ny]
+        (hT1 x,y,z, mT1 x,y,z) optionally sets method 2's T1 recurrence."""
+        nz, ny, _ = self.shape_ext
+        st = np.ascontiguousarray(state, np.float32)
+        if st.shape != (9, 2, nz, ny):
+            raise ValueError(f"state must be (9, 2, {nz}, {ny})")
+        t1 = None
+        if t1rec is not None:
+            t1 = np.ascontiguousarray(t1rec, np.float32)
+            if t1.shape != (6, nz, ny):
+                raise ValueError(f"t1rec must be (6, {nz}, {ny})")
+        L.check(self.lib, self.lib.cgfd_core_write_fault_state(self.handle, fid, L.ptr(st), L.ptr(t1)))
 
     def read_pg(self) -> np.ndarray:
         nz, ny, nx = self.shape_ext
@@ -376,6 +404,15 @@ class Batch:
 
     def set_variant(self, v: int):
         L.check(self.lib, self.lib.cgfd_core_set_variant(self.handle, v))
+
+    def set_zchunk(self, z: int):
+        L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
+
+    def autotune(self, reps: int = 3):
+        return _autotune(self, reps)
+
+    def variant_info(self):
+        return _variant_info(self)
 
     @property
     def active(self) -> int:

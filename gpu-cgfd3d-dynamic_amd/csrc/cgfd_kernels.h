@@ -84,28 +84,10 @@ DEVI FaultDev fdev(const FaultArgs &a, int id) {
   return F;
 }
 
-/* tile lists of the z-queue volume kernel (interior tiles / tiles touching a
-   CFS-PML slab); tile id = bx + ntx*(by + nty*bz) */
-struct ZqTiles {
-  static constexpr int TX = 64, TY = 4;
-  static constexpr int G = 8;  /* y-adjacent tiles per group of the block schedule */
-  int ntx = 0, nty = 0, ntz = 0;
-  int n_int = 0, n_pml = 0;
-  int *d_int = nullptr, *d_pml = nullptr;
-  /* the same tiles in schedule order: x tile, group of G y tiles, z chunk,
-     y tile within the group (slowest to fastest) */
-  int *d_sint = nullptr, *d_spml = nullptr;
-};
-
-void launch_vol_zq(const VolArgs &a, const ZqTiles &t, int minw, int idir, int jdir, int kdir, hipStream_t st);
-void launch_vol_pipe(const VolArgs &a, const ZqTiles &t, int idir, int jdir, int kdir, hipStream_t st);
-void launch_vol_sched(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int minw, int idir, int jdir, int kdir,
-                      hipStream_t st);
 /* variants with separate interior and CFS-PML block kernels launch the PML
    blocks on st_pml (nullptr = st); the two write disjoint points */
 void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,
                    hipStream_t st, hipStream_t st_pml = nullptr);
-void launch_vol_lds(const VolArgs &a, int idir, int jdir, int kdir, hipStream_t st);
 void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
                     hipStream_t st);
 void launch_pg(const Grid &g, const BatArgs &b, const float *wend, const float *wpre, float *PG, float *Dis, float dt,

@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -65,8 +66,6 @@ struct cgfd_core {
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
   int variant = 20;
   int zchunk = 4;
-  ZqTiles zq;                       /* tile lists for the current zchunk */
-  std::map<int, ZqTiles> zq_cache;  /* by zchunk */
   int nvol = 0;
   DevBuf mem;
 
@@ -280,71 +279,6 @@ static void soa9(std::vector<float> &dst, size_t off, const float *src, size_t n
     for (int e = 0; e < 9; e++) dst[off + e * nodes + p] = src[p * 9 + e];
 }
 
-/* tile lists of the z-queue volume kernel over physical planes [0, nvol) */
-static int build_zq_tiles(cgfd_core *c, int nvol) {
-  auto it = c->zq_cache.find(c->zchunk);
-  if (it != c->zq_cache.end()) {
-    c->zq = it->second;
-    return CGFD_OK;
-  }
-  ZqTiles t;
-  const Grid &g = c->g;
-  const int TX = ZqTiles::TX, TY = ZqTiles::TY, zc = c->zchunk;
-  t.ntx = (g.ni + 3 + TX - 1) / TX;  /* tiles from ghost-inclusive column 0 (aligned kernels) */
-  t.nty = (g.nj + TY - 1) / TY;
-  t.ntz = nvol > 0 ? (nvol + zc - 1) / zc : 0;
-  std::vector<int> ti, tp;
-  for (int bz = 0; bz < t.ntz; bz++)
-    for (int by = 0; by < t.nty; by++)
-      for (int bx = 0; bx < t.ntx; bx++) {
-        /* columns of tile bx for the aligned kernels ([64 bx, 64 bx + 63]) and for
-           the ones with origin at the first physical column ([64 bx + 3, 64 bx + 66]) */
-        int i0 = std::max(bx * TX, 3), i1 = std::min(bx * TX + TX + 2, g.ni + 2);
-        int j0 = by * TY + 3, j1 = std::min(by * TY + TY, g.nj) - 1 + 3;
-        int k0 = bz * zc + 3, k1 = std::min(bz * zc + zc, nvol) - 1 + 3;
-        bool pml = false;
-        for (int dim = 0; dim < 3; dim++)
-          for (int side = 0; side < 2; side++) {
-            const PmlFace &f = c->pml[dim][side];
-            if (!f.on) continue;
-            if (i1 >= f.i1 && i0 <= f.i2 && j1 >= f.j1 && j0 <= f.j2 && k1 >= f.k1 && k0 <= f.k2) pml = true;
-          }
-        int id = bx + t.ntx * (by + t.nty * bz);
-        (pml ? tp : ti).push_back(id);
-      }
-  t.n_int = (int)ti.size();
-  t.n_pml = (int)tp.size();
-  /* schedule order of the same two sets */
-  std::vector<char> is_pml((size_t)t.ntx * t.nty * std::max(t.ntz, 1), 0);
-  for (int id : tp) is_pml[id] = 1;
-  std::vector<int> si, sp;
-  const int G = ZqTiles::G, ngy = (t.nty + G - 1) / G;
-  for (int bx = 0; bx < t.ntx; bx++)
-    for (int gy = 0; gy < ngy; gy++)
-      for (int bz = 0; bz < t.ntz; bz++)
-        for (int cy = 0; cy < G; cy++) {
-          int by = gy * G + cy;
-          if (by >= t.nty) continue;
-          int id = bx + t.ntx * (by + t.nty * bz);
-          (is_pml[id] ? sp : si).push_back(id);
-        }
-  int rc;
-  if (t.n_int) {
-    if ((rc = c->alloc(&t.d_int, ti.size()))) return rc;
-    HIPCHK(hipMemcpy(t.d_int, ti.data(), ti.size() * 4, hipMemcpyHostToDevice));
-    if ((rc = c->alloc(&t.d_sint, si.size()))) return rc;
-    HIPCHK(hipMemcpy(t.d_sint, si.data(), si.size() * 4, hipMemcpyHostToDevice));
-  }
-  if (t.n_pml) {
-    if ((rc = c->alloc(&t.d_pml, tp.size()))) return rc;
-    HIPCHK(hipMemcpy(t.d_pml, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
-    if ((rc = c->alloc(&t.d_spml, sp.size()))) return rc;
-    HIPCHK(hipMemcpy(t.d_spml, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
-  }
-  c->zq_cache[c->zchunk] = t;
-  c->zq = t;
-  return CGFD_OK;
-}
 
 /* fault arrays: the per-scenario block (T0 3, fric 4, out 11, T1 21, hT1 3,
    mT1 3 floats and flag_rup, init_t0_flag ints per node) for every scenario,
@@ -470,6 +404,8 @@ static int check_batch(const cgfd_desc *ds, int nscen) {
   return CGFD_OK;
 }
 
+static void apply_auto_variant(cgfd_core *c);
+
 static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   if (!out) return cgfd_set_error(CGFD_ERR_ARG, "null output pointer");
   *out = nullptr;
@@ -593,7 +529,6 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   {
     const int ntop = c->isfree ? (g.nk < 4 ? g.nk : 4) : 0;
     c->nvol = g.nk - ntop;
-    if ((rc = build_zq_tiles(c, c->nvol))) return fail(rc);
   }
   if ((rc = c->alloc(&c->it_dev, 2))) return fail(rc);
   if (hipMemset(c->it_dev, 0, 8) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
@@ -608,6 +543,7 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   }
   if (hipStreamSynchronize(c->st) != hipSuccess || hipGetLastError() != hipSuccess)
     return fail(cgfd_set_error(CGFD_ERR_HIP, "setup kernels failed"));
+  apply_auto_variant(c);  /* the tuned schedule of this shape, if any */
   *out = c;
   return CGFD_OK;
 }
@@ -682,31 +618,15 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.tape_sstride = (long)c->nfault * c->tape_nt * kTapeFloats * c->g.sslice;
 }
 
-/* volume kernel variants: 1 direct loads, 2 LDS z-march, 3..5 z-queue with
-   no VGPR cap / >= 3 waves / >= 4 waves per SIMD, 6 pipelined LDS z-march,
-   7..9 direct loads on an L2-aware block schedule with the interior tiles
-   at the compiler's occupancy / >= 4 / >= 6 waves per SIMD, 10 direct loads
-   with row-major thread order, 11 direct loads in slab order (zchunk y tiles
-   per slab), 12..14 split interior/PML direct kernels (interior with all
-   loads free to move / a fence every 3 fields / >= 8 waves per SIMD and a
-   fence every field; a fence makes a field's taps be consumed before the
-   next field's loads), 15 = 14 with fences in the PML blocks too, 16 = 14
-   with the interior blocks in slab order (zchunk y tiles per slab), 17 = 16
-   with XCD-contiguous block ids for both the interior and the PML blocks,
-   18 = 16 with the interior x taps taken from neighbouring lanes (DPP),
-   19 = 16 with the lean PML-tile kernel (directional rhs parts accumulated
-   per field); 20 and 21 run 19's kernels on one scenario and differ on
-   batches (20: the PML-tile blocks in per-scenario slab sweeps, the interior
-   blocks with the scenarios of a tile back to back; 21: both in
-   per-scenario sweeps) */
+/* volume kernel variants (see cgfd_core_set_variant): 1 direct loads,
+   16 split interior/PML kernels with the interior in slab order, 18 = 16
+   with DPP x taps, 19 = 16 with the lean PML-tile kernel, 20 and 21 = 19 on
+   one scenario and two batch orders */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
-  if (c->bat.nact > 1) {  /* batch kernels: split interior/PML for 16 and 17, else one direct kernel */
-    const int bv = (c->variant == 16 || c->variant == 17 || c->variant >= 19) ? c->variant : 1;
+  if (c->bat.nact > 1) {  /* batch kernels: split interior/PML for 16 and 19..21, else one direct kernel */
+    const int bv = (c->variant == 16 || c->variant >= 19) ? c->variant : 1;
     launch_volume(bv, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
   }
-  else if (c->variant == 6) launch_vol_pipe(va, c->zq, idir, jdir, kdir, c->st);
-  else if (c->variant >= 7 && c->variant <= 9) launch_vol_sched(va, c->bat, c->zq, c->variant - 7, idir, jdir, kdir, c->st);
-  else if (c->variant >= 3 && c->variant <= 5) launch_vol_zq(va, c->zq, c->variant - 1 > 2 ? c->variant - 1 : 0, idir, jdir, kdir, c->st);
   else launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
 }
 
@@ -1323,6 +1243,25 @@ extern "C" int cgfd_core_read_fault_state(cgfd_core *c, int id, float *dst) {
   return cgfd_batch_read_fault_state(c, 0, id, dst);
 }
 
+extern "C" int cgfd_batch_write_fault_state(cgfd_core *c, int s, int id, const float *src, const float *t1rec) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!src) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+  if (id < 0 || id >= c->nfault) return cgfd_set_error(CGFD_ERR_ARG, "fault id %d out of range", id);
+  const size_t ss = c->g.sslice;
+  HIPCHK(hipSetDevice(c->dev));
+  HIPCHK(hipStreamSynchronize(c->st));
+  float *dst = c->flev(scen_flev(c, s)) + s * c->bat.fwstride + (long)id * 18 * ss;
+  HIPCHK(hipMemcpy(dst, src, 18 * ss * 4, hipMemcpyHostToDevice));
+  if (t1rec) {
+    HIPCHK(hipMemcpy(c->flt[id].hT1 + s * c->bat.fpf, t1rec, 3 * ss * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->flt[id].mT1 + s * c->bat.fpf, t1rec + 3 * ss, 3 * ss * 4, hipMemcpyHostToDevice));
+  }
+  return CGFD_OK;
+}
+extern "C" int cgfd_core_write_fault_state(cgfd_core *c, int id, const float *src, const float *t1rec) {
+  return cgfd_batch_write_fault_state(c, 0, id, src, t1rec);
+}
+
 extern "C" int cgfd_batch_read_pg(cgfd_core *c, int s, float *dst) {
   if (int rc = check_scen(c, s)) return rc;
   if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
@@ -1439,29 +1378,100 @@ extern "C" int cgfd_batch_active(cgfd_core *c) {
   return n;
 }
 
+/* ---- autotuned schedules, per problem shape ---- */
+namespace {
+struct TuneKey {
+  int ni, nj, nk, nscen, nfault, isfree, pml;
+  bool operator<(const TuneKey &o) const {
+    const int a[7] = {ni, nj, nk, nscen, nfault, isfree, pml}, b[7] = {o.ni, o.nj, o.nk, o.nscen, o.nfault, o.isfree, o.pml};
+    return std::lexicographical_compare(a, a + 7, b, b + 7);
+  }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;  /* shape -> (variant, zchunk) */
+bool g_tune_file_read = false;
+std::mutex g_tune_mu;
+
+TuneKey tune_key(const cgfd_core *c) {
+  int pml = 0;
+  for (int d = 0; d < 3; d++)
+    for (int e = 0; e < 2; e++) pml |= (c->pml[d][e].on ? 1 : 0) << (2 * d + e);
+  return TuneKey{c->g.ni, c->g.nj, c->g.nk, c->nscen, c->nfault, c->isfree, pml};
+}
+
+bool valid_variant(int v) { return v == 1 || v == 16 || (v >= 18 && v <= 21); }
+
+/* the cache file: one "ni nj nk nscen nfault isfree pml variant zchunk" line
+   per tuned shape; later lines win */
+void read_tune_file() {
+  if (g_tune_file_read) return;
+  g_tune_file_read = true;
+  const char *path = std::getenv("CGFD_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return;
+  TuneKey k;
+  int v, z;
+  while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d", &k.ni, &k.nj, &k.nk, &k.nscen, &k.nfault, &k.isfree, &k.pml,
+                     &v, &z) == 9)
+    if (valid_variant(v) && z >= 4 && z <= 4096) g_tuned[k] = {v, z};
+  std::fclose(f);
+}
+
+void record_tuned(const TuneKey &k, int v, int z) {
+  g_tuned[k] = {v, z};
+  const char *path = std::getenv("CGFD_TUNE_CACHE");
+  if (!path || !*path) return;
+  FILE *f = std::fopen(path, "a");
+  if (!f) return;
+  std::fprintf(f, "%d %d %d %d %d %d %d %d %d\n", k.ni, k.nj, k.nk, k.nscen, k.nfault, k.isfree, k.pml, v, z);
+  std::fclose(f);
+}
+}  // namespace
+
+/* variant 0: the tuned choice for this shape, else 20 (zchunk 4) */
+static void apply_auto_variant(cgfd_core *c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  read_tune_file();
+  auto it = g_tuned.find(tune_key(c));
+  if (it != g_tuned.end()) {
+    c->variant = it->second.first;
+    c->zchunk = it->second.second;
+  } else {
+    c->variant = 20;
+    c->zchunk = 4;
+  }
+}
+
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
-  if (variant == 0) variant = 20;
-  if (variant < 1 || variant > 21) return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d", variant);
-  c->variant = variant;
+  if (variant != 0 && !valid_variant(variant))
+    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 16, 18, 19, 20, 21)", variant);
+  if (variant == 0) apply_auto_variant(c);
+  else c->variant = variant;
   c->drop_graph();
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_get_variant(const cgfd_core *c, int *variant, int *zchunk) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (variant) *variant = c->variant;
+  if (zchunk) *zchunk = c->zchunk;
   return CGFD_OK;
 }
 
 extern "C" int cgfd_core_set_zchunk(cgfd_core *c, int zchunk) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (zchunk < 4 || zchunk > 4096) return cgfd_set_error(CGFD_ERR_ARG, "zchunk %d out of range [4, 4096]", zchunk);
-  HIPCHK(hipSetDevice(c->dev));
   c->zchunk = zchunk;
   c->drop_graph();
-  return build_zq_tiles(c, c->nvol);
+  return CGFD_OK;
 }
 
 extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int *zchunk_out, float *ms_out) {
   if (!c || reps < 1) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
   HIPCHK(hipSetDevice(c->dev));
   c->drop_graph();
-  if (c->nvol <= 0 || c->nscen > 1) {  /* batches have one volume kernel */
+  if (c->nvol <= 0 || c->refresh_active() == 0) {
     if (variant_out) *variant_out = c->variant;
     if (zchunk_out) *zchunk_out = c->zchunk;
     if (ms_out) *ms_out = 0.0f;
@@ -1472,18 +1482,17 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const int variants[19] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19};
-  const int chunks[6] = {4, 8, 16, 32, 64, 128};
+  const bool batch = c->bat.nact > 1;
+  const std::vector<int> variants = batch ? std::vector<int>{20, 19, 21, 16, 1} : std::vector<int>{20, 16, 18, 1};
+  const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
-  for (int vi = 0; vi < 19; vi++)
-    for (int zi = 0; zi < 6; zi++) {
-      int v = variants[vi], z = chunks[zi];
-      if ((v == 1 || v == 10 || (v >= 12 && v <= 15)) && zi > 0) continue;  /* no z chunks */
+  for (int v : variants)
+    for (int zi = 0; zi < 4; zi++) {
+      const int z = chunks[zi];
+      if (v == 1 && zi > 0) continue;  /* no slabs */
       c->variant = v;
       c->zchunk = z;
-      int rc = build_zq_tiles(c, c->nvol);
-      if (rc) return rc;
       /* the four stages' volume launches of one step: they read level pre
          and write only levels A, B and end (and the PML aux tmp/end), which
          the next real step rewrites before reading */
@@ -1518,8 +1527,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   (void)hipEventDestroy(e1);
   c->variant = bv;
   c->zchunk = bz;
-  int rc = build_zq_tiles(c, c->nvol);
-  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    read_tune_file();
+    record_tuned(tune_key(c), bv, bz);
+  }
   if (variant_out) *variant_out = bv;
   if (zchunk_out) *zchunk_out = bz;
   if (ms_out) *ms_out = best;

@@ -81,31 +81,6 @@ __global__ void __launch_bounds__(256) k_vol_bat(VolArgs a, BatArgs b, long nite
 }
 
 /* ------------------------------------------------------------------------
- * Slab-ordered direct kernel: block ids run x tile fastest, then the S
- * y-tiles of a slab, then the plane, then the slab (S = zchunk). The blocks
- * resident at one time cover a few planes of one slab of 4 S rows, so the
- * planes above and below a block are read by blocks running at the same
- * time (the z-neighbour block is ntx*S ids away, on the same XCD when that
- * is a multiple of 8), and each array is touched in contiguous runs of 4 S
- * rows per plane.
- * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256) k_vol_slab(VolArgs a, BatArgs b) {
-  const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
-  const int S = a.zchunk;
-  long L = blockIdx.x;
-  const int bx = (int)(L % ntx);
-  L /= ntx;
-  const int cy = (int)(L % S);
-  L /= S;
-  const int kz = (int)(L % a.nk_run);
-  const int by = (int)(L / a.nk_run) * S + cy;
-  const int i = bx * 64 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
-  if (by >= nty || i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD>(a, b, i, iy + 3, a.k_lo + kz + 3, b.act[0]);
-}
-
-/* ------------------------------------------------------------------------
  * Split direct kernels: blocks whose tile-plane touches no CFS-PML slab run
  * an interior kernel that turns each field's three derivatives into its
  * physical gradient at once and accumulates the rhs (the sums of
@@ -180,27 +155,15 @@ DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int
   rk_point32(a, o, h, woff);
 }
 
-template <int ID, int JD, int KD, int MINW, int G>
-__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1) k_vol_acc(VolArgs a, BatArgs b) {
-  const int k = a.k_lo + (int)blockIdx.z + 3;
-  const int bi0 = blockIdx.x * 64, bj0 = blockIdx.y * 4 + 3;
-  if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
-  const int i = bi0 + (int)threadIdx.x, iy = blockIdx.y * 4 + (int)threadIdx.y;
-  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
-  acc_point<ID, JD, KD, G>(a, b, i, iy + 3, k, b.act[0]);
-}
-
-/* block id -> (x tile, y tile, plane) in slab order (see k_vol_slab). XCD:
-   the grid is a multiple of 8 blocks and block ids L with L % 8 == x (the
-   ones the dispatcher sends to XCD x) take the x-th contiguous eighth of the
-   items, so one XCD's L2 sees whole rows of a slab and the planes above and
-   below: the x, y and z halos of a tile are fetched into that L2 by the
-   tile's own neighbours. false: padding item past the last slab. */
-DEVI bool slab_item(const VolArgs &a, bool xcd, int &bx, int &by, int &kz) {
+/* block id -> (x tile, y tile, plane) in slab order: x tile fastest, then
+   the S = zchunk y tiles of a slab, then the plane, then the slab, so the
+   blocks resident at one time cover a few planes of one slab of 4 S rows and
+   read each other's z and y halos from L2. false: padding item past the last
+   slab. */
+DEVI bool slab_item(const VolArgs &a, int &bx, int &by, int &kz) {
   const int ntx = (a.g.ni + 3 + 63) / 64, nty = (a.g.nj + 3) / 4;
   const int S = a.zchunk;
   long L = blockIdx.x;
-  if (xcd) L = (L % 8) * (long)(gridDim.x / 8) + L / 8;
   bx = (int)(L % ntx);
   L /= ntx;
   const int cy = (int)(L % S);
@@ -210,11 +173,12 @@ DEVI bool slab_item(const VolArgs &a, bool xcd, int &bx, int &by, int &kz) {
   return by < nty;
 }
 
-/* the interior kernel of variant 14 in slab order */
-template <int ID, int JD, int KD, bool XCD>
+/* the interior kernel in slab order: 8 waves per SIMD, a register fence
+   after every field */
+template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256, 8) k_vol_acc_slab(VolArgs a, BatArgs b) {
   int bx, by, kz;
-  if (!slab_item(a, XCD, bx, by, kz)) return;
+  if (!slab_item(a, bx, by, kz)) return;
   const int k = a.k_lo + kz + 3;
   const int bi0 = bx * 64, bj0 = by * 4 + 3;
   if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
@@ -323,26 +287,13 @@ DEVI void acc_dpp_point(const VolArgs &a, const BatArgs &b, int x0, int j, int k
 template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256, 8) k_vol_acc_dpp(VolArgs a, BatArgs b) {
   int bx, by, kz;
-  if (!slab_item(a, false, bx, by, kz)) return;
+  if (!slab_item(a, bx, by, kz)) return;
   const int k = a.k_lo + kz + 3;
   const int bi0 = bx * 64, bj0 = by * 4 + 3;
   if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
   const int iy = by * 4 + (int)threadIdx.y;
   if (iy >= a.g.nj) return;
   acc_dpp_point<ID, JD, KD>(a, b, bi0, iy + 3, k, b.act[0]);
-}
-
-/* the CFS-PML blocks in the same order */
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256) k_vol_pml_slab(VolArgs a, BatArgs b) {
-  int bx, by, kz;
-  if (!slab_item(a, true, bx, by, kz)) return;
-  const int k = a.k_lo + kz + 3;
-  const int bi0 = bx * 64, bj0 = by * 4 + 3;
-  if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
-  const int i = bi0 + (int)threadIdx.x, iy = by * 4 + (int)threadIdx.y;
-  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[0]);
 }
 
 /* ------------------------------------------------------------------------
@@ -529,30 +480,14 @@ __global__ void __launch_bounds__(256, LEAN ? 6 : 1) k_vol_bat_pml(VolArgs a, Ba
   else direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[q]);
 }
 
-template <int ID, int JD, int KD, bool FENCE>
+template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256) k_vol_pmlblk(VolArgs a, BatArgs b) {
   const int k = a.k_lo + (int)blockIdx.z + 3;
   const int bi0 = blockIdx.x * 64, bj0 = blockIdx.y * 4 + 3;
   if (!block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
   const int i = bi0 + (int)threadIdx.x, iy = blockIdx.y * 4 + (int)threadIdx.y;
   if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, true, FENCE>(a, b, i, iy + 3, k, b.act[0]);
-}
-
-/* ------------------------------------------------------------------------
- * Row-major direct kernel: consecutive threads take consecutive physical
- * points of a plane (i fastest, wrapping to the next j), so every array is
- * read and written in runs of whole rows instead of 256-byte tile rows,
- * which keeps HBM pages open. grid = (ceil(ni*nj / 256), nk_run).
- * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256) k_vol_rows(VolArgs a, BatArgs b) {
-  const unsigned t = blockIdx.x * 256u + threadIdx.x;
-  const unsigned ni = (unsigned)a.g.ni;
-  if (t >= ni * (unsigned)a.g.nj) return;
-  const int iy = (int)(t / ni);
-  const int ix = (int)(t - (unsigned)iy * ni);
-  direct_point<ID, JD, KD>(a, b, ix + 3, iy + 3, a.k_lo + (int)blockIdx.y + 3, b.act[0]);
+  direct_point<ID, JD, KD, true, false>(a, b, i, iy + 3, k, b.act[0]);
 }
 
 /* ------------------------------------------------------------------------
@@ -568,80 +503,6 @@ __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a, BatArgs b) {
   if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
   const int j = iy + 3, k = a.k_lo + (int)blockIdx.z + 3;
   direct_point<ID, JD, KD>(a, b, i, j, k, b.act[0]);
-}
-
-/* ------------------------------------------------------------------------
- * The same per-point kernel on an L2-aware block schedule. A work item is one
- * 64 x 4 tile of one plane; items run in the order of a host-built tile list
- * (x tile, group of 8 y-adjacent tiles, z chunk, y tile in the group) with
- * the planes of a chunk fastest. Each XCD (block ids L with L % 8 == x) takes
- * a contiguous range of items, so the blocks resident on one XCD cover a z
- * chunk of 8 y-adjacent tiles: the z and y neighbours a block reads were
- * fetched into that XCD's L2 by its neighbours moments earlier. Interior
- * tiles run without the CFS-PML code (fewer registers); MINW > 0 asks the
- * compiler for at least MINW waves per SIMD.
- * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD, bool PML, int MINW>
-__global__ void __launch_bounds__(256, MINW > 0 ? MINW : 1)
-    k_vol_sched(VolArgs a, BatArgs b, const int *__restrict__ tiles, long nitems, int ntx, int nty) {
-  const long L = blockIdx.x;
-  const long per = (long)(gridDim.x / 8);
-  const long item = (L % 8) * per + L / 8;
-  if (item >= nitems) return;
-  const int Kc = a.zchunk;
-  const int tile = tiles[item / Kc];
-  const int kk = (int)(item % Kc);
-  const int bx = tile % ntx;
-  const int by = (tile / ntx) % nty;
-  const int bz = tile / (ntx * nty);
-  const int kz = bz * Kc + kk;
-  if (kz >= a.nk_run) return;
-  const int i = bx * 64 + (int)threadIdx.x;  /* ghost-inclusive, aligned tiles */
-  const int iy = by * 4 + (int)threadIdx.y;
-  if (i < 3 || i >= a.g.ni + 3 || iy >= a.g.nj) return;
-  direct_point<ID, JD, KD, PML>(a, b, i, iy + 3, a.k_lo + kz + 3, b.act[0]);
-}
-
-template <int ID, int JD, int KD, int MINW>
-static void launch_sched_t(const VolArgs &a, const BatArgs &b, const ZqTiles &t, hipStream_t st) {
-  dim3 blk(64, 4, 1);
-  if (t.n_int > 0) {
-    const long items = (long)t.n_int * a.zchunk;
-    const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, false, MINW>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_sint,
-                       items,
-                       t.ntx, t.nty);
-  }
-  if (t.n_pml > 0) {
-    const long items = (long)t.n_pml * a.zchunk;
-    const long nb = ((items + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_sched<ID, JD, KD, true, 0>), dim3((unsigned)nb), blk, 0, st, a, b, t.d_spml,
-                       items,
-                       t.ntx, t.nty);
-  }
-}
-
-template <int MINW>
-static void launch_sched_w(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int idir, int jdir, int kdir,
-                           hipStream_t st) {
-  switch (idir * 4 + jdir * 2 + kdir) {
-    case 0: launch_sched_t<0, 0, 0, MINW>(a, b, t, st); break;
-    case 1: launch_sched_t<0, 0, 1, MINW>(a, b, t, st); break;
-    case 2: launch_sched_t<0, 1, 0, MINW>(a, b, t, st); break;
-    case 3: launch_sched_t<0, 1, 1, MINW>(a, b, t, st); break;
-    case 4: launch_sched_t<1, 0, 0, MINW>(a, b, t, st); break;
-    case 5: launch_sched_t<1, 0, 1, MINW>(a, b, t, st); break;
-    case 6: launch_sched_t<1, 1, 0, MINW>(a, b, t, st); break;
-    default: launch_sched_t<1, 1, 1, MINW>(a, b, t, st); break;
-  }
-}
-
-/* w: 0 = the compiler's choice, 1 = >= 4 waves/SIMD, 2 = >= 6 waves/SIMD */
-void launch_vol_sched(const VolArgs &a, const BatArgs &b, const ZqTiles &t, int w, int idir, int jdir, int kdir,
-                      hipStream_t st) {
-  if (w == 1) launch_sched_w<4>(a, b, t, idir, jdir, kdir, st);
-  else if (w >= 2) launch_sched_w<6>(a, b, t, idir, jdir, kdir, st);
-  else launch_sched_w<0>(a, b, t, idir, jdir, kdir, st);
 }
 
 /* ------------------------------------------------------------------------
@@ -894,14 +755,10 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_bat<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b, items);
     return;
   }
-  if (variant == 2) {
-    launch_vol_lds(a, ID, JD, KD, st);
-    return;
-  }
   if (variant >= 19) {
     const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
     const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
-    hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, false>), dim3((unsigned)nb), blk, 0, st, a, b);
+    hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
     if (a.pml_on) hipLaunchKernelGGL((k_vol_pml_lean<ID, JD, KD>), agrd, blk, 0, st_pml, a, b);
     return;
   }
@@ -909,39 +766,14 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
     const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
     hipLaunchKernelGGL((k_vol_acc_dpp<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
-    if (a.pml_on) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, false>), agrd, blk, 0, st_pml, a, b);
+    if (a.pml_on) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD>), agrd, blk, 0, st_pml, a, b);
     return;
   }
-  if (variant == 17) {
-    const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
-    const long nb = (((long)agrd.x * a.zchunk * a.nk_run * nslab + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, true>), dim3((unsigned)nb), blk, 0, st, a, b);
-    if (a.pml_on) hipLaunchKernelGGL((k_vol_pml_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st_pml, a, b);
-    return;
-  }
-  if (variant >= 12 && variant <= 16) {
-    if (variant == 16) {
-      const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
-      const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
-      hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD, false>), dim3((unsigned)nb), blk, 0, st, a, b);
-    } else if (variant == 12) hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 0, 0>), agrd, blk, 0, st, a, b);
-    else if (variant == 13) hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 0, 3>), agrd, blk, 0, st, a, b);
-    else hipLaunchKernelGGL((k_vol_acc<ID, JD, KD, 8, 1>), agrd, blk, 0, st, a, b);
-    if (a.pml_on) {
-      if (variant == 15) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, true>), agrd, blk, 0, st_pml, a, b);
-      else hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD, false>), agrd, blk, 0, st_pml, a, b);
-    }
-    return;
-  }
-  if (variant == 11) {
+  if (variant == 16) {
     const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
     const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
-    hipLaunchKernelGGL((k_vol_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
-    return;
-  }
-  if (variant == 10) {
-    dim3 rgrd((unsigned)(((long)a.g.ni * a.g.nj + 255) / 256), a.nk_run, 1);
-    hipLaunchKernelGGL((k_vol_rows<ID, JD, KD>), rgrd, dim3(256), 0, st, a, b);
+    hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
+    if (a.pml_on) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD>), agrd, blk, 0, st_pml, a, b);
     return;
   }
   hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), agrd, blk, 0, st, a, b);

@@ -245,6 +245,14 @@ int cgfd_core_read_fault(cgfd_core *core, int id, float *host_dst);
    (Vx,Vy,Vz,T2x,T2y,T2z,T3x,T3y,T3z; side 0 minus, 1 plus). */
 int cgfd_core_read_fault_state(cgfd_core *core, int id, float *host_dst);
 
+/* Overwrites that split-node state (same layout) for a restart or a
+   prescribed initial condition. t1rec, if not null, is [6][ny*nz]: the T1
+   rate hT1 (x,y,z) and the T1 of the last completed step mT1 (x,y,z) that
+   method 2 uses (fault_wav_t.cu:242-248); null leaves them. The host-side
+   counterpart of the reference's fault_wav_t contents (fault_wav_t.cu:21-78),
+   which its driver only ever zero-initialises. */
+int cgfd_core_write_fault_state(cgfd_core *core, int id, const float *host_src, const float *t1rec);
+
 /* Surface peak values host_dst[15][nx*ny] (wav_t.cu:160-174); zero if the
    top is not free. */
 int cgfd_core_read_pg(cgfd_core *core, float *host_dst);
@@ -292,40 +300,40 @@ int cgfd_core_read_fault_receivers(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
                         float *ms_volume, int *n_volume);
 
-/* Kernel variant selection for the volume stencil (0 = default = 20,
-   1 = direct loads, 2 = LDS-tiled z-march, 3 = z register queue,
-   4 = z queue capped at >= 3 waves/SIMD, 5 = z queue capped at >= 4 waves/SIMD,
-   6 = software-pipelined LDS z-march, 7 = direct loads on an L2-aware
-   block schedule, 8 = the same with >= 4 waves/SIMD on interior tiles,
-   9 = the same with >= 6 waves/SIMD, 10 = direct loads, row-major thread
-   order, 11 = direct loads in slab order with zchunk y-tiles per slab,
-   12..14 = interior blocks on a per-field accumulating kernel (loads free
-   to move / a fence every 3 fields / >= 8 waves per SIMD with a fence every
-   field) and PML blocks on the direct kernel, 15 = 14 with the PML blocks
-   fenced as well, 16 = 14 with the interior blocks in slab order,
-   17 = 16 with XCD-contiguous block ids for the interior and the PML
-   blocks, 18 = 16 with the interior x taps moved between lanes by DPP
-   wave shifts instead of loaded, 19 = 16 with a lean PML-tile kernel that
-   accumulates the directional rhs parts field by field, 20 and 21 = 19 on
-   one scenario). Batches with more than one active scenario run split interior /
-   PML kernels for variant 16 (one slab-ordered sweep per scenario),
-   variants 17 and 19 (the scenarios of a tile back to back, XCD-contiguous,
-   lean PML kernel), variant 20 (interior as 19, lean PML blocks in one
-   slab-ordered sweep per scenario: the fastest), variant 21 (both kernels
-   in per-scenario sweeps), and one direct
-   tile x scenario kernel for every other variant.
-   Returns CGFD_ERR_ARG for an unknown variant. */
+/* Kernel variant selection for the volume stencil. 0 = automatic: the
+   variant and z-chunk cgfd_core_autotune chose for a problem of this shape
+   in this process (or in the cache file named by CGFD_TUNE_CACHE), else 20.
+   1 = one direct-load kernel (27 derivatives per point); 16 = an interior
+   kernel that accumulates each field's gradient into the rhs (64 x 4 tiles
+   in slab order: zchunk y-tiles per slab, then the plane) and a direct
+   kernel for the tiles touching a CFS-PML slab; 18 = 16 with the interior x
+   taps moved between lanes by DPP wave shifts instead of loaded; 19 = 16
+   with a lean PML-tile kernel that accumulates the directional rhs parts
+   field by field; 20 and 21 = 19 on one scenario. Batches with more than
+   one active scenario run split interior / PML kernels for 16 (one
+   slab-ordered sweep per scenario), 19 (the scenarios of a tile back to
+   back, XCD-contiguous, lean PML kernel), 20 (interior as 19, lean PML
+   blocks in one slab-ordered sweep per scenario), 21 (both kernels in
+   per-scenario sweeps), and one direct tile x scenario kernel for 1 and 18.
+   Returns CGFD_ERR_ARG for any other value. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
-/* z planes per block (variants 2..6), per schedule chunk (variants 7..9) or
-   y-tiles per slab (variants 11, 16, 17), 4..4096. */
+/* The schedule the core will run: variant and z-chunk (after variant 0 has
+   been resolved). */
+int cgfd_core_get_variant(const cgfd_core *core, int *variant, int *zchunk);
+
+/* y-tiles per slab of the slab-ordered kernels (16, 18..21), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
-/* Autotuner: times every (variant, z-chunk) candidate on `reps` repetitions
-   of one step's four volume launches and keeps the fastest; *ms_best is the
-   average per launch. The launches read level "pre" and write only levels
-   that the next step rewrites before reading, so the simulation state is
-   unchanged. */
+/* Autotuner: times every candidate (variant, z-chunk) pair -- 1, 16, 18
+   and 19 with z-chunks 4..32 on one scenario; 1, 16, 19, 20 and 21 on a
+   batch -- on `reps` repetitions of one step's four volume launches, keeps
+   the fastest and records it for this problem shape (grid, scenarios,
+   faults, boundaries) so later cores of the same shape start with it, and
+   appends it to the file named by CGFD_TUNE_CACHE if that is set; *ms_best
+   is the average per launch. The launches read level "pre" and write only
+   levels that the next step rewrites before reading, so the simulation
+   state is unchanged. */
 int cgfd_core_autotune(cgfd_core *core, int reps, int *variant, int *zchunk, float *ms_best);
 
 /* ---------------------------------------------------------------------
@@ -347,6 +355,7 @@ int cgfd_batch_read_wavefield(cgfd_core *core, int s, float *host_dst);
 int cgfd_batch_write_wavefield(cgfd_core *core, int s, const float *host_src);
 int cgfd_batch_read_fault(cgfd_core *core, int s, int id, float *host_dst);
 int cgfd_batch_read_fault_state(cgfd_core *core, int s, int id, float *host_dst);
+int cgfd_batch_write_fault_state(cgfd_core *core, int s, int id, const float *host_src, const float *t1rec);
 int cgfd_batch_read_pg(cgfd_core *core, int s, float *host_dst);
 int cgfd_batch_read_stations(cgfd_core *core, int s, float *host_dst, int *nt);
 

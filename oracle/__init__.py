@@ -32,11 +32,13 @@ def load():
             "oracle_write_wave": (i32, [vp, vp]),
             "oracle_read_fault": (i32, [vp, i32, vp]),
             "oracle_read_fault_state": (i32, [vp, i32, vp]),
+            "oracle_write_fault_state": (i32, [vp, i32, vp, vp]),
             "oracle_read_pg": (i32, [vp, vp]),
             "oracle_step_index": (i32, [vp]),
             "oracle_probe_inner": (i32, [vp, vp, i32, i32, i32, vp]),
             "oracle_read_freemats": (i32, [vp, vp]),
             "oracle_probe_friction": (i32, [vp, vp, i32]),
+            "oracle_probe_w2f_f2w": (i32, [vp, C.c_double, vp, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -53,12 +55,25 @@ def _p(a):
 class Oracle:
     """CPU restatement of the core on a boundary descriptor (cgfd.lib.Desc)."""
 
-    def __init__(self, desc, shape_ext, nfault, double=False, fix_q1=False):
+    def __init__(self, desc, shape_ext, nfault, double=False, fix_q1=False, plant=0):
+        """plant != 0 builds the oracle with a deliberate mistake (see
+        oracle_create in cgfd_oracle.cpp); only the tests that show a pin
+        catches that mistake use it."""
         self.lib = load()
-        flags = (1 if double else 0) | (2 if fix_q1 else 0)
+        flags = (1 if double else 0) | (2 if fix_q1 else 0) | ((int(plant) & 0xFF) << 8)
         self.h = self.lib.oracle_create(C.byref(desc), flags)
         self.shape_ext = shape_ext
         self.nfault = nfault
+
+    @classmethod
+    def from_scenario(cls, sc, double=False, fix_q1=False, plant=0):
+        """The oracle on its own FP64 setup of the scenario (setup_ref), not
+        on the product's descriptor."""
+        from .setup_ref import SetupRef
+        ref = SetupRef(sc)
+        o = cls(ref.desc(), sc.shape_ext, len(sc.faults), double=double, fix_q1=fix_q1, plant=plant)
+        o.setup_ref = ref          # keeps the descriptor's arrays alive
+        return o
 
     def run(self, n):
         self.lib.oracle_run(self.h, n)
@@ -92,6 +107,12 @@ class Oracle:
         out = np.empty((9, 2, nz, ny), np.float64)
         assert self.lib.oracle_read_fault_state(self.h, fid, _p(out)) == 0
         return out
+
+    def write_fault_state(self, state, fid=0, t1rec=None):
+        nz, ny, _ = self.shape_ext
+        st = np.ascontiguousarray(state, np.float64).reshape(9, 2, nz, ny)
+        t1 = None if t1rec is None else np.ascontiguousarray(t1rec, np.float64).reshape(6, nz, ny)
+        assert self.lib.oracle_write_fault_state(self.h, fid, _p(st), None if t1 is None else _p(t1)) == 0
 
     def read_pg(self):
         nz, ny, nx = self.shape_ext
@@ -128,3 +149,15 @@ def probe_friction(vals, double=True):
     out = np.zeros(7, np.float64)
     load().oracle_probe_friction(_p(v), _p(out), 1 if double else 0)
     return out
+
+
+def probe_w2f_f2w(metric9, jac, stress6, fix_q1):
+    """wave2fault then fault2wave on one node with the oracle's own per-node
+    steps; returns (stress written back [6] in {xx, yy, zz, yz, xz, xy}
+    order, traction matrix [3][3])."""
+    m = np.ascontiguousarray(metric9, np.float64)
+    s = np.ascontiguousarray(stress6, np.float64)
+    out = np.zeros(6, np.float64)
+    tr = np.zeros(9, np.float64)
+    load().oracle_probe_w2f_f2w(_p(m), float(jac), _p(s), int(bool(fix_q1)), _p(out), _p(tr))
+    return out, tr.reshape(3, 3)

@@ -211,6 +211,9 @@ struct Oracle {
   T dt, t0;
   int imethod, isfree, nfault, it;
   int fix_q1;
+  /* plant: a deliberate mistake for the tests that show a pin catches it
+     (0 = none; see oracle_create). Never set outside those tests. */
+  int plant = 0;
   T rk_a[3], rk_b[4];
   T coef[2][5];
 
@@ -409,6 +412,33 @@ struct Oracle {
       }
   }
 
+  /* M = [grad xi, grad eta, grad zeta] as columns (transform.cu:139-147) */
+  void metric_at(size_t iptr, T metric[3][3]) const {
+    metric[0][0] = xix[iptr]; metric[0][1] = etx[iptr]; metric[0][2] = ztx[iptr];
+    metric[1][0] = xiy[iptr]; metric[1][1] = ety[iptr]; metric[1][2] = zty[iptr];
+    metric[2][0] = xiz[iptr]; metric[2][1] = etz[iptr]; metric[2][2] = ztz[iptr];
+  }
+  /* traction = J (sigma M), transform.cu:150-160 (united == 1) */
+  static void w2f_node(T stress[3][3], T metric[3][3], T jj, T traction[3][3]) {
+    matmul3x3(stress, metric, traction);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) traction[a][b] *= jj;
+  }
+  /* sigma = (traction M^-1) / J, symmetrised, transform.cu:330-366 (united == 0);
+     out = {xx, yy, zz, xy, "Txz" argument, "Tyz" argument} */
+  static void f2w_node(T traction[3][3], T metric[3][3], T jacv, T out[6]) {
+    T rj = (T)(1.0 / jacv);
+    T stress[3][3];
+    invert3x3(metric);
+    matmul3x3(traction, metric, stress);
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) stress[a][b] *= rj;
+    out[0] = stress[0][0]; out[1] = stress[1][1]; out[2] = stress[2][2];
+    out[3] = (T)((stress[0][1] + stress[1][0]) * 0.5);
+    out[4] = (T)((stress[0][2] + stress[2][0]) * 0.5);
+    out[5] = (T)((stress[1][2] + stress[2][1]) * 0.5);
+  }
+
   /* ---- wave2fault_gpu, transform.cu:100-212 ---- */
   void wave2fault(T *w_cur, T *w_rhs, T *fcur, T *frhs) {
     T *Vx = w_cur, *Vy = w_cur + sicmp, *Vz = w_cur + 2 * sicmp;
@@ -430,16 +460,12 @@ struct Oracle {
           if (F.united[pf] != 1) continue;
           size_t iptr = i0 + (iy + nj1) * siy + (iz + nk1) * siz;
           T metric[3][3], stress[3][3], traction[3][3];
-          metric[0][0] = xix[iptr]; metric[0][1] = etx[iptr]; metric[0][2] = ztx[iptr];
-          metric[1][0] = xiy[iptr]; metric[1][1] = ety[iptr]; metric[1][2] = zty[iptr];
-          metric[2][0] = xiz[iptr]; metric[2][1] = etz[iptr]; metric[2][2] = ztz[iptr];
+          metric_at(iptr, metric);
           T jj = jac[iptr];
           stress[0][0] = Txx[iptr]; stress[0][1] = Txy[iptr]; stress[0][2] = Txz[iptr];
           stress[1][0] = Txy[iptr]; stress[1][1] = Tyy[iptr]; stress[1][2] = Tyz[iptr];
           stress[2][0] = Txz[iptr]; stress[2][1] = Tyz[iptr]; stress[2][2] = Tzz[iptr];
-          matmul3x3(stress, metric, traction);
-          for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) traction[a][b] *= jj;
+          w2f_node(stress, metric, jj, traction);
           size_t p3 = pf + 3 * sslice;
           F.T1x[p3] = traction[0][0]; F.T1y[p3] = traction[1][0]; F.T1z[p3] = traction[2][0];
           for (int m = 0; m < 2; m++) {
@@ -451,9 +477,7 @@ struct Oracle {
           stress[0][0] = hTxx[iptr]; stress[0][1] = hTxy[iptr]; stress[0][2] = hTxz[iptr];
           stress[1][0] = hTxy[iptr]; stress[1][1] = hTyy[iptr]; stress[1][2] = hTyz[iptr];
           stress[2][0] = hTxz[iptr]; stress[2][1] = hTyz[iptr]; stress[2][2] = hTzz[iptr];
-          matmul3x3(stress, metric, traction);
-          for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) traction[a][b] *= jj;
+          w2f_node(stress, metric, jj, traction);
           for (int m = 0; m < 2; m++) {
             size_t p = pf + m * sslice;
             fhT2x[p] = traction[0][1]; fhT2y[p] = traction[1][1]; fhT2z[p] = traction[2][1];
@@ -483,11 +507,8 @@ struct Oracle {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
           if (F.united[pf] != 0) continue;
           size_t iptr = i0 + (iy + nj1) * siy + (iz + nk1) * siz;
-          T metric[3][3], stress[3][3], traction[3][3];
-          metric[0][0] = xix[iptr]; metric[0][1] = etx[iptr]; metric[0][2] = ztx[iptr];
-          metric[1][0] = xiy[iptr]; metric[1][1] = ety[iptr]; metric[1][2] = zty[iptr];
-          metric[2][0] = xiz[iptr]; metric[2][1] = etz[iptr]; metric[2][2] = ztz[iptr];
-          T rj = (T)(1.0 / jac[iptr]);
+          T metric[3][3], traction[3][3], sv[6];
+          metric_at(iptr, metric);
           size_t p3 = pf + 3 * sslice, p1 = pf + sslice;
           traction[0][0] = F.T1x[p3]; traction[1][0] = F.T1y[p3]; traction[2][0] = F.T1z[p3];
           traction[0][1] = (T)((fT2x[pf] + fT2x[p1]) * 0.5);
@@ -496,19 +517,20 @@ struct Oracle {
           traction[0][2] = (T)((fT3x[pf] + fT3x[p1]) * 0.5);
           traction[1][2] = (T)((fT3y[pf] + fT3y[p1]) * 0.5);
           traction[2][2] = (T)((fT3z[pf] + fT3z[p1]) * 0.5);
-          invert3x3(metric);
-          matmul3x3(traction, metric, stress);
-          for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) stress[a][b] *= rj;
-          Vx[iptr] = (T)((fVx[pf] + fVx[p1]) * 0.5);
-          Vy[iptr] = (T)((fVy[pf] + fVy[p1]) * 0.5);
-          Vz[iptr] = (T)((fVz[pf] + fVz[p1]) * 0.5);
-          Txx[iptr] = stress[0][0];
-          Tyy[iptr] = stress[1][1];
-          Tzz[iptr] = stress[2][2];
-          Txy[iptr] = (T)((stress[0][1] + stress[1][0]) * 0.5);
-          kTxz[iptr] = (T)((stress[0][2] + stress[2][0]) * 0.5);
-          kTyz[iptr] = (T)((stress[1][2] + stress[2][1]) * 0.5);
+          f2w_node(traction, metric, jac[iptr], sv);
+          if (plant == 6) {  /* planted: the plus side's velocity instead of the side average */
+            Vx[iptr] = fVx[p1]; Vy[iptr] = fVy[p1]; Vz[iptr] = fVz[p1];
+          } else {
+            Vx[iptr] = (T)((fVx[pf] + fVx[p1]) * 0.5);
+            Vy[iptr] = (T)((fVy[pf] + fVy[p1]) * 0.5);
+            Vz[iptr] = (T)((fVz[pf] + fVz[p1]) * 0.5);
+          }
+          Txx[iptr] = sv[0];
+          Tyy[iptr] = sv[1];
+          Tzz[iptr] = sv[2];
+          Txy[iptr] = sv[3];
+          kTxz[iptr] = sv[4];
+          kTyz[iptr] = sv[5];
         }
     }
   }
@@ -581,18 +603,20 @@ struct Oracle {
               Ry[m] = (T)(a_1 * T1y[4 * sslice + pf] + a_2 * T1y[5 * sslice + pf] + a_3 * T1y[6 * sslice + pf]);
               Rz[m] = (T)(a_1 * T1z[4 * sslice + pf] + a_2 * T1z[5 * sslice + pf] + a_3 * T1z[6 * sslice + pf]);
             }
-            Rx[m] = (T)(0.5 * ((2 * m - 1) * Rx[m] + (DyT2x + DzT3x)));
-            Ry[m] = (T)(0.5 * ((2 * m - 1) * Ry[m] + (DyT2y + DzT3y)));
-            Rz[m] = (T)(0.5 * ((2 * m - 1) * Rz[m] + (DyT2z + DzT3z)));
+            int sg = plant == 2 ? (1 - 2 * m) : (2 * m - 1);  /* planted: wrong side sign */
+            Rx[m] = (T)(0.5 * (sg * Rx[m] + (DyT2x + DzT3x)));
+            Ry[m] = (T)(0.5 * (sg * Ry[m] + (DyT2y + DzT3y)));
+            Rz[m] = (T)(0.5 * (sg * Rz[m] + (DyT2z + DzT3z)));
           }
           size_t p1 = pf + sslice;
           T dVx = mVx[p1] - mVx[pf];
           T dVy = mVy[p1] - mVy[pf];
           T dVz = mVz[p1] - mVz[pf];
           T Trial[3];
-          Trial[0] = (T)(2.0 * (Mrho[0] * Mrho[1] * dVx / dt + Mrho[0] * Rx[1] - Mrho[1] * Rx[0]) / (a_0 * (Mrho[0] + Mrho[1])));
-          Trial[1] = (T)(2.0 * (Mrho[0] * Mrho[1] * dVy / dt + Mrho[0] * Ry[1] - Mrho[1] * Ry[0]) / (a_0 * (Mrho[0] + Mrho[1])));
-          Trial[2] = (T)(2.0 * (Mrho[0] * Mrho[1] * dVz / dt + Mrho[0] * Rz[1] - Mrho[1] * Rz[0]) / (a_0 * (Mrho[0] + Mrho[1])));
+          const double two = plant == 1 ? 1.0 : 2.0;  /* planted: a dropped factor */
+          Trial[0] = (T)(two * (Mrho[0] * Mrho[1] * dVx / dt + Mrho[0] * Rx[1] - Mrho[1] * Rx[0]) / (a_0 * (Mrho[0] + Mrho[1])));
+          Trial[1] = (T)(two * (Mrho[0] * Mrho[1] * dVy / dt + Mrho[0] * Ry[1] - Mrho[1] * Ry[0]) / (a_0 * (Mrho[0] + Mrho[1])));
+          Trial[2] = (T)(two * (Mrho[0] * Mrho[1] * dVz / dt + Mrho[0] * Rz[1] - Mrho[1] * Rz[0]) / (a_0 * (Mrho[0] + Mrho[1])));
           T vn[3], vs1[3], vs2[3];
           vs1[0] = F.vec_s1[pf * 3 + 0]; vs1[1] = F.vec_s1[pf * 3 + 1]; vs1[2] = F.vec_s1[pf * 3 + 2];
           vs2[0] = F.vec_s2[pf * 3 + 0]; vs2[1] = F.vec_s2[pf * 3 + 1]; vs2[2] = F.vec_s2[pf * 3 + 2];
@@ -915,10 +939,11 @@ struct Oracle {
             T DxT1[3], DyT2[3], DzT3[3];
             for (int c = 0; c < 3; c++) {
               const T *t = T1[c];
+              const double a0p = plant == 3 ? 0.0 : a_0;  /* planted: a dropped term */
               if (m == 0)
-                DxT1[c] = (T)(a_0 * t[pf + 3 * sslice] - a_1 * t[pf + 2 * sslice] - a_2 * t[pf + 1 * sslice] - a_3 * t[pf + 0 * sslice]);
+                DxT1[c] = (T)(a0p * t[pf + 3 * sslice] - a_1 * t[pf + 2 * sslice] - a_2 * t[pf + 1 * sslice] - a_3 * t[pf + 0 * sslice]);
               else
-                DxT1[c] = (T)(-a_0 * t[pf + 3 * sslice] + a_1 * t[pf + 4 * sslice] + a_2 * t[pf + 5 * sslice] + a_3 * t[pf + 6 * sslice]);
+                DxT1[c] = (T)(-a0p * t[pf + 3 * sslice] + a_1 * t[pf + 4 * sslice] + a_2 * t[pf + 5 * sslice] + a_3 * t[pf + 6 * sslice]);
               const T *p2 = fT2[c] + pf + m * sslice;
               if (F.rupy[pf] == 1) DyT2[c] = mac22(p2, 1, jdir);
               if (F.rupy[pf] == 0) DyT2[c] = macdrp(p2, 1, jdir);
@@ -1011,7 +1036,7 @@ struct Oracle {
           load9(T1p, &F.M2[3][idx]); load9(Vyp, &F.M2[4][idx]); load9(Vzp, &F.M2[5][idx]);
           T dtT1[3];
           auto get_dtT1 = [&]() {
-            if (F.faultgrid[pf] == 1) { dtT1[0] = F.hT1x[pf]; dtT1[1] = F.hT1y[pf]; dtT1[2] = F.hT1z[pf]; }
+            if (F.faultgrid[pf] == 1 && plant != 4) { dtT1[0] = F.hT1x[pf]; dtT1[1] = F.hT1y[pf]; dtT1[2] = F.hT1z[pf]; }
             else { dtT1[0] = 0.0; dtT1[1] = 0.0; dtT1[2] = 0.0; }
           };
           T dxV1[3], dyV1[3], dzV1[3], dxV2[3], dyV2[3], dzV2[3];
@@ -1034,6 +1059,7 @@ struct Oracle {
               setv(dxV2, DxV, 4); setv(dyV2, DyV, 4); setv(dzV2, DzV, 4);
               matmul3x1(P2M[0], dxV2, o1); matmul3x1(P2M[1], dyV2, o2); matmul3x1(P2M[2], dzV2, o3);
               matmul3x1(P2M[3], dyV1, o4); matmul3x1(P2M[4], dzV1, o5);
+              if (plant == 5) for (int c = 0; c < 3; c++) o1[c] = 0;  /* planted: a dropped term */
               for (int c = 0; c < 3; c++) DxV[3][c] = o1[c] + o2[c] + o3[c] - o4[c] - o5[c];
             }
             if (imethod == 2) method2_full();
@@ -1044,6 +1070,7 @@ struct Oracle {
               setv(dyV2, DyV, 4); setv(dzV2, DzV, 4);
               matmul3x1(M2P[0], dxV1, o1); matmul3x1(M2P[1], dyV1, o2); matmul3x1(M2P[2], dzV1, o3);
               matmul3x1(M2P[3], dyV2, o4); matmul3x1(M2P[4], dzV2, o5);
+              if (plant == 5) for (int c = 0; c < 3; c++) o1[c] = 0;
               for (int c = 0; c < 3; c++) DxV[4][c] = o1[c] + o2[c] + o3[c] - o4[c] - o5[c];
             }
             if (imethod == 2) method2_full();
@@ -1402,13 +1429,21 @@ int dispatch(void *hp, F &&f) {
 extern "C" {
 
 /* flags: bit 0 = use FP64, bit 1 = correct the fault2wave Txz/Tyz swap
-   (also taken from desc->compat_flags) */
+   (also taken from desc->compat_flags); bits 8-15 = a planted mistake for
+   the tests that show a pin catches it:
+     1 trial traction without its factor 2 (trial_slipweakening.cu:273-275)
+     2 wrong sign of the side term in R (trial_slipweakening.cu:262-264)
+     3 split-node DxT1 without its a_0 term (fault_gpu.cu:326-359)
+     4 method 2 without the T1 rate term (fault_gpu.cu:662-707)
+     5 method 1 without the inv(D11)D11 DxV term (fault_gpu.cu:636-661)
+     6 fault2wave velocity from the plus side only (transform.cu:341-343) */
 void *oracle_create(const cgfd_desc *d, int flags) {
   Handle *h = new Handle;
   h->dbl = flags & 1;
   int fq = ((flags >> 1) & 1) | (d->compat_flags & CGFD_COMPAT_FIX_F2W_SWAP);
-  if (h->dbl) h->o = new Oracle<double>(d, fq);
-  else h->o = new Oracle<float>(d, fq);
+  int plant = (flags >> 8) & 0xff;
+  if (h->dbl) { auto *o = new Oracle<double>(d, fq); o->plant = plant; h->o = o; }
+  else { auto *o = new Oracle<float>(d, fq); o->plant = plant; h->o = o; }
   return h;
 }
 
@@ -1464,6 +1499,26 @@ int oracle_read_fault_state(void *hp, int id, double *out) {
   });
 }
 
+/* Overwrites fault id's split-node state at level pre (in[9][2][sslice]) and,
+   if t1rec is not null, hT1 and mT1 (t1rec[6][sslice]). */
+int oracle_write_fault_state(void *hp, int id, const double *in, const double *t1rec) {
+  return dispatch(hp, [&](auto *o) {
+    using T = std::remove_pointer_t<decltype(o->w_pre)>;
+    if (id < 0 || id >= o->nfault) return -1;
+    auto *p = o->f_pre + id * o->f_siz_ilevel;
+    for (size_t i = 0; i < o->f_siz_ilevel; i++) p[i] = (T)in[i];
+    if (t1rec) {
+      auto &F = o->flt[id];
+      size_t ss = o->sslice;
+      for (size_t i = 0; i < ss; i++) {
+        F.hT1x[i] = (T)t1rec[i]; F.hT1y[i] = (T)t1rec[ss + i]; F.hT1z[i] = (T)t1rec[2 * ss + i];
+        F.mT1x[i] = (T)t1rec[3 * ss + i]; F.mT1y[i] = (T)t1rec[4 * ss + i]; F.mT1z[i] = (T)t1rec[5 * ss + i];
+      }
+    }
+    return 0;
+  });
+}
+
 int oracle_read_pg(void *hp, double *out) {
   return dispatch(hp, [&](auto *o) { for (size_t i = 0; i < o->PG.size(); i++) out[i] = o->PG[i]; return 0; });
 }
@@ -1482,6 +1537,31 @@ int oracle_probe_inner(void *hp, const double *w9, int idir, int jdir, int kdir,
     for (size_t i = 0; i < r.size(); i++) rhs9[i] = r[i];
     return 0;
   });
+}
+
+/* wave2fault then fault2wave on one node (the oracle's own per-node steps):
+   metric9 = {xi_x, xi_y, xi_z, eta_x, .., zeta_z}, stress6 = {xx, yy, zz, yz,
+   xz, xy} (the wavefield's component order); out6 = the stress fault2wave
+   writes back in the same order, with the Txz/Tyz argument swap unless fix_q1.
+   traction9 (may be null) = the traction matrix [a][b] row-major. */
+int oracle_probe_w2f_f2w(const double *metric9, double jacv, const double *stress6, int fix_q1, double *out6,
+                         double *traction9) {
+  using T = double;
+  T M[3][3], S[3][3], Tr[3][3], sv[6];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) M[c][r] = metric9[3 * r + c];  /* columns = grad xi, grad eta, grad zeta */
+  S[0][0] = stress6[0]; S[1][1] = stress6[1]; S[2][2] = stress6[2];
+  S[1][2] = S[2][1] = stress6[3]; S[0][2] = S[2][0] = stress6[4]; S[0][1] = S[1][0] = stress6[5];
+  Oracle<T>::w2f_node(S, M, jacv, Tr);
+  if (traction9)
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) traction9[3 * a + b] = Tr[a][b];
+  Oracle<T>::f2w_node(Tr, M, jacv, sv);
+  out6[0] = sv[0]; out6[1] = sv[1]; out6[2] = sv[2]; out6[5] = sv[3];
+  /* the Txz argument of the kernel receives the Tyz array unless fixed */
+  if (fix_q1) { out6[4] = sv[4]; out6[3] = sv[5]; }
+  else { out6[3] = sv[4]; out6[4] = sv[5]; }
+  return 0;
 }
 
 /* Free-surface conversion matrices computed at create (bdryfree matVx2Vz2,

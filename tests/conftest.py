@@ -18,8 +18,10 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def built_libs():
     from cgfd import build
-    if not build.LIB_PATH.exists():
+    # rebuild when a source is newer than the library, so a test never runs
+    # a binary older than the tree it ships with
+    if build.core_is_stale():
         build.build_core()
-    if not build.ORACLE_LIB.exists():
+    if build.oracle_is_stale():
         build.build_oracle()
     return build.LIB_PATH, build.ORACLE_LIB

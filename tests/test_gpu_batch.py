@@ -29,13 +29,21 @@ def _plain_with_fields(sc, fields):
     return model.Setup(sc2)
 
 
-@pytest.mark.parametrize("variant", [1, 7, 10, 16, 17, 19, 20, 21])
-def test_batch_matches_plain_runs(built_libs, variant):
+CASES = [(v, "default") for v in (1, 16, 18, 19, 20, 21)] + [(20, "first_retires"), (18, "first_retires"),
+                                                                (20, "ragged_slab"), (21, "ragged_slab")]
+
+
+@pytest.mark.parametrize("variant,case", CASES)
+def test_batch_matches_plain_runs(built_libs, variant, case):
+    """default: 4 scenarios, scenarios 1 and 3 retire first; first_retires:
+    scenario 0 retires first and one scenario runs alone at the end;
+    ragged_slab: nj = 50, so the last slab of 4-row tiles is partial."""
     from cgfd import configs, model
+    import copy
     import oracle
-    sc = configs.small(imethod=2)
+    sc = configs.small(imethod=2, nj=50) if case == "ragged_slab" else configs.small(imethod=2)
     st = model.Setup(sc)
-    nts = [40, 25, 40, 33]
+    nts = [20, 40, 33] if case == "first_retires" else [40, 25, 40, 33]
     fields = [None] + [_scenario_fields(sc, 100 + s) for s in range(1, len(nts))]
     bat = model.Batch(st, list(zip(fields, nts)), variant=variant)
     bat.run(max(nts) + 5)
@@ -52,7 +60,10 @@ def test_batch_matches_plain_runs(built_libs, variant):
         assert r <= 1e-6, f"scenario {s}: batched vs plain wavefield rel L2 {r:.3e}"
         assert rel_l2(fault_phys(fg)[6], fault_phys(fp)[6]) <= 1e-6
         if s == 1:
-            orc = oracle.Oracle(st_s.desc, sc.shape_ext, 1)
+            sc_s = copy.copy(sc)
+            sc_s.faults = [copy.copy(sc.faults[0])]
+            sc_s.faults[0].fields = fields[s][0]
+            orc = oracle.Oracle.from_scenario(sc_s)
             orc.run(nt)
             assert rel_l2(interior(wg), interior(orc.read_wavefield())) <= 1e-3
             assert rel_l2(fault_phys(fg)[6], fault_phys(orc.read_fault())[6]) <= 1e-3

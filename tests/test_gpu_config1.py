@@ -36,7 +36,7 @@ def test_config1_full_run(built_libs):
     core = model.Core(st)
     ijk = sc.stations + 3
     core.set_stations(ijk, nt)
-    orc = oracle.Oracle(st.desc, sc.shape_ext, 1)
+    orc = oracle.Oracle.from_scenario(sc)
     every = 4  # oracle station samples (the GPU records every step)
     rec_o = []
     t0 = time.perf_counter()

@@ -4,9 +4,9 @@ oracle is too slow to run: properties that do not depend on size.
 * cubic fields on an affine grid advance by exactly one RK4 step of the
   closed form (the same pin as the oracle's, now on every tile, XCD slot and
   block of the real launch shape);
-* every volume-kernel variant produces the same rupture on the real
-  configuration (tile lists, XCD-ordered schedules and z chunks cover every
-  point exactly once).
+* every volume-kernel variant the autotuner chooses from produces the same
+  rupture on the real configuration (tiles, slab orders and z chunks cover
+  every point exactly once).
 """
 import gc
 
@@ -73,7 +73,7 @@ def test_fullsize_volume_variants_agree(built_libs):
     st = model.Setup(sc)
     del sc.x, sc.y, sc.z, sc.lam, sc.mu, sc.rho
     ref = None
-    for v, z in ((1, 32), (7, 32), (4, 16), (6, 32)):
+    for v, z in ((1, 4), (20, 4), (16, 16), (18, 32), (19, 8)):
         _log(f"[fullsize-variants] variant {v} zchunk {z}")
         core = model.Core(st, variant=v)
         core.set_zchunk(z)

@@ -18,11 +18,11 @@ import pins
 from helpers import fault_phys, interior, rel_l2
 
 
-def _oracle(sc, double=True):
+def _oracle(sc, double=True, plant=0):
     from cgfd import model
     import oracle
     st = model.Setup(sc)
-    return st, oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=double)
+    return st, oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=double, plant=plant)
 
 
 @pytest.mark.parametrize("double,tol", [(True, 1e-5), (False, 1e-4)])
@@ -226,7 +226,7 @@ def test_cfspml_absorbs_outgoing_pulse(built_libs):
 
 
 # ---------------------------------------------------------------- locked fault
-def _locked_fault_run(imethod, with_fault, sigma, steps=60):
+def _locked_fault_run(imethod, with_fault, sigma, steps=60, plant=0):
     """FP64 oracle: a Gaussian velocity pulse 8 nodes in front of a fault
     whose static friction is 1000 everywhere (never reaches strength)."""
     from cgfd import configs
@@ -235,7 +235,7 @@ def _locked_fault_run(imethod, with_fault, sigma, steps=60):
     i0 = sc.faults[0].x_index + 3
     if not with_fault:
         sc.faults = []
-    st, orc = _oracle(sc, True)
+    st, orc = _oracle(sc, True, plant=plant)
     nz, ny, nx = sc.shape_ext
     kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
     g = np.exp(-(((ii - (i0 - 8)) / sigma) ** 2 + ((jj - ny // 2) / sigma) ** 2 + ((kk - nz // 2) / sigma) ** 2))
@@ -270,6 +270,15 @@ def test_locked_fault_transmits_without_slip(built_libs, imethod):
     w, vmax, vsmax = _locked_fault_run(imethod, True, 2.5)
     assert vmax > 0.05, "the pulse must reach the fault"
     assert vsmax <= 1e-12 * vmax, f"slip rate {vsmax:.3e} on a locked fault (particle velocity {vmax:.3e})"
+
+
+@pytest.mark.parametrize("plant", [1, 2, 3])
+def test_locked_fault_catches_planted_mistakes(built_libs, plant):
+    """The continuity check fails for a trial traction without its factor 2
+    (1), with the wrong sign of the side term (2), and for a split-node
+    velocity row without its a_0 term (3): each opens O(0.1-1) slip."""
+    _, vmax, vsmax = _locked_fault_run(2, True, 2.5, steps=30, plant=plant)
+    assert vsmax > 1e-3 * vmax, f"planted mistake {plant} not caught: {vsmax:.3e} vs {vmax:.3e}"
 
 
 def test_locked_fault_close_to_no_fault(built_libs):
