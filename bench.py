@@ -50,9 +50,11 @@ def read_traffic(profile_dir: Path):
     return float(m.group(1)) if m else None
 
 
-def calib_copy(mbytes: int) -> None:
-    """One device-to-device copy of a known size (the reference point that
-    scales the FETCH_SIZE/WRITE_SIZE counters of a profiling run)."""
+def calib_copy(mbytes: int, reps: int = 1) -> float:
+    """Device-to-device copies of a known size: the reference point that
+    scales the FETCH_SIZE/WRITE_SIZE counters of a profiling run, and, timed
+    with HIP events, the streaming rate a plain copy reaches on this GPU
+    (read + written bytes per second, GB/s)."""
     import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     n = C.c_size_t(mbytes << 20)
@@ -61,18 +63,46 @@ def calib_copy(mbytes: int) -> None:
         raise RuntimeError("hipMalloc failed in calib_copy")
     hip.hipMemset(src, 0, n)
     hip.hipDeviceSynchronize()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    hip.hipEventCreate(C.byref(e0))
+    hip.hipEventCreate(C.byref(e1))
     hip.hipMemcpy(dst, src, n, 3)  # hipMemcpyDeviceToDevice
     hip.hipDeviceSynchronize()
+    hip.hipEventRecord(e0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(dst, src, n, 3, None)
+    hip.hipEventRecord(e1, None)
+    hip.hipEventSynchronize(e1)
+    ms = C.c_float()
+    hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
     hip.hipFree(src)
     hip.hipFree(dst)
-    log(f"[bench] calibration copy of {mbytes} MiB done")
+    gbs = 2.0 * (mbytes << 20) * reps / (ms.value / 1e3) / 1e9 if ms.value > 0 else 0.0
+    log(f"[bench] device copy of {mbytes} MiB x {reps}: {gbs:.0f} GB/s (read + write)")
+    return gbs
+
+
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(threads: int, steps: int = 24):
-    """The oracle (FP32 CPU restatement, OpenMP) on a bounded sample: the same
-    configuration generator at 1/4 linear scale (150 x 300 x 100 nodes)."""
+    """The oracle (FP32 CPU restatement, OpenMP over k-planes) on a bounded
+    sample: the same configuration generator at 1/4 linear scale (150 x 300
+    x 100 nodes). Timed with the -O3 AVX2/FMA build of the restatement when
+    this CPU has those instructions (the checker build is -O2 without
+    contraction)."""
     from cgfd import configs, model
     import oracle
+    fast = oracle.load_fast()
     sc = configs.config2(scale=0.25, steps=steps)
     st = model.Setup(sc)
     n = oracle.Oracle.threads(threads)
@@ -84,15 +114,22 @@ def cpu_baseline(threads: int, steps: int = 24):
     cells = sc.ni * sc.nj * sc.nk * steps
     o.close()
     st.close()
+    build = oracle.active_build()
     return {"value": cells / el / 1e9, "unit": "Gcell-updates/s", "cores": n, "kind": "port",
-            "sample": f"oracle FP32 on the same generator at 1/4 scale ({sc.ni}x{sc.nj}x{sc.nk}), "
-                      f"{steps} steps after 1 warm-up, {n} OpenMP threads, {el:.1f} s"}
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(), "build": build,
+            "sample": f"oracle FP32 ({build}) on the same generator at 1/4 scale ({sc.ni}x{sc.nj}x{sc.nk}), "
+                      f"{steps} steps after 1 warm-up, {n} OpenMP threads of {os.cpu_count()} visible, {el:.1f} s"
+                      + ("" if fast else "; the CPU lacks AVX2/FMA, so the portable build ran")}
 
 
 def batched_leg(steps: int, nscen: int, variant: int, overlap: bool = False):
     """Config-3 ensemble: nscen scenarios on one 400x200x150 grid, batched,
     against scenario 0 alone through the plain path on the same grid, both
-    with the same volume-kernel variant."""
+    with the same volume-kernel variant (0: autotuned for each). Two timed
+    batched runs: every scenario active for all the timed steps, and the
+    config's uneven run lengths scaled into the timed window (U(1000,3000)
+    steps mapped to U(steps/3, steps)), so scenarios retire inside it; the
+    second counts only the scenario-steps actually taken."""
     from cgfd import configs, model
     t0 = time.perf_counter()
     base, fields, nts = configs.config3(nscen=nscen)
@@ -101,6 +138,9 @@ def batched_leg(steps: int, nscen: int, variant: int, overlap: bool = False):
     cells = base.ni * base.nj * base.nk
     # plain: scenario 0 alone
     core = model.Core(st, variant=variant)
+    if variant == 0:
+        core.autotune(2)
+    vplain = core.variant_info()
     core.set_overlap(overlap)
     core.run(8)
     core.sync()
@@ -109,9 +149,12 @@ def batched_leg(steps: int, nscen: int, variant: int, overlap: bool = False):
     core.sync()
     plain = cells * steps / (time.perf_counter() - t) / 1e9
     core.close()
-    # batched: every scenario active for the timed steps (run lengths >= 1000)
-    big = [max(n, steps + 16) for n in nts]
+    # batched, every scenario active for the timed steps
+    big = [n + steps + 16 for n in nts]
     bat = model.Batch(st, [(None if s == 0 else [fields[s]], big[s]) for s in range(nscen)], variant=variant)
+    if variant == 0:
+        bat.autotune(2)
+    vbat = bat.variant_info()
     bat.set_overlap(overlap)
     bat.run(8)
     bat.sync()
@@ -121,12 +164,34 @@ def batched_leg(steps: int, nscen: int, variant: int, overlap: bool = False):
     el = time.perf_counter() - t
     batched = cells * nscen * steps / el / 1e9
     bat.close()
+    # batched, uneven run lengths retiring inside the timed window
+    lo, hi = 1000, 3000
+    short = [8 + max(1, int(round(steps * (1 + 2 * (n - lo) / (hi - lo)) / 3))) for n in nts]
+    bat = model.Batch(st, [(None if s == 0 else [fields[s]], short[s]) for s in range(nscen)], variant=variant)
+    if variant != 0:
+        bat.set_variant(variant)
+    else:
+        bat.set_variant(vbat[0])
+        bat.set_zchunk(vbat[1])
+    bat.run(8)
+    bat.sync()
+    t = time.perf_counter()
+    bat.run(max(short) - 8)
+    bat.sync()
+    el_u = time.perf_counter() - t
+    taken = sum(n - 8 for n in short)
+    uneven = cells * taken / el_u / 1e9
+    bat.close()
     st.close()
     return {"value": round(batched, 4), "unit": "scenario-Gcell-updates/s", "nscen": nscen,
             "grid": [base.ni, base.nj, base.nk], "steps": steps, "ms_per_step": round(el / steps * 1e3, 3),
             "plain_same_grid": round(plain, 4), "per_scenario_vs_plain": round(batched / plain, 3),
-            "volume_kernel_variant": variant,
-            "note": "all scenarios active in the timed steps; retirement is exercised by the GPU tests"}
+            "volume_kernel": {"plain": list(vplain), "batched": list(vbat), "autotuned": variant == 0},
+            "uneven": {"value": round(uneven, 4), "per_scenario_vs_plain": round(uneven / plain, 3),
+                       "timed_steps": max(short) - 8, "scenario_steps": taken,
+                       "run_lengths": f"{min(short) - 8}..{max(short) - 8} timed steps per scenario"},
+            "note": "value: all scenarios active in the timed steps; uneven: scenarios retire inside the timed "
+                    "window, counting only the scenario-steps taken"}
 
 
 def adjoint_leg(steps: int):
@@ -214,16 +279,41 @@ def batched_adjoint_leg(steps: int, nscen: int = 8):
             "per_scenario_forward_plus_adjoint_vs_unbatched": round(per / (t_f + t_a), 3), "finite": finite}
 
 
+def device_setup_leg(sc, st_host, host_s):
+    """The setup again with the metric and the fault coefficients computed
+    on the device (csrc/k_setup.hip), timed end to end (host arrays in, host
+    arrays out, as the descriptor needs them), and its metric compared with
+    the host setup's bit for bit."""
+    import ctypes as C
+    import numpy as np
+    from cgfd import model
+    t0 = time.perf_counter()
+    st_d = model.Setup(sc, device_setup=True)
+    dev_s = time.perf_counter() - t0
+    nz, ny, nx = sc.shape_ext
+    n = 10 * nz * ny * nx
+
+    def view(p):
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n,))
+    same = bool(np.array_equal(view(st_host.desc.metric), view(st_d.desc.metric)))
+    st_d.close()
+    log(f"[bench] device setup in {dev_s:.1f} s (host {host_s:.1f} s), metric identical: {same}")
+    return {"host_s": round(host_s, 2), "device_s": round(dev_s, 2), "metric_bit_identical": same,
+            "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+            "scope": "metric (gd_curv_metric_cal) and fault coefficients (fault_coef_cal) on the device; "
+                     "coordinates, media, PML profiles and fault_set on the host; times include the "
+                     "host<->device copies"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the workload (1.0 = 1200x600x400)")
-    ap.add_argument("--variant", type=int, default=19,
-                    help="volume kernel variant (19 = interior kernel in slab order + lean PML-tile kernel, the one the "
-                         "committed profiles cover; 0 = autotune)")
-    ap.add_argument("--zchunk", type=int, default=4, help="z planes per block / y tiles per slab (0 = default)")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="volume kernel variant (0 = autotune and use the fastest; see cgfd_core_set_variant)")
+    ap.add_argument("--zchunk", type=int, default=0, help="y tiles per slab (0 = default or autotuned)")
     ap.add_argument("--overlap", action="store_true",
                     help="stage fork: interior, PML + top and fault kernels on three streams")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -234,7 +324,9 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--batched-steps", type=int, default=16)
     ap.add_argument("--nscen", type=int, default=32)
-    ap.add_argument("--batched-variant", type=int, default=20)
+    ap.add_argument("--batched-variant", type=int, default=0)
+    ap.add_argument("--no-copy-ceiling", action="store_true")
+    ap.add_argument("--no-setup-leg", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
@@ -259,11 +351,17 @@ def main() -> int:
     log(f"[bench] generated {sc.name} {sc.ni}x{sc.nj}x{sc.nk} in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     st = model.Setup(sc)
-    log(f"[bench] host setup in {time.perf_counter() - t0:.1f} s")
+    host_setup_s = time.perf_counter() - t0
+    log(f"[bench] host setup in {host_setup_s:.1f} s")
+    setup_leg = None
+    if not args.no_setup_leg:
+        setup_leg = device_setup_leg(sc, st, host_setup_s)
     t0 = time.perf_counter()
     core = model.Core(st, variant=args.variant)
     if args.zchunk:
         core.set_zchunk(args.zchunk)
+    # the 64 stations recorded at every step, as the reference's io_recv_keep
+    core.set_stations(sc.stations + 3, args.steps + args.warmup)
     if args.overlap:
         core.set_overlap(True)
     core.sync()
@@ -271,7 +369,7 @@ def main() -> int:
     tuned = None
     if args.variant == 0:
         t0 = time.perf_counter()
-        tuned = core.autotune(3)
+        tuned = core.autotune(2)
         log(f"[bench] autotune: variant {tuned[0]}, zchunk {tuned[1]}, {tuned[2]:.3f} ms per stage-0 launch "
             f"({time.perf_counter() - t0:.1f} s)")
     # free host copies (the core holds everything in HBM)
@@ -298,6 +396,7 @@ def main() -> int:
     bytes_vol = sum(STAGE_BYTES) * vol_pts * (n_vol // launches_per_step)
     achieved = bytes_vol / (ms_vol / 1e3) / 1e9 if ms_vol > 0 else 0.0
     traffic = read_traffic(REPO / "profiles")
+    copy_gbs = None if args.no_copy_ceiling else calib_copy(2048, reps=5)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -314,14 +413,20 @@ def main() -> int:
         "config": {"workload": f"{sc.name}: plain-method rupture, {nj}x{ni}x{nk} nodes (strike x normal x depth) "
                                f"at 50 m, curvilinear, PML 20, free surface, 1 fault, FP32",
                    "grid": [ni, nj, nk], "parallelism": "single GPU", "hbm_frac_768B_model": round(hbm_frac, 4),
-                   "max_slip_m": round(slip_max, 4), "stage_fork": args.overlap,
+                   "max_slip_m": round(slip_max, 4), "stage_fork": args.overlap, "stations": int(len(sc.stations)),
                    "volume_kernel": {"variant": tuned[0] if tuned else args.variant,
                                      "zchunk": tuned[1] if tuned else args.zchunk, "autotuned": tuned is not None}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_vol (volume stencil + PML + RK update)",
-                     "avg_launch_ms": round(ms_vol / max(n_vol, 1), 4)},
+                     "avg_launch_ms": round(ms_vol / max(n_vol, 1), 4),
+                     "traffic_source": "profiles/k_vol_config2_pmc.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE of the "
+                                       "volume dispatches, calibrated with a device copy (not measured in this run)",
+                     "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+                     "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None},
     }
+    if setup_leg:
+        out["setup"] = setup_leg
     core.close()
     st.close()
     del core, st, sc

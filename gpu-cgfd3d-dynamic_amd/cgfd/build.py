@@ -21,8 +21,9 @@ BUILD_DIR = PKG_DIR / "_build"
 LIB_PATH = BUILD_DIR / "libcgfd.so"
 ORACLE_DIR = REPO / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle.so"
+ORACLE_FAST_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle_fast.so"
 
-HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip"]
+HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip", "k_setup.hip"]
 HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp", "cgfd_nc.cpp", "cgfd_sac.cpp"]
 
 # Packed FP32 instructions stay disabled: hipcc otherwise forms v_pk_*_f32
@@ -116,17 +117,28 @@ def build_core(verbose: bool = False, jobs: int | None = None) -> Path:
 
 
 def build_oracle(verbose: bool = False) -> Path:
-    """Compile the CPU restatement used as the checker (OpenMP, -O2)."""
+    """Compile the CPU restatement: the checker (OpenMP, -O2, no FMA
+    contraction) and the same source -O3 for AVX2/FMA, which bench.py times
+    as the CPU baseline."""
+    from concurrent.futures import ThreadPoolExecutor
     out = ORACLE_LIB
     out.parent.mkdir(parents=True, exist_ok=True)
+    src = str(ORACLE_DIR / "cgfd_oracle.cpp")
     tmp = out.with_suffix(".so.tmp")
+    tmp_fast = ORACLE_FAST_LIB.with_suffix(".so.tmp")
     cmd = ["g++", "-O2", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-march=x86-64-v2",
-           "-ffp-contract=off", f"-I{INCLUDE}", str(ORACLE_DIR / "cgfd_oracle.cpp"), "-o", str(tmp)]
+           "-ffp-contract=off", f"-I{INCLUDE}", src, "-o", str(tmp)]
+    cmd_fast = ["g++", "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-march=x86-64-v3",
+                f"-I{INCLUDE}", src, "-o", str(tmp_fast)]
     if verbose:
         print(" ".join(cmd))
+        print(" ".join(cmd_fast))
     digest = _digest(_oracle_deps())
-    _run(cmd)
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        for f in [ex.submit(_run, c) for c in (cmd, cmd_fast)]:
+            f.result()
     os.replace(tmp, out)
+    os.replace(tmp_fast, ORACLE_FAST_LIB)
     out.with_suffix(".stamp").write_text(digest + "\n")
     return out
 

@@ -67,6 +67,7 @@ class SetupParams(C.Structure):
         ("init_stress", (VP * 7) * MAX_FAULTS),
         ("united_margin", I32),
         ("dt", F32), ("t0", F32), ("nt_total", I32), ("device_id", I32),
+        ("metric", VP), ("device_setup", I32),
     ]
 
 

@@ -58,7 +58,12 @@ class Scenario:
 class Setup:
     """Host setup (metric, PML profiles, fault coefficients and masks)."""
 
-    def __init__(self, sc: Scenario, device_id: int = 0):
+    def __init__(self, sc: Scenario, device_id: int = 0, device_setup: bool = False, metric=None):
+        """device_setup: compute the metric and the fault coefficients with
+        the HIP kernels of the setup (same arithmetic as the host loops).
+        metric: an imported metric, (10, nk, nj, ni) at the physical points
+        in the order jac, xi_x..xi_z, eta_x..eta_z, zeta_x..zeta_z, used
+        instead of the one the coordinates give."""
         self.lib = L.load_library()
         self.sc = sc
         keep = []
@@ -89,6 +94,12 @@ class Setup:
                 p.init_stress[fid][q] = arr(f.fields[name])
         p.united_margin = sc.united_margin
         p.dt, p.t0, p.nt_total, p.device_id = sc.dt, sc.t0, sc.nt_total, device_id
+        if metric is not None:
+            m = np.asarray(metric)
+            if m.shape != (10, sc.nk, sc.nj, sc.ni):
+                raise ValueError(f"metric must be (10, nk, nj, ni) = {(10, sc.nk, sc.nj, sc.ni)}, got {m.shape}")
+            p.metric = arr(m)
+        p.device_setup = int(bool(device_setup))
         h = C.c_void_p()
         L.check(self.lib, self.lib.cgfd_setup_create(C.byref(p), C.byref(h)))
         del keep  # the setup copied everything it needs

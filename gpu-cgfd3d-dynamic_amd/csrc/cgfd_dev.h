@@ -106,6 +106,7 @@ struct BatArgs {
   long pstride;          /* CFS-PML auxiliaries */
   long fwstride;         /* fault split-node state levels */
   long fpf, fpi;         /* fault parameters/outputs (floats), flags (ints) */
+  long fpd;              /* FP64 slip accumulators (doubles) */
   long pgstride;         /* PG + displacement */
   long ststride;         /* station records */
 };

@@ -27,6 +27,7 @@ struct FaultDev {
   const float *Fm;               /* [14][9][ny] free-surface row */
   float *T1;                     /* [3][7][sslice] */
   float *hT1, *mT1;              /* [3][sslice] */
+  double *slip64;                /* per scenario [3][sslice]: Slip, Slip1, Slip2 accumulated in FP64 */
 };
 
 struct FaultArgs {
@@ -37,6 +38,7 @@ struct FaultArgs {
   int nfault;
   const FaultDev *ftab;                     /* [nfault] in device memory (scenario 0) */
   long sfp, sfi;                            /* this scenario's offsets into the per-node blocks */
+  long sfd;                                 /* and into the FP64 slip accumulators */
   int istage, idir, jdir, kdir;
   int isfree, imethod, fix_q1;
   float dt, ca, cb, rkb;
@@ -66,6 +68,7 @@ DEVI FaultArgs fault_at(const FaultArgs &a, const BatArgs &b, int s) {
   r.f_end += f;
   r.sfp = (long)s * b.fpf;
   r.sfi = (long)s * b.fpi;
+  r.sfd = (long)s * b.fpd;
   if (r.tape) r.tape += (long)s * a.tape_sstride;
   return r;
 }
@@ -81,6 +84,7 @@ DEVI FaultDev fdev(const FaultArgs &a, int id) {
   F.mT1 += a.sfp;
   F.flag_rup += a.sfi;
   F.init_t0_flag += a.sfi;
+  F.slip64 += a.sfd;
   return F;
 }
 

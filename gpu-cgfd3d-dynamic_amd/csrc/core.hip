@@ -323,6 +323,11 @@ static int upload_faults(cgfd_core *c, const cgfd_desc *ds) {
     F.mT1 = fb + 42 * ss;
     F.flag_rup = ib;
     F.init_t0_flag = ib + ss;
+    /* FP64 slip accumulators, zero (fault_info.cu:945-947) */
+    double *db = nullptr;
+    if ((rc = c->alloc(&db, (size_t)B * 3 * ss))) return rc;
+    HIPCHK(hipMemset(db, 0, (size_t)B * 3 * ss * sizeof(double)));
+    F.slip64 = db;
     /* masks */
     const int *msk[4] = {f.united, f.faultgrid, f.rup_index_y, f.rup_index_z};
     const int *dmsk[4];
@@ -518,6 +523,7 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
     c->bat.fwstride = 4L * c->nfault * 18 * g.sslice;
     c->bat.fpf = 45L * g.sslice;
     c->bat.fpi = 2L * g.sslice;
+    c->bat.fpd = 3L * g.sslice;
     size_t n = (size_t)B * c->bat.fwstride;
     if ((rc = c->alloc(&c->fwav, n))) return fail(rc);
     if (hipMemset(c->fwav, 0, n * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
@@ -797,6 +803,7 @@ std::vector<StatePart> state_parts(cgfd_core *c) {
       v.push_back({F.mT1, 3 * ss * 4});
       v.push_back({F.flag_rup, ss * 4});
       v.push_back({F.init_t0_flag, ss * 4});
+      v.push_back({F.slip64, 3 * ss * 8});
     }
   }
   for (int dim = 0; dim < 3; dim++)
@@ -1319,11 +1326,10 @@ extern "C" int cgfd_batch_read_stations(cgfd_core *c, int s, float *dst, int *nt
     std::memcpy(dst + (size_t)q * rec * 9, &h[(size_t)q * c->st_nt * 9], (size_t)rec * 9 * 4);
   return CGFD_OK;
 }
-extern "C" int cgfd_batch_read_snapshot(cgfd_core *c, int s, const int *box, int flags, float *dst) {
-  if (int rc = check_scen(c, s)) return rc;
-  if (!box || !dst || flags <= 0 || flags > 7) return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot arguments");
+/* validated snapshot box and its size in floats */
+static int snapshot_box(const cgfd_core *c, const int *box, int flags, IoBox &b, size_t &n) {
+  if (!box || flags <= 0 || flags > 7) return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot arguments");
   const Grid &g = c->g;
-  IoBox b;
   const int lo[3] = {3, 3, 3}, hi[3] = {g.ni + 2, g.nj + 2, g.nk + 2};
   for (int a = 0; a < 3; a++) {
     b.s[a] = box[3 * a];
@@ -1334,15 +1340,30 @@ extern "C" int cgfd_batch_read_snapshot(cgfd_core *c, int s, const int *box, int
                             a, b.s[a], b.n[a], b.d[a]);
   }
   const int groups = (flags & 1 ? 3 : 0) + (flags & 2 ? 6 : 0) + (flags & 4 ? 6 : 0);
-  const size_t n = (size_t)b.n[0] * b.n[1] * b.n[2] * groups;
+  n = (size_t)b.n[0] * b.n[1] * b.n[2] * groups;
+  return CGFD_OK;
+}
+
+/* pack the box of scenario s into the device buffer buf and copy it to dst */
+static hipError_t snapshot_copy(cgfd_core *c, int s, const IoBox &b, int flags, size_t n, float *buf, float *dst) {
+  launch_pack_box(c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, c->g, c->lam, c->mu, b, flags, buf, c->st);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(dst, buf, n * 4, hipMemcpyDeviceToHost, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  return e;
+}
+
+extern "C" int cgfd_batch_read_snapshot(cgfd_core *c, int s, const int *box, int flags, float *dst) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot arguments");
+  IoBox b;
+  size_t n = 0;
+  if (int rc = snapshot_box(c, box, flags, b, n)) return rc;
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   float *buf = nullptr;
   HIPCHK(hipMalloc(&buf, n * 4));
-  launch_pack_box(c->wlev(scen_wlev(c, s)) + s * c->bat.wstride, g, c->lam, c->mu, b, flags, buf, c->st);
-  hipError_t e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpyAsync(dst, buf, n * 4, hipMemcpyDeviceToHost, c->st);
-  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  hipError_t e = snapshot_copy(c, s, b, flags, n, buf, dst);
   (void)hipFree(buf);
   if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "snapshot: %s", hipGetErrorString(e));
   return CGFD_OK;
@@ -1398,7 +1419,7 @@ TuneKey tune_key(const cgfd_core *c) {
   return TuneKey{c->g.ni, c->g.nj, c->g.nk, c->nscen, c->nfault, c->isfree, pml};
 }
 
-bool valid_variant(int v) { return v == 1 || v == 16 || (v >= 18 && v <= 21); }
+bool valid_variant(int v) { return v == 1 || v == 16 || (v >= 19 && v <= 21); }
 
 /* the cache file: one "ni nj nk nscen nfault isfree pml variant zchunk" line
    per tuned shape; later lines win */
@@ -1445,7 +1466,7 @@ static void apply_auto_variant(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant != 0 && !valid_variant(variant))
-    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 16, 18, 19, 20, 21)", variant);
+    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 16, 19, 20, 21)", variant);
   if (variant == 0) apply_auto_variant(c);
   else c->variant = variant;
   c->drop_graph();
@@ -1483,7 +1504,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const bool batch = c->bat.nact > 1;
-  const std::vector<int> variants = batch ? std::vector<int>{20, 19, 21, 16, 1} : std::vector<int>{20, 16, 18, 1};
+  const std::vector<int> variants = batch ? std::vector<int>{20, 19, 21, 16, 1} : std::vector<int>{20, 16, 1};
   const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;

@@ -737,9 +737,19 @@ DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
     F.out[3 * ss + pf] = Vs;
     F.out[4 * ss + pf] = Vs1;
     F.out[5 * ss + pf] = Vs2;
-    F.out[6 * ss + pf] += Vs * dt;
-    F.out[7 * ss + pf] += Vs1 * dt;
-    F.out[8 * ss + pf] += Vs2 * dt;
+    /* Slip, Slip1, Slip2 accumulate in FP64; the outputs (and the Slip the
+       friction law reads) are their FP32 values */
+    {
+      double *sd = F.slip64;
+      const double s0 = sd[pf] + (double)(Vs * dt), s1 = sd[ss + pf] + (double)(Vs1 * dt),
+                   s2 = sd[2 * ss + pf] + (double)(Vs2 * dt);
+      sd[pf] = s0;
+      sd[ss + pf] = s1;
+      sd[2 * ss + pf] = s2;
+      F.out[6 * ss + pf] = (float)s0;
+      F.out[7 * ss + pf] = (float)s1;
+      F.out[8 * ss + pf] = (float)s2;
+    }
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       float t = T1[(long)c * 7 * ss + 3 * ss + pf];
@@ -749,7 +759,7 @@ DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
     if (Vs > F.out[9 * ss + pf]) F.out[9 * ss + pf] = Vs;
     if (F.init_t0_flag[pf] == 0) {
       if (Vs > 1e-3) {
-        F.out[10 * ss + pf] = (*a.it_dev + 1) * dt;
+        F.out[10 * ss + pf] = (float)((*a.it_dev + 1) * (double)dt);
         F.init_t0_flag[pf] = 1;
         F.flag_rup[pf] = 1;
       }

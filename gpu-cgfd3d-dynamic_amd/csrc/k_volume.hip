@@ -100,14 +100,15 @@ DEVI bool block_touches_pml(const VolArgs &a, int i0, int i1, int j0, int j1, in
   return false;
 }
 
-/* G > 0: the scheduler may not move instructions across the end of every
-   G-th field, which bounds the loads in flight (and the registers) per wave */
-template <int ID, int JD, int KD, int G>
-DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
+/* The per-point interior update with each field's three derivatives from
+   `deriv(f, dx, dy, dz)`. G > 0: the scheduler may not move instructions
+   across the end of every G-th field, which bounds the loads in flight (and
+   the registers) per wave */
+template <int G, class Deriv>
+DEVI void acc_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn, Deriv deriv) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   const unsigned o = (unsigned)p * 4u;
-  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
   const float *M = a.met;
   const long woff = (long)sn * b.wstride;
   const float xx = ld32nt(M + sc, o), xy = ld32nt(M + 2 * sc, o), xz = ld32nt(M + 3 * sc, o);
@@ -117,8 +118,8 @@ DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int
   float hv0 = 0.f, hv1 = 0.f, hv2 = 0.f;
 #pragma unroll
   for (int f = 0; f < 9; f++) {
-    const float *bc = a.cur + woff + f * sc;
-    const float dx = dfd32<ID>(bc, o, 4u), dy = dfd32<JD>(bc, o, by), dz = dfd32<KD>(bc, o, bz);
+    float dx, dy, dz;
+    deriv(f, o, dx, dy, dz);
     float g0 = xx * dx + ex * dy + zx * dz;  /* d/dx */
     float g1 = xy * dx + ey * dy + zy * dz;  /* d/dy */
     float g2 = xz * dx + ez * dy + zz * dz;  /* d/dz */
@@ -155,6 +156,20 @@ DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int
   rk_point32(a, o, h, woff);
 }
 
+/* interior point with every tap loaded through the cache hierarchy */
+template <int ID, int JD, int KD, int G>
+DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
+  const unsigned by = (unsigned)a.g.siy * 4u, bz = (unsigned)a.g.siz * 4u;
+  const float *cur = a.cur + (long)sn * b.wstride;
+  const long sc = a.g.sicmp;
+  acc_core<G>(a, b, i, j, k, sn, [&](int f, unsigned o, float &dx, float &dy, float &dz) {
+    const float *bc = cur + f * sc;
+    dx = dfd32<ID>(bc, o, 4u);
+    dy = dfd32<JD>(bc, o, by);
+    dz = dfd32<KD>(bc, o, bz);
+  });
+}
+
 /* block id -> (x tile, y tile, plane) in slab order: x tile fastest, then
    the S = zchunk y tiles of a slab, then the plane, then the slab, so the
    blocks resident at one time cover a few planes of one slab of 4 S rows and
@@ -188,115 +203,6 @@ __global__ void __launch_bounds__(256, 8) k_vol_acc_slab(VolArgs a, BatArgs b) {
 }
 
 /* ------------------------------------------------------------------------
- * x taps from the wave's own lanes. A 64 x 4 block is four waves, one row
- * each, lane l at x = bi0 + l, so the x neighbours of a point sit in the
- * neighbouring lanes: DPP wave shifts move the centre values one lane, and
- * the lane shifted in at the wave's edge keeps the `old` operand (bound_ctrl
- * off), which carries the halo value. One load per row brings the four halo
- * values of all nine fields (lane 4f+t: field f, halo t); v_readlane makes
- * them wave-uniform. This replaces four of the five x-tap loads per field.
- * ---------------------------------------------------------------------- */
-DEVI float wave_shl1(float v, float edge) {  /* lane l <- lane l+1; lane 63 <- edge */
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xf, 0xf, false));
-}
-DEVI float wave_shr1(float v, float edge) {  /* lane l <- lane l-1; lane 0 <- edge */
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xf, 0xf, false));
-}
-DEVI float lane_val(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
-
-/* halo t of a wave starting at x0: forward taps (-1..3) need x0-1 and
-   x0+64..66, backward taps (-3..1) x0-3..x0-1 and x0+64 */
-template <int ID>
-DEVI int halo_x(int x0, int t) { return ID ? (t == 0 ? x0 - 1 : x0 + 63 + t) : (t == 3 ? x0 + 64 : x0 - 3 + t); }
-
-template <int ID>
-DEVI float dx_wave(float c, float h0, float h1, float h2, float h3) {
-  if (ID == 1) {
-    const float m1 = wave_shr1(c, h0), p1 = wave_shl1(c, h1), p2 = wave_shl1(p1, h2), p3 = wave_shl1(p2, h3);
-    return dfd5<1>(m1, c, p1, p2, p3);
-  } else {
-    const float m1 = wave_shr1(c, h2), m2 = wave_shr1(m1, h1), m3 = wave_shr1(m2, h0), p1 = wave_shl1(c, h3);
-    return dfd5<0>(m3, m2, m1, c, p1);
-  }
-}
-
-/* acc_point with x taps from DPP; every lane of the wave takes part (lanes
-   outside the physical columns load clamped addresses and store nothing) */
-template <int ID, int JD, int KD>
-DEVI void acc_dpp_point(const VolArgs &a, const BatArgs &b, int x0, int j, int k, int sn) {
-  const int lane = (int)threadIdx.x;
-  const int i = x0 + lane;
-  const int nx = a.g.nx;
-  const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
-  const long row = j * sy + (long)k * sz;
-  const unsigned o = (unsigned)(row + (i < nx ? i : nx - 1)) * 4u;
-  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
-  const float *M = a.met;
-  const long woff = (long)sn * b.wstride;
-  int hx = halo_x<ID>(x0, lane & 3);
-  hx = hx < 0 ? 0 : (hx > nx - 1 ? nx - 1 : hx);
-  const int hf = (lane >> 2) < 8 ? (lane >> 2) : 8;
-  const float hv = ld32(a.cur + woff + hf * sc, (unsigned)(row + hx) * 4u);
-  const float xx = ld32(M + sc, o), xy = ld32(M + 2 * sc, o), xz = ld32(M + 3 * sc, o);
-  const float ex = ld32(M + 4 * sc, o), ey = ld32(M + 5 * sc, o), ez = ld32(M + 6 * sc, o);
-  const float zx = ld32(M + 7 * sc, o), zy = ld32(M + 8 * sc, o), zz = ld32(M + 9 * sc, o);
-  float gvx[3], gvy[3], gvz[3];
-  float hv0 = 0.f, hv1 = 0.f, hv2 = 0.f;
-#pragma unroll
-  for (int f = 0; f < 9; f++) {
-    const float *bc = a.cur + woff + f * sc;
-    const float c = ld32(bc, o);
-    const float dx = dx_wave<ID>(c, lane_val(hv, 4 * f), lane_val(hv, 4 * f + 1), lane_val(hv, 4 * f + 2),
-                                 lane_val(hv, 4 * f + 3));
-    const float dy = dfd32<JD>(bc, o, by), dz = dfd32<KD>(bc, o, bz);
-    float g0 = xx * dx + ex * dy + zx * dz;
-    float g1 = xy * dx + ey * dy + zy * dz;
-    float g2 = xz * dx + ez * dy + zz * dz;
-    __asm__ volatile("" : "+v"(g0), "+v"(g1), "+v"(g2) : : "memory");
-    switch (f) {
-      case VX: gvx[0] = g0; gvx[1] = g1; gvx[2] = g2; break;
-      case VY: gvy[0] = g0; gvy[1] = g1; gvy[2] = g2; break;
-      case VZ: gvz[0] = g0; gvz[1] = g1; gvz[2] = g2; break;
-      case TXX: hv0 += g0; break;
-      case TYY: hv1 += g1; break;
-      case TZZ: hv2 += g2; break;
-      case TYZ: hv1 += g2; hv2 += g1; break;
-      case TXZ: hv0 += g2; hv2 += g0; break;
-      case TXY: hv0 += g1; hv1 += g0; break;
-    }
-  }
-  if (i < 3 || i >= a.g.ni + 3) return;
-  if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
-  const float l = ld32(a.lam, o), m = ld32(a.mu, o), s = ld32(a.slw, o);
-  const float l2m = l + 2.0f * m;
-  float h[9];
-  h[VX] = s * hv0;
-  h[VY] = s * hv1;
-  h[VZ] = s * hv2;
-  h[TXX] = l2m * gvx[0] + l * (gvy[1] + gvz[2]);
-  h[TYY] = l2m * gvy[1] + l * (gvx[0] + gvz[2]);
-  h[TZZ] = l2m * gvz[2] + l * (gvx[0] + gvy[1]);
-  h[TXY] = m * (gvx[1] + gvy[0]);
-  h[TXZ] = m * (gvx[2] + gvz[0]);
-  h[TYZ] = m * (gvy[2] + gvz[1]);
-  rk_point32(a, o, h, woff);
-}
-
-/* the slab-ordered interior kernel with x taps from DPP (whole waves: a
-   wave leaves only when its row is past the last y row) */
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256, 8) k_vol_acc_dpp(VolArgs a, BatArgs b) {
-  int bx, by, kz;
-  if (!slab_item(a, bx, by, kz)) return;
-  const int k = a.k_lo + kz + 3;
-  const int bi0 = bx * 64, bj0 = by * 4 + 3;
-  if (block_touches_pml(a, bi0, bi0 + 63, bj0, bj0 + 3, k)) return;
-  const int iy = by * 4 + (int)threadIdx.y;
-  if (iy >= a.g.nj) return;
-  acc_dpp_point<ID, JD, KD>(a, b, bi0, iy + 3, k, b.act[0]);
-}
-
-/* ------------------------------------------------------------------------
  * Lean CFS-PML point. The correction of face `dim` needs the part of every
  * rhs component that comes from the derivatives along `dim` only (r below;
  * the r[] of sv_curv_col_el_iso_gpu.cu:916-1288). Those parts are
@@ -308,12 +214,11 @@ __global__ void __launch_bounds__(256, 8) k_vol_acc_dpp(VolArgs a, BatArgs b) {
  * term of the x/y faces (k == nk2) never occurs here: with a free surface
  * the top four layers belong to k_vol_top.
  * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD>
-DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
+template <class Deriv>
+DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn, Deriv deriv) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   const unsigned o = (unsigned)p * 4u;
-  const unsigned by = (unsigned)sy * 4u, bz = (unsigned)sz * 4u;
   const float *M = a.met;
   const long woff = (long)sn * b.wstride;
   float e[3][3];  /* rows: xi, eta, zeta derivatives (x, y, z components) */
@@ -330,8 +235,8 @@ DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k,
     for (int c = 0; c < 9; c++) r[d][c] = 0.0f;
 #pragma unroll
   for (int f = 0; f < 9; f++) {
-    const float *bc = a.cur + woff + f * sc;
-    float D0 = dfd32<ID>(bc, o, 4u), D1 = dfd32<JD>(bc, o, by), D2 = dfd32<KD>(bc, o, bz);
+    float D0, D1, D2;
+    deriv(f, o, D0, D1, D2);
     __asm__ volatile("" : "+v"(D0), "+v"(D1), "+v"(D2) : : "memory");
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -404,6 +309,19 @@ DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k,
   }
   if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
   rk_point32(a, o, h, woff);
+}
+
+template <int ID, int JD, int KD>
+DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
+  const unsigned by = (unsigned)a.g.siy * 4u, bz = (unsigned)a.g.siz * 4u;
+  const float *cur = a.cur + (long)sn * b.wstride;
+  const long sc = a.g.sicmp;
+  pml_core(a, b, i, j, k, sn, [&](int f, unsigned o, float &D0, float &D1, float &D2) {
+    const float *bc = cur + f * sc;
+    D0 = dfd32<ID>(bc, o, 4u);
+    D1 = dfd32<JD>(bc, o, by);
+    D2 = dfd32<KD>(bc, o, bz);
+  });
 }
 
 /* at least 6 waves per SIMD: 71 VGPRs without spills (the compiler's own
@@ -725,7 +643,7 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
     return;
   }
-  if (b.nact > 1 && variant >= 16 && variant != 18) {  /* batches, split kernels */
+  if (b.nact > 1 && variant >= 16) {  /* batches, split kernels */
     long items, nb;
     if (variant != 16) {  /* lean PML kernel; 20: its blocks in per-scenario slab sweeps, 21: both kernels */
       const long items0 = (long)agrd.x * agrd.y * a.nk_run * b.nact, nb0 = ((items0 + 7) / 8) * 8;
@@ -760,13 +678,6 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
     hipLaunchKernelGGL((k_vol_acc_slab<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
     if (a.pml_on) hipLaunchKernelGGL((k_vol_pml_lean<ID, JD, KD>), agrd, blk, 0, st_pml, a, b);
-    return;
-  }
-  if (variant == 18) {
-    const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
-    const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
-    hipLaunchKernelGGL((k_vol_acc_dpp<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b);
-    if (a.pml_on) hipLaunchKernelGGL((k_vol_pmlblk<ID, JD, KD>), agrd, blk, 0, st_pml, a, b);
     return;
   }
   if (variant == 16) {

@@ -205,28 +205,37 @@ int NcReader::open(const std::string &path) {
   Cursor c{b};
   c.o = 4;
   c.u32();  /* numrecs */
+  /* every count read from the file is bounded by what is left of the header
+     buffer, and the loops stop at the first failed read */
+  const auto bad = [&]() { return cgfd_set_error(CGFD_ERR_ARG, "%s: header larger than 64 KiB, truncated or malformed",
+                                                 path.c_str()); };
   uint32_t tag = c.u32(), n = c.u32();
   if (tag == kNcDimension)
-    for (uint32_t q = 0; q < n; q++) { c.name(); dimlen_.push_back((long)c.u32()); }
+    for (uint32_t q = 0; q < n && c.ok; q++) { c.name(); dimlen_.push_back((long)c.u32()); }
   tag = c.u32(); n = c.u32();
   if (tag == kNcAttribute)
-    for (uint32_t q = 0; q < n; q++) { c.name(); const uint32_t t = c.u32(), m = c.u32(); c.skip_values(t, m); }
+    for (uint32_t q = 0; q < n && c.ok; q++) { c.name(); const uint32_t t = c.u32(), m = c.u32(); c.skip_values(t, m); }
   tag = c.u32(); n = c.u32();
   if (tag == kNcVariable)
-    for (uint32_t q = 0; q < n; q++) {
+    for (uint32_t q = 0; q < n && c.ok; q++) {
       Var v;
       v.name = c.name();
       const uint32_t nd = c.u32();
-      for (uint32_t d = 0; d < nd; d++) v.dims.push_back((int)c.u32());
+      if (nd > 64) return bad();
+      for (uint32_t d = 0; d < nd && c.ok; d++) {
+        const uint32_t id = c.u32();
+        if (id >= dimlen_.size()) return bad();  /* a dimension id the header never defined */
+        v.dims.push_back((int)id);
+      }
       const uint32_t at = c.u32(), na = c.u32();
       if (at == kNcAttribute)
-        for (uint32_t a = 0; a < na; a++) { c.name(); const uint32_t t = c.u32(), m = c.u32(); c.skip_values(t, m); }
+        for (uint32_t a = 0; a < na && c.ok; a++) { c.name(); const uint32_t t = c.u32(), m = c.u32(); c.skip_values(t, m); }
       v.type = c.u32();
       c.u32();  /* vsize */
       v.begin = off64 ? (int64_t)c.u64() : (int64_t)c.u32();
       vars_.push_back(v);
     }
-  if (!c.ok) return cgfd_set_error(CGFD_ERR_ARG, "%s: header larger than 64 KiB or truncated", path.c_str());
+  if (!c.ok) return bad();
   return CGFD_OK;
 }
 
@@ -237,6 +246,54 @@ int NcReader::shape(const std::string &name, std::vector<long> &dims) const {
       for (int d : v.dims) dims.push_back(dimlen_[d]);
       return CGFD_OK;
     }
+  return cgfd_set_error(CGFD_ERR_ARG, "%s has no variable %s", path_.c_str(), name.c_str());
+}
+
+int NcReader::get_block(const std::string &name, const std::vector<long> &count, std::vector<float> &out) {
+  for (const Var &v : vars_) {
+    if (v.name != name) continue;
+    if (count.size() != v.dims.size() || count.empty())
+      return cgfd_set_error(CGFD_ERR_ARG, "%s: %s has %zu dimensions, %zu asked", path_.c_str(), name.c_str(),
+                            v.dims.size(), count.size());
+    for (size_t d = 0; d < count.size(); d++) {
+      if (dimlen_[v.dims[d]] == 0)
+        return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is a record variable", path_.c_str(), name.c_str());
+      if (count[d] < 1 || count[d] > dimlen_[v.dims[d]])
+        return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is smaller than the grid along dimension %zu (%ld < %ld)",
+                              path_.c_str(), name.c_str(), d, dimlen_[v.dims[d]], count[d]);
+    }
+    if (v.type != kNcFloat && v.type != 6)
+      return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is neither float nor double", path_.c_str(), name.c_str());
+    const size_t sz = v.type == kNcFloat ? 4 : 8;
+    const size_t nd = count.size();
+    const size_t row = (size_t)count[nd - 1], frow = (size_t)dimlen_[v.dims[nd - 1]];
+    size_t nrows = 1;
+    for (size_t d = 0; d + 1 < nd; d++) nrows *= (size_t)count[d];
+    out.resize(nrows * row);
+    std::vector<unsigned char> raw(row * sz);
+    std::vector<size_t> idx(nd, 0);
+    for (size_t r = 0; r < nrows; r++) {
+      /* offset of row (idx[0..nd-2], 0) in the file's full dimensions */
+      size_t off = 0;
+      for (size_t d = 0; d + 1 < nd; d++) off = off * (size_t)dimlen_[v.dims[d]] + idx[d];
+      off = off * frow;
+      if (std::fseek(fp_, (long)(v.begin + (int64_t)(off * sz)), SEEK_SET) != 0 ||
+          std::fread(raw.data(), 1, raw.size(), fp_) != raw.size())
+        return cgfd_set_error(CGFD_ERR_ARG, "%s: short read of %s", path_.c_str(), name.c_str());
+      for (size_t i = 0; i < row; i++) {
+        uint64_t u = 0;
+        for (size_t k = 0; k < sz; k++) u = (u << 8) | raw[i * sz + k];
+        float *o = &out[r * row + i];
+        if (sz == 4) { uint32_t w = (uint32_t)u; std::memcpy(o, &w, 4); }
+        else { double dv; std::memcpy(&dv, &u, 8); *o = (float)dv; }
+      }
+      for (size_t d = nd - 1; d-- > 0;) {  /* next row index */
+        if (++idx[d] < (size_t)count[d]) break;
+        idx[d] = 0;
+      }
+    }
+    return CGFD_OK;
+  }
   return cgfd_set_error(CGFD_ERR_ARG, "%s has no variable %s", path_.c_str(), name.c_str());
 }
 
@@ -269,19 +326,40 @@ int NcReader::get_float(const std::string &name, std::vector<float> &out) {
 }  // namespace cgfd
 
 namespace {
-/* nf float fields of [nk][nj] from a classic file, in order */
+/* nf float fields of the leading [nk][nj] block of each variable (the
+   reference reads that block with nc_get_vara_float from start {0, 0} on
+   one process), in order */
 int read_planes(const char *path, int nj, int nk, const char *const *names, int nf, float *fields) {
   if (!path || !fields || nj <= 0 || nk <= 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
   cgfd::NcReader r;
   if (int rc = r.open(path)) return rc;
   std::vector<float> v;
-  std::vector<long> dims;
   for (int q = 0; q < nf; q++) {
-    if (int rc = r.shape(names[q], dims)) return rc;
-    if (dims.size() != 2 || dims[0] != nk || dims[1] != nj)
-      return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is not [%d][%d] (nk, nj)", path, names[q], nk, nj);
-    if (int rc = r.get_float(names[q], v)) return rc;
+    if (int rc = r.get_block(names[q], {nk, nj}, v)) return rc;
     std::memcpy(fields + (size_t)q * nj * nk, v.data(), (size_t)nj * nk * 4);
+  }
+  return CGFD_OK;
+}
+
+/* nf float fields of [nk][nj][ni]: the leading block when `block` (coordinate
+   import, nc_get_vara_float from start 0), else the variable must be exactly
+   that size (metric and media imports read whole variables, nc_get_var) */
+int read_volumes(const char *path, int ni, int nj, int nk, const char *const *names, int nf, bool block,
+                 float *fields) {
+  if (!path || !fields || ni <= 0 || nj <= 0 || nk <= 0) return cgfd_set_error(CGFD_ERR_ARG, "bad arguments");
+  cgfd::NcReader r;
+  if (int rc = r.open(path)) return rc;
+  std::vector<float> v;
+  std::vector<long> dims;
+  const size_t n = (size_t)ni * nj * nk;
+  for (int q = 0; q < nf; q++) {
+    if (!block) {
+      if (int rc = r.shape(names[q], dims)) return rc;
+      if (dims.size() != 3 || dims[0] != nk || dims[1] != nj || dims[2] != ni)
+        return cgfd_set_error(CGFD_ERR_ARG, "%s: %s is not [%d][%d][%d] (k, j, i)", path, names[q], nk, nj, ni);
+    }
+    if (int rc = r.get_block(names[q], {nk, nj, ni}, v)) return rc;
+    std::memcpy(fields + q * n, v.data(), n * 4);
   }
   return CGFD_OK;
 }
@@ -297,4 +375,24 @@ extern "C" int cgfd_io_read_init_stress(const char *path, int nj, int nk, float 
 extern "C" int cgfd_io_read_fault_coord(const char *path, int nj, int nk, float *xyz) {
   static const char *names[3] = {"x", "y", "z"};
   return read_planes(path, nj, nk, names, 3, xyz);
+}
+
+/* coord_<frame>.nc (gd_curv_coord_import, gd_t.cu:855-923) */
+extern "C" int cgfd_io_read_coord(const char *path, int ni, int nj, int nk, float *xyz) {
+  static const char *names[3] = {"x", "y", "z"};
+  return read_volumes(path, ni, nj, nk, names, 3, true, xyz);
+}
+
+/* metric_<frame>.nc (gd_curv_metric_import, gd_t.cu:1015-1068; variable
+   names of gd_curv_metric_init, gd_t.cu:150-190) */
+extern "C" int cgfd_io_read_metric(const char *path, int ni, int nj, int nk, float *metric) {
+  static const char *names[10] = {"jac", "xi_x", "xi_y", "xi_z", "eta_x", "eta_y", "eta_z", "zeta_x", "zeta_y", "zeta_z"};
+  return read_volumes(path, ni, nj, nk, names, 10, false, metric);
+}
+
+/* media_<frame>.nc (md_import, md_t.cu:229-286; isotropic elastic variable
+   names of md_init, md_t.cu:80-97) */
+extern "C" int cgfd_io_read_media(const char *path, int ni, int nj, int nk, float *media) {
+  static const char *names[3] = {"lambda", "mu", "rho"};
+  return read_volumes(path, ni, nj, nk, names, 3, false, media);
 }

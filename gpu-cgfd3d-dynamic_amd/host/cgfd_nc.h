@@ -59,6 +59,9 @@ class NcReader {
   /* the variable's dimension lengths, or an error if it is absent */
   int shape(const std::string &name, std::vector<long> &dims) const;
   int get_float(const std::string &name, std::vector<float> &out);
+  /* the leading block count[0] x count[1] x .. of a fixed-size variable whose
+     dimensions are at least that large (nc_get_vara with start 0) */
+  int get_block(const std::string &name, const std::vector<long> &count, std::vector<float> &out);
 
  private:
   struct Var { std::string name; std::vector<int> dims; uint32_t type = 0; int64_t begin = 0; };

@@ -6,6 +6,8 @@
  */
 #include "../../include/cgfd_setup.h"
 #include "cgfd_errors.h"
+#include "cgfd_setup_node.h"
+#include "cgfd_setup_dev.h"
 
 #include <cmath>
 #include <cstdio>
@@ -17,74 +19,7 @@
 
 namespace {
 
-/* 3x3 helpers with the reference's arithmetic (src/lib/fdlib_math.cu) */
-inline float norm3f(const float *A) {
-  float x2 = A[0] * A[0], y2 = A[1] * A[1], z2 = A[2] * A[2];
-  return sqrtf(x2 + y2 + z2);
-}
-inline void invert3x3f(float m[][3]) {
-  double inv[3][3];
-  inv[0][0] = (float)(m[1][1] * m[2][2] - m[2][1] * m[1][2]);
-  inv[0][1] = (float)(m[2][1] * m[0][2] - m[0][1] * m[2][2]);
-  inv[0][2] = (float)(m[0][1] * m[1][2] - m[0][2] * m[1][1]);
-  inv[1][0] = (float)(m[1][2] * m[2][0] - m[1][0] * m[2][2]);
-  inv[1][1] = (float)(m[0][0] * m[2][2] - m[2][0] * m[0][2]);
-  inv[1][2] = (float)(m[1][0] * m[0][2] - m[0][0] * m[1][2]);
-  inv[2][0] = (float)(m[1][0] * m[2][1] - m[1][1] * m[2][0]);
-  inv[2][1] = (float)(m[2][0] * m[0][1] - m[0][0] * m[2][1]);
-  inv[2][2] = (float)(m[0][0] * m[1][1] - m[0][1] * m[1][0]);
-  double det = inv[0][0] * m[0][0] + inv[0][1] * m[1][0] + inv[0][2] * m[2][0];
-  det = 1.0 / det;
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) m[i][j] = (float)(inv[i][j] * det);
-}
-inline void matmul3x3f(float A[][3], float B[][3], float C[][3]) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) {
-      C[i][j] = 0.0f;
-      for (int k = 0; k < 3; k++) C[i][j] += A[i][k] * B[k][j];
-    }
-}
-inline void cross3f(const float *A, const float *B, float *C) {
-  C[0] = A[1] * B[2] - A[2] * B[1];
-  C[1] = A[2] * B[0] - A[0] * B[2];
-  C[2] = A[0] * B[1] - A[1] * B[0];
-}
-inline float dot3f(const float *A, const float *B) {
-  float r = 0.0f;
-  for (int i = 0; i < 3; i++) r += A[i] * B[i];
-  return r;
-}
-
-/* 6th-order centred difference, fd_t.h:126-140 (d_4 = 0 is skipped there) */
-inline float center6(const float *p, long s) {
-  const double d1 = -0.01666667, d2 = 0.15, d3 = -0.75, d5 = 0.75, d6 = -0.15, d7 = 0.01666667;
-  return (float)((d1 * p[-3 * s]) + (d2 * p[-2 * s]) + (d3 * p[-s]) + (d5 * p[s]) + (d6 * p[2 * s]) + (d7 * p[3 * s]));
-}
-
-/* Elastic traction-strain matrix D_ab of metric rows a, b at a node.
-   Entry expressions and their evaluation order are those of
-   fault_info.cu:190-369 (all 18 blocks share this pattern). */
-inline void dmat(const float *ea, const float *eb, float lam, float mu, float lam2mu, float D[9]) {
-  D[0] = lam2mu * ea[0] * eb[0] + mu * (ea[1] * eb[1] + ea[2] * eb[2]);
-  D[1] = lam * ea[0] * eb[1] + mu * ea[1] * eb[0];
-  D[2] = lam * ea[0] * eb[2] + mu * ea[2] * eb[0];
-  D[3] = mu * ea[0] * eb[1] + lam * ea[1] * eb[0];
-  D[4] = lam2mu * ea[1] * eb[1] + mu * (ea[0] * eb[0] + ea[2] * eb[2]);
-  D[5] = lam * ea[1] * eb[2] + mu * ea[2] * eb[1];
-  D[6] = mu * ea[0] * eb[2] + lam * ea[2] * eb[0];
-  D[7] = mu * ea[1] * eb[2] + lam * ea[2] * eb[1];
-  D[8] = lam2mu * ea[2] * eb[2] + mu * (ea[0] * eb[0] + ea[1] * eb[1]);
-}
-
-inline void to33(const float *v, float M[][3]) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) M[i][j] = v[3 * i + j];
-}
-inline void from33(float M[][3], float *v) {
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++) v[3 * i + j] = M[i][j];
-}
+using namespace cgfd_setup_math;
 
 struct Grid {
   int ni, nj, nk, nx, ny, nz, ni1, ni2, nj1, nj2, nk1, nk2;
@@ -168,27 +103,14 @@ void mirror_symmetry(const Grid &g, float *v, int ncmp) {
 
 /* gd_curv_metric_cal, gd_t.cu:209-304 */
 void metric_cal(const Grid &g, const float *x3d, const float *y3d, const float *z3d, float *met) {
-  float *jac3d = met, *xi_x = met + g.sicmp, *xi_y = xi_x + g.sicmp, *xi_z = xi_y + g.sicmp;
-  float *et_x = xi_z + g.sicmp, *et_y = et_x + g.sicmp, *et_z = et_y + g.sicmp;
-  float *zt_x = et_z + g.sicmp, *zt_y = zt_x + g.sicmp, *zt_z = zt_y + g.sicmp;
 #pragma omp parallel for schedule(static)
   for (int k = g.nk1; k <= g.nk2; k++)
     for (int j = g.nj1; j <= g.nj2; j++)
       for (int i = g.ni1; i <= g.ni2; i++) {
         size_t iptr = i + j * g.siy + k * g.siz;
-        float vec1[3], vec2[3], vec3[3], vecg[3];
-        vec1[0] = center6(x3d + iptr, 1); vec1[1] = center6(y3d + iptr, 1); vec1[2] = center6(z3d + iptr, 1);
-        vec2[0] = center6(x3d + iptr, g.siy); vec2[1] = center6(y3d + iptr, g.siy); vec2[2] = center6(z3d + iptr, g.siy);
-        vec3[0] = center6(x3d + iptr, g.siz); vec3[1] = center6(y3d + iptr, g.siz); vec3[2] = center6(z3d + iptr, g.siz);
-        cross3f(vec1, vec2, vecg);
-        float jac = dot3f(vecg, vec3);
-        jac3d[iptr] = jac;
-        cross3f(vec2, vec3, vecg);
-        xi_x[iptr] = vecg[0] / jac; xi_y[iptr] = vecg[1] / jac; xi_z[iptr] = vecg[2] / jac;
-        cross3f(vec3, vec1, vecg);
-        et_x[iptr] = vecg[0] / jac; et_y[iptr] = vecg[1] / jac; et_z[iptr] = vecg[2] / jac;
-        cross3f(vec1, vec2, vecg);
-        zt_x[iptr] = vecg[0] / jac; zt_y[iptr] = vecg[1] / jac; zt_z[iptr] = vecg[2] / jac;
+        float o[10];
+        metric_point(x3d, y3d, z3d, iptr, (long)g.siy, (long)g.siz, o);
+        for (int c = 0; c < 10; c++) met[c * g.sicmp + iptr] = o[c];
       }
   geometric_symmetry(g, met, 10);
 }
@@ -329,9 +251,28 @@ int cgfd_setup_create(const cgfd_setup_params *p, cgfd_setup **out) {
     geometric_symmetry(g, s->coords.data(), 3);
     const float *x3d = s->coords.data(), *y3d = x3d + g.sicmp, *z3d = y3d + g.sicmp;
 
-    /* ---- metric ---- */
+    /* ---- metric: imported at the physical points (gd_curv_metric_import,
+       gd_t.cu:1015-1068, then geometric_symmetry), or computed on the host or
+       on the device ---- */
+    const bool dev = p->device_setup != 0;
     s->metric.assign(10 * g.sicmp, 0.0f);
-    metric_cal(g, x3d, y3d, z3d, s->metric.data());
+    if (p->metric) {
+      for (int c = 0; c < 10; c++) {
+        float *dst = s->metric.data() + c * g.sicmp;
+        const float *src_c = p->metric + (size_t)c * g.ni * g.nj * g.nk;
+        for (int k = 0; k < g.nk; k++)
+          for (int j = 0; j < g.nj; j++)
+            for (int i = 0; i < g.ni; i++)
+              dst[(i + 3) + (j + 3) * g.siy + (k + 3) * g.siz] = src_c[i + (size_t)j * g.ni + (size_t)k * g.ni * g.nj];
+      }
+      geometric_symmetry(g, s->metric.data(), 10);
+    } else if (dev) {
+      int rc = cgfd_dev_metric(p->device_id, g.ni, g.nj, g.nk, s->coords.data(), s->metric.data());
+      if (rc != CGFD_OK) { delete s; return rc; }
+      geometric_symmetry(g, s->metric.data(), 10);
+    } else {
+      metric_cal(g, x3d, y3d, z3d, s->metric.data());
+    }
 
     /* ---- media: physical values + mirror_symmetry (md_import, md_t.cu:263-278) ---- */
     s->media.assign(3 * g.sicmp, 0.0f);
@@ -406,152 +347,39 @@ int cgfd_setup_create(const cgfd_setup_params *p, cgfd_setup **out) {
       for (int q = 0; q < 6; q++) F.M2[q].assign(9 * ns, 0.f);
       for (int q = 0; q < 14; q++) F.Fm[q].assign(9 * (size_t)g.ny, 0.f);
 
-      /* fault_coef_cal, fault_info.cu:155-848 */
+      /* fault_coef_cal, fault_info.cu:155-848: per node on the host, or
+         the same node arithmetic on the device */
+      FaultCoefOut out;
+      for (int q = 0; q < 12; q++) out.D[q] = F.D[q].data();
+      for (int q = 0; q < 5; q++) { out.P2M[q] = F.P2M[q].data(); out.M2P[q] = F.M2P[q].data(); }
+      for (int q = 0; q < 6; q++) out.M2[q] = F.M2[q].data();
+      for (int q = 0; q < 14; q++) out.Fm[q] = F.Fm[q].data();
+      out.vec_n = F.vec_n.data(); out.vec_s1 = F.vec_s1.data(); out.vec_s2 = F.vec_s2.data();
+      out.x_et = F.x_et.data(); out.y_et = F.y_et.data(); out.z_et = F.z_et.data();
+      std::vector<float> node_in;
+      if (dev) node_in.assign(kFaultNodeIn * ns, 0.f);
       for (int k = 0; k < g.nz; k++)
         for (int j = 0; j < g.ny; j++) {
           size_t iptr = i0 + j * g.siy + k * g.siz;
           size_t pf = j + (size_t)k * g.ny;
           float rho = rho3d[iptr], mu = mu3d[iptr], lam = lam3d[iptr];
-          float lam2mu = lam + 2.0 * mu;
           float jac = jac3d[iptr];
           float e1[3] = {xi_x[iptr], xi_y[iptr], xi_z[iptr]};
           float e2[3] = {et_x[iptr], et_y[iptr], et_z[iptr]};
           float e3[3] = {zt_x[iptr], zt_y[iptr], zt_z[iptr]};
           F.rho_f[pf] = rho; F.lam_f[pf] = lam; F.mu_f[pf] = mu;
           F.rho_f[pf + ns] = rho; F.lam_f[pf + ns] = lam; F.mu_f[pf + ns] = mu;
-          float d11[9], d12[9], d13[9];
-          dmat(e1, e1, lam, mu, lam2mu, d11);
-          dmat(e1, e2, lam, mu, lam2mu, d12);
-          dmat(e1, e3, lam, mu, lam2mu, d13);
-          const float *rows[3] = {e1, e2, e3};
-          /* stored D21..D33 for side 1 ("_1", minus) then side 2 ("_2", plus);
-             both sides share the medium on one process */
-          float Dst[12][9];
-          for (int sd = 0; sd < 2; sd++)
-            for (int a = 1; a <= 2; a++)
-              for (int b = 0; b < 3; b++) dmat(rows[a], rows[b], lam, mu, lam2mu, Dst[sd * 6 + (a - 1) * 3 + b]);
-          for (int q = 0; q < 9; q++) { d11[q] *= jac; d12[q] *= jac; d13[q] *= jac; }
-          for (int sd = 0; sd < 12; sd++)
-            for (int q = 0; q < 9; q++) Dst[sd][q] *= jac;
-          for (int sd = 0; sd < 12; sd++)
-            for (int q = 0; q < 9; q++) F.D[sd][pf * 9 + q] = Dst[sd][q];
-          float D11[3][3], D12[3][3], D13[3][3], mat1[3][3], mat2[3][3], mat3[3][3];
-          to33(d11, D11); to33(d12, D12); to33(d13, D13);
-          /* method 1, minus -> plus: inv(D11_2) * {D11_1, D12_1, D13_1, D12_2, D13_2} */
-          to33(d11, mat1);
-          invert3x3f(mat1);
-          float (*Mp[5])[3] = {D11, D12, D13, D12, D13};
-          for (int q = 0; q < 5; q++) { matmul3x3f(mat1, Mp[q], mat2); from33(mat2, &F.M2P[q][pf * 9]); }
-          /* plus -> minus: inv(D11_1) * {D11_2, D12_2, D13_2, D12_1, D13_1} */
-          to33(d11, mat1);
-          invert3x3f(mat1);
-          for (int q = 0; q < 5; q++) { matmul3x3f(mat1, Mp[q], mat2); from33(mat2, &F.P2M[q][pf * 9]); }
-          /* method 2: minus T1,Vy,Vz -> Vx then plus (fault_info.cu:528-607) */
-          for (int sd = 0; sd < 2; sd++) {
-            to33(d11, mat1);
-            invert3x3f(mat1);
-            from33(mat1, &F.M2[sd * 3 + 0][pf * 9]);
-            matmul3x3f(mat1, D12, mat2); from33(mat2, &F.M2[sd * 3 + 1][pf * 9]);
-            matmul3x3f(mat1, D13, mat2); from33(mat2, &F.M2[sd * 3 + 2][pf * 9]);
-          }
-          /* strike / normal / dip vectors, fault_info.cu:609-654 */
-          mat3[0][0] = e1[0]; mat3[0][1] = e1[1]; mat3[0][2] = e1[2];
-          mat3[1][0] = e2[0]; mat3[1][1] = e2[1]; mat3[1][2] = e2[2];
-          mat3[2][0] = e3[0]; mat3[2][1] = e3[1]; mat3[2][2] = e3[2];
-          invert3x3f(mat3);
-          F.x_et[pf] = mat3[0][1]; F.y_et[pf] = mat3[1][1]; F.z_et[pf] = mat3[2][1];
-          float vs1[3] = {mat3[0][1], mat3[1][1], mat3[2][1]};
-          float nrm = norm3f(vs1);
-          for (int a = 0; a < 3; a++) vs1[a] /= nrm;
-          float vn[3] = {e1[0], e1[1], e1[2]};
-          nrm = norm3f(vn);
-          for (int a = 0; a < 3; a++) vn[a] /= nrm;
-          float vs2[3];
-          cross3f(vn, vs1, vs2);
-          for (int a = 0; a < 3; a++) {
-            F.vec_n[pf * 3 + a] = vn[a]; F.vec_s1[pf * 3 + a] = vs1[a]; F.vec_s2[pf * 3 + a] = vs2[a];
-          }
-          /* free-surface row, fault_info.cu:656-845 (evaluated whatever the top
-             boundary is, as upstream) */
-          if (k == g.nk2) {
-            float A[3][3], Bm[3][3], Cm[3][3];
-            float Vx2Vz1[3][3], Vy2Vz1[3][3], Vx2Vz2[3][3], Vy2Vz2[3][3];
-            for (int sd = 0; sd < 2; sd++) {
-              const float *D31 = &F.D[sd * 6 + 3][pf * 9], *D32 = &F.D[sd * 6 + 4][pf * 9], *D33 = &F.D[sd * 6 + 5][pf * 9];
-              for (int q = 0; q < 9; q++) { A[q / 3][q % 3] = D33[q]; Bm[q / 3][q % 3] = -D31[q]; Cm[q / 3][q % 3] = -D32[q]; }
-              invert3x3f(A);
-              matmul3x3f(A, Bm, sd == 0 ? Vx2Vz1 : Vx2Vz2);
-              matmul3x3f(A, Cm, sd == 0 ? Vy2Vz1 : Vy2Vz2);
-            }
-            float vn2[3] = {e1[0], e1[1], e1[2]};
-            float s1b[3] = {F.x_et[pf], F.y_et[pf], F.z_et[pf]};
-            float s2b[3];
-            cross3f(vn2, s1b, s2b);
-            nrm = norm3f(s2b);
-            for (int a = 0; a < 3; a++) s2b[a] /= nrm;
-            float g1_2 = 0.0f, g2_2 = 0.0f, g3_2 = 0.0f;
-            float h1_3 = s2b[0] * e1[0] + s2b[1] * e1[1] + s2b[2] * e1[2];
-            float h2_3 = s2b[0] * e2[0] + s2b[1] * e2[1] + s2b[2] * e2[2];
-            float h3_3 = s2b[0] * e3[0] + s2b[1] * e3[1] + s2b[2] * e3[2];
-            nrm = norm3f(vn2);
-            h1_3 = h1_3 / nrm; h2_3 = h2_3 / nrm; h3_3 = h3_3 / nrm;
-            float D11f[3][3], D12f[3][3], D13f[3][3];
-            for (int ii = 0; ii < 3; ii++)
-              for (int jj = 0; jj < 3; jj++) {
-                float To = s1b[ii] * vn2[jj];
-                float D11_12 = mu * g1_2 * To, D12_12 = mu * g2_2 * To, D13_12 = mu * g3_2 * To;
-                float To3 = s2b[ii] * vn2[jj];
-                float D11_13 = mu * h1_3 * To3, D12_13 = mu * h2_3 * To3, D13_13 = mu * h3_3 * To3;
-                D11f[ii][jj] = D11[ii][jj] + D11_12 + D11_13;
-                D12f[ii][jj] = D12[ii][jj] + D12_12 + D12_13;
-                D13f[ii][jj] = D13[ii][jj] + D13_12 + D13_13;
-              }
-            /* both sides share D11f etc. on one process (D11_1 == D11_2) */
-            float m1[3][3], m2[3][3], m3[3][3], m4[3][3];
-            matmul3x3f(D13f, Vx2Vz1, m1);
-            matmul3x3f(D13f, Vy2Vz1, m2);
-            matmul3x3f(D13f, Vx2Vz2, m3);
-            matmul3x3f(D13f, Vy2Vz2, m4);
-            float Vx1f[3][3], Vy1f[3][3], Vx2f[3][3], Vy2f[3][3];
-            for (int ii = 0; ii < 3; ii++)
-              for (int jj = 0; jj < 3; jj++) {
-                Vx1f[ii][jj] = D11f[ii][jj] + m1[ii][jj];
-                Vy1f[ii][jj] = D12f[ii][jj] + m2[ii][jj];
-                Vx2f[ii][jj] = D11f[ii][jj] + m3[ii][jj];
-                Vy2f[ii][jj] = D12f[ii][jj] + m4[ii][jj];
-              }
-            float To[3][3], P2M1f[3][3], P2M2f[3][3], P2M3f[3][3], M2P1f[3][3], M2P2f[3][3], M2P3f[3][3];
-            for (int ii = 0; ii < 3; ii++)
-              for (int jj = 0; jj < 3; jj++) To[ii][jj] = Vx1f[ii][jj];
-            invert3x3f(To);
-            matmul3x3f(To, Vx2f, P2M1f);
-            matmul3x3f(To, Vy2f, P2M2f);
-            matmul3x3f(To, Vy1f, P2M3f);
-            float *jrow = nullptr;
-            (void)jrow;
-            size_t jf = (size_t)j * 9;
-            from33(To, &F.Fm[10][jf]);     /* matT1toVxf_Min */
-            from33(P2M3f, &F.Fm[11][jf]);  /* matVytoVxf_Min */
-            for (int ii = 0; ii < 3; ii++)
-              for (int jj = 0; jj < 3; jj++) To[ii][jj] = Vx2f[ii][jj];
-            invert3x3f(To);
-            matmul3x3f(To, Vx1f, M2P1f);
-            matmul3x3f(To, Vy1f, M2P2f);
-            matmul3x3f(To, Vy2f, M2P3f);
-            from33(To, &F.Fm[12][jf]);     /* matT1toVxf_Plus */
-            from33(M2P3f, &F.Fm[13][jf]);  /* matVytoVxf_Plus */
-            from33(Vx2Vz1, &F.Fm[0][jf]);
-            from33(Vy2Vz1, &F.Fm[1][jf]);
-            from33(Vx2Vz2, &F.Fm[2][jf]);
-            from33(Vy2Vz2, &F.Fm[3][jf]);
-            from33(P2M1f, &F.Fm[4][jf]);
-            from33(P2M2f, &F.Fm[5][jf]);
-            from33(P2M3f, &F.Fm[6][jf]);
-            from33(M2P1f, &F.Fm[7][jf]);
-            from33(M2P2f, &F.Fm[8][jf]);
-            from33(M2P3f, &F.Fm[9][jf]);
+          if (!dev) fault_coef_node(out, pf, (size_t)j, e1, e2, e3, jac, lam, mu, k == g.nk2);
+          else {
+            float *q = &node_in[pf * kFaultNodeIn];
+            for (int a = 0; a < 3; a++) { q[a] = e1[a]; q[3 + a] = e2[a]; q[6 + a] = e3[a]; }
+            q[9] = jac; q[10] = lam; q[11] = mu;
           }
         }
+      if (dev) {
+        int rc = cgfd_dev_fault_coef(p->device_id, g.ny, g.nz, g.nk2, node_in.data(), out);
+        if (rc != CGFD_OK) { delete s; return rc; }
+      }
 
       /* fault_set, fault_info.cu:967-1074 */
       for (int q = 0; q < 7; q++) F.in[q].assign(ns, 0.f);

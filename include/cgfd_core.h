@@ -306,28 +306,27 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    1 = one direct-load kernel (27 derivatives per point); 16 = an interior
    kernel that accumulates each field's gradient into the rhs (64 x 4 tiles
    in slab order: zchunk y-tiles per slab, then the plane) and a direct
-   kernel for the tiles touching a CFS-PML slab; 18 = 16 with the interior x
-   taps moved between lanes by DPP wave shifts instead of loaded; 19 = 16
-   with a lean PML-tile kernel that accumulates the directional rhs parts
-   field by field; 20 and 21 = 19 on one scenario. Batches with more than
-   one active scenario run split interior / PML kernels for 16 (one
-   slab-ordered sweep per scenario), 19 (the scenarios of a tile back to
-   back, XCD-contiguous, lean PML kernel), 20 (interior as 19, lean PML
-   blocks in one slab-ordered sweep per scenario), 21 (both kernels in
-   per-scenario sweeps), and one direct tile x scenario kernel for 1 and 18.
-   Returns CGFD_ERR_ARG for any other value. */
+   kernel for the tiles touching a CFS-PML slab; 19 = 16 with a lean
+   PML-tile kernel that accumulates the directional rhs parts field by field;
+   20 and 21 = 19 on one scenario. Batches with more than one active
+   scenario run split interior / PML kernels for 16 (one slab-ordered sweep
+   per scenario), 19 (the scenarios of a tile back to back, XCD-contiguous,
+   lean PML kernel), 20 (interior as 19, lean PML blocks in one slab-ordered
+   sweep per scenario), 21 (both kernels in per-scenario sweeps), and one
+   direct tile x scenario kernel for 1. Returns CGFD_ERR_ARG for any other
+   value. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
 /* The schedule the core will run: variant and z-chunk (after variant 0 has
    been resolved). */
 int cgfd_core_get_variant(const cgfd_core *core, int *variant, int *zchunk);
 
-/* y-tiles per slab of the slab-ordered kernels (16, 18..21), 4..4096. */
+/* y-tiles per slab of the slab-ordered kernels (16, 19..21), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
-/* Autotuner: times every candidate (variant, z-chunk) pair -- 1, 16, 18
-   and 19 with z-chunks 4..32 on one scenario; 1, 16, 19, 20 and 21 on a
-   batch -- on `reps` repetitions of one step's four volume launches, keeps
+/* Autotuner: times every candidate (variant, z-chunk) pair -- 20, 16 and 1
+   on one scenario; 20, 19, 21, 16 and 1 on a batch; z-chunks 4..32 for the
+   slab-ordered ones -- on `reps` repetitions of one step's four volume launches, keeps
    the fastest and records it for this problem shape (grid, scenarios,
    faults, boundaries) so later cores of the same shape start with it, and
    appends it to the file named by CGFD_TUNE_CACHE if that is set; *ms_best

@@ -84,6 +84,21 @@ int cgfd_io_read_init_stress(const char *path, int nj, int nk, float *fields);
    builds the curvilinear grid around. Host only. */
 int cgfd_io_read_fault_coord(const char *path, int nj, int nk, float *xyz);
 
+/* The grid and media imports of the reference's main (grid_generation_method
+   "import", metric "import", media "import"), host only:
+   cgfd_io_read_coord: coord_<frame>.nc x, y, z -> xyz[3][nk][nj][ni] (the
+     leading block of larger variables, as gd_curv_coord_import, gd_t.cu:855-923);
+   cgfd_io_read_metric: metric_<frame>.nc jac, xi_x .. zeta_z ->
+     metric[10][nk][nj][ni] (gd_curv_metric_import, gd_t.cu:1015-1068);
+   cgfd_io_read_media: media_<frame>.nc lambda, mu, rho -> media[3][nk][nj][ni]
+     (md_import, md_t.cu:229-286; density, not slowness).
+   Variables may be float or double; the metric and media variables must be
+   exactly [nk][nj][ni], as the reference reads them whole. The outputs feed
+   cgfd_setup_params (x, y, z, lambda, mu, rho, metric). */
+int cgfd_io_read_coord(const char *path, int ni, int nj, int nk, float *xyz);
+int cgfd_io_read_metric(const char *path, int ni, int nj, int nk, float *metric);
+int cgfd_io_read_media(const char *path, int ni, int nj, int nk, float *media);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,15 @@ typedef struct cgfd_setup_params {
   float dt, t0;
   int nt_total;
   int device_id;
+  /* optional imported metric (metric_<frame>.nc, cgfd_io_read_metric):
+     jac, xi_x .. zeta_z, ni*nj*nk each, layout as x; NULL: computed from
+     the coordinates (gd_curv_metric_cal). Ghosts come from
+     geometric_symmetry either way (gd_curv_metric_import, gd_t.cu:1064). */
+  const float *metric;
+  /* 1: compute the metric and the fault coefficients with HIP kernels on
+     device_id (the same arithmetic; the results land in the setup's host
+     arrays as before); 0: on the host */
+  int device_setup;
 } cgfd_setup_params;
 
 typedef struct cgfd_setup cgfd_setup;

@@ -13,15 +13,45 @@ from pathlib import Path
 import numpy as np
 
 LIB_PATH = Path(__file__).resolve().parent / "_build" / "libcgfd_oracle.so"
+# the same restatement built -O3 for AVX2/FMA, timed as the CPU baseline only
+FAST_LIB_PATH = Path(__file__).resolve().parent / "_build" / "libcgfd_oracle_fast.so"
 _LIB = None
+_LIB_FILE = None
 
 
-def load():
-    global _LIB
+def _cpu_has_avx2_fma() -> bool:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("flags"):
+                f = set(line.split(":", 1)[1].split())
+                return {"avx2", "fma", "bmi2"} <= f
+    except OSError:
+        pass
+    return False
+
+
+def load_fast() -> bool:
+    """Loads the -O3 AVX2/FMA build for timing if nothing is loaded yet and
+    this CPU can run it; returns whether it is the loaded build."""
+    if _LIB is None and FAST_LIB_PATH.exists() and _cpu_has_avx2_fma():
+        load(FAST_LIB_PATH)
+    return _LIB_FILE == FAST_LIB_PATH
+
+
+def active_build() -> str:
+    if _LIB_FILE == FAST_LIB_PATH:
+        return "g++ -O3 -march=x86-64-v3 -fopenmp"
+    return "g++ -O2 -march=x86-64-v2 -ffp-contract=off -fopenmp"
+
+
+def load(path=None):
+    global _LIB, _LIB_FILE
     if _LIB is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} missing; build with __graft_entry__.build()")
-        lib = C.CDLL(str(LIB_PATH))
+        path = Path(path) if path is not None else LIB_PATH
+        if not path.exists():
+            raise RuntimeError(f"{path} missing; build with __graft_entry__.build()")
+        lib = C.CDLL(str(path))
+        _LIB_FILE = path
         vp, i32 = C.c_void_p, C.c_int
         sig = {
             "oracle_create": (vp, [vp, i32]),
@@ -39,6 +69,7 @@ def load():
             "oracle_read_freemats": (i32, [vp, vp]),
             "oracle_probe_friction": (i32, [vp, vp, i32]),
             "oracle_probe_w2f_f2w": (i32, [vp, C.c_double, vp, i32, vp, vp]),
+            "oracle_ad_gradient": (i32, [vp, i32, vp, i32, i32, vp, vp, i32, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -161,3 +192,29 @@ def probe_w2f_f2w(metric9, jac, stress6, fix_q1):
     tr = np.zeros(9, np.float64)
     load().oracle_probe_w2f_f2w(_p(m), float(jac), _p(s), int(bool(fix_q1)), _p(out), _p(tr))
     return out, tr.reshape(3, 3)
+
+
+def ad_gradient(desc, shape_ext, nfault, w0, nt, sta_ijk, res, rec_fjk=None, res_f=None, fix_q1=False):
+    """FP64 reverse-mode gradient of the oracle (oracle_ad_gradient): returns
+    (J, dJ/dw0 [9][nz][ny][nx], dJ/d(mu_s, mu_d, Dc, C0, T0x, T0y, T0z)
+    [nfault][7][nz][ny]). Small grids only (one thread, a tape of every
+    operation)."""
+    nz, ny, nx = shape_ext
+    w0 = np.ascontiguousarray(w0, np.float64)
+    sta = np.ascontiguousarray(sta_ijk, np.int32).reshape(-1, 3)
+    res = np.ascontiguousarray(res, np.float64)
+    assert res.shape == (sta.shape[0], nt, 9)
+    nrec = 0
+    rec = resf = None
+    if rec_fjk is not None:
+        rec = np.ascontiguousarray(rec_fjk, np.int32).reshape(-1, 3)
+        resf = np.ascontiguousarray(res_f, np.float64)
+        nrec = rec.shape[0]
+        assert resf.shape == (nrec, nt, 9)
+    gw = np.zeros((9, nz, ny, nx), np.float64)
+    gp = np.zeros((max(nfault, 1), 7, nz, ny), np.float64)
+    J = C.c_double()
+    load().oracle_ad_gradient(C.byref(desc), int(bool(fix_q1)), _p(w0), int(nt), sta.shape[0], _p(sta), _p(res), nrec,
+                              None if rec is None else _p(rec), None if resf is None else _p(resf), _p(gw), _p(gp),
+                              C.byref(J))
+    return J.value, gw, gp[:nfault]

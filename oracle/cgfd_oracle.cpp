@@ -16,21 +16,36 @@
  *
  * Each function names the reference lines it restates.
  *
- * What pins it (tests/test_oracle_pins.py, tests/test_abi.py): the interior
- * rhs, FD operators and RK4 composition by the closed-form step of cubic
- * fields; the metric by affine grids; friction by closed-form values at its
- * kinks; the free surface by sigma.n = 0 after surface rupture; the CFS-PML
- * by absorbed energy; the fault frame by orthonormality; FP32 vs FP64; the
- * trial traction and split-node velocity update by a locked fault that
- * transmits a pulse without slip.
- * Parity unpinned (no golden vectors exist and the reference cannot be
- * built here; these rest on line-by-line review and on agreement with the
- * independently written GPU kernels): the split-node stress formulas and
- * the free-surface rows of the fault kernels (fault_velo/stress_F/B), the
- * method-1/2 split-node matrices (fault_coef_cal), the trial traction's
- * dependence on the fault-plane derivatives under slip, wave2fault/
- * fault2wave including the Txz/Tyz swap, the one-step lag of hT1, and the
- * exact stage ordering.
+ * What pins it (tests/test_oracle_pins.py, tests/test_oracle_fault_pins.py,
+ * tests/test_setup_ref.py, tests/test_oracle_adjoint.py, tests/test_abi.py):
+ * the interior rhs, FD operators and RK4 composition by the closed-form step
+ * of cubic fields; the metric by affine grids; friction by closed-form
+ * values at its kinks; the free surface by sigma.n = 0 after surface
+ * rupture; the CFS-PML by absorbed energy; FP32 vs FP64; the trial traction
+ * assembly (its factor 2, the side signs, the DyT2/DzT3 terms) and the
+ * split-node velocity rows by a locked fault that transmits a 3-D pulse
+ * without slip; the slipping path (friction cap, DxT1 rows, fault2wave's
+ * side averages, slip/slip-rate/rupture-time accumulators, free-surface
+ * rows) by a uniform stress drop against the 1-D solution, strike-slip to
+ * the surface and buried dip-slip, both methods; the split-node stress rows
+ * with the method-1/2 matrices, method 2's T1 rate and the free-surface
+ * rows by a uniform strain rate reproduced exactly on Cartesian and sheared
+ * grids; wave2fault/fault2wave by their round trip, with and without the
+ * Txz/Tyz swap; the setup arrays the oracle reads (metric, PML profiles,
+ * fault matrices, masks) by an independent FP64 restatement
+ * (oracle/setup_ref.py) and closed forms, and the oracle runs on that
+ * restatement in the parity tests. Each fault pin has a planted-mistake
+ * control (plant, below) that fails it.
+ * Parity unpinned (line-by-line review against the cited lines only): the
+ * one-step lag of hT1 (a constant T1 rate, the only closed form available,
+ * cannot tell a lagged rate from a current one), and the values of method
+ * 2's top three split-node rows on a fault reaching the surface, where the
+ * reference takes no T1 rate outside its fault grid (the tests show the
+ * rows differ from the exact solution there, as the rule implies, but no
+ * closed form fixes them).
+ * oracle_ad_gradient differentiates the FP64 restatement in reverse mode
+ * (ad::Var below): the FP64 reference adjoint the GPU adjoint is checked
+ * against, itself pinned by FP64 central differences.
  */
 #include "../include/cgfd_core.h"
 
@@ -47,6 +62,84 @@
 #endif
 
 namespace {
+
+/* ---- reverse-mode differentiation of the FP64 restatement ----
+   Oracle<ad::Var> runs the same statements on values that record every
+   arithmetic operation on a tape (one node per result: up to two parents
+   and the local partial derivatives). A reverse sweep over the tape gives
+   the exact FP64 gradient of a scalar function of the outputs with respect
+   to the marked inputs, following the branches (friction law, guards) the
+   FP64 run takes. The tape is global: Oracle<ad::Var> runs on one thread. */
+namespace ad {
+struct Node {
+  int a, b;
+  double da, db;
+};
+std::vector<Node> tape;
+
+struct Var {
+  double v = 0.0;
+  int i = -1;  /* tape index; -1: a constant */
+  Var() = default;
+  Var(double x) : v(x), i(-1) {}  /* NOLINT: constants convert implicitly */
+};
+inline Var mk(double v, int a, double da, int b = -1, double db = 0.0) {
+  if (a < 0 && b < 0) return Var(v);
+  Var r(v);
+  r.i = (int)tape.size();
+  tape.push_back({a, b, da, db});
+  return r;
+}
+inline Var leaf(double v) {
+  Var r(v);
+  r.i = (int)tape.size();
+  tape.push_back({-1, -1, 0.0, 0.0});
+  return r;
+}
+inline Var operator+(Var x, Var y) { return mk(x.v + y.v, x.i, 1.0, y.i, 1.0); }
+inline Var operator-(Var x, Var y) { return mk(x.v - y.v, x.i, 1.0, y.i, -1.0); }
+inline Var operator*(Var x, Var y) { return mk(x.v * y.v, x.i, y.v, y.i, x.v); }
+inline Var operator/(Var x, Var y) { return mk(x.v / y.v, x.i, 1.0 / y.v, y.i, -x.v / (y.v * y.v)); }
+inline Var operator+(Var x, double y) { return mk(x.v + y, x.i, 1.0); }
+inline Var operator+(double x, Var y) { return mk(x + y.v, y.i, 1.0); }
+inline Var operator-(Var x, double y) { return mk(x.v - y, x.i, 1.0); }
+inline Var operator-(double x, Var y) { return mk(x - y.v, y.i, -1.0); }
+inline Var operator*(Var x, double y) { return mk(x.v * y, x.i, y); }
+inline Var operator*(double x, Var y) { return mk(x * y.v, y.i, x); }
+inline Var operator/(Var x, double y) { return mk(x.v / y, x.i, 1.0 / y); }
+inline Var operator/(double x, Var y) { return mk(x / y.v, y.i, -x / (y.v * y.v)); }
+inline Var operator-(Var x) { return mk(-x.v, x.i, -1.0); }
+inline Var &operator+=(Var &x, Var y) { return x = x + y; }
+inline Var &operator-=(Var &x, Var y) { return x = x - y; }
+inline Var &operator*=(Var &x, Var y) { return x = x * y; }
+inline Var &operator/=(Var &x, Var y) { return x = x / y; }
+inline bool operator<(Var x, Var y) { return x.v < y.v; }
+inline bool operator>(Var x, Var y) { return x.v > y.v; }
+inline bool operator<=(Var x, Var y) { return x.v <= y.v; }
+inline bool operator>=(Var x, Var y) { return x.v >= y.v; }
+inline bool operator==(Var x, Var y) { return x.v == y.v; }
+inline bool operator!=(Var x, Var y) { return x.v != y.v; }
+/* sqrt at 0 takes the derivative 0 (|v| of a zero vector) */
+inline Var sqrt(Var x) {
+  double r = std::sqrt(x.v);
+  return mk(r, x.i, r > 0.0 ? 0.5 / r : 0.0);
+}
+inline Var fabs(Var x) { return mk(std::fabs(x.v), x.i, x.v < 0.0 ? -1.0 : 1.0); }
+inline double val(Var x) { return x.v; }
+}  // namespace ad
+
+inline float xsqrt(float x) { return std::sqrt(x); }
+inline double xsqrt(double x) { return std::sqrt(x); }
+inline ad::Var xsqrt(ad::Var x) { return ad::sqrt(x); }
+inline float xabs(float x) { return std::fabs(x); }
+inline double xabs(double x) { return std::fabs(x); }
+inline ad::Var xabs(ad::Var x) { return ad::fabs(x); }
+inline double val(double x) { return x; }
+inline double val(float x) { return x; }
+using ad::val;
+/* the type the reference's FP64 intermediates take: double, or the AD type */
+template <class T> struct Wide { using type = double; };
+template <> struct Wide<ad::Var> { using type = ad::Var; };
 
 /* ---- fd_t.h:9-12 fault traction-image coefficients ---- */
 const double a_0 = 1.2500392377;
@@ -119,10 +212,10 @@ template <class T> inline T vecc(const T *p, int f) { return f == 1 ? vecc_f(p) 
 /* ---- src/lib/fdlib_math.cu ---- */
 template <class T> inline T norm3(const T *A) {  /* :4-11 */
   T x2 = A[0] * A[0], y2 = A[1] * A[1], z2 = A[2] * A[2];
-  return std::sqrt(x2 + y2 + z2);
+  return xsqrt(x2 + y2 + z2);
 }
 template <class T> inline void invert3x3(T m[][3]) {  /* :14-43, double cofactors */
-  double inv[3][3];
+  typename Wide<T>::type inv[3][3];
   inv[0][0] = (T)(m[1][1] * m[2][2] - m[2][1] * m[1][2]);
   inv[0][1] = (T)(m[2][1] * m[0][2] - m[0][1] * m[2][2]);
   inv[0][2] = (T)(m[0][1] * m[1][2] - m[0][2] * m[1][1]);
@@ -132,7 +225,7 @@ template <class T> inline void invert3x3(T m[][3]) {  /* :14-43, double cofactor
   inv[2][0] = (T)(m[1][0] * m[2][1] - m[1][1] * m[2][0]);
   inv[2][1] = (T)(m[2][0] * m[0][1] - m[0][0] * m[2][1]);
   inv[2][2] = (T)(m[0][0] * m[1][1] - m[0][1] * m[1][0]);
-  double det = inv[0][0] * m[0][0] + inv[0][1] * m[1][0] + inv[0][2] * m[2][0];
+  typename Wide<T>::type det = inv[0][0] * m[0][0] + inv[0][1] * m[1][0] + inv[0][2] * m[2][0];
   det = 1.0 / det;
   for (int i = 0; i < 3; i++)
     for (int j = 0; j < 3; j++) m[i][j] = (T)(inv[i][j] * det);
@@ -187,7 +280,7 @@ int friction_point(const T *Trial, T jacvec, const T *vn, const T *T0, T mus, T 
   else { Tau_s = Ts0; flag = 0; }
   T Tau[3];
   if (ifchange == 1) {
-    if (std::fabs(Ts0) < 1e-1) {
+    if (xabs(Ts0) < 1e-1) {
       for (int a = 0; a < 3; a++) Tau[a] = Tau_n * vn[a];
     } else {
       for (int a = 0; a < 3; a++) Tau[a] = Tau_n * vn[a] + (Tau_s / Ts0) * Ts[a];
@@ -454,6 +547,7 @@ struct Oracle {
       T *fT3x = fcmp(fcur, id, 6), *fT3y = fcmp(fcur, id, 7), *fT3z = fcmp(fcur, id, 8);
       T *fhT2x = fcmp(frhs, id, 3), *fhT2y = fcmp(frhs, id, 4), *fhT2z = fcmp(frhs, id, 5);
       T *fhT3x = fcmp(frhs, id, 6), *fhT3y = fcmp(frhs, id, 7), *fhT3z = fcmp(frhs, id, 8);
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -502,6 +596,7 @@ struct Oracle {
       T *fVx = fcmp(fl, id, 0), *fVy = fcmp(fl, id, 1), *fVz = fcmp(fl, id, 2);
       T *fT2x = fcmp(fl, id, 3), *fT2y = fcmp(fl, id, 4), *fT2z = fcmp(fl, id, 5);
       T *fT3x = fcmp(fl, id, 6), *fT3y = fcmp(fl, id, 7), *fT3z = fcmp(fl, id, 8);
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -546,6 +641,7 @@ struct Oracle {
       T *fT3x = fcmp(fcur, id, 6), *fT3y = fcmp(fcur, id, 7), *fT3z = fcmp(fcur, id, 8);
       T *mVx = fcmp(fpre, id, 0), *mVy = fcmp(fpre, id, 1), *mVz = fcmp(fpre, id, 2);
       T *T1x = F.T1x.data(), *T1y = F.T1y.data(), *T1z = F.T1z.data();
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -698,6 +794,7 @@ struct Oracle {
     const T *Ta[3][3] = {{Txx, Txy, Txz}, {Txy, Tyy, Tyz}, {Txz, Tyz, Tzz}};
     for (int k = k_min; k <= nk2; k++) {
       int n_free = nk2 - k - fz[0];
+#pragma omp parallel for schedule(static)
       for (int iy = 0; iy < nj; iy++)
         for (int ix = 0; ix < ni; ix++) {
           size_t iptr = (ix + ni1) + (iy + nj1) * siy + k * siz;
@@ -737,6 +834,7 @@ struct Oracle {
     T *hTyz = rhs + 6 * sicmp, *hTxz = rhs + 7 * sicmp, *hTxy = rhs + 8 * sicmp;
     for (int n = 0; n < 3; n++) {
       int k = nk2 - n;
+#pragma omp parallel for schedule(static)
       for (int iy = 0; iy < nj; iy++)
         for (int ix = 0; ix < ni; ix++) {
           size_t iptr = (ix + ni1) + (iy + nj1) * siy + k * siz;
@@ -800,6 +898,7 @@ struct Oracle {
         const T *my = dim == 0 ? xiy : (dim == 1 ? ety : zty);
         const T *mz = dim == 0 ? xiz : (dim == 1 ? etz : ztz);
         const T *mat = dim == 0 ? matVx2Vz.data() : matVy2Vz.data();
+#pragma omp parallel for schedule(static)
         for (int iz = 0; iz < f.cn; iz++)
           for (int iy = 0; iy < f.bn; iy++)
             for (int ix = 0; ix < f.an; ix++) {
@@ -888,6 +987,7 @@ struct Oracle {
       T *fT3[3] = {fcmp(fcur, id, 6), fcmp(fcur, id, 7), fcmp(fcur, id, 8)};
       T *fhV[3] = {fcmp(frhs, id, 0), fcmp(frhs, id, 1), fcmp(frhs, id, 2)};
       T *T1[3] = {F.T1x.data(), F.T1y.data(), F.T1z.data()};
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -997,6 +1097,7 @@ struct Oracle {
       T *fV[3] = {fcmp(fcur, id, 0), fcmp(fcur, id, 1), fcmp(fcur, id, 2)};
       T *fhT2[3] = {fcmp(frhs, id, 3), fcmp(frhs, id, 4), fcmp(frhs, id, 5)};
       T *fhT3[3] = {fcmp(frhs, id, 6), fcmp(frhs, id, 7), fcmp(frhs, id, 8)};
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -1229,6 +1330,7 @@ struct Oracle {
     for (int id = 0; id < nfault; id++) {
       Fault &F = flt[id];
       size_t off = (size_t)id * f_siz_ilevel;
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -1257,6 +1359,7 @@ struct Oracle {
   void stage_update(T c, int istage) {
     for (int id = 0; id < nfault; id++) {
       Fault &F = flt[id];
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -1272,6 +1375,7 @@ struct Oracle {
     for (int id = 0; id < nfault; id++) {
       Fault &F = flt[id];
       T *fVx = fcmp(fend, id, 0), *fVy = fcmp(fend, id, 1), *fVz = fcmp(fend, id, 2);
+#pragma omp parallel for schedule(static)
       for (int iz = 0; iz < nk; iz++)
         for (int iy = 0; iy < nj; iy++) {
           size_t pf = (iy + nj1) + (iz + nk1) * ny;
@@ -1281,7 +1385,7 @@ struct Oracle {
           const T *s1 = &F.vec_s1[pf * 3], *s2 = &F.vec_s2[pf * 3];
           T Vs1 = dVx * s1[0] + dVy * s1[1] + dVz * s1[2];
           T Vs2 = dVx * s2[0] + dVy * s2[1] + dVz * s2[2];
-          T Vs = std::sqrt(Vs2 * Vs2 + Vs1 * Vs1);
+          T Vs = xsqrt(Vs2 * Vs2 + Vs1 * Vs1);
           F.Vs[pf] = Vs; F.Vs1[pf] = Vs1; F.Vs2[pf] = Vs2;
           F.Slip[pf] += Vs * dt; F.Slip1[pf] += Vs1 * dt; F.Slip2[pf] += Vs2 * dt;
           size_t p3 = pf + 3 * sslice;
@@ -1314,27 +1418,27 @@ struct Oracle {
       for (int ix = 0; ix < ni; ix++) {
         size_t iptr = (ix + ni1) + (iy + nj1) * siy + nk2 * siz;
         size_t i1 = (ix + ni1) + (iy + nj1) * siy;
-        T Ax = std::fabs((Vx1[iptr] - Vx0[iptr]) / dt);
-        T Ay = std::fabs((Vy1[iptr] - Vy0[iptr]) / dt);
-        T Az = std::fabs((Vz1[iptr] - Vz0[iptr]) / dt);
+        T Ax = xabs((Vx1[iptr] - Vx0[iptr]) / dt);
+        T Ay = xabs((Vy1[iptr] - Vy0[iptr]) / dt);
+        T Az = xabs((Vz1[iptr] - Vz0[iptr]) / dt);
         Dx[i1] += (T)(0.5 * (Vx1[iptr] + Vx0[iptr]) * dt);
         Dy[i1] += (T)(0.5 * (Vy1[iptr] + Vy0[iptr]) * dt);
         Dz[i1] += (T)(0.5 * (Vz1[iptr] + Vz0[iptr]) * dt);
-        T Vv = std::sqrt(Vx1[iptr] * Vx1[iptr] + Vy1[iptr] * Vy1[iptr] + Vz1[iptr] * Vz1[iptr]);
-        T Vh = std::sqrt(Vx1[iptr] * Vx1[iptr] + Vy1[iptr] * Vy1[iptr]);
-        T A = std::sqrt(Ax * Ax + Ay * Ay + Az * Az);
-        T Ah = std::sqrt(Ax * Ax + Ay * Ay);
-        T Dd = std::sqrt(Dx[i1] * Dx[i1] + Dy[i1] * Dy[i1] + Dz[i1] * Dz[i1]);
-        T Dh = std::sqrt(Dx[i1] * Dx[i1] + Dy[i1] * Dy[i1]);
-        if (PGVx[i1] < std::fabs(Vx1[iptr])) PGVx[i1] = std::fabs(Vx1[iptr]);
-        if (PGVy[i1] < std::fabs(Vy1[iptr])) PGVy[i1] = std::fabs(Vy1[iptr]);
-        if (PGVz[i1] < std::fabs(Vz1[iptr])) PGVz[i1] = std::fabs(Vz1[iptr]);
+        T Vv = xsqrt(Vx1[iptr] * Vx1[iptr] + Vy1[iptr] * Vy1[iptr] + Vz1[iptr] * Vz1[iptr]);
+        T Vh = xsqrt(Vx1[iptr] * Vx1[iptr] + Vy1[iptr] * Vy1[iptr]);
+        T A = xsqrt(Ax * Ax + Ay * Ay + Az * Az);
+        T Ah = xsqrt(Ax * Ax + Ay * Ay);
+        T Dd = xsqrt(Dx[i1] * Dx[i1] + Dy[i1] * Dy[i1] + Dz[i1] * Dz[i1]);
+        T Dh = xsqrt(Dx[i1] * Dx[i1] + Dy[i1] * Dy[i1]);
+        if (PGVx[i1] < xabs(Vx1[iptr])) PGVx[i1] = xabs(Vx1[iptr]);
+        if (PGVy[i1] < xabs(Vy1[iptr])) PGVy[i1] = xabs(Vy1[iptr]);
+        if (PGVz[i1] < xabs(Vz1[iptr])) PGVz[i1] = xabs(Vz1[iptr]);
         if (PGAx[i1] < Ax) PGAx[i1] = Ax;
         if (PGAy[i1] < Ay) PGAy[i1] = Ay;
         if (PGAz[i1] < Az) PGAz[i1] = Az;
-        if (PGDx[i1] < std::fabs(Dx[i1])) PGDx[i1] = std::fabs(Dx[i1]);
-        if (PGDy[i1] < std::fabs(Dy[i1])) PGDy[i1] = std::fabs(Dy[i1]);
-        if (PGDz[i1] < std::fabs(Dz[i1])) PGDz[i1] = std::fabs(Dz[i1]);
+        if (PGDx[i1] < xabs(Dx[i1])) PGDx[i1] = xabs(Dx[i1]);
+        if (PGDy[i1] < xabs(Dy[i1])) PGDy[i1] = xabs(Dy[i1]);
+        if (PGDz[i1] < xabs(Dz[i1])) PGDz[i1] = xabs(Dz[i1]);
         if (PGV[i1] < Vv) PGV[i1] = Vv;
         if (PGVh[i1] < Vh) PGVh[i1] = Vh;
         if (PGA[i1] < A) PGA[i1] = A;
@@ -1561,6 +1665,82 @@ int oracle_probe_w2f_f2w(const double *metric9, double jacv, const double *stres
   /* the Txz argument of the kernel receives the Tyz array unless fixed */
   if (fix_q1) { out6[4] = sv[4]; out6[3] = sv[5]; }
   else { out6[3] = sv[4]; out6[4] = sv[5]; }
+  return 0;
+}
+
+/* FP64 reverse-mode gradient of
+     J = sum_t sum_s res[s][t] . w_t(station s)
+       + sum_t sum_r res_f[r][t][3..8] . (Vs, Vs1, Vs2, Slip, Slip1, Slip2)_t(receiver r)
+   over nt steps from the initial wavefield w0 (host layout [9][nz][ny][nx]),
+   where w_t is the wavefield and out_t the fault outputs at the start of
+   step t (what io_recv_keep / io_fault_recv_keep record), with respect to
+   w0 (g_w0, same layout) and to mu_s, mu_d, Dc, C0, T0x, T0y, T0z of every
+   fault node (g_par[nfault][7][sslice]). sta_ijk: ghost-inclusive (i,j,k)
+   per station; rec_fjk: (fault id, ghost-inclusive j, k) per receiver;
+   res_f may be null. Runs on one thread; the tape needs ~24 bytes per
+   arithmetic operation of the run, so only small grids. */
+int oracle_ad_gradient(const cgfd_desc *d, int fix_q1, const double *w0, int nt, int nsta, const int *sta_ijk,
+                       const double *res, int nrec, const int *rec_fjk, const double *res_f, double *g_w0,
+                       double *g_par, double *J) {
+#ifdef _OPENMP
+  const int nthr = omp_get_max_threads();
+  omp_set_num_threads(1);
+#endif
+  ad::tape.clear();
+  {
+    Oracle<ad::Var> o(d, fix_q1);
+    const size_t nw = 9 * o.sicmp, ss = o.sslice;
+    std::vector<int> iw(nw), ip((size_t)o.nfault * 7 * ss);
+    for (size_t i = 0; i < nw; i++) {
+      o.w_pre[i] = ad::leaf(w0[i]);
+      iw[i] = o.w_pre[i].i;
+    }
+    for (int id = 0; id < o.nfault; id++) {
+      auto &F = o.flt[id];
+      std::vector<ad::Var> *par[7] = {&F.mu_s, &F.mu_d, &F.Dc, &F.C0, &F.T0x, &F.T0y, &F.T0z};
+      for (int q = 0; q < 7; q++)
+        for (size_t p = 0; p < ss; p++) {
+          (*par[q])[p] = ad::leaf((*par[q])[p].v);
+          ip[((size_t)id * 7 + q) * ss + p] = (*par[q])[p].i;
+        }
+    }
+    ad::Var Jv(0.0);
+    for (int t = 0; t < nt; t++) {
+      for (int sidx = 0; sidx < nsta; sidx++) {
+        const size_t iptr = sta_ijk[3 * sidx] + sta_ijk[3 * sidx + 1] * o.siy + (size_t)sta_ijk[3 * sidx + 2] * o.siz;
+        for (int c = 0; c < 9; c++) {
+          const double r = res[((size_t)sidx * nt + t) * 9 + c];
+          if (r != 0.0) Jv += r * o.w_pre[c * o.sicmp + iptr];
+        }
+      }
+      for (int r = 0; res_f && r < nrec; r++) {
+        auto &F = o.flt[rec_fjk[3 * r]];
+        const size_t pf = rec_fjk[3 * r + 1] + (size_t)rec_fjk[3 * r + 2] * o.ny;
+        for (int v = 3; v < 9; v++) {
+          const double w = res_f[((size_t)r * nt + t) * 9 + v];
+          if (w != 0.0) Jv += w * F.out[v * ss + pf];
+        }
+      }
+      o.step();
+    }
+    *J = Jv.v;
+    std::vector<double> adj(ad::tape.size(), 0.0);
+    if (Jv.i >= 0) adj[Jv.i] = 1.0;
+    for (long n = (long)ad::tape.size() - 1; n >= 0; n--) {
+      const double g = adj[n];
+      if (g == 0.0) continue;
+      const ad::Node &nd = ad::tape[n];
+      if (nd.a >= 0) adj[nd.a] += nd.da * g;
+      if (nd.b >= 0) adj[nd.b] += nd.db * g;
+    }
+    for (size_t i = 0; i < nw; i++) g_w0[i] = adj[iw[i]];
+    for (size_t i = 0; i < ip.size(); i++) g_par[i] = adj[ip[i]];
+  }
+  ad::tape.clear();
+  ad::tape.shrink_to_fit();
+#ifdef _OPENMP
+  omp_set_num_threads(nthr);
+#endif
   return 0;
 }
 

@@ -25,3 +25,18 @@ def built_libs():
     if build.oracle_is_stale():
         build.build_oracle()
     return build.LIB_PATH, build.ORACLE_LIB
+
+
+def pytest_runtest_logstart(nodeid, location):
+    """On the GPU box, note each test's start in gpurun_out/test_progress.log
+    (a run shows it is alive even while pytest captures the output)."""
+    root = os.environ.get("GRAFT_REPO_ROOT")
+    if not root:
+        return
+    try:
+        d = os.path.join(root, "gpurun_out")
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, "test_progress.log"), "a") as f:
+            f.write(f"start {nodeid}\n")
+    except OSError:
+        pass

@@ -17,3 +17,20 @@ def rel_l2(a, b):
 def fault_phys(f, g=3):
     """Physical nodes of fault-plane arrays (last two axes nz, ny)."""
     return f[..., g:-g, g:-g]
+
+
+def progress(msg):
+    """A progress line: printed, and on the GPU box also appended to
+    gpurun_out/test_progress.log, so a long test shows it is alive even
+    when pytest captures its output."""
+    import os
+    print(msg, flush=True)
+    root = os.environ.get("GRAFT_REPO_ROOT")
+    if root:
+        try:
+            d = os.path.join(root, "gpurun_out")
+            os.makedirs(d, exist_ok=True)
+            with open(os.path.join(d, "test_progress.log"), "a") as f:
+                f.write(msg + "\n")
+        except OSError:
+            pass

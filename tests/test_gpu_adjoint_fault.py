@@ -10,11 +10,16 @@ through the C ABI (cgfd_adj_forward / cgfd_adj_backward_fault).
 * Rupturing fault: J_tr = 1/2 sum (T(theta) - target)^2 over the station
   traces and J_vs = 1/2 sum (V_s(theta) - V_s target)^2 over the slip rate
   at fault receivers, as functions of the friction parameters theta; the
-  adjoint gradients along seeded smooth directions of
-  mu_s, mu_d, Dc and the initial shear traction against FP64 central
-  differences of J computed by the FP64 oracle. The friction law is
-  differentiated on the branch the FP32 forward run took; the difference
-  quotient is of the FP64 discrete map.
+  adjoint gradients along 20 seeded smooth directions each of mu_s, mu_d,
+  Dc and the initial shear traction against FP64 central differences of J
+  computed by the FP64 oracle, to 1e-3. The friction law is differentiated
+  on the branch the FP32 forward run took and the difference quotient is of
+  the FP64 discrete map, which is piecewise smooth: a step that moves a
+  node's rupture onset or its crossing of Dc by one time step inside the
+  difference interval adds a jump. Steps of about 1e-6 of each parameter
+  (a few FP32 units in the last place; the quotient uses the perturbation
+  actually taken) make that vanishingly unlikely, and FP64 keeps the
+  quotient accurate to ~1e-9.
 * A core with a fault and no tape is refused.
 """
 import ctypes as C
@@ -86,11 +91,10 @@ def test_fault_adjoint_linear_identity_locked(built_libs, imethod, nfault, free_
 
 
 def _traces_fp64(sc, nt, frec):
-    """Station traces and fault-receiver slip rates of the FP64 oracle."""
-    from cgfd import model
+    """Station traces and fault-receiver slip rates of the FP64 oracle (on
+    its own setup of the scenario)."""
     import oracle
-    st = model.Setup(sc)
-    orc = oracle.Oracle(st.desc, sc.shape_ext, len(sc.faults), double=True)
+    orc = oracle.Oracle.from_scenario(sc, double=True)
     ijk = sc.stations + 3
     rec, vs = [], []
     for _ in range(nt):
@@ -100,8 +104,39 @@ def _traces_fp64(sc, nt, frec):
         vs.append([f[3, k, j] for (_, j, k) in frec])
         orc.run(1)
     orc.close()
-    st.close()
     return np.transpose(np.array(rec, np.float64), (1, 0, 2)), np.array(vs, np.float64).T
+
+
+def _j_fp64(sc, name, d, eps, nt, frec, target, vs_target):
+    """(J_traces, J_slip_rate) of the FP64 oracle with `name` moved by eps d
+    (runs in a worker process)."""
+    import copy
+    s2 = copy.deepcopy(sc)
+    s2.faults[0].fields[name] = (s2.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32)
+    rec, vs = _traces_fp64(s2, nt, frec)
+    return np.array([0.5 * float(np.sum((rec - target) ** 2)), 0.5 * float(np.sum((vs[:, 1:] - vs_target[:, 1:]) ** 2))])
+
+
+def _smooth_branch_mask(sc, nt, fr_out_gpu, rel=1e-4):
+    """Fault nodes at which the friction law stays on one smooth branch for
+    differentiation: over the FP64 run, Slip is never within rel * Dc of Dc
+    at a step (the mu_s/mu_d switch of the weakening law) and the rupture
+    onset step is the one the FP32 GPU run took. At the other nodes the FP64
+    map the difference quotient sees and the FP32 branches the adjoint
+    follows can take different sides of a kink."""
+    import oracle
+    orc = oracle.Oracle.from_scenario(sc, double=True)
+    dc = sc.faults[0].fields["Dc"].astype(np.float64)
+    near = np.zeros((sc.nk, sc.nj), bool)
+    for _ in range(nt):
+        orc.run(1)
+        slip = orc.read_fault(0)[6, 3:-3, 3:-3]
+        near |= np.abs(slip - dc) < rel * dc
+    t0_64 = orc.read_fault(0)[10, 3:-3, 3:-3]
+    orc.close()
+    t0_32 = fr_out_gpu[10, 3:-3, 3:-3].astype(np.float64)
+    same_onset = np.abs(t0_64 - t0_32) < 0.5 * sc.dt
+    return (~near & same_onset).astype(np.float64)
 
 
 def _fault_receivers(sc):
@@ -123,8 +158,10 @@ def _direction(sc, name, seed):
     return d
 
 
-# parameter, step size of the difference quotient (units of the parameter)
-STEPS = [("mu_s", 2e-4), ("mu_d", 2e-4), ("Dc", 2e-5), ("Ty", 2e3)]
+# parameter, step size of the difference quotient (units of the parameter):
+# about 1e-6 of mu (0.5-0.7), Dc (0.1 m) and the shear traction (~70 MPa)
+STEPS = [("mu_s", 1e-6), ("mu_d", 1e-6), ("Dc", 1e-7), ("Ty", 70.0)]
+NDIR = 20
 
 
 @pytest.mark.parametrize("imethod", [2, 1])
@@ -163,26 +200,41 @@ def test_fault_adjoint_parameter_gradient_vs_fp64(built_libs, imethod):
     _, gf_vs = core.adjoint_fault(np.zeros_like(r), rf)
     assert np.isfinite(gf_tr).all() and np.isfinite(gf_vs).all()
 
-    def j_fp64(name, d, eps):
-        s2 = copy.deepcopy(sc)
-        s2.faults[0].fields[name] = (s2.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32)
-        rec, vs = _traces_fp64(s2, nt, frec)
-        return np.array([0.5 * float(np.sum((rec - target) ** 2)), 0.5 * float(np.sum((vs[:, 1:] - vs_target[:, 1:]) ** 2))])
-
-    worst = 0.0
+    import fdpool
+    mask = _smooth_branch_mask(sc, nt, fr_out_gpu=core.read_fault(0))
+    print(f"[fault adjoint method {imethod}] directions restricted to {int(mask.sum())} of "
+          f"{int((_direction(sc, 'Dc', 1) != 0).sum())} fault-grid nodes (the rest cross Dc within 1e-4 Dc of a "
+          f"step or start slipping at a different step in FP32 and FP64)", flush=True)
+    jobs, dirs = [], []
     for q, (name, eps) in enumerate(STEPS):
-        d = _direction(sc, name, 200 + q)
-        # the step actually taken in FP32 parameters
-        plus = (sc.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32).astype(np.float64)
-        minus = (sc.faults[0].fields[name].astype(np.float64) - eps * d).astype(np.float32).astype(np.float64)
-        fd = j_fp64(name, d, eps) - j_fp64(name, d, -eps)
-        for t, (lab, gf) in enumerate((("traces", gf_tr), ("slip rate", gf_vs))):
-            ad = float(np.sum(gf[0, PARAMS.index(name)].astype(np.float64) * (plus - minus)))
-            rel = abs(ad - fd[t]) / max(abs(fd[t]), 1e-300)
-            worst = max(worst, rel)
-            print(f"[fault adjoint method {imethod}] {lab} d{name}: adjoint {ad:.8e}, FP64 central difference "
-                  f"{fd[t]:.8e}, rel {rel:.2e}", flush=True)
-    assert worst <= 2e-2
+        for k in range(NDIR):
+            d = _direction(sc, name, 200 + 100 * q + k) * mask
+            dirs.append((q, name, eps, k, d))
+            for sgn in (1.0, -1.0):
+                jobs.append((sc, name, d, sgn * eps, nt, frec, target, vs_target))
+    print(f"[fault adjoint method {imethod}] {len(jobs)} FP64 oracle runs ...", flush=True)
+    Js = fdpool.run_all(_j_fp64, jobs)
+    worst = 0.0
+    errs_by = {}
+    for n, (q, name, eps, k, d) in enumerate(dirs):
+        errs = errs_by.setdefault(name, [])
+        if True:
+            # the step actually taken in FP32 parameters
+            plus = (sc.faults[0].fields[name].astype(np.float64) + eps * d).astype(np.float32).astype(np.float64)
+            minus = (sc.faults[0].fields[name].astype(np.float64) - eps * d).astype(np.float32).astype(np.float64)
+            fd = Js[2 * n] - Js[2 * n + 1]
+            for t, (lab, gf) in enumerate((("traces", gf_tr), ("slip rate", gf_vs))):
+                ad = float(np.sum(gf[0, PARAMS.index(name)].astype(np.float64) * (plus - minus)))
+                rel = abs(ad - fd[t]) / max(abs(fd[t]), 1e-300)
+                errs.append(rel)
+                if k < 2:
+                    print(f"[fault adjoint method {imethod}] {lab} d{name} direction {k}: adjoint {ad:.8e}, FP64 "
+                          f"central difference {fd[t]:.8e}, rel {rel:.2e}", flush=True)
+    for name, errs in errs_by.items():
+        print(f"[fault adjoint method {imethod}] d{name}: {NDIR} directions x 2 functionals, worst rel "
+              f"{max(errs):.2e}, median {np.median(errs):.2e}", flush=True)
+        worst = max(worst, max(errs))
+    assert worst <= 1e-3
     core.close()
     st.close()
 

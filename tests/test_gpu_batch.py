@@ -29,7 +29,7 @@ def _plain_with_fields(sc, fields):
     return model.Setup(sc2)
 
 
-CASES = [(v, "default") for v in (1, 16, 18, 19, 20, 21)] + [(20, "first_retires"), (18, "first_retires"),
+CASES = [(v, "default") for v in (1, 16, 19, 20, 21)] + [(20, "first_retires"), (1, "first_retires"),
                                                                 (20, "ragged_slab"), (21, "ragged_slab")]
 
 

@@ -104,8 +104,14 @@ def test_init_stress_reader(built_libs, tmp_path):
     assert rc == 0, lib.cgfd_last_error()
     for q, n in enumerate(("Tx", "Ty", "Tz", "mu_s", "mu_d", "Dc", "C0")):
         assert np.array_equal(out[q], vals[n]), n
+    # the reference reads the leading nk x nj block (nc_get_vara_float from
+    # start 0): a smaller grid reads a block, a larger one is an error
+    small = np.zeros((7, nk - 2, nj - 3), np.float32)
+    rc = lib.cgfd_io_read_init_stress(str(path).encode(), nj - 3, nk - 2, small.ctypes.data_as(C.c_void_p))
+    assert rc == 0, lib.cgfd_last_error()
+    assert np.array_equal(small[1], vals["Ty"][:nk - 2, :nj - 3])
     rc = lib.cgfd_io_read_init_stress(str(path).encode(), nj + 1, nk, out.ctypes.data_as(C.c_void_p))
-    assert rc == -1 and b"not [9][18]" in lib.cgfd_last_error()
+    assert rc == -1 and b"smaller than the grid" in lib.cgfd_last_error()
 
 
 def test_fault_coord_reader(built_libs, tmp_path):
