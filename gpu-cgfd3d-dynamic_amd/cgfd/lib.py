@@ -131,6 +131,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_load_state": (I32, [VP, VP, I32]),
         "cgfd_core_autotune": (I32, [VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(F32)]),
         "cgfd_core_get_variant": (I32, [VP, C.POINTER(I32), C.POINTER(I32)]),
+        "cgfd_core_read_snapshot_noalloc": (I32, [VP, VP, I32, VP]),
         "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),
         "cgfd_batch_create": (I32, [C.POINTER(Desc), I32, C.POINTER(VP)]),
         "cgfd_batch_active": (I32, [VP]),

@@ -410,6 +410,7 @@ static int check_batch(const cgfd_desc *ds, int nscen) {
 }
 
 static void apply_auto_variant(cgfd_core *c);
+static int scratch_setup(int dev);
 
 static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   if (!out) return cgfd_set_error(CGFD_ERR_ARG, "null output pointer");
@@ -430,6 +431,7 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   auto fail = [&](int code) { cgfd_core_destroy(c); return code; };
   c->dev = d->device_id;
   if (hipSetDevice(c->dev) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "hipSetDevice failed"));
+  if ((rc = scratch_setup(c->dev))) return fail(rc);
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
     return fail(cgfd_set_error(CGFD_ERR_HIP, "hipStreamCreate failed"));
   for (int q = 0; q < 2; q++)
@@ -1365,6 +1367,51 @@ extern "C" int cgfd_batch_read_snapshot(cgfd_core *c, int s, const int *box, int
   HIPCHK(hipMalloc(&buf, n * 4));
   hipError_t e = snapshot_copy(c, s, b, flags, n, buf, dst);
   (void)hipFree(buf);
+  if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "snapshot: %s", hipGetErrorString(e));
+  return CGFD_OK;
+}
+
+/* ---- the allocation-free snapshot reader ----
+   The packing buffer is this file-scope device array. Every GPU the library
+   runs on holds its own copy (32 MiB) until the process ends; its address on
+   a device comes from hipGetSymbolAddress with that device current (at core
+   creation), and host code only ever uses that address. All calls on one
+   GPU share the array, so calls that overlap on one GPU race; nothing in it
+   needs clearing (a call writes every value it copies out). */
+constexpr size_t kScratchFloats = (size_t)8 << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+static float *g_scratch[64] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_setup(int dev) {
+  if (dev < 0 || dev >= 64) return cgfd_set_error(CGFD_ERR_ARG, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return CGFD_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return cgfd_set_error(CGFD_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_scratch[dev] = (float *)p;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_snapshot_noalloc(cgfd_core *c, const int *box, int flags, float *dst) {
+  if (int rc = check_scen(c, 0)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "bad snapshot arguments");
+  IoBox b;
+  size_t n = 0;
+  if (int rc = snapshot_box(c, box, flags, b, n)) return rc;
+  if (n > kScratchFloats)
+    return cgfd_set_error(CGFD_ERR_ARG, "snapshot of %zu values exceeds the allocation-free maximum of %zu (32 MiB)", n,
+                          kScratchFloats);
+  HIPCHK(hipSetDevice(c->dev));
+  float *buf = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    buf = c->dev >= 0 && c->dev < 64 ? g_scratch[c->dev] : nullptr;
+  }
+  if (!buf) return cgfd_set_error(CGFD_ERR_STATE, "scratch of device %d not set up", c->dev);
+  HIPCHK(hipStreamSynchronize(c->st));
+  const hipError_t e = snapshot_copy(c, 0, b, flags, n, buf, dst);
   if (e != hipSuccess) return cgfd_set_error(CGFD_ERR_HIP, "snapshot: %s", hipGetErrorString(e));
   return CGFD_OK;
 }

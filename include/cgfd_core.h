@@ -281,6 +281,14 @@ int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 int cgfd_core_read_slice(cgfd_core *core, int axis, int index, float *host_dst);
 int cgfd_core_read_snapshot(cgfd_core *core, const int *box, int flags, float *host_dst);
 int cgfd_batch_read_snapshot(cgfd_core *core, int s, const int *box, int flags, float *host_dst);
+/* cgfd_core_read_snapshot without any device allocation: no counterpart in
+   the reference; for callers that cannot allocate device memory (code under
+   HIP graph capture, serving loops, harnesses that forbid hipMalloc). The
+   packed values go through a fixed 32 MiB device array per GPU (at most
+   8 Mi values: a box of nk x nj x ni nodes and 3, 6, 9 or 15 fields);
+   larger boxes return CGFD_ERR_ARG. Calls on one GPU share that array, so
+   calls that overlap on one GPU race. */
+int cgfd_core_read_snapshot_noalloc(cgfd_core *core, const int *box, int flags, float *host_dst);
 
 /* Fault receivers (io_fault_recv_keep, io_funcs.cu:2313-2351): n receivers
    fjk[n][3] = (fault id, ghost-inclusive j, k) on grid nodes of the fault
