@@ -84,6 +84,18 @@ def calib_copy(mbytes: int, reps: int = 1) -> float:
     return gbs
 
 
+def stream_ceiling(ni: int, nj: int, nk: int, reps: int = 5) -> float:
+    """GB/s of a kernel that streams an RK stage-1 point's 39 reads and 18
+    writes over the same grid with no stencil (cgfd_calib_stream)."""
+    import ctypes as C
+    from cgfd import lib as L
+    lib = L.load_library()
+    g = C.c_float()
+    L.check(lib, lib.cgfd_calib_stream(0, ni, nj, nk, 39, 18, reps, C.byref(g)))
+    log(f"[bench] stage-1 data movement without the stencil, {ni}x{nj}x{nk}: {g.value:.0f} GB/s")
+    return float(g.value)
+
+
 def _cpu_model() -> str:
     try:
         for line in Path("/proc/cpuinfo").read_text().splitlines():
@@ -397,6 +409,9 @@ def main() -> int:
     achieved = bytes_vol / (ms_vol / 1e3) / 1e9 if ms_vol > 0 else 0.0
     traffic = read_traffic(REPO / "profiles")
     copy_gbs = None if args.no_copy_ceiling else calib_copy(2048, reps=5)
+    core.close()
+    st.close()
+    stream_gbs = None if args.no_copy_ceiling else stream_ceiling(ni, nj, nk)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -423,12 +438,15 @@ def main() -> int:
                      "traffic_source": "profiles/k_vol_config2_pmc.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE of the "
                                        "volume dispatches, calibrated with a device copy (not measured in this run)",
                      "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
-                     "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None},
+                     "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None,
+                     "stream_ceiling_gbs": round(stream_gbs, 1) if stream_gbs else None,
+                     "frac_of_stream_ceiling": round(achieved / stream_gbs, 4) if stream_gbs else None,
+                     "stream_ceiling_note": "cgfd_calib_stream: 39 reads + 18 writes per point (an RK stage-1 "
+                                            "point's arrays) with no stencil, same grid and tiles, measured in "
+                                            "this run"},
     }
     if setup_leg:
         out["setup"] = setup_leg
-    core.close()
-    st.close()
     del core, st, sc
     if not args.no_batched:
         log("[bench] batched ensemble ...")

@@ -23,8 +23,9 @@ ORACLE_DIR = REPO / "oracle"
 ORACLE_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle.so"
 ORACLE_FAST_LIB = ORACLE_DIR / "_build" / "libcgfd_oracle_fast.so"
 
-HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip", "k_setup.hip"]
-HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp", "cgfd_nc.cpp", "cgfd_sac.cpp"]
+HIP_SOURCES = ["core.hip", "k_volume.hip", "k_fault.hip", "k_adjoint.hip", "k_adj_fault.hip", "k_io.hip", "k_setup.hip",
+               "k_calib.hip"]
+HOST_SOURCES = ["cgfd_setup.cpp", "cgfd_errors.cpp", "cgfd_nc.cpp", "cgfd_sac.cpp", "cgfd_stations.cpp"]
 
 # Packed FP32 instructions stay disabled: hipcc otherwise forms v_pk_*_f32
 # from scalar float code.

@@ -18,6 +18,7 @@ NCMP = 9
 NFAULT_OUT = 11
 NPG = 15
 COMPAT_FIX_F2W_SWAP = 1
+COMPAT_FIX_STATION_INTERP = 2
 
 ERR = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_NOMEM", -4: "ERR_STATE", -5: "ERR_NAN", -6: "ERR_NODEV"}
 
@@ -69,6 +70,21 @@ class SetupParams(C.Structure):
         ("dt", F32), ("t0", F32), ("nt_total", I32), ("device_id", I32),
         ("metric", VP), ("device_setup", I32),
     ]
+
+
+STATION_NAME_MAX = 256
+
+
+class Station(C.Structure):
+    """cgfd_station (include/cgfd_io.h)"""
+    _fields_ = [("name", C.c_char * STATION_NAME_MAX), ("i", I32), ("j", I32), ("k", I32),
+                ("di", F32), ("dj", F32), ("dk", F32), ("x", F32), ("y", F32), ("z", F32)]
+
+
+class FaultStation(C.Structure):
+    """cgfd_fault_station (include/cgfd_io.h)"""
+    _fields_ = [("name", C.c_char * STATION_NAME_MAX), ("fault_id", I32), ("j", I32), ("k", I32),
+                ("dj", F32), ("dk", F32), ("x", F32), ("y", F32), ("z", F32)]
 
 
 _LIB = None
@@ -130,6 +146,11 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_save_state": (I32, [VP, VP, I32]),
         "cgfd_core_load_state": (I32, [VP, VP, I32]),
         "cgfd_core_autotune": (I32, [VP, I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(F32)]),
+        "cgfd_calib_stream": (I32, [I32, I32, I32, I32, I32, I32, I32, C.POINTER(F32)]),
+        "cgfd_core_set_stations_interp": (I32, [VP, I32, VP, VP, I32]),
+        "cgfd_core_set_fault_receivers_interp": (I32, [VP, I32, VP, VP, I32]),
+        "cgfd_io_read_stations": (I32, [C.c_char_p, VP, VP, I32, C.POINTER(I32), C.POINTER(I32)]),
+        "cgfd_io_read_fault_stations": (I32, [C.c_char_p, VP, VP, I32, C.POINTER(I32)]),
         "cgfd_core_get_variant": (I32, [VP, C.POINTER(I32), C.POINTER(I32)]),
         "cgfd_core_read_snapshot_noalloc": (I32, [VP, VP, I32, VP]),
         "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),

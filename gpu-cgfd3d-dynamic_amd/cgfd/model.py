@@ -107,6 +107,33 @@ class Setup:
         self.desc = L.Desc()
         L.check(self.lib, self.lib.cgfd_setup_get_desc(h, C.byref(self.desc)))
 
+    def read_stations(self, path) -> list:
+        """The station list file at path, located on this grid
+        (cgfd_io_read_stations): dicts with name, ijk (ghost-inclusive base
+        node), dijk (increments) and xyz, in file order, stations off the
+        grid dropped."""
+        n_file, n_loc = C.c_int(), C.c_int()
+        L.check(self.lib, self.lib.cgfd_io_read_stations(str(path).encode(), self.handle, None, 0, C.byref(n_file),
+                                                         C.byref(n_loc)))
+        arr = (L.Station * max(n_file.value, 1))()
+        L.check(self.lib, self.lib.cgfd_io_read_stations(str(path).encode(), self.handle, arr, n_file.value,
+                                                         C.byref(n_file), C.byref(n_loc)))
+        return [{"name": s.name.decode(), "ijk": (s.i, s.j, s.k), "dijk": (s.di, s.dj, s.dk), "xyz": (s.x, s.y, s.z)}
+                for s in arr[:n_loc.value]]
+
+    def read_fault_stations(self, path) -> list:
+        """The fault station file at path (cgfd_io_read_fault_stations):
+        dicts with name, fault_id, jk (ghost-inclusive base node), djk and
+        xyz."""
+        n = C.c_int()
+        L.check(self.lib, self.lib.cgfd_io_read_fault_stations(str(path).encode(), self.handle, None, 0,
+                                                               C.byref(n)))
+        arr = (L.FaultStation * max(n.value, 1))()
+        L.check(self.lib, self.lib.cgfd_io_read_fault_stations(str(path).encode(), self.handle, arr, n.value,
+                                                               C.byref(n)))
+        return [{"name": s.name.decode(), "fault_id": s.fault_id, "jk": (s.j, s.k), "djk": (s.dj, s.dk),
+                 "xyz": (s.x, s.y, s.z)} for s in arr[:n.value]]
+
     def dt_max(self) -> float:
         v = C.c_float()
         L.check(self.lib, self.lib.cgfd_setup_dt_max(self.handle, C.byref(v)))
@@ -232,11 +259,17 @@ class Core:
         L.check(self.lib, self.lib.cgfd_core_read_snapshot(self.handle, L.ptr(b), flags, L.ptr(out)))
         return out
 
-    def set_fault_receivers(self, fjk_ext: np.ndarray, nt_max: int):
+    def set_fault_receivers(self, fjk_ext: np.ndarray, nt_max: int, djk=None):
         """Receivers at (fault id, ghost-inclusive j, k) nodes of the fault
-        planes, recording Tn..Slip2 at every step start."""
+        planes, recording Tn..Slip2 at every step start; djk (n, 2):
+        increments towards j+1, k+1 (bilinear between the nodes)."""
         a = np.ascontiguousarray(fjk_ext, np.int32).reshape(-1, 3)
-        L.check(self.lib, self.lib.cgfd_core_set_fault_receivers(self.handle, a.shape[0], L.ptr(a), nt_max))
+        if djk is None:
+            L.check(self.lib, self.lib.cgfd_core_set_fault_receivers(self.handle, a.shape[0], L.ptr(a), nt_max))
+        else:
+            d = np.ascontiguousarray(djk, np.float32).reshape(-1, 2)
+            L.check(self.lib, self.lib.cgfd_core_set_fault_receivers_interp(self.handle, a.shape[0], L.ptr(a),
+                                                                             L.ptr(d), nt_max))
         self._nfr = a.shape[0]
         self._nfr_nt = nt_max
 
@@ -339,9 +372,17 @@ class Core:
         L.check(self.lib, self.lib.cgfd_core_read_pg(self.handle, L.ptr(out)))
         return out
 
-    def set_stations(self, ijk_ext: np.ndarray, nt_max: int):
+    def set_stations(self, ijk_ext: np.ndarray, nt_max: int, dijk=None):
+        """Stations at ghost-inclusive (i, j, k) nodes; dijk (n, 3):
+        increments towards i+1, j+1, k+1 (the reference's eight-point
+        interpolation, cgfd_core_set_stations_interp)."""
         a = np.ascontiguousarray(ijk_ext, np.int32).reshape(-1, 3)
-        L.check(self.lib, self.lib.cgfd_core_set_stations(self.handle, a.shape[0], L.ptr(a), nt_max))
+        if dijk is None:
+            L.check(self.lib, self.lib.cgfd_core_set_stations(self.handle, a.shape[0], L.ptr(a), nt_max))
+        else:
+            d = np.ascontiguousarray(dijk, np.float32).reshape(-1, 3)
+            L.check(self.lib, self.lib.cgfd_core_set_stations_interp(self.handle, a.shape[0], L.ptr(a), L.ptr(d),
+                                                                      nt_max))
         self._nst = a.shape[0]
         self._ntmax = nt_max
 

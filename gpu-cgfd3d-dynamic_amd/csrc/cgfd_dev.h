@@ -91,6 +91,21 @@ struct VolArgs {
   FaultSkip fs;
 };
 
+/* the box-split schedule of the volume update: the interior box (points of
+   no CFS-PML face, x bounds on 32-column boundaries) and the bands around
+   it; ghost-inclusive, half-open ranges */
+struct VolBox {
+  int x0, x1, y0, y1, k0, k1;
+  int ntx, nty;  /* tiles of the box; ntx = 0: empty */
+};
+struct VolBands {
+  int n;  /* regions */
+  int x0[6], x1[6], y0[6], y1[6], k0[6], k1[6];
+  int ox[6];          /* x origin of a region's 32-wide tiles (32-aligned) */
+  int ntx[6], nty[6]; /* 32 x 8 tiles per plane */
+  long cum[7];        /* block prefix sums over the regions */
+};
+
 /* ---------------------------------------------------------------------- */
 /* Scenarios of a batch share the grid, metric, media, CFS-PML profiles and
    fault coefficients; each has its own wavefield levels, PML auxiliaries,

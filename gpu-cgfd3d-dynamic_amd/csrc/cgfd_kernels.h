@@ -142,8 +142,8 @@ struct AdjFaultArgs {
 void launch_adjf_begin(const AdjFaultArgs &a, hipStream_t st);
 /* residuals of fault-receiver sample `sample` ([nrec][nt][9], Tn..Slip2) into
    the adjoint of the outputs the step before it produced */
-void launch_adjf_receivers(const AdjFaultArgs &a, const int *rec, int nrec, const float *res, int nt, int sample,
-                           hipStream_t st);
+void launch_adjf_receivers(const AdjFaultArgs &a, const int *rec, const float *fac, int nterm, int nrec,
+                           const float *res, int nt, int sample, hipStream_t st);
 void launch_adjf_stage_pre(const AdjFaultArgs &a, hipStream_t st);
 void launch_adjf_stage_post(const AdjFaultArgs &a, hipStream_t st);
 void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
@@ -154,12 +154,12 @@ void launch_adj_top(int mode, int idir, int jdir, int kdir, const Grid &g, const
                     const float *E, float beta, float *T, const AdjPml &pm, hipStream_t st);
 void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
                       const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st);
-void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int n, long sicmp, float *P,
-                       hipStream_t st);
-void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
-                            int nt_max, const int *it_dev, int it0, hipStream_t st);
-void launch_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
-                         int it0, long ss, hipStream_t st);
+void launch_adj_inject(const long *term, const float *fac, int nterm, int nsta, const float *res, int nt, int n,
+                       long sicmp, float *P, hipStream_t st);
+void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *term, const float *fac,
+                            int nterm, int n, float *buf, int nt_max, const int *it_dev, int it0, hipStream_t st);
+void launch_record_fault(const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n, float *buf,
+                         int nt_max, const int *it_dev, int it0, long ss, hipStream_t st);
 /* output boxes (k_io.hip): start, count, stride per axis, ghost-inclusive */
 struct IoBox {
   int s[3], n[3], d[3];

@@ -63,7 +63,7 @@ struct cgfd_core {
   Grid g{};
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
-  int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, nan_every = 0;
+  int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, fix_sta = 0, nan_every = 0;
   int variant = 20;
   int zchunk = 4;
   int nvol = 0;
@@ -102,15 +102,18 @@ struct cgfd_core {
   float *PG = nullptr, *Dis = nullptr;
   float *stage = nullptr;  /* nx*ny*nz floats: host-layout staging for volume copies */
 
-  long *st_idx = nullptr;
-  float *st_buf = nullptr;
-  size_t st_idx_cap = 0, st_buf_cap = 0;
-  int st_n = 0, st_nt = 0, st_it0 = 0;
-  /* fault receivers: (fault id, slice index) pairs, [n][nt][9] records */
+  /* stations: base node per station (st_idx), and per station st_nterm
+     terms of the recorded sum (node index, factors fx fy fz) */
+  long *st_idx = nullptr, *st_term = nullptr;
+  float *st_buf = nullptr, *st_fac = nullptr;
+  size_t st_idx_cap = 0, st_buf_cap = 0, st_term_cap = 0, st_fac_cap = 0;
+  int st_n = 0, st_nt = 0, st_it0 = 0, st_nterm = 1;
+  /* fault receivers: fr_nterm (fault id, slice index) pairs and factors
+     (fy, fz) per receiver, [n][nt][9] records */
   int *fr_idx = nullptr;
-  float *fr_buf = nullptr;
-  size_t fr_idx_cap = 0, fr_buf_cap = 0;
-  int fr_n = 0, fr_nt = 0, fr_it0 = 0;
+  float *fr_buf = nullptr, *fr_fac = nullptr;
+  size_t fr_idx_cap = 0, fr_buf_cap = 0, fr_fac_cap = 0;
+  int fr_n = 0, fr_nt = 0, fr_it0 = 0, fr_nterm = 1;
 
   /* eight steps (one cycle of the operator pairs; an even number of level
      swaps) captured once and replayed */
@@ -452,6 +455,7 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   c->imethod = d->imethod;
   c->nfault = d->number_fault;
   c->fix_q1 = (d->compat_flags & CGFD_COMPAT_FIX_F2W_SWAP) ? 1 : 0;
+  c->fix_sta = (d->compat_flags & CGFD_COMPAT_FIX_STATION_INTERP) ? 1 : 0;
   c->nan_every = d->qc_check_nan_every;
   const long sc = g.sicmp;
   const int B = nscen;
@@ -646,10 +650,11 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
   const int nvol = nk - ntop;
   if (c->refresh_active() == 0) return CGFD_OK;  /* every scenario of the batch has retired */
   if (c->st_n > 0)
-    launch_record_stations(c->bat, c->wlev(c->lv_pre), c->g.sicmp, c->st_idx, c->st_n, c->st_buf, c->st_nt,
-                           c->it_dev, c->st_it0, c->st);
+    launch_record_stations(c->bat, c->wlev(c->lv_pre), c->g.sicmp, c->st_term, c->st_fac, c->st_nterm, c->st_n,
+                           c->st_buf, c->st_nt, c->it_dev, c->st_it0, c->st);
   if (c->fr_n > 0)
-    launch_record_fault(c->d_ftab, c->fr_idx, c->fr_n, c->fr_buf, c->fr_nt, c->it_dev, c->fr_it0, c->g.sslice, c->st);
+    launch_record_fault(c->d_ftab, c->fr_idx, c->fr_fac, c->fr_nterm, c->fr_n, c->fr_buf, c->fr_nt, c->it_dev,
+                        c->fr_it0, c->g.sslice, c->st);
   for (int s = 0; s < 4; s++) {
     const int idir = (kPairs[ipair][0] + s) % 2;
     const int jdir = (kPairs[ipair][1] + s) % 2;
@@ -1019,7 +1024,7 @@ int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residu
       const int ipair = it % 8;
       fa.n = n;
       fa.wE = E;
-      launch_adjf_receivers(fa, c->fr_idx, c->fr_n, resf, nt, n + 1, c->st);
+      launch_adjf_receivers(fa, c->fr_idx, c->fr_fac, c->fr_nterm, c->fr_n, resf, nt, n + 1, c->st);
       launch_adjf_begin(fa, c->st);
       for (int s = 3; s >= 0; s--) {
         const int idir = (kPairs[ipair][0] + s) % 2, jdir = (kPairs[ipair][1] + s) % 2, kdir = (kPairs[ipair][2] + s) % 2;
@@ -1040,7 +1045,7 @@ int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residu
         launch_adjf_stage_post(fa, c->st);
       }
     }
-    launch_adj_inject(c->st_idx, c->st_n, res, nt, n, sc, P, c->st);
+    launch_adj_inject(c->st_term, c->st_fac, c->st_nterm, c->st_n, res, nt, n, sc, P, c->st);
     std::swap(E, P);
     std::swap(pm.le, pm.lp);
     std::swap(fa.le, fa.lp);
@@ -1082,32 +1087,60 @@ extern "C" int cgfd_batch_adj_backward(cgfd_core *c, int s, const float *residua
   return adj_sweep(c, s, residual, nullptr, nt, grad, grad_fault);
 }
 
-extern "C" int cgfd_core_set_fault_receivers(cgfd_core *c, int n, const int *fjk, int nt_max) {
+/* io_fault_recv_keep (io_funcs.cu:2313-2351): a receiver between nodes is
+   the bilinear sum buff0 Ly1 Lz1 + buff1 Ly2 Lz1 + buff2 Ly1 Lz2 +
+   buff3 Ly2 Lz2 over (j, k), (j+1, k), (j, k+1), (j+1, k+1), with Ly2 = dj,
+   Ly1 = 1 - dj; djk = NULL: on the nodes themselves */
+static int set_fault_receivers(cgfd_core *c, int n, const int *fjk, const float *djk, int nt_max) {
   if (!c || n < 0 || (n > 0 && (!fjk || nt_max <= 0))) return cgfd_set_error(CGFD_ERR_ARG, "bad receiver arguments");
   if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "fault receivers: single-scenario cores only");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  std::vector<int> idx(2 * (size_t)n);
+  const int nterm = djk ? 4 : 1;
+  std::vector<int> idx(2 * (size_t)n * nterm);
+  std::vector<float> fac(2 * (size_t)n * nterm);
   for (int r = 0; r < n; r++) {
     const int id = fjk[3 * r], j = fjk[3 * r + 1], k = fjk[3 * r + 2];
     if (id < 0 || id >= c->nfault || j < 3 || j > c->g.nj + 2 || k < 3 || k > c->g.nk + 2)
       return cgfd_set_error(CGFD_ERR_ARG, "fault receiver %d (fault %d, j=%d, k=%d) is outside the fault planes", r, id,
                             j, k);
-    idx[2 * r] = id;
-    idx[2 * r + 1] = j + k * c->g.ny;
+    const float ly2 = djk ? djk[2 * r] : 0.0f, lz2 = djk ? djk[2 * r + 1] : 0.0f;
+    const float ly1 = (float)(1.0 - ly2), lz1 = (float)(1.0 - lz2);
+    const int dj[4] = {0, 1, 0, 1}, dk[4] = {0, 0, 1, 1};
+    const float fy[4] = {ly1, ly2, ly1, ly2}, fz[4] = {lz1, lz1, lz2, lz2};
+    for (int u = 0; u < nterm; u++) {
+      const size_t q = (size_t)r * nterm + u;
+      idx[2 * q] = id;
+      idx[2 * q + 1] = (j + dj[u]) + (k + dk[u]) * c->g.ny;
+      fac[2 * q] = nterm == 1 ? 1.0f : fy[u];
+      fac[2 * q + 1] = nterm == 1 ? 1.0f : fz[u];
+    }
   }
   int rc;
   if (n > 0) {
     if ((rc = c->grow(&c->fr_idx, &c->fr_idx_cap, idx.size()))) return rc;
     HIPCHK(hipMemcpy(c->fr_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
+    if ((rc = c->grow(&c->fr_fac, &c->fr_fac_cap, fac.size()))) return rc;
+    HIPCHK(hipMemcpy(c->fr_fac, fac.data(), fac.size() * 4, hipMemcpyHostToDevice));
     if ((rc = c->grow(&c->fr_buf, &c->fr_buf_cap, (size_t)n * nt_max * 9))) return rc;
     HIPCHK(hipMemset(c->fr_buf, 0, (size_t)n * nt_max * 9 * 4));
   }
   c->fr_n = n;
   c->fr_nt = nt_max;
+  c->fr_nterm = nterm;
   c->fr_it0 = c->it_host;
   c->drop_graph();
   return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_fault_receivers(cgfd_core *c, int n, const int *fjk, int nt_max) {
+  return set_fault_receivers(c, n, fjk, nullptr, nt_max);
+}
+
+extern "C" int cgfd_core_set_fault_receivers_interp(cgfd_core *c, int n, const int *fjk, const float *djk,
+                                                    int nt_max) {
+  if (n > 0 && !djk) return cgfd_set_error(CGFD_ERR_ARG, "null increments");
+  return set_fault_receivers(c, n, fjk, djk, nt_max);
 }
 
 extern "C" int cgfd_core_read_fault_receivers(cgfd_core *c, float *dst, int *nt) {
@@ -1286,30 +1319,66 @@ extern "C" int cgfd_batch_read_pg(cgfd_core *c, int s, float *dst) {
 }
 extern "C" int cgfd_core_read_pg(cgfd_core *c, float *dst) { return cgfd_batch_read_pg(c, 0, dst); }
 
-extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int nt_max) {
+/* io_recv_keep (io_funcs.cu:427-469): a station between nodes is the sum
+   of eight terms buff[t] Lx Ly Lz over the cell's corners t = i + 2 j + 4 k,
+   Lx2 = di, Lx1 = 1 - di, summed in corner order, except that the reference
+   takes corner 5 for the term of corner 4 (io_funcs.cu:459-460), which
+   CGFD_COMPAT_FIX_STATION_INTERP corrects; dijk = NULL: on the nodes */
+static int set_stations(cgfd_core *c, int n, const int *ijk, const float *dijk, int nt_max) {
   if (!c || n < 0 || (n > 0 && (!ijk || nt_max <= 0))) return cgfd_set_error(CGFD_ERR_ARG, "bad station arguments");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  std::vector<long> idx(n);
+  const int nterm = dijk ? 8 : 1;
+  std::vector<long> idx(n), term((size_t)n * nterm);
+  std::vector<float> fac((size_t)n * nterm * 3);
   for (int s = 0; s < n; s++) {
     int i = ijk[3 * s], j = ijk[3 * s + 1], k = ijk[3 * s + 2];
     if (i < 3 || i > c->g.ni + 2 || j < 3 || j > c->g.nj + 2 || k < 3 || k > c->g.nk + 2)
       return cgfd_set_error(CGFD_ERR_ARG, "station %d at (%d,%d,%d) is outside the physical grid", s, i, j, k);
     idx[s] = i + (long)j * c->g.siy + (long)k * c->g.siz;
+    if (nterm == 1) {
+      term[s] = idx[s];
+      fac[3 * s] = fac[3 * s + 1] = fac[3 * s + 2] = 1.0f;
+      continue;
+    }
+    const float lx2 = dijk[3 * s], ly2 = dijk[3 * s + 1], lz2 = dijk[3 * s + 2];
+    const float lx[2] = {(float)(1.0 - lx2), lx2}, ly[2] = {(float)(1.0 - ly2), ly2}, lz[2] = {(float)(1.0 - lz2), lz2};
+    for (int u = 0; u < 8; u++) {
+      const int src = (u == 4 && !c->fix_sta) ? 5 : u;  /* the corner the term reads */
+      const int a = src & 1, b = (src >> 1) & 1, e = src >> 2;
+      term[(size_t)s * 8 + u] = idx[s] + a + (long)b * c->g.siy + (long)e * c->g.siz;
+      fac[((size_t)s * 8 + u) * 3] = lx[u & 1];
+      fac[((size_t)s * 8 + u) * 3 + 1] = ly[(u >> 1) & 1];
+      fac[((size_t)s * 8 + u) * 3 + 2] = lz[u >> 2];
+    }
   }
   int rc;
   c->bat.ststride = (long)n * nt_max * 9;
   if (n > 0) {
     if ((rc = c->grow(&c->st_idx, &c->st_idx_cap, (size_t)n))) return rc;
     HIPCHK(hipMemcpy(c->st_idx, idx.data(), (size_t)n * sizeof(long), hipMemcpyHostToDevice));
+    if ((rc = c->grow(&c->st_term, &c->st_term_cap, term.size()))) return rc;
+    HIPCHK(hipMemcpy(c->st_term, term.data(), term.size() * sizeof(long), hipMemcpyHostToDevice));
+    if ((rc = c->grow(&c->st_fac, &c->st_fac_cap, fac.size()))) return rc;
+    HIPCHK(hipMemcpy(c->st_fac, fac.data(), fac.size() * 4, hipMemcpyHostToDevice));
     if ((rc = c->grow(&c->st_buf, &c->st_buf_cap, (size_t)c->nscen * n * nt_max * 9))) return rc;
     HIPCHK(hipMemset(c->st_buf, 0, (size_t)c->nscen * n * nt_max * 9 * 4));
   }
   c->st_n = n;
   c->st_nt = nt_max;
+  c->st_nterm = nterm;
   c->st_it0 = c->it_host;
   c->drop_graph();
   return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_stations(cgfd_core *c, int n, const int *ijk, int nt_max) {
+  return set_stations(c, n, ijk, nullptr, nt_max);
+}
+
+extern "C" int cgfd_core_set_stations_interp(cgfd_core *c, int n, const int *ijk, const float *dijk, int nt_max) {
+  if (n > 0 && !dijk) return cgfd_set_error(CGFD_ERR_ARG, "null increments");
+  return set_stations(c, n, ijk, dijk, nt_max);
 }
 
 extern "C" int cgfd_batch_read_stations(cgfd_core *c, int s, float *dst, int *nt) {
@@ -1466,7 +1535,7 @@ TuneKey tune_key(const cgfd_core *c) {
   return TuneKey{c->g.ni, c->g.nj, c->g.nk, c->nscen, c->nfault, c->isfree, pml};
 }
 
-bool valid_variant(int v) { return v == 1 || v == 16 || (v >= 19 && v <= 21); }
+bool valid_variant(int v) { return v == 1 || v == 16 || (v >= 19 && v <= 21) || v == 23 || v == 24; }
 
 /* the cache file: one "ni nj nk nscen nfault isfree pml variant zchunk" line
    per tuned shape; later lines win */
@@ -1513,7 +1582,7 @@ static void apply_auto_variant(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant != 0 && !valid_variant(variant))
-    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 16, 19, 20, 21)", variant);
+    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 16, 19, 20, 21, 23, 24)", variant);
   if (variant == 0) apply_auto_variant(c);
   else c->variant = variant;
   c->drop_graph();
@@ -1551,7 +1620,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const bool batch = c->bat.nact > 1;
-  const std::vector<int> variants = batch ? std::vector<int>{20, 19, 21, 16, 1} : std::vector<int>{20, 16, 1};
+  const std::vector<int> variants = batch ? std::vector<int>{20, 23, 24, 19, 21, 16, 1} : std::vector<int>{20, 23, 24, 16, 1};
   const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;

@@ -208,20 +208,23 @@ __global__ void __launch_bounds__(256) k_adjf_begin(AdjFaultArgs a) {
 /* fault-receiver residuals of one sample (the outputs after the step being
    reversed): Slip, Slip1, Slip2 into their running adjoints, Vs, Vs1, Vs2
    into this step's; welded nodes output nothing */
-__global__ void k_adjf_receivers(AdjFaultArgs a, const int *rec, int nrec, const float *res, int nt, int sample) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrec) return;
-  const int id = rec[2 * r];
-  const long pf = rec[2 * r + 1], ss = a.g.sslice;
-  if (a.ftab[id].united[pf] != 0) return;
+__global__ void k_adjf_receivers(AdjFaultArgs a, const int *rec, const float *fac, int nterm, int nrec,
+                                 const float *res, int nt, int sample) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrec * nterm) return;
+  const int r = t / nterm;
+  const int id = rec[2 * t];
+  const long pf = rec[2 * t + 1], ss = a.g.sslice;
+  const float w = fac[2 * t] * fac[2 * t + 1];
+  if (w == 0.0f || a.ftab[id].united[pf] != 0) return;
   float *blk = adj_block(a, id);
   const float *q = res + ((long)r * nt + sample) * 9;
-  aadd(blk + kOffVsr * ss + pf, q[3]);
-  aadd(blk + (kOffVsr + 1) * ss + pf, q[4]);
-  aadd(blk + (kOffVsr + 2) * ss + pf, q[5]);
-  aadd(blk + kOffSlip * ss + pf, q[6]);
-  aadd(blk + kOffSlip12 * ss + pf, q[7]);
-  aadd(blk + (kOffSlip12 + 1) * ss + pf, q[8]);
+  aadd(blk + kOffVsr * ss + pf, q[3] * w);
+  aadd(blk + (kOffVsr + 1) * ss + pf, q[4] * w);
+  aadd(blk + (kOffVsr + 2) * ss + pf, q[5] * w);
+  aadd(blk + kOffSlip * ss + pf, q[6] * w);
+  aadd(blk + kOffSlip12 * ss + pf, q[7] * w);
+  aadd(blk + (kOffSlip12 + 1) * ss + pf, q[8] * w);
 }
 
 /* RK structure of the split-node levels at reverse stage MODE (the
@@ -792,10 +795,11 @@ void launch_adjf_begin(const AdjFaultArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(k_adjf_begin, node_grid(a), dim3(64, 4), 0, st, a);
 }
 
-void launch_adjf_receivers(const AdjFaultArgs &a, const int *rec, int nrec, const float *res, int nt, int sample,
-                           hipStream_t st) {
+void launch_adjf_receivers(const AdjFaultArgs &a, const int *rec, const float *fac, int nterm, int nrec,
+                           const float *res, int nt, int sample, hipStream_t st) {
   if (a.nfault <= 0 || nrec <= 0 || !res) return;
-  hipLaunchKernelGGL(k_adjf_receivers, dim3((nrec + 63) / 64), dim3(64), 0, st, a, rec, nrec, res, nt, sample);
+  hipLaunchKernelGGL(k_adjf_receivers, dim3((nrec * nterm + 63) / 64), dim3(64), 0, st, a, rec, fac, nterm, nrec, res,
+                     nt, sample);
 }
 
 void launch_adjf_stage_pre(const AdjFaultArgs &a, hipStream_t st) {

@@ -434,13 +434,19 @@ void launch_adj_top(int mode, int idir, int jdir, int kdir, const Grid &g, const
   }
 }
 
-/* P += residual row n at the stations (all nine components) */
-__global__ void k_adj_inject(const long *__restrict__ idx, int nsta, const float *__restrict__ res, int nt, int n,
-                             long sicmp, float *P) {
+/* P += residual row n at the stations (all nine components), the transpose
+   of the recording: each term's node takes the residual times the term's
+   weight fx fy fz (a station on a node: one term of weight 1) */
+__global__ void k_adj_inject(const long *__restrict__ term, const float *__restrict__ fac, int nterm, int nsta,
+                             const float *__restrict__ res, int nt, int n, long sicmp, float *P) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nsta * 9) return;
-  const int s = t / 9, c = t % 9;
-  atomicAdd(P + c * sicmp + idx[s], res[((long)s * nt + n) * 9 + c]);
+  if (t >= nsta * 9 * nterm) return;
+  const int u = t % nterm, sc = t / nterm;
+  const int s = sc / 9, c = sc % 9;
+  const float *f = fac + ((long)s * nterm + u) * 3;
+  const float w = (f[0] * f[1]) * f[2];
+  if (w == 0.0f) return;
+  atomicAdd(P + c * sicmp + term[(long)s * nterm + u], res[((long)s * nt + n) * 9 + c] * w);
 }
 
 void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu, const float *slw, const float *X,
@@ -489,10 +495,12 @@ void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, con
   }
 }
 
-void launch_adj_inject(const long *idx, int nsta, const float *res, int nt, int n, long sicmp, float *P,
-                       hipStream_t st) {
-  const int nthr = nsta * 9;
-  hipLaunchKernelGGL(k_adj_inject, dim3((nthr + 255) / 256), dim3(256), 0, st, idx, nsta, res, nt, n, sicmp, P);
+void launch_adj_inject(const long *term, const float *fac, int nterm, int nsta, const float *res, int nt, int n,
+                       long sicmp, float *P, hipStream_t st) {
+  const int nthr = nsta * 9 * nterm;
+  if (nthr <= 0) return;
+  hipLaunchKernelGGL(k_adj_inject, dim3((nthr + 255) / 256), dim3(256), 0, st, term, fac, nterm, nsta, res, nt, n,
+                     sicmp, P);
 }
 
 }  // namespace cgfd

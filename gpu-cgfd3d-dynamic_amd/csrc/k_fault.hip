@@ -803,16 +803,26 @@ __global__ void k_nan_check(const float *p, long n, int *flag) {
 }
 
 /* receivers (io_recv_keep, drv_rk_curv_col.cu:215): 9 components of level pre */
-__global__ void k_record(BatArgs b, const float *w, long sicmp, const long *idx, int n, float *buf, int nt_max,
-                         const int *it_dev, int it0) {
+/* stations: each of the nine components is a sum of nterm terms
+   ((v * fx) * fy) * fz in the order of io_recv_keep (io_funcs.cu:452-462);
+   one term with unit factors is a station on a node. FMA contraction is off
+   so the sums round as the reference's host arithmetic does. */
+__global__ void k_record(BatArgs b, const float *w, long sicmp, const long *term, const float *fac, int nterm, int n,
+                         float *buf, int nt_max, const int *it_dev, int it0) {
+#pragma clang fp contract(off)
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 9) return;
   int s = t / 9, c = t % 9;
   int it = *it_dev - it0;
   if (it < 0 || it >= nt_max) return;
+  const long *tm = term + (long)s * nterm;
+  const float *f = fac + (long)s * nterm * 3;
   for (int q = 0; q < b.nact; q++) {
     const long sc = b.act[q];
-    buf[sc * b.ststride + ((long)s * nt_max + it) * 9 + c] = w[sc * b.wstride + c * sicmp + idx[s]];
+    const float *wc = w + sc * b.wstride + c * sicmp;
+    float acc = ((wc[tm[0]] * f[0]) * f[1]) * f[2];
+    for (int u = 1; u < nterm; u++) acc = acc + ((wc[tm[u]] * f[3 * u]) * f[3 * u + 1]) * f[3 * u + 2];
+    buf[sc * b.ststride + ((long)s * nt_max + it) * 9 + c] = acc;
   }
 }
 
@@ -861,29 +871,35 @@ void launch_nan_check(const float *p, long n, int *flag, hipStream_t st) {
   hipLaunchKernelGGL(k_nan_check, dim3((unsigned)blocks), dim3(256), 0, st, p, n, flag);
 }
 /* fault receivers: the nine outputs Tn, Ts1, Ts2, Vs, Vs1, Vs2, Slip, Slip1,
-   Slip2 at each receiver's node, recorded at the step start
-   (io_fault_recv_keep, io_funcs.cu:2313-2351, at grid nodes) */
-__global__ void k_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
-                               int it0, long ss) {
+   Slip2 at each receiver, recorded at the step start as the sum of nterm
+   terms (v * fy) * fz over (fault id, node) pairs (io_fault_recv_keep,
+   io_funcs.cu:2313-2351; one term with unit factors on a node) */
+__global__ void k_record_fault(const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n, float *buf,
+                               int nt_max, const int *it_dev, int it0, long ss) {
+#pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 9) return;
   const int r = t / 9, v = t % 9;
   const int it = *it_dev - it0;
   if (it < 0 || it >= nt_max) return;
-  buf[((long)r * nt_max + it) * 9 + v] = ftab[rec[2 * r]].out[v * ss + rec[2 * r + 1]];
+  const int *tm = rec + 2L * r * nterm;
+  const float *f = fac + 2L * r * nterm;
+  float acc = (ftab[tm[0]].out[v * ss + tm[1]] * f[0]) * f[1];
+  for (int u = 1; u < nterm; u++) acc = acc + (ftab[tm[2 * u]].out[v * ss + tm[2 * u + 1]] * f[2 * u]) * f[2 * u + 1];
+  buf[((long)r * nt_max + it) * 9 + v] = acc;
 }
-void launch_record_fault(const FaultDev *ftab, const int *rec, int n, float *buf, int nt_max, const int *it_dev,
-                         int it0, long ss, hipStream_t st) {
+void launch_record_fault(const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n, float *buf,
+                         int nt_max, const int *it_dev, int it0, long ss, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_record_fault, dim3((n * 9 + 255) / 256), dim3(256), 0, st, ftab, rec, n, buf, nt_max, it_dev,
-                     it0, ss);
+  hipLaunchKernelGGL(k_record_fault, dim3((n * 9 + 255) / 256), dim3(256), 0, st, ftab, rec, fac, nterm, n, buf,
+                     nt_max, it_dev, it0, ss);
 }
-void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *idx, int n, float *buf,
-                            int nt_max, const int *it_dev, int it0, hipStream_t st) {
+void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *term, const float *fac,
+                            int nterm, int n, float *buf, int nt_max, const int *it_dev, int it0, hipStream_t st) {
   if (n <= 0) return;
   int t = n * 9;
-  hipLaunchKernelGGL(k_record, dim3((t + 255) / 256), dim3(256), 0, st, b, w, sicmp, idx, n, buf, nt_max, it_dev,
-                     it0);
+  hipLaunchKernelGGL(k_record, dim3((t + 255) / 256), dim3(256), 0, st, b, w, sicmp, term, fac, nterm, n, buf, nt_max,
+                     it_dev, it0);
 }
 
 }  // namespace cgfd

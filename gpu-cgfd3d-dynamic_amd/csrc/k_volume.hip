@@ -17,6 +17,8 @@
  * nodes are left to the fault kernel, which overwrites their rhs in the
  * reference (sv_curv_col_el_iso_fault_gpu.cu:11-189).
  */
+#include <algorithm>
+
 #include "cgfd_dev.h"
 #include "cgfd_kernels.h"
 #include "k_vol_common.h"
@@ -337,6 +339,133 @@ __global__ void __launch_bounds__(256, 6) k_vol_pml_lean(VolArgs a, BatArgs b) {
 }
 
 /* ------------------------------------------------------------------------
+ * Box split (variants 23 and 24, one scenario). The interior kernel covers
+ * exactly the box of points that no CFS-PML face contains, with its x
+ * bounds moved inwards to multiples of 32 columns (whole 128-byte row
+ * segments), in TW x 256/TW tiles (64 x 4 for 23, 32 x 8 for 24: fewer
+ * y-halo rows per point) and slab order. The band kernel covers the rest,
+ * the up to six slabs around the box, in 32 x 8 tiles, so an x-PML band of
+ * 21 columns fills 29 of a tile's 32 lanes and no interior column takes the
+ * PML path except the few between a PML edge and the next 32-column
+ * boundary (the 64-wide tiles of variant 20 send up to 43 interior columns
+ * per row through it).
+ * ---------------------------------------------------------------------- */
+template <int ID, int JD, int KD, int TW>
+__global__ void __launch_bounds__(256, 8) k_vol_acc_box(VolArgs a, BatArgs b, VolBox x) {
+  constexpr int TH = 256 / TW;
+  const int S = a.zchunk;
+  long L = blockIdx.x;
+  const int bx = (int)(L % x.ntx);
+  L /= x.ntx;
+  const int cy = (int)(L % S);
+  L /= S;
+  const int nkb = x.k1 - x.k0;
+  const int kz = (int)(L % nkb);
+  const int by = (int)(L / nkb) * S + cy;
+  if (by >= x.nty) return;
+  const int t = (int)threadIdx.x;
+  const int i = x.x0 + bx * TW + (t % TW), j = x.y0 + by * TH + t / TW;
+  if (i >= x.x1 || j >= x.y1) return;
+  acc_point<ID, JD, KD, 1>(a, b, i, j, x.k0 + kz, b.act[0]);
+}
+
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256, 6) k_vol_pml_band(VolArgs a, BatArgs b, VolBands r) {
+  const long L0 = blockIdx.x;
+  int q = 0;
+  while (q + 1 < r.n && L0 >= r.cum[q + 1]) q++;
+  long L = L0 - r.cum[q];
+  const int bx = (int)(L % r.ntx[q]);
+  L /= r.ntx[q];
+  const int by = (int)(L % r.nty[q]);
+  L /= r.nty[q];
+  const int t = (int)threadIdx.x;
+  const int i = r.ox[q] + bx * 32 + (t & 31), j = r.y0[q] + by * 8 + (t >> 5);
+  if (i < r.x0[q] || i >= r.x1[q] || j >= r.y1[q]) return;
+  pml_acc_point<ID, JD, KD>(a, b, i, j, r.k0[q] + (int)L, b.act[0]);
+}
+
+/* batches on the box split: the interior box's (tile, scenario) pairs with
+   the scenario fastest and block ids XCD-contiguous (the shared metric and
+   media lines of a tile serve every scenario from one L2), the bands in one
+   sweep per scenario (each scenario's halos come back from L2). nitems: the
+   box's tiles x scenarios; the grid is padded to a multiple of 8. */
+template <int ID, int JD, int KD, int TW>
+__global__ void __launch_bounds__(256, 8) k_vol_bat_acc_box(VolArgs a, BatArgs b, VolBox x, long nitems) {
+  constexpr int TH = 256 / TW;
+  const long L = blockIdx.x;
+  const long item = (L % 8) * (long)(gridDim.x / 8) + L / 8;
+  if (item >= nitems) return;
+  const int q = (int)(item % b.nact);
+  const long tile = item / b.nact;
+  const int bx = (int)(tile % x.ntx);
+  const int by = (int)((tile / x.ntx) % x.nty);
+  const int kz = (int)(tile / ((long)x.ntx * x.nty));
+  const int t = (int)threadIdx.x;
+  const int i = x.x0 + bx * TW + (t % TW), j = x.y0 + by * TH + t / TW;
+  if (i >= x.x1 || j >= x.y1) return;
+  acc_point<ID, JD, KD, 1>(a, b, i, j, x.k0 + kz, b.act[q]);
+}
+
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256, 6) k_vol_bat_pml_band(VolArgs a, BatArgs b, VolBands r) {
+  const long per = r.cum[r.n];
+  const int sq = (int)(blockIdx.x / per);
+  const long L0 = blockIdx.x % per;
+  int q = 0;
+  while (q + 1 < r.n && L0 >= r.cum[q + 1]) q++;
+  long L = L0 - r.cum[q];
+  const int bx = (int)(L % r.ntx[q]);
+  L /= r.ntx[q];
+  const int by = (int)(L % r.nty[q]);
+  L /= r.nty[q];
+  const int t = (int)threadIdx.x;
+  const int i = r.ox[q] + bx * 32 + (t & 31), j = r.y0[q] + by * 8 + (t >> 5);
+  if (i < r.x0[q] || i >= r.x1[q] || j >= r.y1[q]) return;
+  pml_acc_point<ID, JD, KD>(a, b, i, j, r.k0[q] + (int)L, b.act[sq]);
+}
+
+/* the interior box and the bands around it for one launch's planes */
+static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands &r) {
+  const Grid &g = a.g;
+  const int kv0 = a.k_lo + 3, kv1 = a.k_lo + 3 + a.nk_run;
+  int xl = 3, xr = g.ni + 3, yl = 3, yr = g.nj + 3, kl = kv0, kr = kv1;
+  if (a.pml_on) {
+    const PmlFace(&P)[3][2] = a.pml;
+    if (P[0][0].on) xl = std::max(xl, P[0][0].i2 + 1);
+    if (P[0][1].on) xr = std::min(xr, P[0][1].i1);
+    if (P[1][0].on) yl = std::max(yl, P[1][0].j2 + 1);
+    if (P[1][1].on) yr = std::min(yr, P[1][1].j1);
+    if (P[2][0].on) kl = std::max(kl, P[2][0].k2 + 1);
+    if (P[2][1].on) kr = std::min(kr, P[2][1].k1);
+  }
+  xl = (xl + 31) & ~31;
+  xr &= ~31;
+  r = VolBands{};
+  auto add = [&](int x0, int x1, int y0, int y1, int k0, int k1) {
+    if (x1 <= x0 || y1 <= y0 || k1 <= k0) return;
+    const int q = r.n++;
+    r.x0[q] = x0; r.x1[q] = x1; r.y0[q] = y0; r.y1[q] = y1; r.k0[q] = k0; r.k1[q] = k1;
+    r.ox[q] = x0 & ~31;
+    r.ntx[q] = (x1 - r.ox[q] + 31) / 32;
+    r.nty[q] = (y1 - y0 + 7) / 8;
+    r.cum[q + 1] = r.cum[q] + (long)r.ntx[q] * r.nty[q] * (k1 - k0);
+  };
+  if (xr <= xl || yr <= yl || kr <= kl) {
+    x = VolBox{3, 3, 3, 3, kv0, kv0, 0, 0};
+    add(3, g.ni + 3, 3, g.nj + 3, kv0, kv1);
+    return;
+  }
+  x = VolBox{xl, xr, yl, yr, kl, kr, (xr - xl + tw - 1) / tw, (yr - yl + 256 / tw - 1) / (256 / tw)};
+  add(3, xl, 3, g.nj + 3, kv0, kv1);
+  add(xr, g.ni + 3, 3, g.nj + 3, kv0, kv1);
+  add(xl, xr, 3, yl, kv0, kv1);
+  add(xl, xr, yr, g.nj + 3, kv0, kv1);
+  add(xl, xr, yl, yr, kv0, kl);
+  add(xl, xr, yl, yr, kr, kv1);
+}
+
+/* ------------------------------------------------------------------------
  * Batches on the split interior / CFS-PML kernels. ORDER 0: (tile, scenario)
  * pairs with the scenario fastest, XCD-contiguous, so a tile's metric and
  * media lines serve every scenario from one L2. ORDER 1: one slab-ordered
@@ -643,6 +772,23 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
     return;
   }
+  if (b.nact > 1 && (variant == 23 || variant == 24)) {  /* batches on the box split */
+    const int tw = variant == 23 ? 64 : 32;
+    VolBox x;
+    VolBands r;
+    box_split(a, tw, x, r);
+    if (x.ntx > 0) {
+      const long items = (long)x.ntx * x.nty * (x.k1 - x.k0) * b.nact, nb = ((items + 7) / 8) * 8;
+      if (tw == 64)
+        hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 64>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
+      else
+        hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 32>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
+    }
+    if (r.n > 0)
+      hipLaunchKernelGGL((k_vol_bat_pml_band<ID, JD, KD>), dim3((unsigned)(r.cum[r.n] * b.nact)), dim3(256), 0, st_pml,
+                         a, b, r);
+    return;
+  }
   if (b.nact > 1 && variant >= 16) {  /* batches, split kernels */
     long items, nb;
     if (variant != 16) {  /* lean PML kernel; 20: its blocks in per-scenario slab sweeps, 21: both kernels */
@@ -671,6 +817,21 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     const long items = (long)agrd.x * agrd.y * a.nk_run * b.nact;
     const long nb = ((items + 7) / 8) * 8;
     hipLaunchKernelGGL((k_vol_bat<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b, items);
+    return;
+  }
+  if (variant == 23 || variant == 24) {
+    const int tw = variant == 23 ? 64 : 32;
+    VolBox x;
+    VolBands r;
+    box_split(a, tw, x, r);
+    if (x.ntx > 0) {
+      const long nslab = ((long)x.nty + a.zchunk - 1) / a.zchunk;
+      const long nb = (long)x.ntx * a.zchunk * (x.k1 - x.k0) * nslab;
+      if (tw == 64) hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 64>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
+      else hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 32>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
+    }
+    if (r.n > 0)
+      hipLaunchKernelGGL((k_vol_pml_band<ID, JD, KD>), dim3((unsigned)r.cum[r.n]), dim3(256), 0, st_pml, a, b, r);
     return;
   }
   if (variant >= 19) {

@@ -120,6 +120,10 @@ typedef struct cgfd_desc {
 #define CGFD_COMPAT_FIX_F2W_SWAP 1  /* write sigma_xz into Txz and sigma_yz into Tyz
                                        in fault2wave; the reference swaps them
                                        (transform.cu:274-276 vs :295) */
+#define CGFD_COMPAT_FIX_STATION_INTERP 2  /* interpolate stations between nodes
+                                             with corner 4 for its term; the
+                                             reference reads corner 5 twice
+                                             (io_funcs.cu:459-460) */
 
 typedef struct cgfd_core cgfd_core;
 
@@ -262,6 +266,13 @@ int cgfd_core_read_pg(cgfd_core *core, float *host_dst);
    (io_recv_keep, drv_rk_curv_col.cu:215) into a device buffer of nt_max
    samples. */
 int cgfd_core_set_stations(cgfd_core *core, int n, const int *ijk, int nt_max);
+/* Stations between nodes (io_recv_keep, io_funcs.cu:427-469): base node
+   ijk[n][3] (ghost-inclusive) and increments dijk[n][3] towards i+1, j+1,
+   k+1 (cgfd_io_read_stations gives both); each component is the trilinear
+   sum over the cell's eight corners, in the reference's term order and
+   FP32 rounding, with its corner-5 term twice unless compat_flags has
+   CGFD_COMPAT_FIX_STATION_INTERP. Increments 0 give the node values. */
+int cgfd_core_set_stations_interp(cgfd_core *core, int n, const int *ijk, const float *dijk, int nt_max);
 /* host_dst[n][nt_recorded][9]; returns the number of recorded samples in *nt */
 int cgfd_core_read_stations(cgfd_core *core, float *host_dst, int *nt);
 
@@ -295,10 +306,15 @@ int cgfd_core_read_snapshot_noalloc(cgfd_core *core, const int *box, int flags, 
    planes. The core records the nine fault outputs Tn, Ts1, Ts2, Vs, Vs1,
    Vs2, Slip, Slip1, Slip2 at every step before the step is taken (as the
    reference's call at drv_rk_curv_col.cu:217) into nt_max samples; reading
-   gives host_dst[n][nt_recorded][9]. The reference interpolates between
-   nodes (dj, dk); receivers here sit on nodes, its dj = dk = 0 case.
+   gives host_dst[n][nt_recorded][9]. Receivers here sit on nodes (the
+   reference's dj = dk = 0 case); cgfd_core_set_fault_receivers_interp
+   places them between nodes.
    Single-scenario cores. */
 int cgfd_core_set_fault_receivers(cgfd_core *core, int n, const int *fjk, int nt_max);
+/* Fault receivers between nodes: increments djk[n][2] towards j+1, k+1
+   (cgfd_io_read_fault_stations); each output is the bilinear sum of
+   io_fault_recv_keep over the four nodes. */
+int cgfd_core_set_fault_receivers_interp(cgfd_core *core, int n, const int *fjk, const float *djk, int nt_max);
 int cgfd_core_read_fault_receivers(cgfd_core *core, float *host_dst, int *nt);
 
 /* Timing support: runs nsteps and returns the total elapsed device time of
@@ -342,6 +358,16 @@ int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
    levels that the next step rewrites before reading, so the simulation
    state is unchanged. */
 int cgfd_core_autotune(cgfd_core *core, int reps, int *variant, int *zchunk, float *ms_best);
+
+/* Measurement support, no counterpart in the reference: the streaming rate
+   (GB/s of bytes read + written) of a kernel that has every point of an
+   ni x nj x nk grid (the core's padded array layout, 64 x 4 tiles,
+   non-temporal accesses) read nread arrays and write nwrite arrays once,
+   averaged over reps launches on device_id. nread = 39, nwrite = 18 is the
+   data movement of an RK stage 1 or 2 point without the stencil: the
+   ceiling the volume update's roofline fraction is compared with. Allocates
+   (nread + nwrite) x (ni+6)(nj+6)(nk+6) floats for the duration. */
+int cgfd_calib_stream(int device_id, int ni, int nj, int nk, int nread, int nwrite, int reps, float *gbs);
 
 /* ---------------------------------------------------------------------
  * Batched ensembles: nscen rupture scenarios on one grid. The scenarios

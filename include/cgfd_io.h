@@ -18,6 +18,7 @@
 #define CGFD_IO_H
 
 #include "cgfd_core.h"
+#include "cgfd_setup.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -98,6 +99,51 @@ int cgfd_io_read_fault_coord(const char *path, int nj, int nk, float *xyz);
 int cgfd_io_read_coord(const char *path, int ni, int nj, int nk, float *xyz);
 int cgfd_io_read_metric(const char *path, int ni, int nj, int nk, float *metric);
 int cgfd_io_read_media(const char *path, int ni, int nj, int nk, float *media);
+
+/* Station lists (host only).
+   cgfd_io_read_stations reads the reference's station file
+   (io_recv_read_locate, io_funcs.cu:23-320): a count, then one line per
+   station "name flag_coord flag_depth x y z" ('#' lines and empty lines
+   skipped). flag_coord 0: x, y, z are grid indices counted from 1, z counted
+   down from the free surface from 0 when flag_depth is 1; the station sits
+   at the truncated index with the fractional rest as its increments.
+   flag_coord 1: x, y, z are coordinates (z a depth below the surface when
+   flag_depth is 1, converted with the top-layer cell that contains x, y,
+   gd_t.cu:1522-1612); the station goes to the nearest grid node and, inside
+   the hexahedral cell around it that contains the point, to the nearest
+   sample of a 101^3 trilinear lattice (gd_t.cu:1365-1503, 1694-1775).
+   Stations outside the physical grid are dropped, as the reference drops
+   them with a warning. The reference's locating kernels test flag_coord == 0
+   where its host code expects 1 (io_funcs.cu:511, 533 against :83), which
+   leaves coordinate stations at uninitialised indices; this reads them as
+   the host code intends. out = NULL returns the count of the file in
+   *n_file only; otherwise out holds room for *n_file stations, and
+   *n_located are written, in file order. The increments (di, dj, dk) and
+   base nodes feed cgfd_core_set_stations_interp. */
+#define CGFD_STATION_NAME_MAX 256
+typedef struct cgfd_station {
+  char name[CGFD_STATION_NAME_MAX];
+  int i, j, k;        /* ghost-inclusive base node */
+  float di, dj, dk;   /* increments towards i+1, j+1, k+1 */
+  float x, y, z;      /* coordinates: as given, or of the base node */
+} cgfd_station;
+int cgfd_io_read_stations(const char *path, const cgfd_setup *setup, cgfd_station *out, int max_n, int *n_file,
+                          int *n_located);
+
+/* Fault stations (io_fault_recv_read_locate, io_funcs.cu:2170-2300): a
+   count, then "name fault_id j k" per line, fault_id, j and k counted from
+   1, j and k fractional; the base node is the floor, the rest the
+   increments. A station off the fault plane is an error (CGFD_ERR_ARG), as
+   in the reference. x, y, z are the coordinates of the base node. */
+typedef struct cgfd_fault_station {
+  char name[CGFD_STATION_NAME_MAX];
+  int fault_id;       /* from 0 */
+  int j, k;           /* ghost-inclusive base node on the fault plane */
+  float dj, dk;
+  float x, y, z;
+} cgfd_fault_station;
+int cgfd_io_read_fault_stations(const char *path, const cgfd_setup *setup, cgfd_fault_station *out, int max_n,
+                                int *n_file);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,7 @@ def load(path=None):
             "oracle_write_wave": (i32, [vp, vp]),
             "oracle_read_fault": (i32, [vp, i32, vp]),
             "oracle_read_fault_state": (i32, [vp, i32, vp]),
+            "oracle_read_kink_margin": (i32, [vp, i32, vp]),
             "oracle_write_fault_state": (i32, [vp, i32, vp, vp]),
             "oracle_read_pg": (i32, [vp, vp]),
             "oracle_step_index": (i32, [vp]),
@@ -138,6 +139,16 @@ class Oracle:
         out = np.empty((9, 2, nz, ny), np.float64)
         assert self.lib.oracle_read_fault_state(self.h, fid, _p(out)) == 0
         return out
+
+    def read_kink_margin(self, fid=0):
+        """(4, nz, ny): the smallest relative distance of each friction
+        branch decision (shear vs strength, slip vs Dc, normal traction vs 0,
+        |Ts| vs the 0.1 guard) from its switch point over all trial
+        evaluations so far; 1e300 where the trial never ran."""
+        nz, ny, _ = self.shape_ext
+        out = np.empty((nz, ny, 4), np.float64)
+        assert self.lib.oracle_read_kink_margin(self.h, fid, _p(out)) == 0
+        return np.moveaxis(out, -1, 0).copy()
 
     def write_fault_state(self, state, fid=0, t1rec=None):
         nz, ny, _ = self.shape_ext

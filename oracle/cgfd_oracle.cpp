@@ -46,6 +46,10 @@
  * oracle_ad_gradient differentiates the FP64 restatement in reverse mode
  * (ad::Var below): the FP64 reference adjoint the GPU adjoint is checked
  * against, itself pinned by FP64 central differences.
+ * oracle_read_kink_margin reports how close each node's friction branch
+ * decisions came to their switch points over a run (instrumentation of
+ * friction_point for the gradient tests, which compare derivatives only
+ * where the piecewise-smooth map is smooth).
  */
 #include "../include/cgfd_core.h"
 
@@ -259,10 +263,13 @@ template <class T> inline void load9(T M[][3], const T *src) {
    trial traction -> local traction -> normal/shear split, no opening,
    friction coefficient, strength cap, |Ts| < 0.1 guard. vn must be the unit
    normal. Writes the fault traction T1 (slot 3) and tTn/tTs1/tTs2; returns
-   flag_rup. */
+   flag_rup. margin (optional, test instrumentation): the smallest relative
+   distance so far of each branch decision from its switch point, [0] shear
+   vs strength (|Ts| - Tau_c) / Tau_c, [1] slip vs Dc, [2] normal traction vs
+   0 (relative to |traction|), [3] |Ts| vs the 0.1 guard. */
 template <class T>
 int friction_point(const T *Trial, T jacvec, const T *vn, const T *T0, T mus, T mud, T Slip, T Dc, T C0,
-                   const T *vs1, const T *vs2, T *T1c, T &tTn, T &tTs1, T &tTs2) {
+                   const T *vs1, const T *vs2, T *T1c, T &tTn, T &tTs1, T &tTs2, double *margin = nullptr) {
   T TL[3], Ts[3];
   for (int a = 0; a < 3; a++) TL[a] = Trial[a] / jacvec + T0[a];
   T Tn0 = dot3(TL, vn);
@@ -276,6 +283,13 @@ int friction_point(const T *Trial, T jacvec, const T *vn, const T *T0, T mus, T 
   if (Slip <= Dc) friction = mus - (mus - mud) * Slip / Dc;
   else friction = mud;
   T Tau_c = -friction * Tau_n + C0;
+  if (margin) {
+    const double tc = val(Tau_c), ts = val(Ts0), tn = val(Tn0), dc = val(Dc), sl = val(Slip);
+    const double tl = std::sqrt(val(TL[0]) * val(TL[0]) + val(TL[1]) * val(TL[1]) + val(TL[2]) * val(TL[2]));
+    const double m[4] = {std::fabs(ts - tc) / std::max(std::fabs(tc), 1.0), std::fabs(sl - dc) / std::max(dc, 1e-30),
+                         std::fabs(tn) / std::max(tl, 1.0), std::fabs(ts - 0.1) / 0.1};
+    for (int q = 0; q < 4; q++) margin[q] = std::min(margin[q], m[q]);
+  }
   if (Ts0 >= Tau_c) { Tau_s = Tau_c; flag = 1; ifchange = 1; }
   else { Tau_s = Ts0; flag = 0; }
   T Tau[3];
@@ -335,6 +349,7 @@ struct Oracle {
     int i0;
     std::vector<T> T0x, T0y, T0z, mu_s, mu_d, Dc, C0;
     std::vector<int> united, faultgrid, rupy, rupz, flag_rup, init_t0_flag;
+    std::vector<double> kink;  /* [sslice][4] branch margins of friction_point */
     std::vector<T> out;  /* [11][sslice] */
     T *Tn, *Ts1, *Ts2, *Vs, *Vs1, *Vs2, *Slip, *Slip1, *Slip2, *Peak_vs, *Init_t0;
     std::vector<T> tTn, tTs1, tTs2;
@@ -422,6 +437,7 @@ struct Oracle {
       cpi(F.rupy, fd.rup_index_y, sslice); cpi(F.rupz, fd.rup_index_z, sslice);
       F.flag_rup.assign(sslice, 0);       /* fault_info.cu:1069 */
       F.init_t0_flag.assign(sslice, 0);   /* fault_info.cu:960 */
+      F.kink.assign(4 * sslice, 1e300);
       F.out.assign(11 * sslice, (T)0);
       T *o = F.out.data();
       F.Tn = o; F.Ts1 = o + sslice; F.Ts2 = o + 2 * sslice; F.Vs = o + 3 * sslice;
@@ -724,7 +740,7 @@ struct Oracle {
           T T0v[3] = {F.T0x[pf], F.T0y[pf], F.T0z[pf]};
           T T1c[3], tn, ts1, ts2;
           F.flag_rup[pf] = friction_point<T>(Trial, jacvec, vn, T0v, F.mu_s[pf], F.mu_d[pf], F.Slip[pf], F.Dc[pf],
-                                             F.C0[pf], vs1, vs2, T1c, tn, ts1, ts2);
+                                             F.C0[pf], vs1, vs2, T1c, tn, ts1, ts2, &F.kink[4 * pf]);
           size_t p3 = pf + 3 * sslice;
           T1x[p3] = T1c[0]; T1y[p3] = T1c[1]; T1z[p3] = T1c[2];
           F.tTs1[pf] = ts1;
@@ -1619,6 +1635,18 @@ int oracle_write_fault_state(void *hp, int id, const double *in, const double *t
         F.mT1x[i] = (T)t1rec[3 * ss + i]; F.mT1y[i] = (T)t1rec[4 * ss + i]; F.mT1z[i] = (T)t1rec[5 * ss + i];
       }
     }
+    return 0;
+  });
+}
+
+/* The smallest relative distance of each friction branch decision from its
+   switch point over all trial evaluations since creation, out[sslice][4]
+   (see friction_point); 1e300 at nodes that never reached the trial. */
+int oracle_read_kink_margin(void *hp, int id, double *out) {
+  return dispatch(hp, [&](auto *o) {
+    if (id < 0 || id >= o->nfault) return -1;
+    auto &k = o->flt[id].kink;
+    for (size_t i = 0; i < k.size(); i++) out[i] = k[i];
     return 0;
   });
 }

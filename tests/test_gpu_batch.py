@@ -29,19 +29,28 @@ def _plain_with_fields(sc, fields):
     return model.Setup(sc2)
 
 
-CASES = [(v, "default") for v in (1, 16, 19, 20, 21)] + [(20, "first_retires"), (1, "first_retires"),
-                                                                (20, "ragged_slab"), (21, "ragged_slab")]
+CASES = [(v, "default") for v in (1, 16, 19, 20, 21, 23, 24)] + [(20, "first_retires"), (1, "first_retires"),
+                                                                    (24, "first_retires"), (23, "ragged_slab"),
+                                                                    (20, "ragged_slab"), (21, "ragged_slab"),
+                                                                    (23, "wide"), (24, "wide")]
 
 
 @pytest.mark.parametrize("variant,case", CASES)
 def test_batch_matches_plain_runs(built_libs, variant, case):
     """default: 4 scenarios, scenarios 1 and 3 retire first; first_retires:
     scenario 0 retires first and one scenario runs alone at the end;
-    ragged_slab: nj = 50, so the last slab of 4-row tiles is partial."""
+    ragged_slab: nj = 50, so the last slab of 4-row tiles is partial; wide:
+    ni = 110, wide enough for the box split's interior box (columns 32..95)
+    next to its bands."""
     from cgfd import configs, model
     import copy
     import oracle
-    sc = configs.small(imethod=2, nj=50) if case == "ragged_slab" else configs.small(imethod=2)
+    if case == "ragged_slab":
+        sc = configs.small(imethod=2, nj=50)
+    elif case == "wide":
+        sc = configs.small(imethod=2, ni=110, nj=50)
+    else:
+        sc = configs.small(imethod=2)
     st = model.Setup(sc)
     nts = [20, 40, 33] if case == "first_retires" else [40, 25, 40, 33]
     fields = [None] + [_scenario_fields(sc, 100 + s) for s in range(1, len(nts))]

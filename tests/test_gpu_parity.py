@@ -143,7 +143,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 16, 19, 20, 21])
+@pytest.mark.parametrize("variant", [1, 16, 19, 20, 21, 23, 24])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation the autotuner chooses from."""
     from cgfd import configs, model
@@ -183,7 +183,7 @@ def test_graph_replay_matches_eager(built_libs):
     assert np.array_equal(cg.read_stations(), ce.read_stations())
 
 
-@pytest.mark.parametrize("variant", [16, 19, 20, 1])
+@pytest.mark.parametrize("variant", [16, 19, 20, 1, 23, 24])
 def test_stage_fork_matches_serial(built_libs, variant):
     """The stage fork (interior blocks, CFS-PML blocks + top layers, fault
     kernels on three streams) against every kernel on one stream: the
@@ -283,7 +283,7 @@ def test_locked_fault_transmits_without_slip_gpu(built_libs, imethod):
     st.close()
 
 
-@pytest.mark.parametrize("variant", [20, 16, 1])
+@pytest.mark.parametrize("variant", [20, 16, 1, 23, 24])
 def test_ragged_grid_matches_oracle(built_libs, variant):
     """A grid whose sizes are multiples of none of the tile sizes (37 x 45 x
     27: partial 64-wide x tiles, a partial last row of 4-row y tiles, z
@@ -344,4 +344,29 @@ def test_slices_and_snapshots_match_the_wavefield(built_libs):
     assert err <= 1e-6
     assert np.array_equal(core.read_snapshot(box, 4), snap[9:])
     core.close()
+    st.close()
+
+
+@pytest.mark.parametrize("variant", [23, 24])
+def test_box_split_ragged_grid_matches_oracle(built_libs, variant):
+    """The box-split schedule where its interior box is not empty and no
+    size is a multiple of a tile: 133 x 61 x 35 nodes with PML 5 and a free
+    surface give an interior box of columns 32..127 (three 32-wide or one
+    and a half 64-wide tiles), partial 4- or 8-row tiles and six-sided
+    bands around it, with the fault plane inside the box."""
+    from cgfd import configs, model
+    import oracle
+    sc = configs.small(ni=133, nj=61, nk=35, pml=5, imethod=2, curvilinear=True)
+    st = model.Setup(sc)
+    core = model.Core(st, variant=variant)
+    core.set_zchunk(4)
+    orc = oracle.Oracle.from_scenario(sc)
+    core.run(24)
+    orc.run(24)
+    r = rel_l2(interior(core.read_wavefield()), interior(orc.read_wavefield()))
+    rs = rel_l2(fault_phys(core.read_fault(0))[6], fault_phys(orc.read_fault(0))[6])
+    print(f"[box split {variant}] wavefield {r:.2e}, slip {rs:.2e}", flush=True)
+    assert r <= TOL and rs <= TOL
+    core.close()
+    orc.close()
     st.close()

@@ -164,3 +164,36 @@ def test_plane_column_shows_the_txz_tyz_swap(built_libs, fix_q1):
     assert np.abs(w1[6][c] - w1[7][c]).max() > 0.1 * nrm
     for cmp in (3, 4, 5, 8):
         assert np.abs(w[cmp][c] - w1[cmp][c]).max() <= 2e-4 * nrm
+
+
+def test_kink_margins_of_the_friction_branches(built_libs):
+    """The branch margins the adjoint tests select smooth nodes with. Under
+    the uniform stress drop (mu_s = mu_d = 0.5, sigma_n = 100 MPa, shear 10
+    MPa above strength) the first trial evaluation sees the initial traction,
+    so the shear-vs-strength margin is at most 10 / 50 = 0.2, and it is the
+    same at every node away from the slipping region's edges; the welded
+    nodes never reach the trial (1e300); a locked fault (mu = 1000) keeps
+    every node far from its switch."""
+    import copy
+    import oracle
+    sc = stress_drop_scenario("strike", 2)
+    orc = oracle.Oracle.from_scenario(sc, double=True)
+    orc.run(3)
+    m = orc.read_kink_margin(0)
+    orc.close()
+    act = m[0] < 1e300
+    ks, js = np.nonzero(act)
+    assert act.sum() > 1000 and np.all(m[:, 0:3, :] == 1e300)
+    inner = m[0][ks.min() + 6:ks.max() - 5, js.min() + 6:js.max() - 5]
+    assert np.all(inner < 1e300)
+    assert inner.max() - inner.min() <= 1e-12
+    assert 0.15 < inner.min() <= 0.2 + 1e-12
+    assert m[1][act].min() > 0.5        # slip stays far below Dc
+    locked = copy.deepcopy(sc)
+    locked.faults[0].fields["mu_s"][:] = 1000.0
+    locked.faults[0].fields["mu_d"][:] = 1000.0
+    orc = oracle.Oracle.from_scenario(locked, double=True)
+    orc.run(3)
+    ml = orc.read_kink_margin(0)[0]
+    orc.close()
+    assert ml[ml < 1e300].min() > 0.99
