@@ -64,7 +64,7 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, fix_sta = 0, nan_every = 0;
-  int variant = 20;
+  int variant = 23;
   int zchunk = 4;
   int nvol = 0;
   DevBuf mem;
@@ -1565,7 +1565,8 @@ void record_tuned(const TuneKey &k, int v, int z) {
 }
 }  // namespace
 
-/* variant 0: the tuned choice for this shape, else 20 (zchunk 4) */
+/* variant 0: the tuned choice for this shape, else the box split 23 with
+   8 y-tiles per slab (the fastest untuned choice on every shape measured) */
 static void apply_auto_variant(cgfd_core *c) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   read_tune_file();
@@ -1574,8 +1575,8 @@ static void apply_auto_variant(cgfd_core *c) {
     c->variant = it->second.first;
     c->zchunk = it->second.second;
   } else {
-    c->variant = 20;
-    c->zchunk = 4;
+    c->variant = 23;
+    c->zchunk = 8;
   }
 }
 
@@ -1620,7 +1621,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   const bool batch = c->bat.nact > 1;
-  const std::vector<int> variants = batch ? std::vector<int>{20, 23, 24, 19, 21, 16, 1} : std::vector<int>{20, 23, 24, 16, 1};
+  const std::vector<int> variants = batch ? std::vector<int>{23, 24, 20, 19, 21, 16, 1} : std::vector<int>{23, 24, 20, 16, 1};
   const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;
