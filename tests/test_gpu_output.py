@@ -1,7 +1,10 @@
 """Output files in the reference's NetCDF layouts (include/cgfd_io.h) from a
 short rupture: fault frames every 5 steps with Peak_vs/Init_t0 at the end,
 x/y/z slices, and the PG file, read back with scipy's NetCDF reader and
-compared bit for bit with what the core returns at the same steps."""
+compared bit for bit with what the core returns at the same steps, and with
+the FP32 oracle run in step with it (the reference's io_fault_nc_put,
+io_slice_nc_put and PG_slice_output values at those steps, to the parity
+tolerance)."""
 import ctypes as C
 
 import numpy as np
@@ -17,9 +20,13 @@ def _ok(core, rc):
 def test_fault_slice_pg_files(built_libs, tmp_path):
     from cgfd import configs, model
     from scipy.io import netcdf_file
+    import oracle
+    from helpers import rel_l2
     sc = configs.small(imethod=2)
     st = model.Setup(sc)
     core = model.Core(st)
+    orc = oracle.Oracle.from_scenario(sc)
+    o_faults, o_slc = [], []
     lib, h = core.lib, core.handle
     i0 = sc.faults[0].x_index + 3
     d = str(tmp_path).encode()
@@ -46,7 +53,12 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
             slc.append([core.read_slice(ax, idx) for ax, idx in slices])
             _ok(core, lib.cgfd_io_snap_put(sn, h, it // 5, C.c_float(it * dt)))
             snaps.append(core.read_snapshot(box, 7))
+            o_faults.append(orc.read_fault(0)[:9, 3:-3, 3:-3].copy())
+            ow = orc.read_wavefield()
+            o_slc.append([ow[:, :, :, idx] if ax == 0 else (ow[:, :, idx, :] if ax == 1 else ow[:, idx, :, :])
+                          for ax, idx in slices])
         core.run(1)
+        orc.run(1)
     _ok(core, lib.cgfd_io_fault_end(fo, h))
     for o in so + [sn]:
         _ok(core, lib.cgfd_io_close(o))
@@ -65,6 +77,12 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
         assert np.array_equal(f.variables["Init_t0"][:], fend[10, 3:-3, 3:-3])
         assert int(f.i_index_with_ghosts_in_this_thread) == i0
         assert float(np.abs(f.variables["Slip"][3]).max()) > 0.0
+        # the frames against the oracle's fault outputs at the same steps
+        for q in (0, 3, 6):
+            r = rel_l2(np.asarray(f.variables[names[q]][3], np.float64), o_faults[3][q])
+            assert r <= 1e-3, (names[q], r)
+        ofend = orc.read_fault(0)
+        assert rel_l2(np.asarray(f.variables["Peak_vs"][:], np.float64), ofend[9, 3:-3, 3:-3]) <= 1e-3
     wn = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy"]
     for s_, (ax, idx) in enumerate(slices):
         fn = ["slicex_i", "slicey_j", "slicez_k"][ax] + f"{idx - 2}_px0_py0_pz0.nc"
@@ -72,6 +90,9 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
             for q, n in enumerate(wn):
                 for r in range(4):
                     assert np.array_equal(f.variables[n][r], slc[r][s_][q]), (fn, n, r)
+            # the physical part of the last frame against the oracle's wavefield
+            got = np.stack([np.asarray(f.variables[n][3], np.float64) for n in wn])
+            assert rel_l2(got, o_slc[3][s_][:, 3:-3, 3:-3]) <= 1e-3, fn
     with netcdf_file(str(tmp_path / "snap_001_px0_py0_pz0.nc"), "r", mmap=False) as f:
         assert (f.dimensions["k"], f.dimensions["j"], f.dimensions["i"]) == (8, 7, 9)
         order = ["Vx", "Vy", "Vz", "Txx", "Tyy", "Tzz", "Tyz", "Txz", "Txy", "Exx", "Eyy", "Ezz", "Eyz", "Exz", "Exy"]
@@ -89,6 +110,10 @@ def test_fault_slice_pg_files(built_libs, tmp_path):
         assert np.array_equal(f.variables["PGV"][:], buf[0])
         assert np.array_equal(f.variables["PGDz"][:], buf[14])
         assert float(buf[0].max()) > 0.0
+        opg = orc.read_pg()
+        assert rel_l2(np.asarray(f.variables["PGV"][:], np.float64), opg[0]) <= 1e-3
+        assert rel_l2(np.asarray(f.variables["PGDz"][:], np.float64), opg[14]) <= 1e-3
+    orc.close()
     core.close()
     st.close()
 
