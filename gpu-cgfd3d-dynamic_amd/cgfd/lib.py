@@ -151,6 +151,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_set_fault_receivers_interp": (I32, [VP, I32, VP, VP, I32]),
         "cgfd_io_read_stations": (I32, [C.c_char_p, VP, VP, I32, C.POINTER(I32), C.POINTER(I32)]),
         "cgfd_io_read_fault_stations": (I32, [C.c_char_p, VP, VP, I32, C.POINTER(I32)]),
+        "cgfd_io_line_stations": (I32, [VP, C.c_char_p, VP, VP, I32, VP, C.POINTER(I32)]),
         "cgfd_core_get_variant": (I32, [VP, C.POINTER(I32), C.POINTER(I32)]),
         "cgfd_core_read_snapshot_noalloc": (I32, [VP, VP, I32, VP]),
         "cgfd_drv_rk_curv_col_allstep": (I32, [C.POINTER(Desc), VP, VP, VP]),

@@ -121,6 +121,19 @@ class Setup:
         return [{"name": s.name.decode(), "ijk": (s.i, s.j, s.k), "dijk": (s.di, s.dj, s.dk), "xyz": (s.x, s.y, s.z)}
                 for s in arr[:n_loc.value]]
 
+    def line_stations(self, name: str, start, incre, count: int) -> list:
+        """A receiver line (cgfd_io_line_stations): start, incre = global
+        physical (i, j, k) from 0; the nodes inside the grid as station
+        dicts named <name>.no<n>."""
+        st = np.ascontiguousarray(start, np.int32)
+        inc = np.ascontiguousarray(incre, np.int32)
+        arr = (L.Station * max(count, 1))()
+        n = C.c_int()
+        L.check(self.lib, self.lib.cgfd_io_line_stations(self.handle, name.encode(), L.ptr(st), L.ptr(inc), count, arr,
+                                                         C.byref(n)))
+        return [{"name": s.name.decode(), "ijk": (s.i, s.j, s.k), "dijk": (s.di, s.dj, s.dk), "xyz": (s.x, s.y, s.z)}
+                for s in arr[:n.value]]
+
     def read_fault_stations(self, path) -> list:
         """The fault station file at path (cgfd_io_read_fault_stations):
         dicts with name, fault_id, jk (ghost-inclusive base node), djk and

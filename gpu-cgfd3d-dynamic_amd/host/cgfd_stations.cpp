@@ -4,6 +4,7 @@
  *
  *   station list   io_recv_read_locate, io_funcs.cu:23-320
  *   fault stations io_fault_recv_read_locate, io_funcs.cu:2170-2300
+ *   receiver lines io_line_locate, io_funcs.cu:312-425 (SAC names :771)
  *   locator        gd_curv_coord_to_local_indx, gd_t.cu:1365-1503;
  *                  gd_curv_depth_to_axis, gd_t.cu:1522-1612;
  *                  gd_curv_coord2index_sample, gd_t.cu:1694-1775;
@@ -413,5 +414,30 @@ extern "C" int cgfd_io_read_fault_stations(const char *path, const cgfd_setup *s
     const long p = g.at(d.fault[fid].fault_x_index + 3, s.j, s.k);
     s.x = g.x[p]; s.y = g.y[p]; s.z = g.z[p];
   }
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_io_line_stations(const cgfd_setup *setup, const char *name, const int *start, const int *incre,
+                                     int count, cgfd_station *out, int *n_out) {
+  if (!setup || !name || !start || !incre || !n_out || count < 0 || (count > 0 && !out))
+    return cgfd_set_error(CGFD_ERR_ARG, "cgfd_io_line_stations: bad arguments");
+  *n_out = 0;
+  Ext g;
+  if (int rc = ext_of(setup, g)) return rc;
+  int n = 0;
+  for (int ipt = 0; ipt < count; ipt++) {
+    const int gi = start[0] + ipt * incre[0], gj = start[1] + ipt * incre[1], gk = start[2] + ipt * incre[2];
+    if (gi < 0 || gi > g.ni - 1 || gj < 0 || gj > g.nj - 1 || gk < 0 || gk > g.nk - 1) continue;
+    cgfd_station &s = out[n++];
+    const int w = std::snprintf(s.name, CGFD_STATION_NAME_MAX, "%s.no%d", name, ipt);
+    if (w < 0 || w >= CGFD_STATION_NAME_MAX)
+      return cgfd_set_error(CGFD_ERR_ARG, "receiver line %s: name longer than %d characters", name,
+                            CGFD_STATION_NAME_MAX - 1);
+    s.i = gi + 3; s.j = gj + 3; s.k = gk + 3;
+    s.di = s.dj = s.dk = 0.0f;
+    const long p = g.at(s.i, s.j, s.k);
+    s.x = g.x[p]; s.y = g.y[p]; s.z = g.z[p];
+  }
+  *n_out = n;
   return CGFD_OK;
 }

@@ -145,6 +145,17 @@ typedef struct cgfd_fault_station {
 int cgfd_io_read_fault_stations(const char *path, const cgfd_setup *setup, cgfd_fault_station *out, int max_n,
                                 int *n_file);
 
+/* A receiver line of the par file (receiver_line_index_start / _incre /
+   _count, io_line_locate, io_funcs.cu:312-425): the nodes start + n incre
+   (global physical indices from 0) for n < count that lie in the grid, in
+   out (room for count) as stations named "<name>.no<n>" -- the names the
+   reference gives their SAC files (io_line_output_sac, :771), so
+   cgfd_io_stations_sac writes <name>.no<n>.<cmp>.sac -- with zero
+   increments and the node's coordinates. io_line_keep records the nodes'
+   values each step, as cgfd_core_set_stations(_interp) does. */
+int cgfd_io_line_stations(const cgfd_setup *setup, const char *name, const int *start, const int *incre, int count,
+                          cgfd_station *out, int *n_out);
+
 #ifdef __cplusplus
 }
 #endif

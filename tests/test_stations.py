@@ -120,3 +120,30 @@ def test_fault_stations(built_libs, tmp_path):
     with pytest.raises(RuntimeError, match="outside the calculation area"):
         st.read_fault_stations(bad)
     st.close()
+
+
+def test_receiver_line(built_libs):
+    """A receiver line (io_line_locate, io_funcs.cu:312-425): nodes start + n
+    incre inside the grid, named <line>.no<n> as the reference names their
+    SAC files, with their coordinates; nodes past the grid are skipped."""
+    sc, st = _setup()
+    got = st.line_stations("line_x", (2, 10, sc.nk - 1), (5, 0, 0), 10)
+    n_in = len([n for n in range(10) if 2 + 5 * n <= sc.ni - 1])
+    assert len(got) == n_in and n_in < 10
+    assert [s["name"] for s in got] == [f"line_x.no{n}" for n in range(n_in)]
+    xyz = st.coords()
+    for n, s in enumerate(got):
+        assert s["ijk"] == (2 + 5 * n + 3, 13, sc.nk + 2) and s["dijk"] == (0.0, 0.0, 0.0)
+        assert s["xyz"] == tuple(float(xyz[c, sc.nk + 2, 13, 2 + 5 * n + 3]) for c in range(3))
+    st.close()
+
+
+def test_receiver_line_keeps_the_point_numbers(built_libs):
+    """io_line_locate numbers a kept node by its place on the line
+    (recv_seq = ipt, io_funcs.cu:412), skipped nodes included."""
+    sc, st = _setup()
+    got = st.line_stations("l2", (-8, 4, 5), (5, 1, 0), 4)
+    assert [s["name"] for s in got] == ["l2.no2", "l2.no3"]
+    assert [s["ijk"] for s in got] == [(2 + 3, 6 + 3, 5 + 3), (7 + 3, 7 + 3, 5 + 3)]
+    assert st.line_stations("empty", (0, 0, 0), (1, 1, 1), 0) == []
+    st.close()
