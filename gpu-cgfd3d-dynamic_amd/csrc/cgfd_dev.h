@@ -103,7 +103,10 @@ struct VolBands {
   int x0[6], x1[6], y0[6], y1[6], k0[6], k1[6];
   int ox[6];          /* x origin of a region's 32-wide tiles (32-aligned) */
   int ntx[6], nty[6]; /* 32 x 8 tiles per plane */
-  long cum[7];        /* block prefix sums over the regions */
+  int sy[6];          /* y tiles per slab: a region's blocks run x tile
+                         fastest, then the sy y tiles of a slab, then the
+                         plane, then the slab */
+  long cum[7];        /* block prefix sums over the regions (padded slabs) */
 };
 
 /* ---------------------------------------------------------------------- */

@@ -369,20 +369,33 @@ __global__ void __launch_bounds__(256, 8) k_vol_acc_box(VolArgs a, BatArgs b, Vo
   acc_point<ID, JD, KD, 1>(a, b, i, j, x.k0 + kz, b.act[0]);
 }
 
-template <int ID, int JD, int KD>
-__global__ void __launch_bounds__(256, 6) k_vol_pml_band(VolArgs a, BatArgs b, VolBands r) {
-  const long L0 = blockIdx.x;
+/* block L0 of the bands -> the thread's point; false: a lane outside its
+   region or a padding block past a region's last slab. Within a region the
+   blocks run in slab order, so the planes whose z taps a block reads are
+   still being worked on by resident blocks when it runs. */
+DEVI bool band_point(const VolBands &r, long L0, int &i, int &j, int &k) {
   int q = 0;
   while (q + 1 < r.n && L0 >= r.cum[q + 1]) q++;
   long L = L0 - r.cum[q];
   const int bx = (int)(L % r.ntx[q]);
   L /= r.ntx[q];
-  const int by = (int)(L % r.nty[q]);
-  L /= r.nty[q];
+  const int cy = (int)(L % r.sy[q]);
+  L /= r.sy[q];
+  const int nkq = r.k1[q] - r.k0[q];
+  const int kz = (int)(L % nkq);
+  const int by = (int)(L / nkq) * r.sy[q] + cy;
   const int t = (int)threadIdx.x;
-  const int i = r.ox[q] + bx * 32 + (t & 31), j = r.y0[q] + by * 8 + (t >> 5);
-  if (i < r.x0[q] || i >= r.x1[q] || j >= r.y1[q]) return;
-  pml_acc_point<ID, JD, KD>(a, b, i, j, r.k0[q] + (int)L, b.act[0]);
+  i = r.ox[q] + bx * 32 + (t & 31);
+  j = r.y0[q] + by * 8 + (t >> 5);
+  k = r.k0[q] + kz;
+  return by < r.nty[q] && i >= r.x0[q] && i < r.x1[q] && j < r.y1[q];
+}
+
+template <int ID, int JD, int KD>
+__global__ void __launch_bounds__(256, 6) k_vol_pml_band(VolArgs a, BatArgs b, VolBands r) {
+  int i, j, k;
+  if (!band_point(r, blockIdx.x, i, j, k)) return;
+  pml_acc_point<ID, JD, KD>(a, b, i, j, k, b.act[0]);
 }
 
 /* batches on the box split: the interior box's (tile, scenario) pairs with
@@ -411,19 +424,12 @@ template <int ID, int JD, int KD>
 __global__ void __launch_bounds__(256, 6) k_vol_bat_pml_band(VolArgs a, BatArgs b, VolBands r) {
   const long per = r.cum[r.n];
   const int sq = (int)(blockIdx.x / per);
-  const long L0 = blockIdx.x % per;
-  int q = 0;
-  while (q + 1 < r.n && L0 >= r.cum[q + 1]) q++;
-  long L = L0 - r.cum[q];
-  const int bx = (int)(L % r.ntx[q]);
-  L /= r.ntx[q];
-  const int by = (int)(L % r.nty[q]);
-  L /= r.nty[q];
-  const int t = (int)threadIdx.x;
-  const int i = r.ox[q] + bx * 32 + (t & 31), j = r.y0[q] + by * 8 + (t >> 5);
-  if (i < r.x0[q] || i >= r.x1[q] || j >= r.y1[q]) return;
-  pml_acc_point<ID, JD, KD>(a, b, i, j, r.k0[q] + (int)L, b.act[sq]);
+  int i, j, k;
+  if (!band_point(r, blockIdx.x % per, i, j, k)) return;
+  pml_acc_point<ID, JD, KD>(a, b, i, j, k, b.act[sq]);
 }
+
+constexpr int kBandSlabBlocks = 192;
 
 /* the interior box and the bands around it for one launch's planes */
 static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands &r) {
@@ -449,7 +455,13 @@ static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands &r) {
     r.ox[q] = x0 & ~31;
     r.ntx[q] = (x1 - r.ox[q] + 31) / 32;
     r.nty[q] = (y1 - y0 + 7) / 8;
-    r.cum[q + 1] = r.cum[q] + (long)r.ntx[q] * r.nty[q] * (k1 - k0);
+    /* a plane of a slab holds at most kBandSlabBlocks blocks, so the seven
+       planes a z derivative reads fit among the ~1500 blocks resident at 6
+       waves per SIMD (a whole plane of the bottom band of a 600 x 1200 grid
+       is 2850 blocks) */
+    r.sy[q] = std::max(1, std::min(r.nty[q], kBandSlabBlocks / r.ntx[q]));
+    const long nslab = (r.nty[q] + r.sy[q] - 1) / r.sy[q];
+    r.cum[q + 1] = r.cum[q] + (long)r.ntx[q] * r.sy[q] * nslab * (k1 - k0);
   };
   if (xr <= xl || yr <= yl || kr <= kl) {
     x = VolBox{3, 3, 3, 3, kv0, kv0, 0, 0};
