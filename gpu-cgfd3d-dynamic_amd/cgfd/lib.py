@@ -119,6 +119,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_core_run_timed": (I32, [VP, I32, C.POINTER(F32), C.POINTER(F32), C.POINTER(I32)]),
         "cgfd_core_set_variant": (I32, [VP, I32]),
         "cgfd_core_set_zchunk": (I32, [VP, I32]),
+        "cgfd_core_set_vertical_columns": (I32, [VP, I32]),
+        "cgfd_core_get_vertical_columns": (I32, [VP, C.POINTER(I32), C.POINTER(I32)]),
         "cgfd_core_set_graph": (I32, [VP, I32]),
         "cgfd_core_set_overlap": (I32, [VP, I32]),
         "cgfd_core_state_bytes": (I32, [VP, C.POINTER(C.c_size_t)]),

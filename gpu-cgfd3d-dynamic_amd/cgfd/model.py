@@ -320,6 +320,17 @@ class Core:
     def set_zchunk(self, z: int):
         L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
 
+    def set_vertical_columns(self, on: bool):
+        """on: the five-term metric form on grids of vertical columns (the
+        default); off: the nine-term form everywhere."""
+        L.check(self.lib, self.lib.cgfd_core_set_vertical_columns(self.handle, int(on)))
+
+    def vertical_columns(self):
+        """(the grid has the vertical-column form, the volume update uses it)"""
+        g, u = C.c_int(), C.c_int()
+        L.check(self.lib, self.lib.cgfd_core_get_vertical_columns(self.handle, C.byref(g), C.byref(u)))
+        return bool(g.value), bool(u.value)
+
     def autotune(self, reps: int = 3):
         """Pick the fastest volume-kernel variant and z chunk for this grid
         (recorded for every later core of the same shape)."""
@@ -472,6 +483,17 @@ class Batch:
 
     def set_zchunk(self, z: int):
         L.check(self.lib, self.lib.cgfd_core_set_zchunk(self.handle, z))
+
+    def set_vertical_columns(self, on: bool):
+        """on: the five-term metric form on grids of vertical columns (the
+        default); off: the nine-term form everywhere."""
+        L.check(self.lib, self.lib.cgfd_core_set_vertical_columns(self.handle, int(on)))
+
+    def vertical_columns(self):
+        """(the grid has the vertical-column form, the volume update uses it)"""
+        g, u = C.c_int(), C.c_int()
+        L.check(self.lib, self.lib.cgfd_core_get_vertical_columns(self.handle, C.byref(g), C.byref(u)))
+        return bool(g.value), bool(u.value)
 
     def autotune(self, reps: int = 3):
         return _autotune(self, reps)

@@ -88,6 +88,10 @@ struct VolArgs {
   int k_lo;          /* first physical k offset handled */
   int nk_run;        /* number of k planes handled */
   int zchunk;        /* planes per block (LDS kernel) */
+  int vcol;          /* 1: the grid's columns are vertical (xi_y, xi_z, eta_x
+                        and eta_z below 2^-32 of xi_x and eta_y at every
+                        physical point): the box split skips those four
+                        metric arrays */
   FaultSkip fs;
 };
 

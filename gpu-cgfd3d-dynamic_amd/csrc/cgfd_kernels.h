@@ -94,6 +94,10 @@ void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, in
                    hipStream_t st, hipStream_t st_pml = nullptr);
 void launch_dvh2dvz(const Grid &g, const float *met, const float *lam, const float *mu, float *mx, float *my,
                     hipStream_t st);
+/* *flag (device, zeroed by the caller) becomes nonzero if at some physical
+   point |xi_y| or |xi_z| exceeds 2^-32 |xi_x|, or |eta_x| or |eta_z| exceeds
+   2^-32 |eta_y| (or one of them is NaN) */
+void launch_vcol_check(const Grid &g, const float *met, int *flag, hipStream_t st);
 void launch_pg(const Grid &g, const BatArgs &b, const float *wend, const float *wpre, float *PG, float *Dis, float dt,
                hipStream_t st);
 

@@ -64,6 +64,8 @@ struct cgfd_core {
   int ni = 0, nj = 0, nk = 0;
   float dt = 0, t0 = 0;
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, fix_sta = 0, nan_every = 0;
+  int vcol_grid = 0;  /* xi_y, xi_z, eta_x, eta_z below 2^-32 of xi_x, eta_y everywhere */
+  int vcol_on = 1;    /* use the vertical-column form where the grid has it */
   int variant = 23;
   int zchunk = 4;
   int nvol = 0;
@@ -542,8 +544,8 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
     const int ntop = c->isfree ? (g.nk < 4 ? g.nk : 4) : 0;
     c->nvol = g.nk - ntop;
   }
-  if ((rc = c->alloc(&c->it_dev, 2))) return fail(rc);
-  if (hipMemset(c->it_dev, 0, 8) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
+  if ((rc = c->alloc(&c->it_dev, 3))) return fail(rc);
+  if (hipMemset(c->it_dev, 0, 12) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
   c->nan_flag = c->it_dev + 1;
   if (c->isfree) {
     c->bat.pgstride = 18L * g.siz;
@@ -553,8 +555,12 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
     c->Dis = c->PG + 15 * g.siz;
     launch_dvh2dvz(g, c->met, c->lam, c->mu, c->matVx2Vz, c->matVy2Vz, c->st);
   }
-  if (hipStreamSynchronize(c->st) != hipSuccess || hipGetLastError() != hipSuccess)
+  launch_vcol_check(g, c->met, c->it_dev + 2, c->st);
+  int not_vertical = 1;
+  if (hipMemcpyAsync(&not_vertical, c->it_dev + 2, 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess || hipGetLastError() != hipSuccess)
     return fail(cgfd_set_error(CGFD_ERR_HIP, "setup kernels failed"));
+  c->vcol_grid = not_vertical == 0;
   apply_auto_variant(c);  /* the tuned schedule of this shape, if any */
   *out = c;
   return CGFD_OK;
@@ -604,6 +610,7 @@ void fill_vol(cgfd_core *c, VolArgs &v, int istage, const float *cur, float *tne
   v.fs.nfault = c->nfault;
   for (int q = 0; q < c->nfault; q++) { v.fs.i0[q] = c->flt[q].i0; v.fs.united[q] = c->flt[q].united; }
   v.zchunk = c->zchunk;
+  v.vcol = c->vcol_grid && c->vcol_on;
 }
 
 void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int kdir, float *wcur, float *wnew,
@@ -1594,6 +1601,20 @@ extern "C" int cgfd_core_get_variant(const cgfd_core *c, int *variant, int *zchu
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant) *variant = c->variant;
   if (zchunk) *zchunk = c->zchunk;
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_set_vertical_columns(cgfd_core *c, int on) {
+  if (!c || (on != 0 && on != 1)) return cgfd_set_error(CGFD_ERR_ARG, "cgfd_core_set_vertical_columns: bad arguments");
+  c->vcol_on = on;
+  c->drop_graph();
+  return CGFD_OK;
+}
+
+extern "C" int cgfd_core_get_vertical_columns(const cgfd_core *c, int *grid, int *in_use) {
+  if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
+  if (grid) *grid = c->vcol_grid;
+  if (in_use) *in_use = c->vcol_grid && c->vcol_on && (c->variant == 23 || c->variant == 24);
   return CGFD_OK;
 }
 

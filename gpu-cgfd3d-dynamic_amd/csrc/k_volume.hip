@@ -105,16 +105,26 @@ DEVI bool block_touches_pml(const VolArgs &a, int i0, int i1, int j0, int j1, in
 /* The per-point interior update with each field's three derivatives from
    `deriv(f, dx, dy, dz)`. G > 0: the scheduler may not move instructions
    across the end of every G-th field, which bounds the loads in flight (and
-   the registers) per wave */
-template <int G, class Deriv>
+   the registers) per wave.
+   VC: the grid's columns are vertical (x depends on i alone, y on j alone):
+   xi_y, xi_z, eta_x and eta_z vanish, and the metric computed from such a
+   grid holds there only the rounding of its finite differences (about
+   1e-15 of xi_x and eta_y; the core checks on the device at create that
+   they stay below 2^-32 of them at every point). Those four arrays are not
+   read and their products not formed. The gradients are explicit fused
+   chains in which a zero term would drop out exactly; the omitted products
+   lie 2^-8 of an ulp or more below the kept ones, so both forms agree
+   to the last bit at nearly every point and within an ulp elsewhere. */
+template <int G, bool VC, class Deriv>
 DEVI void acc_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn, Deriv deriv) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   const unsigned o = (unsigned)p * 4u;
   const float *M = a.met;
   const long woff = (long)sn * b.wstride;
-  const float xx = ld32nt(M + sc, o), xy = ld32nt(M + 2 * sc, o), xz = ld32nt(M + 3 * sc, o);
-  const float ex = ld32nt(M + 4 * sc, o), ey = ld32nt(M + 5 * sc, o), ez = ld32nt(M + 6 * sc, o);
+  const float xx = ld32nt(M + sc, o), ey = ld32nt(M + 5 * sc, o);
+  const float xy = VC ? 0.0f : ld32nt(M + 2 * sc, o), xz = VC ? 0.0f : ld32nt(M + 3 * sc, o);
+  const float ex = VC ? 0.0f : ld32nt(M + 4 * sc, o), ez = VC ? 0.0f : ld32nt(M + 6 * sc, o);
   const float zx = ld32nt(M + 7 * sc, o), zy = ld32nt(M + 8 * sc, o), zz = ld32nt(M + 9 * sc, o);
   float gvx[3], gvy[3], gvz[3];
   float hv0 = 0.f, hv1 = 0.f, hv2 = 0.f;
@@ -122,9 +132,16 @@ DEVI void acc_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
   for (int f = 0; f < 9; f++) {
     float dx, dy, dz;
     deriv(f, o, dx, dy, dz);
-    float g0 = xx * dx + ex * dy + zx * dz;  /* d/dx */
-    float g1 = xy * dx + ey * dy + zy * dz;  /* d/dy */
-    float g2 = xz * dx + ez * dy + zz * dz;  /* d/dz */
+    float g0, g1, g2;  /* d/dx, d/dy, d/dz */
+    if (VC) {
+      g0 = __builtin_fmaf(zx, dz, xx * dx);
+      g1 = __builtin_fmaf(zy, dz, ey * dy);
+      g2 = zz * dz;
+    } else {
+      g0 = __builtin_fmaf(zx, dz, __builtin_fmaf(ex, dy, xx * dx));
+      g1 = __builtin_fmaf(zy, dz, __builtin_fmaf(ey, dy, xy * dx));
+      g2 = __builtin_fmaf(zz, dz, __builtin_fmaf(ez, dy, xz * dx));
+    }
     if (G > 0 && (f + 1) % G == 0) {
       /* the gradient exists in registers here and later loads stay below:
          this field's taps are consumed before the next field's are issued */
@@ -159,12 +176,12 @@ DEVI void acc_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
 }
 
 /* interior point with every tap loaded through the cache hierarchy */
-template <int ID, int JD, int KD, int G>
+template <int ID, int JD, int KD, int G, bool VC = false>
 DEVI void acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
   const unsigned by = (unsigned)a.g.siy * 4u, bz = (unsigned)a.g.siz * 4u;
   const float *cur = a.cur + (long)sn * b.wstride;
   const long sc = a.g.sicmp;
-  acc_core<G>(a, b, i, j, k, sn, [&](int f, unsigned o, float &dx, float &dy, float &dz) {
+  acc_core<G, VC>(a, b, i, j, k, sn, [&](int f, unsigned o, float &dx, float &dy, float &dz) {
     const float *bc = cur + f * sc;
     dx = dfd32<ID>(bc, o, 4u);
     dy = dfd32<JD>(bc, o, by);
@@ -216,7 +233,14 @@ __global__ void __launch_bounds__(256, 8) k_vol_acc_slab(VolArgs a, BatArgs b) {
  * term of the x/y faces (k == nk2) never occurs here: with a free surface
  * the top four layers belong to k_vol_top.
  * ---------------------------------------------------------------------- */
-template <class Deriv>
+/* VC: vertical columns (as in acc_core): e[0][1], e[0][2], e[1][0] and
+   e[1][2] vanish, are not read, and their terms are left out of the sums */
+template <bool VC>
+DEVI constexpr bool met_zero(int d, int c) {
+  return VC && ((d == 0 && c != 0) || (d == 1 && c != 1));
+}
+
+template <bool VC, class Deriv>
 DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn, Deriv deriv) {
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
@@ -227,7 +251,7 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
 #pragma unroll
   for (int d = 0; d < 3; d++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) e[d][c] = ld32nt(M + (1 + 3 * d + c) * sc, o);
+    for (int c = 0; c < 3; c++) e[d][c] = met_zero<VC>(d, c) ? 0.0f : ld32nt(M + (1 + 3 * d + c) * sc, o);
   const float l = ld32nt(a.lam, o), m = ld32nt(a.mu, o), s = ld32nt(a.slw, o);
   const float l2m = l + 2.0f * m;
   float r[3][9];
@@ -244,24 +268,28 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
     for (int d = 0; d < 3; d++) {
       const float Dd = d == 0 ? D0 : (d == 1 ? D1 : D2);
       const float e1 = e[d][0], e2 = e[d][1], e3 = e[d][2];
+      const bool n1 = !met_zero<VC>(d, 0), n2 = !met_zero<VC>(d, 1), n3 = !met_zero<VC>(d, 2);
       switch (f) {
-        case TXX: r[d][VX] += s * e1 * Dd; break;
-        case TYY: r[d][VY] += s * e2 * Dd; break;
-        case TZZ: r[d][VZ] += s * e3 * Dd; break;
-        case TXY: r[d][VX] += s * e2 * Dd; r[d][VY] += s * e1 * Dd; break;
-        case TXZ: r[d][VX] += s * e3 * Dd; r[d][VZ] += s * e1 * Dd; break;
-        case TYZ: r[d][VY] += s * e3 * Dd; r[d][VZ] += s * e2 * Dd; break;
+        case TXX: if (n1) r[d][VX] += s * e1 * Dd; break;
+        case TYY: if (n2) r[d][VY] += s * e2 * Dd; break;
+        case TZZ: if (n3) r[d][VZ] += s * e3 * Dd; break;
+        case TXY: if (n2) r[d][VX] += s * e2 * Dd; if (n1) r[d][VY] += s * e1 * Dd; break;
+        case TXZ: if (n3) r[d][VX] += s * e3 * Dd; if (n1) r[d][VZ] += s * e1 * Dd; break;
+        case TYZ: if (n3) r[d][VY] += s * e3 * Dd; if (n2) r[d][VZ] += s * e2 * Dd; break;
         case VX:
-          r[d][TXX] += l2m * e1 * Dd; r[d][TYY] += l * e1 * Dd; r[d][TZZ] += l * e1 * Dd;
-          r[d][TXY] += m * e2 * Dd; r[d][TXZ] += m * e3 * Dd;
+          if (n1) { r[d][TXX] += l2m * e1 * Dd; r[d][TYY] += l * e1 * Dd; r[d][TZZ] += l * e1 * Dd; }
+          if (n2) r[d][TXY] += m * e2 * Dd;
+          if (n3) r[d][TXZ] += m * e3 * Dd;
           break;
         case VY:
-          r[d][TXX] += l * e2 * Dd; r[d][TYY] += l2m * e2 * Dd; r[d][TZZ] += l * e2 * Dd;
-          r[d][TXY] += m * e1 * Dd; r[d][TYZ] += m * e3 * Dd;
+          if (n2) { r[d][TXX] += l * e2 * Dd; r[d][TYY] += l2m * e2 * Dd; r[d][TZZ] += l * e2 * Dd; }
+          if (n1) r[d][TXY] += m * e1 * Dd;
+          if (n3) r[d][TYZ] += m * e3 * Dd;
           break;
         case VZ:
-          r[d][TXX] += l * e3 * Dd; r[d][TYY] += l * e3 * Dd; r[d][TZZ] += l2m * e3 * Dd;
-          r[d][TXZ] += m * e1 * Dd; r[d][TYZ] += m * e2 * Dd;
+          if (n3) { r[d][TXX] += l * e3 * Dd; r[d][TYY] += l * e3 * Dd; r[d][TZZ] += l2m * e3 * Dd; }
+          if (n1) r[d][TXZ] += m * e1 * Dd;
+          if (n2) r[d][TYZ] += m * e2 * Dd;
           break;
       }
     }
@@ -313,12 +341,12 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
   rk_point32(a, o, h, woff);
 }
 
-template <int ID, int JD, int KD>
+template <int ID, int JD, int KD, bool VC = false>
 DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
   const unsigned by = (unsigned)a.g.siy * 4u, bz = (unsigned)a.g.siz * 4u;
   const float *cur = a.cur + (long)sn * b.wstride;
   const long sc = a.g.sicmp;
-  pml_core(a, b, i, j, k, sn, [&](int f, unsigned o, float &D0, float &D1, float &D2) {
+  pml_core<VC>(a, b, i, j, k, sn, [&](int f, unsigned o, float &D0, float &D1, float &D2) {
     const float *bc = cur + f * sc;
     D0 = dfd32<ID>(bc, o, 4u);
     D1 = dfd32<JD>(bc, o, by);
@@ -350,7 +378,7 @@ __global__ void __launch_bounds__(256, 6) k_vol_pml_lean(VolArgs a, BatArgs b) {
  * boundary (the 64-wide tiles of variant 20 send up to 43 interior columns
  * per row through it).
  * ---------------------------------------------------------------------- */
-template <int ID, int JD, int KD, int TW>
+template <int ID, int JD, int KD, int TW, bool VC>
 __global__ void __launch_bounds__(256, 8) k_vol_acc_box(VolArgs a, BatArgs b, VolBox x) {
   constexpr int TH = 256 / TW;
   const int S = a.zchunk;
@@ -366,7 +394,7 @@ __global__ void __launch_bounds__(256, 8) k_vol_acc_box(VolArgs a, BatArgs b, Vo
   const int t = (int)threadIdx.x;
   const int i = x.x0 + bx * TW + (t % TW), j = x.y0 + by * TH + t / TW;
   if (i >= x.x1 || j >= x.y1) return;
-  acc_point<ID, JD, KD, 1>(a, b, i, j, x.k0 + kz, b.act[0]);
+  acc_point<ID, JD, KD, 1, VC>(a, b, i, j, x.k0 + kz, b.act[0]);
 }
 
 /* block L0 of the bands -> the thread's point; false: a lane outside its
@@ -391,11 +419,11 @@ DEVI bool band_point(const VolBands &r, long L0, int &i, int &j, int &k) {
   return by < r.nty[q] && i >= r.x0[q] && i < r.x1[q] && j < r.y1[q];
 }
 
-template <int ID, int JD, int KD>
+template <int ID, int JD, int KD, bool VC>
 __global__ void __launch_bounds__(256, 6) k_vol_pml_band(VolArgs a, BatArgs b, VolBands r) {
   int i, j, k;
   if (!band_point(r, blockIdx.x, i, j, k)) return;
-  pml_acc_point<ID, JD, KD>(a, b, i, j, k, b.act[0]);
+  pml_acc_point<ID, JD, KD, VC>(a, b, i, j, k, b.act[0]);
 }
 
 /* batches on the box split: the interior box's (tile, scenario) pairs with
@@ -403,7 +431,7 @@ __global__ void __launch_bounds__(256, 6) k_vol_pml_band(VolArgs a, BatArgs b, V
    media lines of a tile serve every scenario from one L2), the bands in one
    sweep per scenario (each scenario's halos come back from L2). nitems: the
    box's tiles x scenarios; the grid is padded to a multiple of 8. */
-template <int ID, int JD, int KD, int TW>
+template <int ID, int JD, int KD, int TW, bool VC>
 __global__ void __launch_bounds__(256, 8) k_vol_bat_acc_box(VolArgs a, BatArgs b, VolBox x, long nitems) {
   constexpr int TH = 256 / TW;
   const long L = blockIdx.x;
@@ -417,16 +445,16 @@ __global__ void __launch_bounds__(256, 8) k_vol_bat_acc_box(VolArgs a, BatArgs b
   const int t = (int)threadIdx.x;
   const int i = x.x0 + bx * TW + (t % TW), j = x.y0 + by * TH + t / TW;
   if (i >= x.x1 || j >= x.y1) return;
-  acc_point<ID, JD, KD, 1>(a, b, i, j, x.k0 + kz, b.act[q]);
+  acc_point<ID, JD, KD, 1, VC>(a, b, i, j, x.k0 + kz, b.act[q]);
 }
 
-template <int ID, int JD, int KD>
+template <int ID, int JD, int KD, bool VC>
 __global__ void __launch_bounds__(256, 6) k_vol_bat_pml_band(VolArgs a, BatArgs b, VolBands r) {
   const long per = r.cum[r.n];
   const int sq = (int)(blockIdx.x / per);
   int i, j, k;
   if (!band_point(r, blockIdx.x % per, i, j, k)) return;
-  pml_acc_point<ID, JD, KD>(a, b, i, j, k, b.act[sq]);
+  pml_acc_point<ID, JD, KD, VC>(a, b, i, j, k, b.act[sq]);
 }
 
 constexpr int kBandSlabBlocks = 192;
@@ -774,6 +802,36 @@ __global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const fl
 
 /* ---------------------------------------------------------------------- */
 /* host launchers */
+/* the box split (variants 23, 64-wide interior tiles, and 24, 32-wide): one
+   scenario or a batch */
+template <int ID, int JD, int KD, bool VC>
+static void launch_box_split(int tw, const VolArgs &a, const BatArgs &b, hipStream_t st, hipStream_t st_pml) {
+  VolBox x;
+  VolBands r;
+  box_split(a, tw, x, r);
+  if (b.nact > 1) {
+    if (x.ntx > 0) {
+      const long items = (long)x.ntx * x.nty * (x.k1 - x.k0) * b.nact, nb = ((items + 7) / 8) * 8;
+      if (tw == 64)
+        hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 64, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
+      else
+        hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 32, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
+    }
+    if (r.n > 0)
+      hipLaunchKernelGGL((k_vol_bat_pml_band<ID, JD, KD, VC>), dim3((unsigned)(r.cum[r.n] * b.nact)), dim3(256), 0,
+                         st_pml, a, b, r);
+    return;
+  }
+  if (x.ntx > 0) {
+    const long nslab = ((long)x.nty + a.zchunk - 1) / a.zchunk;
+    const long nb = (long)x.ntx * a.zchunk * (x.k1 - x.k0) * nslab;
+    if (tw == 64) hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 64, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
+    else hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 32, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
+  }
+  if (r.n > 0)
+    hipLaunchKernelGGL((k_vol_pml_band<ID, JD, KD, VC>), dim3((unsigned)r.cum[r.n]), dim3(256), 0, st_pml, a, b, r);
+}
+
 template <int ID, int JD, int KD>
 static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top, hipStream_t st, hipStream_t st_pml) {
   dim3 blk(64, 4, 1);
@@ -784,21 +842,9 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
     return;
   }
-  if (b.nact > 1 && (variant == 23 || variant == 24)) {  /* batches on the box split */
-    const int tw = variant == 23 ? 64 : 32;
-    VolBox x;
-    VolBands r;
-    box_split(a, tw, x, r);
-    if (x.ntx > 0) {
-      const long items = (long)x.ntx * x.nty * (x.k1 - x.k0) * b.nact, nb = ((items + 7) / 8) * 8;
-      if (tw == 64)
-        hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 64>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
-      else
-        hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 32>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
-    }
-    if (r.n > 0)
-      hipLaunchKernelGGL((k_vol_bat_pml_band<ID, JD, KD>), dim3((unsigned)(r.cum[r.n] * b.nact)), dim3(256), 0, st_pml,
-                         a, b, r);
+  if (variant == 23 || variant == 24) {
+    if (a.vcol) launch_box_split<ID, JD, KD, true>(variant == 23 ? 64 : 32, a, b, st, st_pml);
+    else launch_box_split<ID, JD, KD, false>(variant == 23 ? 64 : 32, a, b, st, st_pml);
     return;
   }
   if (b.nact > 1 && variant >= 16) {  /* batches, split kernels */
@@ -831,21 +877,6 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     hipLaunchKernelGGL((k_vol_bat<ID, JD, KD>), dim3((unsigned)nb), blk, 0, st, a, b, items);
     return;
   }
-  if (variant == 23 || variant == 24) {
-    const int tw = variant == 23 ? 64 : 32;
-    VolBox x;
-    VolBands r;
-    box_split(a, tw, x, r);
-    if (x.ntx > 0) {
-      const long nslab = ((long)x.nty + a.zchunk - 1) / a.zchunk;
-      const long nb = (long)x.ntx * a.zchunk * (x.k1 - x.k0) * nslab;
-      if (tw == 64) hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 64>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
-      else hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 32>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
-    }
-    if (r.n > 0)
-      hipLaunchKernelGGL((k_vol_pml_band<ID, JD, KD>), dim3((unsigned)r.cum[r.n]), dim3(256), 0, st_pml, a, b, r);
-    return;
-  }
   if (variant >= 19) {
     const long nslab = ((a.g.nj + 3) / 4 + a.zchunk - 1) / a.zchunk;
     const long nb = (long)agrd.x * a.zchunk * a.nk_run * nslab;
@@ -861,6 +892,27 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     return;
   }
   hipLaunchKernelGGL((k_vol_direct<ID, JD, KD>), agrd, blk, 0, st, a, b);
+}
+
+/* one lane per physical point of a row: x depends on i alone and y on j
+   alone when xi_y, xi_z, eta_x and eta_z vanish; computed from such a grid
+   they hold the rounding of the finite differences (about 1e-15 of xi_x
+   and eta_y), while any tilt of the columns puts them many orders above
+   2^-32 (a NaN fails the test too) */
+__global__ void __launch_bounds__(256) k_vcol_check(Grid g, const float *__restrict__ met, int *flag) {
+  const int i = blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= g.ni) return;
+  const long p = (i + 3) + (long)(blockIdx.y + 3) * g.siy + (long)(blockIdx.z + 3) * g.siz;
+  const float tiny = 0x1p-32f;
+  const float xx = fabsf(met[1 * g.sicmp + p]), ey = fabsf(met[5 * g.sicmp + p]);
+  const float a = fabsf(met[2 * g.sicmp + p]), b = fabsf(met[3 * g.sicmp + p]);
+  const float c = fabsf(met[4 * g.sicmp + p]), d = fabsf(met[6 * g.sicmp + p]);
+  if (!(a <= tiny * xx && b <= tiny * xx && c <= tiny * ey && d <= tiny * ey)) *flag = 1;
+}
+
+void launch_vcol_check(const Grid &g, const float *met, int *flag, hipStream_t st) {
+  hipLaunchKernelGGL(k_vcol_check, dim3((unsigned)((g.ni + 255) / 256), (unsigned)g.nj, (unsigned)g.nk), dim3(256), 0, st,
+                     g, met, flag);
 }
 
 void launch_volume(int variant, const VolArgs &a, const BatArgs &b, int idir, int jdir, int kdir, int top,

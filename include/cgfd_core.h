@@ -353,6 +353,23 @@ int cgfd_core_get_variant(const cgfd_core *core, int *variant, int *zchunk);
 /* y-tiles per slab of the slab-ordered kernels (16, 19..21, 23, 24), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
+/* Vertical-column metric form, no counterpart in the reference (whose
+   kernels read all nine metric terms, sv_curv_col_el_iso_gpu.cu:332-340).
+   On grids whose x depends on i alone and y on j alone (flat grids and
+   grids of vertical columns under topography) xi_y, xi_z, eta_x and eta_z
+   vanish; the metric computed from such a grid holds there only the
+   rounding of its finite differences, about 1e-15 of xi_x and eta_y. At
+   create the core checks on the device whether they stay below 2^-32 of
+   xi_x and eta_y at every physical point. If so, the box-split schedules
+   (23, 24) read only the other five terms and omit the four products,
+   which agrees with the nine-term form to the last bit at nearly every
+   point and within an ulp elsewhere, with four fewer arrays per point. on = 0 makes the core use the nine-term form
+   regardless; on = 1 (the default) uses the five-term form where the grid
+   has it. *grid: whether the grid has it; *in_use: whether the volume
+   update uses it. */
+int cgfd_core_set_vertical_columns(cgfd_core *core, int on);
+int cgfd_core_get_vertical_columns(const cgfd_core *core, int *grid, int *in_use);
+
 /* Autotuner: times every candidate (variant, z-chunk) pair -- 23, 24, 20,
    16 and 1 on one scenario; 23, 24, 20, 19, 21, 16 and 1 on a batch;
    z-chunks 4..32 for the slab-ordered ones -- on `reps` repetitions of one step's four volume launches, keeps
