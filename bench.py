@@ -28,15 +28,23 @@ sys.path.insert(0, str(REPO))
 
 METRIC = "Gcell-updates/s + %HBM peak; batched vs plain; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per point of one volume-stencil launch (FP32, RK4 with
-# each array touched once per stage): stage 0 reads cur(=pre) 36 + metric 36 +
-# media 12 and writes tmp 36 + end 36; stages 1-2 also read pre 36 + end 36;
-# stage 3 reads cur 36 + 48 + end 36 and writes end 36.
-STAGE_BYTES = (156, 228, 228, 156)
-BYTES_PER_CELL_UPDATE = sum(STAGE_BYTES)  # 768
-# on a grid of vertical columns four of the nine metric terms are zero and
-# the volume update reads the other five: 16 B less per point and stage
-STAGE_BYTES_VCOL = tuple(b - 16 for b in STAGE_BYTES)  # 704 per cell-update
+# The reference's RK4 bookkeeping touches, per point and stage (FP32): stage 0
+# reads cur(=pre) 36 + metric 36 + media 12 and writes tmp 36 + end 36;
+# stages 1-2 also read pre 36 + end 36; stage 3 reads cur 36 + 48 + end 36 and
+# writes end 36: 768 B per cell-update, the figure the HBM fraction of the
+# whole step is quoted on (hbm_frac_768B_model).
+BYTES_PER_CELL_UPDATE = 768
+# The arrays one launch of this volume update reads and writes per point
+# (reads, writes): the stage input, the metric (five terms on a grid of
+# vertical columns, else nine), three media arrays and the levels the stage
+# combines (stages 1-2: the step's start; stage 3: the start and the outputs
+# of stages 0 and 1); one output level. End accumulation is deferred to
+# stage 3, so stages 0-2 write one level.
+STAGE_ARRAYS = {True: ((17, 9), (26, 9), (26, 9), (44, 9)), False: ((21, 9), (30, 9), (30, 9), (48, 9))}
+
+
+def stage_bytes(vcol: bool):
+    return tuple(4 * (r + w) for r, w in STAGE_ARRAYS[vcol])  # 596 or 660 per cell-update
 
 
 def log(msg: str) -> None:
@@ -87,16 +95,25 @@ def calib_copy(mbytes: int, reps: int = 1) -> float:
     return gbs
 
 
-def stream_ceiling(ni: int, nj: int, nk: int, reps: int = 5) -> float:
-    """GB/s of a kernel that streams an RK stage-1 point's 39 reads and 18
-    writes over the same grid with no stencil (cgfd_calib_stream)."""
+def stream_ceiling(ni: int, nj: int, nk: int, vcol: bool, reps: int = 5):
+    """(GB/s, per-stage GB/s): a kernel that streams each RK stage's arrays
+    per point (STAGE_ARRAYS) over the same grid with no stencil
+    (cgfd_calib_stream); the step's rate is its bytes over the sum of the
+    stages' times at those rates."""
     import ctypes as C
     from cgfd import lib as L
     lib = L.load_library()
-    g = C.c_float()
-    L.check(lib, lib.cgfd_calib_stream(0, ni, nj, nk, 39, 18, reps, C.byref(g)))
-    log(f"[bench] stage-1 data movement without the stencil, {ni}x{nj}x{nk}: {g.value:.0f} GB/s")
-    return float(g.value)
+    rates = {}
+    for r, w in set(STAGE_ARRAYS[vcol]):
+        g = C.c_float()
+        L.check(lib, lib.cgfd_calib_stream(0, ni, nj, nk, r, w, reps, C.byref(g)))
+        rates[(r, w)] = float(g.value)
+    per_stage = [rates[rw] for rw in STAGE_ARRAYS[vcol]]
+    byts = stage_bytes(vcol)
+    ceiling = sum(byts) / sum(b / g for b, g in zip(byts, per_stage))
+    log(f"[bench] the stages' data movement without the stencil, {ni}x{nj}x{nk}: "
+        f"{', '.join(f'{g:.0f}' for g in per_stage)} GB/s; step {ceiling:.0f} GB/s")
+    return ceiling, per_stage
 
 
 def _cpu_model() -> str:
@@ -400,7 +417,7 @@ def main() -> int:
     import numpy as np
     slip_max = float(np.nanmax(w[6]))
     vcol_grid, vcol_used = core.vertical_columns()
-    stage_bytes = STAGE_BYTES_VCOL if vcol_used else STAGE_BYTES
+    sbytes = stage_bytes(vcol_used)
 
     ni, nj, nk = sc.ni, sc.nj, sc.nk
     cells = ni * nj * nk * args.steps
@@ -410,13 +427,13 @@ def main() -> int:
     nvol_planes = nk - 4  # the top four layers go to the free-surface kernel
     vol_pts = ni * nj * nvol_planes
     launches_per_step = 4
-    bytes_vol = sum(stage_bytes) * vol_pts * (n_vol // launches_per_step)
+    bytes_vol = sum(sbytes) * vol_pts * (n_vol // launches_per_step)
     achieved = bytes_vol / (ms_vol / 1e3) / 1e9 if ms_vol > 0 else 0.0
     traffic = read_traffic(REPO / "profiles")
     copy_gbs = None if args.no_copy_ceiling else calib_copy(2048, reps=5)
     core.close()
     st.close()
-    stream_gbs = None if args.no_copy_ceiling else stream_ceiling(ni, nj, nk)
+    stream_gbs, stream_stages = (None, None) if args.no_copy_ceiling else stream_ceiling(ni, nj, nk, vcol_used)
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -439,9 +456,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_vol (volume stencil + PML + RK update)",
-                     "bytes_per_cell_update": sum(stage_bytes),
-                     "metric_form": ("five terms (vertical columns: xi_y, xi_z, eta_x, eta_z below 2^-32 of xi_x, eta_y)" if vcol_used
-                                     else "nine terms"),
+                     "bytes_per_cell_update": sum(sbytes),
+                     "metric_form": ("five terms (vertical columns: xi_y, xi_z, eta_x, eta_z below 2^-32 of "
+                                     "xi_x, eta_y)" if vcol_used else "nine terms"),
                      "avg_launch_ms": round(ms_vol / max(n_vol, 1), 4),
                      "traffic_source": "profiles/k_vol_config2_pmc.txt: rocprofv3 FETCH_SIZE + WRITE_SIZE of the "
                                        "volume dispatches, calibrated with a device copy (not measured in this run)",
@@ -449,9 +466,11 @@ def main() -> int:
                      "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None,
                      "stream_ceiling_gbs": round(stream_gbs, 1) if stream_gbs else None,
                      "frac_of_stream_ceiling": round(achieved / stream_gbs, 4) if stream_gbs else None,
-                     "stream_ceiling_note": "cgfd_calib_stream: 39 reads + 18 writes per point (an RK stage-1 "
-                                            "point's arrays) with no stencil, same grid and tiles, measured in "
-                                            "this run"},
+                     "stream_ceiling_stages_gbs": [round(g, 1) for g in stream_stages] if stream_stages else None,
+                     "stream_ceiling_note": "cgfd_calib_stream: each RK stage's reads + writes per point "
+                                            f"{list(STAGE_ARRAYS[vcol_used])} with no stencil, same grid and "
+                                            "tiles, measured in this run; the step's rate is its bytes over the "
+                                            "stages' times at those rates"},
     }
     if setup_leg:
         out["setup"] = setup_leg

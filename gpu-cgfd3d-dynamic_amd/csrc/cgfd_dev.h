@@ -73,7 +73,9 @@ struct VolArgs {
   const float *__restrict__ cur;   /* stage input level, [9][sicmp] */
   const float *__restrict__ pre;   /* level at t_n */
   float *__restrict__ tnew;        /* next stage input (stages 0..2) */
-  float *__restrict__ end;         /* level at t_{n+1} */
+  float *__restrict__ end;         /* level A, the output of stage 0: stage 3
+                                      writes the level at t_{n+1} over it */
+  const float *__restrict__ lb;    /* level B, the output of stage 1 (read at stage 3) */
   const float *__restrict__ met;   /* [10][sicmp]: jac, xi, eta, zeta */
   const float *__restrict__ lam;
   const float *__restrict__ mu;
@@ -149,6 +151,19 @@ DEVI float dfd(const float *__restrict__ p, long s) {
    serves all nine fields and all metric arrays, instead of a 64-bit address
    pair per load. Valid while one field array is below 4 GiB (checked at
    create). */
+/* The RK4 step's end value from its start value p, the inputs of stages 1,
+   2 and 3 (A = p + dt h0 / 2, B = p + dt h1 / 2, C = p + dt h2) and
+   t = dt h3 / 6: p + dt (h0 + 2 h1 + 2 h2 + h3) / 6, with dt h_s taken back
+   from the stage inputs (the subtractions A - p and the like are exact
+   whenever the stage increment is at most half the value, Sterbenz). The reference accumulates
+   end += b_s dt h_s after every stage (drv_rk_curv_col.cu:329-568); this
+   form needs only the stage inputs, so stages 0..2 write one level instead
+   of two, and agrees with it to a few ulps of the value. */
+DEVI float rk_end(float p, float A, float B, float C, float t) {
+  const float s = __builtin_fmaf(2.0f, B - p, (A - p) + (C - p));
+  return p + (s / 3.0f + t);
+}
+
 DEVI float ld32(const float *base, unsigned off) { return *(const float *)((const char *)base + off); }
 DEVI void st32(float *base, unsigned off, float v) { *(float *)((char *)base + off) = v; }
 /* the same for data a stage touches once (RK levels, metric, media): marked

@@ -33,7 +33,9 @@ struct FaultDev {
 struct FaultArgs {
   Grid g;
   const float *met, *lam, *mu, *slw, *matVx2Vz, *matVy2Vz;
-  float *w_cur, *w_pre, *w_tnew, *w_end;
+  float *w_cur, *w_pre, *w_tnew;
+  float *w_end;                             /* level A: t_{n+1} is written over it at stage 3 */
+  const float *w_b;                         /* level B (read at stage 3) */
   float *f_cur, *f_pre, *f_tnew, *f_end;   /* level bases; fault id at +id*18*sslice */
   int nfault;
   const FaultDev *ftab;                     /* [nfault] in device memory (scenario 0) */
@@ -62,6 +64,7 @@ DEVI FaultArgs fault_at(const FaultArgs &a, const BatArgs &b, int s) {
   r.w_pre += w;
   if (r.w_tnew) r.w_tnew += w;
   r.w_end += w;
+  if (r.w_b) r.w_b += w;
   r.f_cur += f;
   r.f_pre += f;
   if (r.f_tnew) r.f_tnew += f;

@@ -72,7 +72,9 @@ struct cgfd_core {
   DevBuf mem;
 
   float *wave = nullptr;   /* 4 levels x 9 x sicmp */
-  int lv_pre = 0, lv_a = 1, lv_b = 2, lv_end = 3;
+  /* wavefield levels: the step's start (pre) and the outputs of stages 0,
+     1, 2 (A, B, C); stage 3 writes t_{n+1} over A, and pre and A swap */
+  int lv_pre = 0, lv_a = 1, lv_b = 2, lv_c = 3;
   float *met = nullptr, *lam = nullptr, *mu = nullptr, *slw = nullptr;
   float *matVx2Vz = nullptr, *matVy2Vz = nullptr;
 
@@ -597,7 +599,8 @@ void fill_vol(cgfd_core *c, VolArgs &v, int istage, const float *cur, float *tne
   v.cur = cur;
   v.pre = c->wlev(c->lv_pre);
   v.tnew = tnew;
-  v.end = c->wlev(c->lv_end);
+  v.end = c->wlev(c->lv_a);
+  v.lb = c->wlev(c->lv_b);
   v.met = c->met; v.lam = c->lam; v.mu = c->mu; v.slw = c->slw;
   v.matVx2Vz = c->matVx2Vz; v.matVy2Vz = c->matVy2Vz;
   for (int dim = 0; dim < 3; dim++)
@@ -618,7 +621,8 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.g = c->g;
   f.met = c->met; f.lam = c->lam; f.mu = c->mu; f.slw = c->slw;
   f.matVx2Vz = c->matVx2Vz; f.matVy2Vz = c->matVy2Vz;
-  f.w_cur = wcur; f.w_pre = c->wlev(c->lv_pre); f.w_tnew = wnew; f.w_end = c->wlev(c->lv_end);
+  f.w_cur = wcur; f.w_pre = c->wlev(c->lv_pre); f.w_tnew = wnew; f.w_end = c->wlev(c->lv_a);
+  f.w_b = c->wlev(c->lv_b);
   f.f_cur = fcur; f.f_pre = c->flev(c->fl_pre); f.f_tnew = fnew; f.f_end = c->flev(c->fl_end);
   f.nfault = c->nfault;
   f.ftab = c->d_ftab;
@@ -666,8 +670,8 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
     const int idir = (kPairs[ipair][0] + s) % 2;
     const int jdir = (kPairs[ipair][1] + s) % 2;
     const int kdir = (kPairs[ipair][2] + s) % 2;
-    const int lcur = s == 0 ? c->lv_pre : (s == 2 ? c->lv_b : c->lv_a);
-    const int lnew = s == 0 ? c->lv_a : (s == 1 ? c->lv_b : c->lv_a);
+    const int lcur = s == 0 ? c->lv_pre : (s == 1 ? c->lv_a : (s == 2 ? c->lv_b : c->lv_c));
+    const int lnew = s == 0 ? c->lv_a : (s == 1 ? c->lv_b : c->lv_c);
     float *wcur = c->wlev(lcur);
     float *wnew = s < 3 ? c->wlev(lnew) : nullptr;
     float *fcur = nullptr, *fnew = nullptr;
@@ -712,11 +716,12 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
       (*nev)++;
     }
   }
-  if (c->isfree) launch_pg(c->g, c->bat, c->wlev(c->lv_end), c->wlev(c->lv_pre), c->PG, c->Dis, c->dt, c->st);
+  if (c->isfree) launch_pg(c->g, c->bat, c->wlev(c->lv_a), c->wlev(c->lv_pre), c->PG, c->Dis, c->dt, c->st);
   launch_step_end(c->it_dev, c->st);
   HIPCHK(hipGetLastError());
-  /* swap pre <-> end (drv_rk_curv_col.cu:622-635) */
-  std::swap(c->lv_pre, c->lv_end);
+  /* the new step starts from t_{n+1}, written over level A (the reference
+     swaps pre and end, drv_rk_curv_col.cu:622-635) */
+  std::swap(c->lv_pre, c->lv_a);
   std::swap(c->fl_pre, c->fl_end);
   for (int dim = 0; dim < 3; dim++)
     for (int side = 0; side < 2; side++) std::swap(c->pml[dim][side].pre, c->pml[dim][side].end);
@@ -1226,7 +1231,7 @@ static int scen_steps(const cgfd_core *c, int s) {
    step, so a scenario that retired an odd number of steps ago has its final
    state in today's "end" level. */
 static int scen_wlev(const cgfd_core *c, int s) {
-  return ((c->it_host - scen_steps(c, s)) % 2 == 0) ? c->lv_pre : c->lv_end;
+  return ((c->it_host - scen_steps(c, s)) % 2 == 0) ? c->lv_pre : c->lv_a;
 }
 static int scen_flev(const cgfd_core *c, int s) {
   return ((c->it_host - scen_steps(c, s)) % 2 == 0) ? c->fl_pre : c->fl_end;
@@ -1653,11 +1658,11 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
       c->variant = v;
       c->zchunk = z;
       /* the four stages' volume launches of one step: they read level pre
-         and write only levels A, B and end (and the PML aux tmp/end), which
+         and write only levels A, B and C (and the PML aux tmp/end), which
          the next real step rewrites before reading */
       VolArgs va[4];
-      const int lc[4] = {c->lv_pre, c->lv_a, c->lv_b, c->lv_a};
-      const int ln[4] = {c->lv_a, c->lv_b, c->lv_a, -1};
+      const int lc[4] = {c->lv_pre, c->lv_a, c->lv_b, c->lv_c};
+      const int ln[4] = {c->lv_a, c->lv_b, c->lv_c, -1};
       for (int s = 0; s < 4; s++) {
         fill_vol(c, va[s], s, c->wlev(lc[s]), ln[s] >= 0 ? c->wlev(ln[s]) : nullptr);
         va[s].k_lo = 0;

@@ -4,11 +4,13 @@
  * side, the core's array pitch) reads nread arrays once and writes nwrite
  * arrays once, with the volume kernels' row pitch (the row length rounded
  * up to 32 floats), 64 x 4 tiles starting at ghost-inclusive column 0 and
- * non-temporal accesses. A stage-1/2 point of the RK update reads 39 arrays (stage input,
- * metric, media, pre and end levels) and writes 18 (the next stage input and
- * end), so this rate is what the volume update's bytes could stream at on
- * this GPU with no neighbour reuse at all: the ceiling its roofline fraction
- * is read against, measured in the same process.
+ * non-temporal accesses. Each RK stage of the volume update reads and writes
+ * a fixed set of arrays per point (stage 1 on a grid of vertical columns: the
+ * stage input, five metric terms, three media arrays and the step's start
+ * level, 26 reads, and its output level, 9 writes), so these rates, one per
+ * stage, are what the volume update's bytes could stream at on this GPU
+ * with no neighbour reuse at all: the ceiling its roofline fraction is read
+ * against, measured in the same process.
  */
 #include <hip/hip_runtime.h>
 
@@ -73,12 +75,20 @@ extern "C" int cgfd_calib_stream(int device_id, int ni, int nj, int nk, int nrea
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e == hipSuccess) {
       const dim3 grid((unsigned)((ni + 3 + 63) / 64), (unsigned)((nj + 3) / 4), (unsigned)nk), blk(64, 4, 1);
-      const bool fixed = nread == 39 && nwrite == 18;
       auto launch = [&]() {
-        if (fixed)
-          hipLaunchKernelGGL((k_calib_stream_fixed<39, 18>), grid, blk, 0, 0, src, dst, ni, nj, nx, ny, sicmp);
-        else
-          hipLaunchKernelGGL(k_calib_stream, grid, blk, 0, 0, src, dst, nread, nwrite, ni, nj, nx, ny, sicmp);
+#define CGFD_CALIB_CASE(R, W)                                                                                 \
+  if (nread == R && nwrite == W) {                                                                            \
+    hipLaunchKernelGGL((k_calib_stream_fixed<R, W>), grid, blk, 0, 0, src, dst, ni, nj, nx, ny, sicmp);      \
+    return;                                                                                                   \
+  }
+        /* the RK stages' array counts (reads of the stage input, metric
+           with five or nine terms, media, the levels the stage combines;
+           writes of its output level) and the earlier scheme's 39 + 18 */
+        CGFD_CALIB_CASE(17, 9) CGFD_CALIB_CASE(26, 9) CGFD_CALIB_CASE(44, 9)
+        CGFD_CALIB_CASE(21, 9) CGFD_CALIB_CASE(30, 9) CGFD_CALIB_CASE(48, 9)
+        CGFD_CALIB_CASE(39, 18)
+#undef CGFD_CALIB_CASE
+        hipLaunchKernelGGL(k_calib_stream, grid, blk, 0, 0, src, dst, nread, nwrite, ni, nj, nx, ny, sicmp);
       };
       launch();
       e = hipEventRecord(e0, 0);

@@ -298,21 +298,14 @@ DEVI void hooke(const float *met, long sc, long p, float l, float m, const float
   h[3] = m * (xz * Dx[1] + xy * Dx[2] + ez * Dy[1] + ey * Dy[2] + zz * Dz[1] + zy * Dz[2]); /* Tyz */
 }
 
-/* RK update of wavefield components [c0, c0+nc) at p with rhs h[] */
+/* RK update of wavefield components [c0, c0+nc) at p with rhs h[], as the
+   volume kernels do it (rk_point) */
 DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
   const long sc = a.g.sicmp;
   for (int q = 0; q < nc; q++) {
     long o = (long)(c0 + q) * sc + p;
-    if (a.istage == 0) {
-      float pv = a.w_pre[o];
-      a.w_tnew[o] = pv + a.ca * h[q];
-      a.w_end[o] = pv + a.cb * h[q];
-    } else if (a.istage < 3) {
-      a.w_tnew[o] = a.w_pre[o] + a.ca * h[q];
-      a.w_end[o] += a.cb * h[q];
-    } else {
-      a.w_end[o] += a.cb * h[q];
-    }
+    if (a.istage < 3) a.w_tnew[o] = a.w_pre[o] + a.ca * h[q];
+    else a.w_end[o] = rk_end(a.w_pre[o], a.w_end[o], a.w_b[o], a.w_cur[o], a.cb * h[q]);
   }
 }
 

@@ -120,50 +120,43 @@ DEVI bool fault_column(const FaultSkip &fs, int i, long pf) {
   return false;
 }
 
-/* RK stage update of the nine wavefield components at p */
+/* RK stage update of the nine wavefield components at p: stages 0..2 write
+   the next stage input pre + a_s dt h (wav_update, wav_t.cu:218-226),
+   stage 3 the level at t_{n+1} over level A (rk_end) */
 DEVI void rk_point(const VolArgs &a, long p, const float *h, long woff = 0) {
   const long sc = a.g.sicmp;
   p += woff;
-  if (a.istage == 0) {
+  if (a.istage < 3) {
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
-      float pv = a.pre[c * sc + p];
-      a.tnew[c * sc + p] = pv + a.ca * h[c];
-      a.end[c * sc + p] = pv + a.cb * h[c];
-    }
-  } else if (a.istage < 3) {
-#pragma unroll
-    for (int c = 0; c < 9; c++) {
-      a.tnew[c * sc + p] = a.pre[c * sc + p] + a.ca * h[c];
-      a.end[c * sc + p] += a.cb * h[c];
-    }
+    for (int c = 0; c < 9; c++) a.tnew[c * sc + p] = a.pre[c * sc + p] + a.ca * h[c];
   } else {
 #pragma unroll
-    for (int c = 0; c < 9; c++) a.end[c * sc + p] += a.cb * h[c];
+    for (int c = 0; c < 9; c++) {
+      const long q = c * sc + p;
+      a.end[q] = rk_end(a.pre[q], a.end[q], a.lb[q], a.cur[q], a.cb * h[c]);
+    }
   }
 }
 
-/* rk_point with 32-bit byte offsets (o = 4 p) from the scenario's levels */
+/* rk_point with 32-bit byte offsets (o = 4 p) from the scenario's levels;
+   the stage input's own value at stage 3 comes from the cache (the stencil
+   has just read it) */
 DEVI void rk_point32(const VolArgs &a, unsigned o, const float *h, long woff = 0) {
   const long sc = a.g.sicmp;
   const float *pre = a.pre + woff;
-  float *tnew = a.tnew + woff, *end = a.end + woff;
-  if (a.istage == 0) {
+  if (a.istage < 3) {
+    float *tnew = a.tnew + woff;
 #pragma unroll
-    for (int c = 0; c < 9; c++) {
-      float pv = ld32nt(pre + c * sc, o);
-      st32nt(tnew + c * sc, o, pv + a.ca * h[c]);
-      st32nt(end + c * sc, o, pv + a.cb * h[c]);
-    }
-  } else if (a.istage < 3) {
-#pragma unroll
-    for (int c = 0; c < 9; c++) {
-      st32nt(tnew + c * sc, o, ld32nt(pre + c * sc, o) + a.ca * h[c]);
-      st32nt(end + c * sc, o, ld32nt(end + c * sc, o) + a.cb * h[c]);
-    }
+    for (int c = 0; c < 9; c++) st32nt(tnew + c * sc, o, ld32nt(pre + c * sc, o) + a.ca * h[c]);
   } else {
+    float *end = a.end + woff;
+    const float *lb = a.lb + woff, *cur = a.cur + woff;
 #pragma unroll
-    for (int c = 0; c < 9; c++) st32nt(end + c * sc, o, ld32nt(end + c * sc, o) + a.cb * h[c]);
+    for (int c = 0; c < 9; c++) {
+      const float v = rk_end(ld32nt(pre + c * sc, o), ld32nt(end + c * sc, o), ld32nt(lb + c * sc, o),
+                             ld32(cur + c * sc, o), a.cb * h[c]);
+      st32nt(end + c * sc, o, v);
+    }
   }
 }
 

@@ -385,10 +385,13 @@ int cgfd_core_autotune(cgfd_core *core, int reps, int *variant, int *zchunk, flo
    (GB/s of bytes read + written) of a kernel that has every point of an
    ni x nj x nk grid (the core's padded array layout, 64 x 4 tiles,
    non-temporal accesses) read nread arrays and write nwrite arrays once,
-   averaged over reps launches on device_id. nread = 39, nwrite = 18 is the
-   data movement of an RK stage 1 or 2 point without the stencil: the
-   ceiling the volume update's roofline fraction is compared with. Allocates
-   (nread + nwrite) x (ni+6)(nj+6)(nk+6) floats for the duration. */
+   averaged over reps launches on device_id. With each RK stage's counts
+   (17, 26, 26 and 44 reads and 9 writes per point on a grid of vertical
+   columns; 21, 30, 30 and 48 with the nine-term metric) this is the data
+   movement of the volume update without the stencil: the ceiling its
+   roofline fraction is compared with. Those counts (and 39 + 18) run fully
+   unrolled kernels, other counts a loop. Allocates (nread + nwrite) x
+   (ni+6)(nj+6)(nk+6) floats for the duration. */
 int cgfd_calib_stream(int device_id, int ni, int nj, int nk, int nread, int nwrite, int reps, float *gbs);
 
 /* ---------------------------------------------------------------------
