@@ -57,7 +57,9 @@ struct PmlFace {
   int an, bn;                  /* slab extent in x and y */
   long n;                      /* slab points */
   const float *A, *B, *D;      /* profiles along the face's axis */
-  float *pre, *tmp, *end;      /* aux levels, [9][n] each */
+  float *pre, *la, *lb, *lc;   /* aux levels, [9][n] each: the step's start and
+                                  the outputs of stages 0, 1, 2 (stage 3 writes
+                                  t_{n+1} over la), as the wavefield's */
 };
 
 /* the fault columns the volume kernels leave to the fault kernel */

@@ -487,7 +487,7 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   if ((rc = c->alloc(&c->matVx2Vz, (size_t)2 * 9 * g.siz))) return fail(rc);
   c->matVy2Vz = c->matVx2Vz + 9 * g.siz;
   if (hipMemset(c->matVx2Vz, 0, (size_t)2 * 9 * g.siz * 4) != hipSuccess) return fail(cgfd_set_error(CGFD_ERR_HIP, "memset failed"));
-  /* PML: profiles shared; auxiliaries (3 levels x 9 x slab) of all faces in
+  /* PML: profiles shared; auxiliaries (4 levels x 9 x slab) of all faces in
      one block per scenario */
   long paux = 0;
   for (int dim = 0; dim < 3; dim++)
@@ -495,7 +495,7 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
       if (!d->pml_is_sides[dim][side]) continue;
       long n = (long)(d->pml_ni2[dim][side] - d->pml_ni1[dim][side] + 1) *
                (d->pml_nj2[dim][side] - d->pml_nj1[dim][side] + 1) * (d->pml_nk2[dim][side] - d->pml_nk1[dim][side] + 1);
-      paux += 27 * n;
+      paux += 36 * n;
     }
   float *auxblk = nullptr;
   c->bat.pstride = paux;
@@ -525,8 +525,8 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
         return fail(cgfd_set_error(CGFD_ERR_HIP, "PML coefficient upload failed"));
       f.A = cf; f.B = cf + np; f.D = cf + 2 * np;
       float *aux = auxblk + paoff;
-      paoff += 27 * f.n;
-      f.pre = aux; f.tmp = aux + 9 * f.n; f.end = aux + 18 * f.n;
+      paoff += 36 * f.n;
+      f.pre = aux; f.la = aux + 9 * f.n; f.lb = aux + 18 * f.n; f.lc = aux + 27 * f.n;
     }
   /* faults */
   if (c->nfault > 0) {
@@ -724,7 +724,7 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
   std::swap(c->lv_pre, c->lv_a);
   std::swap(c->fl_pre, c->fl_end);
   for (int dim = 0; dim < 3; dim++)
-    for (int side = 0; side < 2; side++) std::swap(c->pml[dim][side].pre, c->pml[dim][side].end);
+    for (int side = 0; side < 2; side++) std::swap(c->pml[dim][side].pre, c->pml[dim][side].la);
   c->it_host++;
   if (c->nan_every > 0 && (c->it_host % c->nan_every) == 0) {
     for (int q = 0; q < c->bat.nact; q++)
@@ -1658,7 +1658,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
       c->variant = v;
       c->zchunk = z;
       /* the four stages' volume launches of one step: they read level pre
-         and write only levels A, B and C (and the PML aux tmp/end), which
+         and write only levels A, B and C (and the PML auxiliaries' la, lb, lc), which
          the next real step rewrites before reading */
       VolArgs va[4];
       const int lc[4] = {c->lv_pre, c->lv_a, c->lv_b, c->lv_c};

@@ -28,6 +28,33 @@ DEVI void rhs_interior(const float *Dx, const float *Dy, const float *Dz, float 
   h[TYZ] = m * (xz * Dx[VY] + xy * Dx[VZ] + ez * Dy[VY] + ey * Dy[VZ] + zz * Dz[VY] + zy * Dz[VZ]);
 }
 
+
+/* The auxiliary variables of a CFS-PML face follow the wavefield's RK
+   bookkeeping (rk_point): stage s reads level pre, la, lb or lc and writes
+   the next one; stage 3 writes t_{n+1} over la. */
+DEVI const float *aux_input(const PmlFace &f, int istage) {
+  return istage == 0 ? f.pre : (istage == 1 ? f.la : (istage == 2 ? f.lb : f.lc));
+}
+
+/* av: the stage input's values at slab point ia (already read), ar: the
+   auxiliary rhs */
+DEVI void aux_update(const PmlFace &f, const VolArgs &a, long poff, long ia, const float *av, const float *ar) {
+  const float *pre = f.pre + poff;
+  if (a.istage < 3) {
+    float *out = (a.istage == 0 ? f.la : (a.istage == 1 ? f.lb : f.lc)) + poff;
+#pragma unroll
+    for (int c = 0; c < 9; c++) out[c * f.n + ia] = (a.istage == 0 ? av[c] : pre[c * f.n + ia]) + a.ca * ar[c];
+  } else {
+    float *la = f.la + poff;
+    const float *lb = f.lb + poff;
+#pragma unroll
+    for (int c = 0; c < 9; c++) {
+      const long q = c * f.n + ia;
+      la[q] = rk_end(pre[q], la[q], lb[q], av[c], a.cb * ar[c]);
+    }
+  }
+}
+
 /* CFS-PML correction of every face containing (i,j,k) plus the auxiliary
    RK update (sv_curv_col_el_iso_gpu.cu:916-1288, drv_rk_curv_col.cu:368-567) */
 DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *Dx, const float *Dy, const float *Dz,
@@ -58,8 +85,7 @@ DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *
       r[TXY] = m * (e2 * D[VX] + e1 * D[VY]);
       r[TXZ] = m * (e3 * D[VX] + e1 * D[VZ]);
       r[TYZ] = m * (e3 * D[VY] + e2 * D[VZ]);
-      float *const fpre = f.pre + poff, *const ftmp = f.tmp + poff, *const fend = f.end + poff;
-      const float *auxc = (a.istage == 0 ? fpre : ftmp);
+      const float *auxc = aux_input(f, a.istage) + poff;
       float av[9], ar[9];
 #pragma unroll
       for (int c = 0; c < 9; c++) av[c] = auxc[c * f.n + ia];
@@ -90,23 +116,7 @@ DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *
         }
       }
       /* auxiliary RK update; the slab point is owned by this thread only */
-      if (a.istage == 0) {
-#pragma unroll
-        for (int c = 0; c < 9; c++) {
-          float pv = fpre[c * f.n + ia];
-          ftmp[c * f.n + ia] = pv + a.ca * ar[c];
-          fend[c * f.n + ia] = pv + a.cb * ar[c];
-        }
-      } else if (a.istage < 3) {
-#pragma unroll
-        for (int c = 0; c < 9; c++) {
-          ftmp[c * f.n + ia] = fpre[c * f.n + ia] + a.ca * ar[c];
-          fend[c * f.n + ia] += a.cb * ar[c];
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 9; c++) fend[c * f.n + ia] += a.cb * ar[c];
-      }
+      aux_update(f, a, poff, ia, av, ar);
     }
   }
 }

@@ -309,32 +309,15 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
       const int ic = dim == 0 ? i - F.i1 : (dim == 1 ? j - F.j1 : k - F.k1);
       const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
       const float cB1 = cB - 1.0f;
-      float *const fpre = F.pre + poff, *const ftmp = F.tmp + poff, *const fend = F.end + poff;
-      const float *auxc = (a.istage == 0 ? fpre : ftmp);
-      float ar[9];
+      const float *auxc = aux_input(F, a.istage) + poff;
+      float av[9], ar[9];
 #pragma unroll
       for (int c = 0; c < 9; c++) {
-        const float av = auxc[c * F.n + ia];
-        h[c] += cB1 * r[dim][c] - cB * av;
-        ar[c] = cD * r[dim][c] - cA * av;
+        av[c] = auxc[c * F.n + ia];
+        h[c] += cB1 * r[dim][c] - cB * av[c];
+        ar[c] = cD * r[dim][c] - cA * av[c];
       }
-      if (a.istage == 0) {
-#pragma unroll
-        for (int c = 0; c < 9; c++) {
-          const float pv = fpre[c * F.n + ia];
-          ftmp[c * F.n + ia] = pv + a.ca * ar[c];
-          fend[c * F.n + ia] = pv + a.cb * ar[c];
-        }
-      } else if (a.istage < 3) {
-#pragma unroll
-        for (int c = 0; c < 9; c++) {
-          ftmp[c * F.n + ia] = fpre[c * F.n + ia] + a.ca * ar[c];
-          fend[c * F.n + ia] += a.cb * ar[c];
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 9; c++) fend[c * F.n + ia] += a.cb * ar[c];
-      }
+      aux_update(F, a, poff, ia, av, ar);
     }
   }
   if (a.fs.nfault > 0 && fault_column(a.fs, i, j + (long)k * a.g.ny)) return;
