@@ -159,8 +159,10 @@ void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu,
 void launch_adj_top(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *lam,
                     const float *mu, const float *slw, const float *mx, const float *my, const float *X, float alpha,
                     const float *E, float beta, float *T, const AdjPml &pm, hipStream_t st);
+/* vcol: the grid's columns are vertical (the gather reads the xi and eta
+   metric rows' one nonzero term each) */
 void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
-                      const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st);
+                      const float *E, float *A, float *B, float *P, const AdjPml &pm, int vcol, hipStream_t st);
 void launch_adj_inject(const long *term, const float *fac, int nterm, int nsta, const float *res, int nt, int n,
                        long sicmp, float *P, hipStream_t st);
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *term, const float *fac,

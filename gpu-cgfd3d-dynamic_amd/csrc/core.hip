@@ -1050,7 +1050,7 @@ int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residu
         fa.wT = T; fa.wX = X; fa.wE = E;
         launch_adjf_stage_pre(fa, c->st);
         launch_adj_prep(s, g, c->lam, c->mu, c->slw, X, ca, E, cb, c->adj_sg, pm, c->isfree, kdir, c->st);
-        launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, pm, c->st);
+        launch_adj_stage(s, idir, jdir, kdir, g, c->met, c->adj_sg, E, A, B, P, pm, c->vcol_grid && c->vcol_on, c->st);
         if (c->isfree)
           launch_adj_top(s, idir, jdir, kdir, g, c->met, c->lam, c->mu, c->slw, c->matVx2Vz, c->matVy2Vz, X, ca, E,
                          cb, T, pm, c->st);

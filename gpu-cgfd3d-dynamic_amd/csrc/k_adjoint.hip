@@ -147,34 +147,57 @@ DEVI float tap_coef(int t) {
              : (t == 0 ? CGFD_CB0 : t == 1 ? CGFD_CB1 : t == 2 ? CGFD_CB2 : t == 3 ? CGFD_CB3 : CGFD_CB4);
 }
 
-/* add w * kappa(., d)(p) for all nine fields; M: metric row d at p */
+/* add w * kappa(., d)(p) for all nine fields; M: metric row d at p.
+   AX = 0 or 1: only M[AX] is nonzero (the xi and eta rows on a grid of
+   vertical columns, see acc_core), and only the S and G entries it
+   multiplies are read; AX = -1: the full row */
+template <int AX>
 DEVI void add_kappa(float *lm, float w, const float *__restrict__ SG, long sc, long p, float M0, float M1, float M2) {
   const float Sx = SG[p], Sy = SG[sc + p], Sz = SG[2 * sc + p];
   const float *G = SG + 3 * sc + p;
-  lm[VX] += w * (M0 * G[0] + M1 * G[sc] + M2 * G[2 * sc]);
-  lm[VY] += w * (M0 * G[3 * sc] + M1 * G[4 * sc] + M2 * G[5 * sc]);
-  lm[VZ] += w * (M0 * G[6 * sc] + M1 * G[7 * sc] + M2 * G[8 * sc]);
-  lm[TXX] += w * (M0 * Sx);
-  lm[TYY] += w * (M1 * Sy);
-  lm[TZZ] += w * (M2 * Sz);
-  lm[TXY] += w * (M0 * Sy + M1 * Sx);
-  lm[TXZ] += w * (M0 * Sz + M2 * Sx);
-  lm[TYZ] += w * (M1 * Sz + M2 * Sy);
+  if (AX == 0) {
+    lm[VX] += w * (M0 * G[0]);
+    lm[VY] += w * (M0 * G[3 * sc]);
+    lm[VZ] += w * (M0 * G[6 * sc]);
+    lm[TXX] += w * (M0 * Sx);
+    lm[TXY] += w * (M0 * Sy);
+    lm[TXZ] += w * (M0 * Sz);
+  } else if (AX == 1) {
+    lm[VX] += w * (M1 * G[sc]);
+    lm[VY] += w * (M1 * G[4 * sc]);
+    lm[VZ] += w * (M1 * G[7 * sc]);
+    lm[TYY] += w * (M1 * Sy);
+    lm[TXY] += w * (M1 * Sx);
+    lm[TYZ] += w * (M1 * Sz);
+  } else {
+    lm[VX] += w * (M0 * G[0] + M1 * G[sc] + M2 * G[2 * sc]);
+    lm[VY] += w * (M0 * G[3 * sc] + M1 * G[4 * sc] + M2 * G[5 * sc]);
+    lm[VZ] += w * (M0 * G[6 * sc] + M1 * G[7 * sc] + M2 * G[8 * sc]);
+    lm[TXX] += w * (M0 * Sx);
+    lm[TYY] += w * (M1 * Sy);
+    lm[TZZ] += w * (M2 * Sz);
+    lm[TXY] += w * (M0 * Sy + M1 * Sx);
+    lm[TXZ] += w * (M0 * Sz + M2 * Sx);
+    lm[TYZ] += w * (M1 * Sz + M2 * Sy);
+  }
 }
 
 /* the per-face extra S and G of the faces along `dim` at tap (ti, tj, tk) */
+template <int AX>
 DEVI void add_face_kappa(float *lm, float w, const AdjPml &pm, int dim, int ti, int tj, int tk, float M0, float M1,
                          float M2) {
 #pragma unroll
   for (int side = 0; side < 2; side++) {
     const AdjFace &F = pm.f[2 * dim + side];
     if (!in_face(F, ti, tj, tk)) continue;
-    add_kappa(lm, w, F.dsg, F.n, face_idx(F, ti, tj, tk), M0, M1, M2);
+    add_kappa<AX>(lm, w, F.dsg, F.n, face_idx(F, ti, tj, tk), M0, M1, M2);
   }
 }
 
-/* MODE 3: A = L^T;  2: B = L^T, P = E + A;  1: A = L^T, P += B;  0: P += A + L^T */
-template <int ID, int JD, int KD, int MODE>
+/* MODE 3: A = L^T;  2: B = L^T, P = E + A;  1: A = L^T, P += B;  0: P += A + L^T.
+   VC: vertical columns (xi_y, xi_z, eta_x, eta_z left out, as the forward
+   box split does) */
+template <int ID, int JD, int KD, int MODE, bool VC>
 __global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restrict__ met,
                                                    const float *__restrict__ SG, const float *__restrict__ E,
                                                    float *A, float *B, float *P, AdjPml pm) {
@@ -190,18 +213,18 @@ __global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restri
 #pragma unroll
   for (int t = 0; t < 5; t++) {
     const long p = q - (ox + t);
-    const float M0 = met[sc + p], M1 = met[2 * sc + p], M2 = met[3 * sc + p];
-    add_kappa(lm, tap_coef<ID>(t), SG, sc, p, M0, M1, M2);
-    if (pm.on) add_face_kappa(lm, tap_coef<ID>(t), pm, 0, i - (ox + t), j, k, M0, M1, M2);
+    const float M0 = met[sc + p], M1 = VC ? 0.0f : met[2 * sc + p], M2 = VC ? 0.0f : met[3 * sc + p];
+    add_kappa<VC ? 0 : -1>(lm, tap_coef<ID>(t), SG, sc, p, M0, M1, M2);
+    if (pm.on) add_face_kappa<VC ? 0 : -1>(lm, tap_coef<ID>(t), pm, 0, i - (ox + t), j, k, M0, M1, M2);
   }
   __asm__ volatile("" : "+v"(lm[0]), "+v"(lm[1]), "+v"(lm[2]), "+v"(lm[3]), "+v"(lm[4]), "+v"(lm[5]),
                    "+v"(lm[6]), "+v"(lm[7]), "+v"(lm[8]) : : "memory");
 #pragma unroll
   for (int t = 0; t < 5; t++) {
     const long p = q - (long)(oy + t) * sy;
-    const float M0 = met[4 * sc + p], M1 = met[5 * sc + p], M2 = met[6 * sc + p];
-    add_kappa(lm, tap_coef<JD>(t), SG, sc, p, M0, M1, M2);
-    if (pm.on) add_face_kappa(lm, tap_coef<JD>(t), pm, 1, i, j - (oy + t), k, M0, M1, M2);
+    const float M0 = VC ? 0.0f : met[4 * sc + p], M1 = met[5 * sc + p], M2 = VC ? 0.0f : met[6 * sc + p];
+    add_kappa<VC ? 1 : -1>(lm, tap_coef<JD>(t), SG, sc, p, M0, M1, M2);
+    if (pm.on) add_face_kappa<VC ? 1 : -1>(lm, tap_coef<JD>(t), pm, 1, i, j - (oy + t), k, M0, M1, M2);
   }
   __asm__ volatile("" : "+v"(lm[0]), "+v"(lm[1]), "+v"(lm[2]), "+v"(lm[3]), "+v"(lm[4]), "+v"(lm[5]),
                    "+v"(lm[6]), "+v"(lm[7]), "+v"(lm[8]) : : "memory");
@@ -209,8 +232,8 @@ __global__ void __launch_bounds__(256) k_adj_stage(Grid g, const float *__restri
   for (int t = 0; t < 5; t++) {
     const long p = q - (long)(oz + t) * sz;
     const float M0 = met[7 * sc + p], M1 = met[8 * sc + p], M2 = met[9 * sc + p];
-    add_kappa(lm, tap_coef<KD>(t), SG, sc, p, M0, M1, M2);
-    if (pm.on) add_face_kappa(lm, tap_coef<KD>(t), pm, 2, i, j, k - (oz + t), M0, M1, M2);
+    add_kappa<-1>(lm, tap_coef<KD>(t), SG, sc, p, M0, M1, M2);
+    if (pm.on) add_face_kappa<-1>(lm, tap_coef<KD>(t), pm, 2, i, j, k - (oz + t), M0, M1, M2);
   }
 #pragma unroll
   for (int f = 0; f < 9; f++) {
@@ -469,30 +492,37 @@ void launch_adj_prep(int mode, const Grid &g, const float *lam, const float *mu,
   }
 }
 
-template <int ID, int JD, int KD>
+template <int ID, int JD, int KD, bool VC>
 static void adj_stage_dir(int mode, const Grid &g, const float *met, const float *SG, const float *E, float *A,
                           float *B, float *P, const AdjPml &pm, hipStream_t st) {
   dim3 blk(64, 4, 1), grd((g.ni + 63) / 64, (g.nj + 3) / 4, g.nk);
   switch (mode) {
-    case 3: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 3>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
-    case 2: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 2>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
-    case 1: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 1>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
-    default: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 0>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    case 3: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 3, VC>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    case 2: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 2, VC>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    case 1: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 1, VC>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+    default: hipLaunchKernelGGL((k_adj_stage<ID, JD, KD, 0, VC>), grd, blk, 0, st, g, met, SG, E, A, B, P, pm); break;
+  }
+}
+
+template <bool VC>
+static void adj_stage_vc(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
+                         const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st) {
+  switch (idir * 4 + jdir * 2 + kdir) {
+    case 0: adj_stage_dir<0, 0, 0, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 1: adj_stage_dir<0, 0, 1, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 2: adj_stage_dir<0, 1, 0, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 3: adj_stage_dir<0, 1, 1, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 4: adj_stage_dir<1, 0, 0, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 5: adj_stage_dir<1, 0, 1, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    case 6: adj_stage_dir<1, 1, 0, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
+    default: adj_stage_dir<1, 1, 1, VC>(mode, g, met, SG, E, A, B, P, pm, st); break;
   }
 }
 
 void launch_adj_stage(int mode, int idir, int jdir, int kdir, const Grid &g, const float *met, const float *SG,
-                      const float *E, float *A, float *B, float *P, const AdjPml &pm, hipStream_t st) {
-  switch (idir * 4 + jdir * 2 + kdir) {
-    case 0: adj_stage_dir<0, 0, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    case 1: adj_stage_dir<0, 0, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    case 2: adj_stage_dir<0, 1, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    case 3: adj_stage_dir<0, 1, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    case 4: adj_stage_dir<1, 0, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    case 5: adj_stage_dir<1, 0, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    case 6: adj_stage_dir<1, 1, 0>(mode, g, met, SG, E, A, B, P, pm, st); break;
-    default: adj_stage_dir<1, 1, 1>(mode, g, met, SG, E, A, B, P, pm, st); break;
-  }
+                      const float *E, float *A, float *B, float *P, const AdjPml &pm, int vcol, hipStream_t st) {
+  if (vcol) adj_stage_vc<true>(mode, idir, jdir, kdir, g, met, SG, E, A, B, P, pm, st);
+  else adj_stage_vc<false>(mode, idir, jdir, kdir, g, met, SG, E, A, B, P, pm, st);
 }
 
 void launch_adj_inject(const long *term, const float *fac, int nterm, int nsta, const float *res, int nt, int n,

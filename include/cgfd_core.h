@@ -361,7 +361,8 @@ int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
    rounding of its finite differences, about 1e-15 of xi_x and eta_y. At
    create the core checks on the device whether they stay below 2^-32 of
    xi_x and eta_y at every physical point. If so, the box-split schedules
-   (23, 24) read only the other five terms and omit the four products,
+   (23, 24) and the adjoint's transposed gather read only the other five
+   terms and omit the four products,
    which agrees with the nine-term form to the last bit at nearly every
    point and within an ulp elsewhere, with four fewer arrays per point. on = 0 makes the core use the nine-term form
    regardless; on = 1 (the default) uses the five-term form where the grid
