@@ -25,6 +25,8 @@ struct FaultDev {
   const float *P2M, *M2P;        /* [5][9][sslice] */
   const float *M2;               /* [6][9][sslice] T1toVx_Min,VytoVx_Min,VztoVx_Min,.._Plus */
   const float *Fm;               /* [14][9][ny] free-surface row */
+  const float *fmet;             /* [10][sslice] jac and metric on the plane's column i0 (a copy of the
+                                    grid's metric there, contiguous along j for the node kernels) */
   float *T1;                     /* [3][7][sslice] */
   float *hT1, *mT1;              /* [3][sslice] */
   double *slip64;                /* per scenario [3][sslice]: Slip, Slip1, Slip2 accumulated in FP64 */
@@ -43,6 +45,7 @@ struct FaultArgs {
   long sfd;                                 /* and into the FP64 slip accumulators */
   int istage, idir, jdir, kdir;
   int isfree, imethod, fix_q1;
+  int vcol;                                 /* vertical columns: xi_y, xi_z, eta_x, eta_z left out */
   float dt, ca, cb, rkb;
   const int *it_dev;                        /* completed steps (device counter) */
   /* friction tape of the adjoint (null: off): per fault [tape_nt][16][sslice],
@@ -105,6 +108,8 @@ void launch_pg(const Grid &g, const BatArgs &b, const float *wend, const float *
                hipStream_t st);
 
 void launch_wave2fault(const FaultArgs &a, const BatArgs &b, hipStream_t st);
+/* fmet[c][j + k ny] = met[c] at (i0, j, k), every ghost-inclusive (j, k) */
+void launch_fault_metric(const Grid &g, const float *met, int i0, float *fmet, hipStream_t st);
 void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st);
 void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st);
 void launch_step_end(int *it_dev, hipStream_t st);

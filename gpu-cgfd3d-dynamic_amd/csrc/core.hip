@@ -386,7 +386,13 @@ static int upload_faults(cgfd_core *c, const cgfd_desc *ds) {
     if ((rc = c->alloc(&dp, h.size()))) return rc;
     HIPCHK(hipMemcpy(dp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     F.Fm = dp;
+    /* the metric on the plane's column, contiguous along j */
+    if ((rc = c->alloc(&dp, 10 * ss))) return rc;
+    launch_fault_metric(c->g, c->met, F.i0, dp, c->st);
+    F.fmet = dp;
   }
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipGetLastError());
   return CGFD_OK;
 }
 
@@ -623,6 +629,7 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.matVx2Vz = c->matVx2Vz; f.matVy2Vz = c->matVy2Vz;
   f.w_cur = wcur; f.w_pre = c->wlev(c->lv_pre); f.w_tnew = wnew; f.w_end = c->wlev(c->lv_a);
   f.w_b = c->wlev(c->lv_b);
+  f.vcol = c->vcol_grid && c->vcol_on;
   f.f_cur = fcur; f.f_pre = c->flev(c->fl_pre); f.f_tnew = fnew; f.f_end = c->flev(c->fl_end);
   f.nfault = c->nfault;
   f.ftab = c->d_ftab;

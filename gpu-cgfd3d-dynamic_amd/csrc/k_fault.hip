@@ -41,6 +41,26 @@ DEVI void metric_cols(const float *met, long sc, long p, float M[3][3]) {
   M[2][0] = met[3 * sc + p]; M[2][1] = met[6 * sc + p]; M[2][2] = met[9 * sc + p];
 }
 
+/* the same from the fault's copy of the metric on its column, [10][sslice] */
+DEVI void metric_cols_f(const float *fm, long ss, long pf, float M[3][3]) {
+  M[0][0] = fm[ss + pf];     M[0][1] = fm[4 * ss + pf]; M[0][2] = fm[7 * ss + pf];
+  M[1][0] = fm[2 * ss + pf]; M[1][1] = fm[5 * ss + pf]; M[1][2] = fm[8 * ss + pf];
+  M[2][0] = fm[3 * ss + pf]; M[2][1] = fm[6 * ss + pf]; M[2][2] = fm[9 * ss + pf];
+}
+
+__global__ void __launch_bounds__(256) k_fault_metric(Grid g, const float *__restrict__ met, int i0, float *fmet) {
+  const int j = blockIdx.x * 256 + (int)threadIdx.x, k = blockIdx.y;
+  if (j >= g.ny) return;
+  const long p = i0 + (long)j * g.siy + (long)k * g.siz, pf = j + (long)k * g.ny;
+#pragma unroll
+  for (int c = 0; c < 10; c++) fmet[c * g.sslice + pf] = met[c * g.sicmp + p];
+}
+
+void launch_fault_metric(const Grid &g, const float *met, int i0, float *fmet, hipStream_t st) {
+  hipLaunchKernelGGL(k_fault_metric, dim3((unsigned)((g.ny + 255) / 256), (unsigned)g.nz), dim3(256), 0, st, g, met,
+                     i0, fmet);
+}
+
 /* ---------------------------------------------------------------------- */
 DEVI void k_wave2fault_body(const FaultArgs &a, int id) {
   const int iy = blockIdx.x * blockDim.x + threadIdx.x;
@@ -53,8 +73,8 @@ DEVI void k_wave2fault_body(const FaultArgs &a, int id) {
   const long p = F.i0 + (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
   const float *W = a.w_cur;
   float M[3][3], S[3][3], Tr[3][3];
-  metric_cols(a.met, sc, p, M);
-  float jac = a.met[p];
+  metric_cols_f(F.fmet, ss, pf, M);
+  float jac = F.fmet[pf];
   float txx = W[TXX * sc + p], tyy = W[TYY * sc + p], tzz = W[TZZ * sc + p];
   float tyz = W[TYZ * sc + p], txz = W[TXZ * sc + p], txy = W[TXY * sc + p];
   S[0][0] = txx; S[0][1] = txy; S[0][2] = txz;
@@ -91,8 +111,8 @@ __global__ void __launch_bounds__(256) k_wave2fault(FaultArgs a, BatArgs b) {
 DEVI void fault2wave_node(const FaultArgs &a, const FaultDev &F, long pf, long p, float *w, const float *fl) {
   const long sc = a.g.sicmp, ss = a.g.sslice;
   float M[3][3], Tr[3][3], S[3][3];
-  metric_cols(a.met, sc, p, M);
-  float rj = (float)(1.0 / a.met[p]);
+  metric_cols_f(F.fmet, ss, pf, M);
+  float rj = (float)(1.0 / F.fmet[pf]);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     Tr[c][0] = F.T1[(long)c * 7 * ss + 3 * ss + pf];
@@ -150,10 +170,18 @@ DEVI void k_trial_t1_body(const FaultArgs &a, int id) {
   const float *W = a.w_cur;
   const float *met = a.met;
   const long q = F.i0 + (2 * m - 1) * l + (long)(iy + 3) * a.g.siy + (long)(iz + 3) * a.g.siz;
-  float x1 = met[sc + q], x2 = met[2 * sc + q], x3 = met[3 * sc + q], jl = met[q];
-  float txx = W[TXX * sc + q], tyy = W[TYY * sc + q], tzz = W[TZZ * sc + q];
-  float tyz = W[TYZ * sc + q], txz = W[TXZ * sc + q], txy = W[TXY * sc + q];
+  const float x1 = met[sc + q], jl = met[q];
+  const float txx = W[TXX * sc + q], txz = W[TXZ * sc + q], txy = W[TXY * sc + q];
   const long slot = 3 + (2 * m - 1) * l;
+  if (a.vcol) {
+    /* vertical columns: xi_y and xi_z vanish (see the volume kernels) */
+    F.T1[slot * ss + pf] = jl * (x1 * txx);
+    F.T1[7 * ss + slot * ss + pf] = jl * (x1 * txy);
+    F.T1[14 * ss + slot * ss + pf] = jl * (x1 * txz);
+    return;
+  }
+  const float x2 = met[2 * sc + q], x3 = met[3 * sc + q];
+  const float tyy = W[TYY * sc + q], tzz = W[TZZ * sc + q], tyz = W[TYZ * sc + q];
   F.T1[slot * ss + pf] = jl * (x1 * txx + x2 * txy + x3 * txz);
   F.T1[7 * ss + slot * ss + pf] = jl * (x1 * txy + x2 * tyy + x3 * tyz);
   F.T1[14 * ss + slot * ss + pf] = jl * (x1 * txz + x2 * tyz + x3 * tzz);
@@ -186,7 +214,7 @@ DEVI void k_trial_body(const FaultArgs &a, int id) {
   float Mrho[2], R[2][3];
 #pragma unroll
   for (int m = 0; m < 2; m++) {
-    float jac0 = met[p0];
+    float jac0 = F.fmet[pf];
     float rho = F.rho_f[m * ss + pf];
     Mrho[m] = (float)(0.5 * jac0 * rho);
     float t1[3][3];  /* [comp][l-1]: the traction columns k_trial_t1 wrote */
@@ -229,9 +257,9 @@ DEVI void k_trial_body(const FaultArgs &a, int id) {
   }
   float vs1[3] = {F.vs[pf], F.vs[ss + pf], F.vs[2 * ss + pf]};
   float vs2[3] = {F.vs[3 * ss + pf], F.vs[4 * ss + pf], F.vs[5 * ss + pf]};
-  float vn[3] = {met[sc + p0], met[2 * sc + p0], met[3 * sc + p0]};
+  float vn[3] = {F.fmet[ss + pf], F.fmet[2 * ss + pf], F.fmet[3 * ss + pf]};
   float vn0 = norm3(vn);
-  float jacvec = met[p0] * vn0;
+  float jacvec = F.fmet[pf] * vn0;
 #pragma unroll
   for (int c = 0; c < 3; c++) vn[c] /= vn0;
   float T0[3] = {F.T0[pf], F.T0[ss + pf], F.T0[2 * ss + pf]};
@@ -319,15 +347,17 @@ DEVI void rk_cols(const FaultArgs &a, long p, int c0, int nc, const float *h) {
    loads of a node's x taps are adjacent addresses. */
 constexpr int kColNodes = 10;  /* nodes per wavefront in the column kernel */
 
-/* Traction vectors J (grad xi_d . T) a block of kColNodes nodes x 4 rows
-   needs, computed once each into LDS: along x at the 13 columns i0-6..i0+6,
-   along y at the 6 columns over rows j-3..j+12, along z at the 6 columns
-   over planes k-3..k+6 (the union of the 7-point stencils of the block's
-   lanes, each of which would otherwise evaluate 21 of them). */
-constexpr int kColRows = 4;                                       /* blockDim.y */
-constexpr int kTvX = 13 * kColNodes * kColRows;                  /* 520 */
-constexpr int kTvY = 6 * (kColNodes + 6) * kColRows;             /* 384 */
-constexpr int kTvZ = 6 * kColNodes * (kColRows + 6);             /* 600 */
+/* Traction vectors J (grad xi_d . T) a block of kColNodes nodes x kColRows
+   rows needs, computed once each into LDS: along x at the 13 columns
+   i0-6..i0+6, along y at the 6 columns over rows j-3..j+12, along z at the 6
+   columns over planes k-3..k+kColRows+2 (the union of the 7-point stencils
+   of the block's lanes, each of which would otherwise evaluate 21 of them).
+   Eight rows per block: the z halo planes are 6 per 8 rows instead of 6 per
+   4, and the block's traction vectors take 32 KB of LDS. */
+constexpr int kColRows = 8;                                       /* blockDim.y */
+constexpr int kTvX = 13 * kColNodes * kColRows;                  /* 1040 */
+constexpr int kTvY = 6 * (kColNodes + 6) * kColRows;             /* 768 */
+constexpr int kTvZ = 6 * kColNodes * (kColRows + 6);             /* 840 */
 constexpr int kTv = kTvX + kTvY + kTvZ;
 
 template <int IDIR>
@@ -367,11 +397,24 @@ DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
       float o0 = 0.0f, o1 = 0.0f, o2 = 0.0f;
       if (x >= 0 && x < a.g.nx && jg >= 0 && jg < a.g.ny && kg >= 0 && kg < a.g.nz) {
         const long q = x + (long)jg * sy + (long)kg * sz;
-        const float jq = met[q], a1 = met[(1 + 3 * d) * sc + q], a2 = met[(2 + 3 * d) * sc + q],
-                    a3 = met[(3 + 3 * d) * sc + q];
-        o0 = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
-        o1 = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
-        o2 = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+        const float jq = met[q];
+        if (a.vcol && d == 0) {         /* vertical columns: xi_y, xi_z vanish */
+          const float a1 = met[sc + q];
+          o0 = jq * (a1 * Txx[q]);
+          o1 = jq * (a1 * Txy[q]);
+          o2 = jq * (a1 * Txz[q]);
+        } else if (a.vcol && d == 1) {  /* eta_x, eta_z vanish */
+          const float a2 = met[5 * sc + q];
+          o0 = jq * (a2 * Txy[q]);
+          o1 = jq * (a2 * Tyy[q]);
+          o2 = jq * (a2 * Tyz[q]);
+        } else {
+          const float a1 = met[(1 + 3 * d) * sc + q], a2 = met[(2 + 3 * d) * sc + q],
+                      a3 = met[(3 + 3 * d) * sc + q];
+          o0 = jq * (a1 * Txx[q] + a2 * Txy[q] + a3 * Txz[q]);
+          o1 = jq * (a1 * Txy[q] + a2 * Tyy[q] + a3 * Tyz[q]);
+          o2 = jq * (a1 * Txz[q] + a2 * Tyz[q] + a3 * Tzz[q]);
+        }
       }
       tv[0][v] = o0;
       tv[1][v] = o1;
@@ -521,7 +564,7 @@ DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
 }
 
 template <int IDIR>
-__global__ void __launch_bounds__(256) k_fault_cols(FaultArgs a, BatArgs b) {
+__global__ void __launch_bounds__(64 * kColRows) k_fault_cols(FaultArgs a, BatArgs b) {
   const int id = (int)blockIdx.z % a.nfault;
   k_fault_cols_body<IDIR>(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
 }
@@ -560,7 +603,7 @@ DEVI void k_fault_nodes_body(const FaultArgs &a, int id) {
   float fhV[2][3];
 #pragma unroll
   for (int m = 0; m < 2; m++) {
-    float rrhojac = (float)(1.0 / (F.rho_f[m * ss + pf] * met[p0]));
+    float rrhojac = (float)(1.0 / (F.rho_f[m * ss + pf] * F.fmet[pf]));
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       const float *t = T1 + (long)c * 7 * ss + pf;
@@ -834,7 +877,8 @@ void launch_trial(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
 }
 void launch_fault_update(const FaultArgs &a, const BatArgs &b, hipStream_t st) {
   if (a.nfault <= 0) return;
-  dim3 cblk(64, 4, 1), cgrd((a.g.nj + kColNodes - 1) / kColNodes, (a.g.nk + 3) / 4, a.nfault * b.nact);
+  dim3 cblk(64, kColRows, 1),
+      cgrd((a.g.nj + kColNodes - 1) / kColNodes, (a.g.nk + kColRows - 1) / kColRows, a.nfault * b.nact);
   dim3 nblk(64, 4, 1), ngrd((a.g.nj + 63) / 64, (a.g.nk + 3) / 4, a.nfault * b.nact);
   if (a.idir == 1) {
     hipLaunchKernelGGL(k_fault_cols<1>, cgrd, cblk, 0, st, a, b);

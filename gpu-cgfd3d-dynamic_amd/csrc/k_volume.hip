@@ -581,7 +581,7 @@ __global__ void __launch_bounds__(256) k_vol_direct(VolArgs a, BatArgs b) {
  * ---------------------------------------------------------------------- */
 __device__ const int kOpIdx[2][5] = {{-3, -2, -1, 0, 1}, {-1, 0, 1, 2, 3}};
 
-template <int ID, int JD, int KD>
+template <int ID, int JD, int KD, bool VC>
 DEVI void top_point(const VolArgs &a, int i, int j, int k, long woff, long poff) {
   const float *__restrict__ const cur = a.cur + woff;
   const int nk2 = a.g.nk2;
@@ -599,8 +599,10 @@ DEVI void top_point(const VolArgs &a, int i, int j, int k, long woff, long poff)
   const float *jac = M, *xi_x = M + sc, *xi_y = M + 2 * sc, *xi_z = M + 3 * sc;
   const float *et_x = M + 4 * sc, *et_y = M + 5 * sc, *et_z = M + 6 * sc;
   const float *zt_x = M + 7 * sc, *zt_y = M + 8 * sc, *zt_z = M + 9 * sc;
-  float xx = xi_x[p], xy = xi_y[p], xz = xi_z[p];
-  float ex = et_x[p], ey = et_y[p], ez = et_z[p];
+  /* VC: vertical columns, xi_y, xi_z, eta_x, eta_z vanish (see acc_core) */
+  constexpr bool vc = VC;
+  float xx = xi_x[p], xy = vc ? 0.0f : xi_y[p], xz = vc ? 0.0f : xi_z[p];
+  float ex = vc ? 0.0f : et_x[p], ey = et_y[p], ez = vc ? 0.0f : et_z[p];
   float zx = zt_x[p], zy = zt_y[p], zz = zt_z[p];
   float l = a.lam[p], m = a.mu[p], s = a.slw[p];
   float h[9];
@@ -623,9 +625,9 @@ DEVI void top_point(const VolArgs &a, int i, int j, int k, long woff, long poff)
 #pragma unroll
         for (int n = 0; n < 5; n++) {
           long q = p + fx[n];
-          vx[n] = jac[q] * (xi_x[q] * A[q] + xi_y[q] * B[q] + xi_z[q] * C[q]);
+          vx[n] = vc ? jac[q] * (xi_x[q] * A[q]) : jac[q] * (xi_x[q] * A[q] + xi_y[q] * B[q] + xi_z[q] * C[q]);
           q = p + fy[n] * sy;
-          ve[n] = jac[q] * (et_x[q] * A[q] + et_y[q] * B[q] + et_z[q] * C[q]);
+          ve[n] = vc ? jac[q] * (et_y[q] * B[q]) : jac[q] * (et_x[q] * A[q] + et_y[q] * B[q] + et_z[q] * C[q]);
         }
 #pragma unroll
         for (int n = 0; n < 5; n++) {
@@ -679,14 +681,14 @@ DEVI void top_point(const VolArgs &a, int i, int j, int k, long woff, long poff)
 }
 
 /* grid z = plane x scenario */
-template <int ID, int JD, int KD>
+template <int ID, int JD, int KD, bool VC>
 __global__ void __launch_bounds__(256) k_vol_top(VolArgs a, BatArgs b) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y * blockDim.y + threadIdx.y;
   if (ix >= a.g.ni || iy >= a.g.nj) return;
   const int kz = (int)blockIdx.z % a.nk_run;
   const long sn = b.act[blockIdx.z / a.nk_run];
-  top_point<ID, JD, KD>(a, ix + 3, iy + 3, a.k_lo + kz + 3, sn * b.wstride, sn * b.pstride);
+  top_point<ID, JD, KD, VC>(a, ix + 3, iy + 3, a.k_lo + kz + 3, sn * b.wstride, sn * b.pstride);
 }
 
 /* ------------------------------------------------------------------------
@@ -822,7 +824,8 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
   dim3 agrd((a.g.ni + 3 + 63) / 64, (a.g.nj + 3) / 4, a.nk_run);  /* aligned tiles */
   if (top) {
     dim3 tgrd(grd.x, grd.y, (unsigned)(a.nk_run * b.nact));
-    hipLaunchKernelGGL((k_vol_top<ID, JD, KD>), tgrd, blk, 0, st, a, b);
+    if (a.vcol) hipLaunchKernelGGL((k_vol_top<ID, JD, KD, true>), tgrd, blk, 0, st, a, b);
+    else hipLaunchKernelGGL((k_vol_top<ID, JD, KD, false>), tgrd, blk, 0, st, a, b);
     return;
   }
   if (variant == 23 || variant == 24) {
