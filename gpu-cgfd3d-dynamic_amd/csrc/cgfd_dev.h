@@ -50,12 +50,21 @@ struct Grid {
   long sslice;       /* ny*nz */
 };
 
-/* one CFS-PML face (bdrypml_t, bdry_t.h:97-123) */
+/* one CFS-PML face (bdrypml_t, bdry_t.h:97-123). A face spanning the grid
+   in x (the y and z faces) stores its rows from the 32-column boundary at or
+   below i1 with a pitch of whole 32-column segments, so a tile row of the
+   volume kernels reads its auxiliaries in the same 128-byte lines as its
+   wavefield; the narrow x faces store their columns packed. */
+__host__ __device__ inline void pml_row_layout(int dim, int i1, int i2, int &ax0, int &an) {
+  if (dim == 0) { ax0 = i1; an = i2 - i1 + 1; }
+  else { ax0 = i1 & ~31; an = (i2 + 1 - ax0 + 31) & ~31; }
+}
 struct PmlFace {
   int on;
   int i1, i2, j1, j2, k1, k2;  /* ghost-inclusive index ranges */
-  int an, bn;                  /* slab extent in x and y */
-  long n;                      /* slab points */
+  int an, bn;                  /* row pitch (x) and rows per plane (y) of the slab's arrays */
+  int ax0;                     /* x index of a row's first stored column */
+  long n;                      /* slab entries (an x bn x planes) */
   const float *A, *B, *D;      /* profiles along the face's axis */
   float *pre, *la, *lb, *lc;   /* aux levels, [9][n] each: the step's start and
                                   the outputs of stages 0, 1, 2 (stage 3 writes

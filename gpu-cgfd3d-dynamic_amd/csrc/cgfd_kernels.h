@@ -119,8 +119,8 @@ void launch_nan_check(const float *p, long n, int *flag, hipStream_t st);
 /* discrete adjoint of the volume update (k_adjoint.hip) */
 struct AdjFace {          /* one CFS-PML face of the adjoint */
   int on, dim;
-  int i1, i2, j1, j2, k1, k2, an, bn;
-  long n;                 /* slab points */
+  int i1, i2, j1, j2, k1, k2, an, bn, ax0;  /* row layout as the forward face's (PmlFace) */
+  long n;                 /* slab entries */
   const float *A, *B, *D; /* profiles along the face's axis */
   float *lev;             /* auxiliary adjoint levels E, A, B, P (by index), [9][n] each */
   float *dsg;             /* extra S and G of the face, [12][n] */

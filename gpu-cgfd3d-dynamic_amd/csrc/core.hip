@@ -499,8 +499,10 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
   for (int dim = 0; dim < 3; dim++)
     for (int side = 0; side < 2; side++) {
       if (!d->pml_is_sides[dim][side]) continue;
-      long n = (long)(d->pml_ni2[dim][side] - d->pml_ni1[dim][side] + 1) *
-               (d->pml_nj2[dim][side] - d->pml_nj1[dim][side] + 1) * (d->pml_nk2[dim][side] - d->pml_nk1[dim][side] + 1);
+      int ax0, an;
+      pml_row_layout(dim, d->pml_ni1[dim][side], d->pml_ni2[dim][side], ax0, an);
+      long n = (long)an * (d->pml_nj2[dim][side] - d->pml_nj1[dim][side] + 1) *
+               (d->pml_nk2[dim][side] - d->pml_nk1[dim][side] + 1);
       paux += 36 * n;
     }
   float *auxblk = nullptr;
@@ -519,10 +521,11 @@ static int core_create(const cgfd_desc *ds, int nscen, cgfd_core **out) {
       f.i1 = d->pml_ni1[dim][side]; f.i2 = d->pml_ni2[dim][side];
       f.j1 = d->pml_nj1[dim][side]; f.j2 = d->pml_nj2[dim][side];
       f.k1 = d->pml_nk1[dim][side]; f.k2 = d->pml_nk2[dim][side];
-      f.an = f.i2 - f.i1 + 1; f.bn = f.j2 - f.j1 + 1;
+      pml_row_layout(dim, f.i1, f.i2, f.ax0, f.an);
+      f.bn = f.j2 - f.j1 + 1;
       int cn = f.k2 - f.k1 + 1;
       f.n = (long)f.an * f.bn * cn;
-      int np = dim == 0 ? f.an : (dim == 1 ? f.bn : cn);
+      int np = dim == 0 ? f.i2 - f.i1 + 1 : (dim == 1 ? f.bn : cn);
       float *cf;
       if ((rc = c->alloc(&cf, (size_t)3 * np))) return fail(rc);
       if (hipMemcpy(cf, d->pml_A[dim][side], np * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -995,7 +998,7 @@ int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residu
         a.dim = dim;
         if (!f.on) continue;
         a.i1 = f.i1; a.i2 = f.i2; a.j1 = f.j1; a.j2 = f.j2; a.k1 = f.k1; a.k2 = f.k2;
-        a.an = f.an; a.bn = f.bn; a.n = f.n;
+        a.an = f.an; a.ax0 = f.ax0; a.bn = f.bn; a.n = f.n;
         a.A = f.A; a.B = f.B; a.D = f.D;
         a.lev = c->adj_pml + off;
         a.dsg = c->adj_pml + off + 36 * f.n;

@@ -75,7 +75,7 @@ DEVI bool in_face(const AdjFace &F, int i, int j, int k) {
   return F.on && i >= F.i1 && i <= F.i2 && j >= F.j1 && j <= F.j2 && k >= F.k1 && k <= F.k2;
 }
 DEVI long face_idx(const AdjFace &F, int i, int j, int k) {
-  return (long)(k - F.k1) * ((long)F.bn * F.an) + (long)(j - F.j1) * F.an + (i - F.i1);
+  return (long)(k - F.k1) * ((long)F.bn * F.an) + (long)(j - F.j1) * F.an + (i - F.ax0);
 }
 
 /* S and G from c = alpha X + beta E at physical points (X may be null);

@@ -67,7 +67,7 @@ DEVI void pml_point(const VolArgs &a, int i, int j, int k, long p, const float *
       const PmlFace &f = a.pml[dim][side];
       if (!f.on) continue;
       if (i < f.i1 || i > f.i2 || j < f.j1 || j > f.j2 || k < f.k1 || k > f.k2) continue;
-      long ia = (long)(k - f.k1) * ((long)f.bn * f.an) + (long)(j - f.j1) * f.an + (i - f.i1);
+      long ia = (long)(k - f.k1) * ((long)f.bn * f.an) + (long)(j - f.j1) * f.an + (i - f.ax0);
       int ic = dim == 0 ? i - f.i1 : (dim == 1 ? j - f.j1 : k - f.k1);
       float cD = f.D[ic], cA = f.A[ic], cB = f.B[ic];
       float cB1 = cB - 1.0f;

@@ -305,7 +305,7 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
       const PmlFace &F = a.pml[dim][side];
       if (!F.on) continue;
       if (i < F.i1 || i > F.i2 || j < F.j1 || j > F.j2 || k < F.k1 || k > F.k2) continue;
-      const long ia = (long)(k - F.k1) * ((long)F.bn * F.an) + (long)(j - F.j1) * F.an + (i - F.i1);
+      const long ia = (long)(k - F.k1) * ((long)F.bn * F.an) + (long)(j - F.j1) * F.an + (i - F.ax0);
       const int ic = dim == 0 ? i - F.i1 : (dim == 1 ? j - F.j1 : k - F.k1);
       const float cD = F.D[ic], cA = F.A[ic], cB = F.B[ic];
       const float cB1 = cB - 1.0f;
