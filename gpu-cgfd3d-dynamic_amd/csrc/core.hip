@@ -651,16 +651,11 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
   f.tape_sstride = (long)c->nfault * c->tape_nt * kTapeFloats * c->g.sslice;
 }
 
-/* volume kernel variants (see cgfd_core_set_variant): 1 direct loads,
-   16 split interior/PML kernels with the interior in slab order, 18 = 16
-   with DPP x taps, 19 = 16 with the lean PML-tile kernel, 20 and 21 = 19 on
-   one scenario and two batch orders */
+/* volume kernel variants (see cgfd_core_set_variant): 1 direct loads; 20
+   an interior kernel in slab order and a lean kernel for the tiles touching
+   a CFS-PML slab; 23 and 24 the box split */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
-  if (c->bat.nact > 1) {  /* batch kernels: split interior/PML for 16 and 19..21, else one direct kernel */
-    const int bv = (c->variant == 16 || c->variant >= 19) ? c->variant : 1;
-    launch_volume(bv, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
-  }
-  else launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
+  launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
 }
 
 /* one time step; if ev != nullptr, event pairs bracket the volume launches */
@@ -1557,7 +1552,7 @@ TuneKey tune_key(const cgfd_core *c) {
   return TuneKey{c->g.ni, c->g.nj, c->g.nk, c->nscen, c->nfault, c->isfree, pml};
 }
 
-bool valid_variant(int v) { return v == 1 || v == 16 || (v >= 19 && v <= 21) || v == 23 || v == 24; }
+bool valid_variant(int v) { return v == 1 || v == 20 || v == 23 || v == 24; }
 
 /* the cache file: one "ni nj nk nscen nfault isfree pml variant zchunk" line
    per tuned shape; later lines win */
@@ -1605,7 +1600,7 @@ static void apply_auto_variant(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant != 0 && !valid_variant(variant))
-    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 16, 19, 20, 21, 23, 24)", variant);
+    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 20, 23, 24)", variant);
   if (variant == 0) apply_auto_variant(c);
   else c->variant = variant;
   c->drop_graph();
@@ -1656,8 +1651,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const bool batch = c->bat.nact > 1;
-  const std::vector<int> variants = batch ? std::vector<int>{23, 24, 20, 19, 21, 16, 1} : std::vector<int>{23, 24, 20, 16, 1};
+  const std::vector<int> variants = {23, 24, 20, 1};
   const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;

@@ -327,30 +327,27 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
 /* Kernel variant selection for the volume stencil. 0 = automatic: the
    variant and z-chunk cgfd_core_autotune chose for a problem of this shape
    in this process (or in the cache file named by CGFD_TUNE_CACHE), else 23.
-   1 = one direct-load kernel (27 derivatives per point); 16 = an interior
+   1 = one direct-load kernel (27 derivatives per point); 20 = an interior
    kernel that accumulates each field's gradient into the rhs (64 x 4 tiles
-   in slab order: zchunk y-tiles per slab, then the plane) and a direct
-   kernel for the tiles touching a CFS-PML slab; 19 = 16 with a lean
-   PML-tile kernel that accumulates the directional rhs parts field by field;
-   20 and 21 = 19 on one scenario; 23 and 24 = the box split: the
-   accumulating interior kernel over exactly the points no CFS-PML face
-   contains (x bounds on 32-column boundaries; 64 x 4 tiles for 23, 32 x 8
-   for 24; slab order) and the lean PML kernel over the up to six slabs
-   around them in 32 x 8 tiles. Batches with more than one active scenario
-   run split interior / PML kernels for 16 (one slab-ordered sweep per
-   scenario), 19 (the scenarios of a tile back to back, XCD-contiguous, lean
-   PML kernel), 20 (interior as 19, lean PML blocks in one slab-ordered
-   sweep per scenario), 21 (both kernels in per-scenario sweeps), 23 and 24
-   (the box split: interior tiles with the scenarios back to back,
-   XCD-contiguous; bands in one sweep per scenario), and one direct tile x
-   scenario kernel for 1. Returns CGFD_ERR_ARG for any other value. */
+   in slab order: zchunk y-tiles per slab, then the plane) and a lean kernel
+   for the tiles touching a CFS-PML slab that accumulates the directional
+   rhs parts field by field; 23 and 24 = the box split: the accumulating
+   interior kernel over exactly the points no CFS-PML face contains (x
+   bounds on 32-column boundaries; 64 x 4 tiles for 23, 32 x 8 for 24; slab
+   order) and the lean PML kernel over the up to six slabs around them in
+   32 x 8 tiles. Batches with more than one active scenario run, for 20,
+   the interior tiles with the scenarios back to back (XCD-contiguous) and
+   the PML tiles in one slab-ordered sweep per scenario; for 23 and 24 the
+   box split the same way (interior tiles with the scenarios back to back,
+   bands in one sweep per scenario); for 1 one direct tile x scenario
+   kernel. Returns CGFD_ERR_ARG for any other value. */
 int cgfd_core_set_variant(cgfd_core *core, int variant);
 
 /* The schedule the core will run: variant and z-chunk (after variant 0 has
    been resolved). */
 int cgfd_core_get_variant(const cgfd_core *core, int *variant, int *zchunk);
 
-/* y-tiles per slab of the slab-ordered kernels (16, 19..21, 23, 24), 4..4096. */
+/* y-tiles per slab of the slab-ordered kernels (20, 23, 24), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
 /* Vertical-column metric form, no counterpart in the reference (whose
@@ -362,18 +359,19 @@ int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
    create the core checks on the device whether they stay below 2^-32 of
    xi_x and eta_y at every physical point. If so, the box-split schedules
    (23, 24) and the adjoint's transposed gather read only the other five
-   terms and omit the four products,
-   which agrees with the nine-term form to the last bit at nearly every
-   point and within an ulp elsewhere, with four fewer arrays per point. on = 0 makes the core use the nine-term form
-   regardless; on = 1 (the default) uses the five-term form where the grid
-   has it. *grid: whether the grid has it; *in_use: whether the volume
+   terms and omit the four products, which agrees with the nine-term form
+   to the last bit at nearly every point and within an ulp elsewhere, with
+   four fewer arrays per point; the free-surface and fault kernels take the
+   same form for their traction vectors. on = 0 makes the core use the
+   nine-term form regardless; on = 1 (the default) uses the five-term form
+   where the grid has it. *grid: whether the grid has it; *in_use: whether the volume
    update uses it. */
 int cgfd_core_set_vertical_columns(cgfd_core *core, int on);
 int cgfd_core_get_vertical_columns(const cgfd_core *core, int *grid, int *in_use);
 
-/* Autotuner: times every candidate (variant, z-chunk) pair -- 23, 24, 20,
-   16 and 1 on one scenario; 23, 24, 20, 19, 21, 16 and 1 on a batch;
-   z-chunks 4..32 for the slab-ordered ones -- on `reps` repetitions of one step's four volume launches, keeps
+/* Autotuner: times every candidate (variant, z-chunk) pair -- 23, 24, 20
+   and 1, z-chunks 4..32 for the slab-ordered ones -- on `reps` repetitions
+   of one step's four volume launches, keeps
    the fastest and records it for this problem shape (grid, scenarios,
    faults, boundaries) so later cores of the same shape start with it, and
    appends it to the file named by CGFD_TUNE_CACHE if that is set; *ms_best

@@ -143,7 +143,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 16, 19, 20, 21, 23, 24])
+@pytest.mark.parametrize("variant", [1, 20, 23, 24])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation the autotuner chooses from."""
     from cgfd import configs, model
@@ -183,7 +183,7 @@ def test_graph_replay_matches_eager(built_libs):
     assert np.array_equal(cg.read_stations(), ce.read_stations())
 
 
-@pytest.mark.parametrize("variant", [16, 19, 20, 1, 23, 24])
+@pytest.mark.parametrize("variant", [20, 1, 23, 24])
 def test_stage_fork_matches_serial(built_libs, variant):
     """The stage fork (interior blocks, CFS-PML blocks + top layers, fault
     kernels on three streams) against every kernel on one stream: the
@@ -283,7 +283,7 @@ def test_locked_fault_transmits_without_slip_gpu(built_libs, imethod):
     st.close()
 
 
-@pytest.mark.parametrize("variant", [20, 16, 1, 23, 24])
+@pytest.mark.parametrize("variant", [20, 1, 23, 24])
 def test_ragged_grid_matches_oracle(built_libs, variant):
     """A grid whose sizes are multiples of none of the tile sizes (37 x 45 x
     27: partial 64-wide x tiles, a partial last row of 4-row y tiles, z
