@@ -123,7 +123,7 @@ struct AdjFace {          /* one CFS-PML face of the adjoint */
   long n;                 /* slab entries */
   const float *A, *B, *D; /* profiles along the face's axis */
   float *lev;             /* auxiliary adjoint levels E, A, B, P (by index), [9][n] each */
-  float *dsg;             /* extra S and G of the face, [12][n] */
+  float *dsg;             /* extra S and G of the face, [9][n] */
 };
 struct AdjPml {
   int on;

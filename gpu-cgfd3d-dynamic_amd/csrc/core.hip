@@ -966,11 +966,11 @@ int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residu
   long npml = 0;  /* floats of the PML adjoint block */
   for (int dim = 0; dim < 3; dim++)
     for (int side = 0; side < 2; side++)
-      if (c->pml[dim][side].on) npml += 48 * c->pml[dim][side].n;
+      if (c->pml[dim][side].on) npml += 45 * c->pml[dim][side].n;
   if (!c->adj) {
     if ((rc = c->alloc(&c->adj, (size_t)4 * lv))) return rc;
-    if ((rc = c->alloc(&c->adj_sg, (size_t)12 * sc))) return rc;
-    HIPCHK(hipMemset(c->adj_sg, 0, (size_t)12 * sc * 4));  /* ghost entries stay zero */
+    if ((rc = c->alloc(&c->adj_sg, (size_t)9 * sc))) return rc;
+    HIPCHK(hipMemset(c->adj_sg, 0, (size_t)9 * sc * 4));  /* ghost entries stay zero */
     if (npml > 0 && (rc = c->alloc(&c->adj_pml, (size_t)npml))) return rc;
   }
   const size_t nadjf = (size_t)c->nfault * kAdjFaultFloats * ss;
@@ -998,7 +998,7 @@ int adj_sweep(cgfd_core *c, int scen, const float *residual, const float *residu
         a.lev = c->adj_pml + off;
         a.dsg = c->adj_pml + off + 36 * f.n;
         HIPCHK(hipMemsetAsync(a.lev, 0, (size_t)36 * f.n * 4, c->st));
-        off += 48 * f.n;
+        off += 45 * f.n;
       }
   }
   AdjFaultArgs fa{};

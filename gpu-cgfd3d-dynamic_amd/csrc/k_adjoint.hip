@@ -54,21 +54,21 @@
 
 namespace cgfd {
 
-/* S (3) and G (9) of an adjoint rhs c with the media at the point */
+/* S (3) and the 6 distinct entries of G of an adjoint rhs c with the media
+   at the point: G is symmetric (gamma(Vx, y) = gamma(Vy, x) = mu c_xy and
+   so on), so a point stores 9 values, not 12 */
+constexpr int kSG = 9;
 DEVI void sg_of(const float *c, float l, float m, float s, float *o) {
   const float l2m = l + 2.0f * m;
   o[0] = s * c[VX];
   o[1] = s * c[VY];
   o[2] = s * c[VZ];
   o[3] = l2m * c[TXX] + l * c[TYY] + l * c[TZZ];  /* gamma(Vx, x) */
-  o[4] = m * c[TXY];                              /* gamma(Vx, y) */
-  o[5] = m * c[TXZ];                              /* gamma(Vx, z) */
-  o[6] = m * c[TXY];                              /* gamma(Vy, x) */
-  o[7] = l * c[TXX] + l2m * c[TYY] + l * c[TZZ];  /* gamma(Vy, y) */
-  o[8] = m * c[TYZ];                              /* gamma(Vy, z) */
-  o[9] = m * c[TXZ];                              /* gamma(Vz, x) */
-  o[10] = m * c[TYZ];                             /* gamma(Vz, y) */
-  o[11] = l * c[TXX] + l * c[TYY] + l2m * c[TZZ]; /* gamma(Vz, z) */
+  o[4] = l * c[TXX] + l2m * c[TYY] + l * c[TZZ];  /* gamma(Vy, y) */
+  o[5] = l * c[TXX] + l * c[TYY] + l2m * c[TZZ];  /* gamma(Vz, z) */
+  o[6] = m * c[TXY];                              /* gamma(Vx, y) = gamma(Vy, x) */
+  o[7] = m * c[TXZ];                              /* gamma(Vx, z) = gamma(Vz, x) */
+  o[8] = m * c[TYZ];                              /* gamma(Vy, z) = gamma(Vz, y) */
 }
 
 DEVI bool in_face(const AdjFace &F, int i, int j, int k) {
@@ -93,24 +93,24 @@ __global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restric
   const long p = i + j * g.siy + (long)k * g.siz;
   if (pm.fs.nfault > 0 && fault_column(pm.fs, i, j + (long)k * g.ny)) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) SG[q * sc + p] = 0.0f;
+    for (int q = 0; q < kSG; q++) SG[q * sc + p] = 0.0f;
     return;
   }
   float c[9];
 #pragma unroll
   for (int f = 0; f < 9; f++) c[f] = beta * E[f * sc + p] + (X ? alpha * X[f * sc + p] : 0.0f);
   const float l = lam[p], m = mu[p], s = slw[p];
-  float o[12];
+  float o[kSG];
   sg_of(c, l, m, s, o);
   if (isfree) {
     /* what timg (velocity rhs) and vlow (stress rhs) replace at the top */
     if (k >= g.nk2 - (kdir ? 3 : 1)) o[0] = o[1] = o[2] = 0.0f;
     if (k >= g.nk2 - 2)
 #pragma unroll
-      for (int q = 3; q < 12; q++) o[q] = 0.0f;
+      for (int q = 3; q < kSG; q++) o[q] = 0.0f;
   }
 #pragma unroll
-  for (int q = 0; q < 12; q++) SG[q * sc + p] = o[q];
+  for (int q = 0; q < kSG; q++) SG[q * sc + p] = o[q];
   if (!pm.on) return;
 #pragma unroll 1
   for (int fc = 0; fc < 6; fc++) {
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) k_adj_prep(Grid g, const float *__restric
     }
     sg_of(dc, l, m, s, o);
 #pragma unroll
-    for (int q = 0; q < 12; q++) F.dsg[q * n + ia] = o[q];
+    for (int q = 0; q < kSG; q++) F.dsg[q * n + ia] = o[q];
   }
 }
 
@@ -154,25 +154,26 @@ DEVI float tap_coef(int t) {
 template <int AX>
 DEVI void add_kappa(float *lm, float w, const float *__restrict__ SG, long sc, long p, float M0, float M1, float M2) {
   const float Sx = SG[p], Sy = SG[sc + p], Sz = SG[2 * sc + p];
-  const float *G = SG + 3 * sc + p;
+  const float *G = SG + 3 * sc + p;  /* xx, yy, zz, xy, xz, yz (sg_of) */
   if (AX == 0) {
     lm[VX] += w * (M0 * G[0]);
     lm[VY] += w * (M0 * G[3 * sc]);
-    lm[VZ] += w * (M0 * G[6 * sc]);
+    lm[VZ] += w * (M0 * G[4 * sc]);
     lm[TXX] += w * (M0 * Sx);
     lm[TXY] += w * (M0 * Sy);
     lm[TXZ] += w * (M0 * Sz);
   } else if (AX == 1) {
-    lm[VX] += w * (M1 * G[sc]);
-    lm[VY] += w * (M1 * G[4 * sc]);
-    lm[VZ] += w * (M1 * G[7 * sc]);
+    lm[VX] += w * (M1 * G[3 * sc]);
+    lm[VY] += w * (M1 * G[sc]);
+    lm[VZ] += w * (M1 * G[5 * sc]);
     lm[TYY] += w * (M1 * Sy);
     lm[TXY] += w * (M1 * Sx);
     lm[TYZ] += w * (M1 * Sz);
   } else {
-    lm[VX] += w * (M0 * G[0] + M1 * G[sc] + M2 * G[2 * sc]);
-    lm[VY] += w * (M0 * G[3 * sc] + M1 * G[4 * sc] + M2 * G[5 * sc]);
-    lm[VZ] += w * (M0 * G[6 * sc] + M1 * G[7 * sc] + M2 * G[8 * sc]);
+    const float Gxx = G[0], Gyy = G[sc], Gzz = G[2 * sc], Gxy = G[3 * sc], Gxz = G[4 * sc], Gyz = G[5 * sc];
+    lm[VX] += w * (M0 * Gxx + M1 * Gxy + M2 * Gxz);
+    lm[VY] += w * (M0 * Gxy + M1 * Gyy + M2 * Gyz);
+    lm[VZ] += w * (M0 * Gxz + M1 * Gyz + M2 * Gzz);
     lm[TXX] += w * (M0 * Sx);
     lm[TYY] += w * (M1 * Sy);
     lm[TZZ] += w * (M2 * Sz);
