@@ -360,6 +360,8 @@ def main() -> int:
     ap.add_argument("--no-copy-ceiling", action="store_true")
     ap.add_argument("--no-setup-leg", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0, help="MiB of a device copy to run first (profiling)")
+    ap.add_argument("--united-margin", type=int, default=None,
+                    help="welded margin of the fault plane (default: PML layers + 3; the reference hard-codes 40)")
     ap.add_argument("--cpu-threads", type=int, default=min(os.cpu_count() or 1, 16))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -379,7 +381,7 @@ def main() -> int:
         return 0
     from cgfd import configs, model
     t0 = time.perf_counter()
-    sc = configs.config2(scale=args.scale, steps=args.steps + args.warmup)
+    sc = configs.config2(scale=args.scale, steps=args.steps + args.warmup, united_margin=args.united_margin)
     log(f"[bench] generated {sc.name} {sc.ni}x{sc.nj}x{sc.nk} in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     st = model.Setup(sc)
@@ -451,6 +453,7 @@ def main() -> int:
                                f"at 50 m, curvilinear, PML 20, free surface, 1 fault, FP32",
                    "grid": [ni, nj, nk], "parallelism": "single GPU", "hbm_frac_768B_model": round(hbm_frac, 4),
                    "max_slip_m": round(slip_max, 4), "stage_fork": args.overlap, "stations": int(len(sc.stations)),
+                   "united_margin": int(sc.united_margin),
                    "volume_kernel": {"variant": tuned[0] if tuned else args.variant,
                                      "zchunk": tuned[1] if tuned else args.zchunk, "autotuned": tuned is not None}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

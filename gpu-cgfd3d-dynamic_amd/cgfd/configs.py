@@ -192,11 +192,14 @@ def small(ni=36, nj=48, nk=28, pml=5, imethod=2, free_top=True, curvilinear=Fals
     return sc
 
 
-def config2(scale: float = 1.0, steps: int = 4000, seed: int = 2002, imethod: int = 2) -> Scenario:
+def config2(scale: float = 1.0, steps: int = 4000, seed: int = 2002, imethod: int = 2,
+            united_margin: int | None = None) -> Scenario:
     """Plain method at one GPU's real shape: 1200x600x400 at 50 m (reference
     axes ni=600, nj=1200, nk=400), curvilinear with 6 Gaussian hills,
     vertically stretched (50 -> 75 m), layered medium with 5% noise, a
-    40x18 km fault with von Karman shear stress, PML 20, dt 0.004."""
+    40x18 km fault with von Karman shear stress, PML 20, dt 0.004.
+    united_margin: the welded margin of the fault plane (fault_info.cu:
+    1044-1060 hard-codes 40); by default PML layers + 3."""
     rng = _rng(seed)
     dh = 50.0
     ni, nj, nk = int(600 * scale), int(1200 * scale), int(400 * scale)
@@ -241,8 +244,8 @@ def config2(scale: float = 1.0, steps: int = 4000, seed: int = 2002, imethod: in
     return Scenario(ni=ni, nj=nj, nk=nk, x=x, y=y, z=z, lam=lam, mu=mu, rho=rho, free_top=True,
                     pml_layers=layers, imethod=imethod,
                     faults=[FaultSpec(x_index=i0, grid=(j1, j2, k1, k2), fields=f)],
-                    united_margin=pml + 3, dt=0.004, nt_total=steps + 1, stations=st,
-                    name=f"config2x{scale:g}" if scale != 1.0 else "config2")
+                    united_margin=pml + 3 if united_margin is None else united_margin, dt=0.004,
+                    nt_total=steps + 1, stations=st, name=f"config2x{scale:g}" if scale != 1.0 else "config2")
 
 
 def config3(nscen: int = 32, scale: float = 1.0, steps_scale: float = 1.0, seed: int = 3000, imethod: int = 2,
