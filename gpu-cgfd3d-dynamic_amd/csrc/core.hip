@@ -1552,7 +1552,7 @@ TuneKey tune_key(const cgfd_core *c) {
   return TuneKey{c->g.ni, c->g.nj, c->g.nk, c->nscen, c->nfault, c->isfree, pml};
 }
 
-bool valid_variant(int v) { return v == 1 || v == 20 || v == 23 || v == 24; }
+bool valid_variant(int v) { return v == 1 || v == 20 || v == 23 || v == 24 || v == 32; }
 
 /* the cache file: one "ni nj nk nscen nfault isfree pml variant zchunk" line
    per tuned shape; later lines win */
@@ -1600,7 +1600,7 @@ static void apply_auto_variant(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant != 0 && !valid_variant(variant))
-    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 20, 23, 24)", variant);
+    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 20, 23, 24, 32)", variant);
   if (variant == 0) apply_auto_variant(c);
   else c->variant = variant;
   c->drop_graph();
@@ -1624,7 +1624,7 @@ extern "C" int cgfd_core_set_vertical_columns(cgfd_core *c, int on) {
 extern "C" int cgfd_core_get_vertical_columns(const cgfd_core *c, int *grid, int *in_use) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (grid) *grid = c->vcol_grid;
-  if (in_use) *in_use = c->vcol_grid && c->vcol_on && (c->variant == 23 || c->variant == 24);
+  if (in_use) *in_use = c->vcol_grid && c->vcol_on && (c->variant == 23 || c->variant == 24 || c->variant == 32);
   return CGFD_OK;
 }
 
@@ -1651,7 +1651,8 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  const std::vector<int> variants = {23, 24, 20, 1};
+  /* 32 differs from 23 on batches only */
+  const std::vector<int> variants = c->bat.nact > 1 ? std::vector<int>{23, 32, 24, 20, 1} : std::vector<int>{23, 24, 20, 1};
   const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;

@@ -434,6 +434,28 @@ __global__ void __launch_bounds__(256, 8) k_vol_bat_acc_box(VolArgs a, BatArgs b
   acc_point<ID, JD, KD, 1, VC>(a, b, i, j, x.k0 + kz, b.act[q]);
 }
 
+/* variant 32: the interior box in one slab-ordered sweep per scenario, as
+   the bands (each scenario's stencil halos come back from L2 as on one
+   scenario; the shared metric and media are read once per scenario) */
+template <int ID, int JD, int KD, bool VC>
+__global__ void __launch_bounds__(256, 8) k_vol_bat_acc_box_slab(VolArgs a, BatArgs b, VolBox x, long per) {
+  const int S = a.zchunk;
+  const int q = (int)(blockIdx.x / per);
+  long L = blockIdx.x % per;
+  const int bx = (int)(L % x.ntx);
+  L /= x.ntx;
+  const int cy = (int)(L % S);
+  L /= S;
+  const int nkb = x.k1 - x.k0;
+  const int kz = (int)(L % nkb);
+  const int by = (int)(L / nkb) * S + cy;
+  if (by >= x.nty) return;
+  const int t = (int)threadIdx.x;
+  const int i = x.x0 + (bx << 6) + (t & 63), j = x.y0 + by * 4 + (t >> 6);
+  if (i >= x.x1 || j >= x.y1) return;
+  acc_point<ID, JD, KD, 1, VC>(a, b, i, j, x.k0 + kz, b.act[q]);
+}
+
 template <int ID, int JD, int KD, bool VC>
 __global__ void __launch_bounds__(256, 7) k_vol_bat_pml_band(VolArgs a, BatArgs b, VolBands r) {
   const long per = r.cum[r.n];
@@ -782,12 +804,19 @@ __global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const fl
 /* the box split (variants 23, 64-wide interior tiles, and 24, 32-wide): one
    scenario or a batch */
 template <int ID, int JD, int KD, bool VC>
-static void launch_box_split(int tw, const VolArgs &a, const BatArgs &b, hipStream_t st, hipStream_t st_pml) {
+static void launch_box_split(int tw, bool sweeps, const VolArgs &a, const BatArgs &b, hipStream_t st,
+                             hipStream_t st_pml) {
   VolBox x;
   VolBands r;
   box_split(a, tw, x, r);
+  if (b.nact > 1 && sweeps && x.ntx > 0) {  /* variant 32: per-scenario slab sweeps of the box (64 x 4 tiles) */
+    const long nslab = ((long)x.nty + a.zchunk - 1) / a.zchunk;
+    const long per = (long)x.ntx * a.zchunk * (x.k1 - x.k0) * nslab;
+    hipLaunchKernelGGL((k_vol_bat_acc_box_slab<ID, JD, KD, VC>), dim3((unsigned)(per * b.nact)), dim3(256), 0, st, a,
+                       b, x, per);
+  }
   if (b.nact > 1) {
-    if (x.ntx > 0) {
+    if (x.ntx > 0 && !sweeps) {
       const long items = (long)x.ntx * x.nty * (x.k1 - x.k0) * b.nact, nb = ((items + 7) / 8) * 8;
       if (tw == 64)
         hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 64, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
@@ -820,9 +849,10 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     else hipLaunchKernelGGL((k_vol_top<ID, JD, KD, false>), tgrd, blk, 0, st, a, b);
     return;
   }
-  if (variant == 23 || variant == 24) {
-    if (a.vcol) launch_box_split<ID, JD, KD, true>(variant == 23 ? 64 : 32, a, b, st, st_pml);
-    else launch_box_split<ID, JD, KD, false>(variant == 23 ? 64 : 32, a, b, st, st_pml);
+  if (variant == 23 || variant == 24 || variant == 32) {
+    const int tw = variant == 24 ? 32 : 64;
+    if (a.vcol) launch_box_split<ID, JD, KD, true>(tw, variant == 32, a, b, st, st_pml);
+    else launch_box_split<ID, JD, KD, false>(tw, variant == 32, a, b, st, st_pml);
     return;
   }
   if (b.nact > 1 && variant == 20) {  /* batches, split kernels */
