@@ -240,8 +240,24 @@ DEVI constexpr bool met_zero(int d, int c) {
   return VC && ((d == 0 && c != 0) || (d == 1 && c != 1));
 }
 
-template <bool VC, class Deriv>
+/* DM: the directions whose CFS-PML faces can contain the point (bit d for
+   direction d; the caller's region guarantees no face of another direction
+   does). The directional rhs parts are kept for those directions only;
+   the others are summed into one remaining part. */
+template <int DM>
+DEVI constexpr int pml_nslot() {
+  return ((DM & 1) + ((DM >> 1) & 1) + ((DM >> 2) & 1)) + (DM == 7 ? 0 : 1);
+}
+template <int DM>
+DEVI constexpr int pml_slot(int d) {
+  int s = 0;
+  for (int e = 0; e < d; e++) s += (DM >> e) & 1;
+  return ((DM >> d) & 1) ? s : pml_nslot<DM>() - 1;
+}
+
+template <bool VC, int DM, class Deriv>
 DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn, Deriv deriv) {
+  constexpr int NS = pml_nslot<DM>();
   const long sc = a.g.sicmp, sy = a.g.siy, sz = a.g.siz;
   const long p = i + j * sy + (long)k * sz;
   const unsigned o = (unsigned)p * 4u;
@@ -254,9 +270,9 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
     for (int c = 0; c < 3; c++) e[d][c] = met_zero<VC>(d, c) ? 0.0f : ld32nt(M + (1 + 3 * d + c) * sc, o);
   const float l = ld32nt(a.lam, o), m = ld32nt(a.mu, o), s = ld32nt(a.slw, o);
   const float l2m = l + 2.0f * m;
-  float r[3][9];
+  float r[NS][9];
 #pragma unroll
-  for (int d = 0; d < 3; d++)
+  for (int d = 0; d < NS; d++)
 #pragma unroll
     for (int c = 0; c < 9; c++) r[d][c] = 0.0f;
 #pragma unroll
@@ -269,37 +285,44 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
       const float Dd = d == 0 ? D0 : (d == 1 ? D1 : D2);
       const float e1 = e[d][0], e2 = e[d][1], e3 = e[d][2];
       const bool n1 = !met_zero<VC>(d, 0), n2 = !met_zero<VC>(d, 1), n3 = !met_zero<VC>(d, 2);
+      float *const rd = r[pml_slot<DM>(d)];
       switch (f) {
-        case TXX: if (n1) r[d][VX] += s * e1 * Dd; break;
-        case TYY: if (n2) r[d][VY] += s * e2 * Dd; break;
-        case TZZ: if (n3) r[d][VZ] += s * e3 * Dd; break;
-        case TXY: if (n2) r[d][VX] += s * e2 * Dd; if (n1) r[d][VY] += s * e1 * Dd; break;
-        case TXZ: if (n3) r[d][VX] += s * e3 * Dd; if (n1) r[d][VZ] += s * e1 * Dd; break;
-        case TYZ: if (n3) r[d][VY] += s * e3 * Dd; if (n2) r[d][VZ] += s * e2 * Dd; break;
+        case TXX: if (n1) rd[VX] += s * e1 * Dd; break;
+        case TYY: if (n2) rd[VY] += s * e2 * Dd; break;
+        case TZZ: if (n3) rd[VZ] += s * e3 * Dd; break;
+        case TXY: if (n2) rd[VX] += s * e2 * Dd; if (n1) rd[VY] += s * e1 * Dd; break;
+        case TXZ: if (n3) rd[VX] += s * e3 * Dd; if (n1) rd[VZ] += s * e1 * Dd; break;
+        case TYZ: if (n3) rd[VY] += s * e3 * Dd; if (n2) rd[VZ] += s * e2 * Dd; break;
         case VX:
-          if (n1) { r[d][TXX] += l2m * e1 * Dd; r[d][TYY] += l * e1 * Dd; r[d][TZZ] += l * e1 * Dd; }
-          if (n2) r[d][TXY] += m * e2 * Dd;
-          if (n3) r[d][TXZ] += m * e3 * Dd;
+          if (n1) { rd[TXX] += l2m * e1 * Dd; rd[TYY] += l * e1 * Dd; rd[TZZ] += l * e1 * Dd; }
+          if (n2) rd[TXY] += m * e2 * Dd;
+          if (n3) rd[TXZ] += m * e3 * Dd;
           break;
         case VY:
-          if (n2) { r[d][TXX] += l * e2 * Dd; r[d][TYY] += l2m * e2 * Dd; r[d][TZZ] += l * e2 * Dd; }
-          if (n1) r[d][TXY] += m * e1 * Dd;
-          if (n3) r[d][TYZ] += m * e3 * Dd;
+          if (n2) { rd[TXX] += l * e2 * Dd; rd[TYY] += l2m * e2 * Dd; rd[TZZ] += l * e2 * Dd; }
+          if (n1) rd[TXY] += m * e1 * Dd;
+          if (n3) rd[TYZ] += m * e3 * Dd;
           break;
         case VZ:
-          if (n3) { r[d][TXX] += l * e3 * Dd; r[d][TYY] += l * e3 * Dd; r[d][TZZ] += l2m * e3 * Dd; }
-          if (n1) r[d][TXZ] += m * e1 * Dd;
-          if (n2) r[d][TYZ] += m * e2 * Dd;
+          if (n3) { rd[TXX] += l * e3 * Dd; rd[TYY] += l * e3 * Dd; rd[TZZ] += l2m * e3 * Dd; }
+          if (n1) rd[TXZ] += m * e1 * Dd;
+          if (n2) rd[TYZ] += m * e2 * Dd;
           break;
       }
     }
   }
   float h[9];
 #pragma unroll
-  for (int c = 0; c < 9; c++) h[c] = r[0][c] + r[1][c] + r[2][c];
+  for (int c = 0; c < 9; c++) {
+    h[c] = r[0][c];
+#pragma unroll
+    for (int q = 1; q < NS; q++) h[c] += r[q][c];
+  }
   const long poff = (long)sn * b.pstride;
 #pragma unroll
   for (int dim = 0; dim < 3; dim++) {
+    if (!((DM >> dim) & 1)) continue;
+    const float *const rdim = r[pml_slot<DM>(dim)];
 #pragma unroll 1
     for (int side = 0; side < 2; side++) {
       const PmlFace &F = a.pml[dim][side];
@@ -314,8 +337,8 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
 #pragma unroll
       for (int c = 0; c < 9; c++) {
         av[c] = auxc[c * F.n + ia];
-        h[c] += cB1 * r[dim][c] - cB * av[c];
-        ar[c] = cD * r[dim][c] - cA * av[c];
+        h[c] += cB1 * rdim[c] - cB * av[c];
+        ar[c] = cD * rdim[c] - cA * av[c];
       }
       aux_update(F, a, poff, ia, av, ar);
     }
@@ -324,12 +347,12 @@ DEVI void pml_core(const VolArgs &a, const BatArgs &b, int i, int j, int k, int 
   rk_point32(a, o, h, woff);
 }
 
-template <int ID, int JD, int KD, bool VC = false>
+template <int ID, int JD, int KD, bool VC = false, int DM = 7>
 DEVI void pml_acc_point(const VolArgs &a, const BatArgs &b, int i, int j, int k, int sn) {
   const unsigned by = (unsigned)a.g.siy * 4u, bz = (unsigned)a.g.siz * 4u;
   const float *cur = a.cur + (long)sn * b.wstride;
   const long sc = a.g.sicmp;
-  pml_core<VC>(a, b, i, j, k, sn, [&](int f, unsigned o, float &D0, float &D1, float &D2) {
+  pml_core<VC, DM>(a, b, i, j, k, sn, [&](int f, unsigned o, float &D0, float &D1, float &D2) {
     const float *bc = cur + f * sc;
     D0 = dfd32<ID>(bc, o, 4u);
     D1 = dfd32<JD>(bc, o, by);
@@ -405,11 +428,11 @@ DEVI bool band_point(const VolBands &r, long L0, int &i, int &j, int &k) {
 /* 7 waves per SIMD: 72 VGPRs and 8-24 bytes of spills per lane; 1.3% faster
    than the 6 waves the compiler picks without the bound (75 VGPRs), while
    8 waves (64 VGPRs, 40-56 bytes of spills) lose it again */
-template <int ID, int JD, int KD, bool VC>
+template <int ID, int JD, int KD, bool VC, int DM>
 __global__ void __launch_bounds__(256, 7) k_vol_pml_band(VolArgs a, BatArgs b, VolBands r) {
   int i, j, k;
   if (!band_point(r, blockIdx.x, i, j, k)) return;
-  pml_acc_point<ID, JD, KD, VC>(a, b, i, j, k, b.act[0]);
+  pml_acc_point<ID, JD, KD, VC, DM>(a, b, i, j, k, b.act[0]);
 }
 
 /* batches on the box split: the interior box's (tile, scenario) pairs with
@@ -456,19 +479,27 @@ __global__ void __launch_bounds__(256, 8) k_vol_bat_acc_box_slab(VolArgs a, BatA
   acc_point<ID, JD, KD, 1, VC>(a, b, i, j, x.k0 + kz, b.act[q]);
 }
 
-template <int ID, int JD, int KD, bool VC>
+template <int ID, int JD, int KD, bool VC, int DM>
 __global__ void __launch_bounds__(256, 7) k_vol_bat_pml_band(VolArgs a, BatArgs b, VolBands r) {
   const long per = r.cum[r.n];
   const int sq = (int)(blockIdx.x / per);
   int i, j, k;
   if (!band_point(r, blockIdx.x % per, i, j, k)) return;
-  pml_acc_point<ID, JD, KD, VC>(a, b, i, j, k, b.act[sq]);
+  pml_acc_point<ID, JD, KD, VC, DM>(a, b, i, j, k, b.act[sq]);
 }
 
 constexpr int kBandSlabBlocks = 192;
 
+/* the bands in groups by the CFS-PML face directions their points can lie
+   in (the band kernel's DM): the x bands (any face), the y bands above the
+   bottom and below the top z slab (y faces only), the y bands' corners in
+   those slabs (y and z faces) and the z slabs inside the box's x and y
+   range (z faces only) */
+constexpr int kBandGroups = 4;
+constexpr int kBandDM[kBandGroups] = {7, 2, 6, 4};
+
 /* the interior box and the bands around it for one launch's planes */
-static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands &r) {
+static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands (&rg)[kBandGroups]) {
   const Grid &g = a.g;
   const int kv0 = a.k_lo + 3, kv1 = a.k_lo + 3 + a.nk_run;
   int xl = 3, xr = g.ni + 3, yl = 3, yr = g.nj + 3, kl = kv0, kr = kv1;
@@ -483,9 +514,10 @@ static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands &r) {
   }
   xl = (xl + 31) & ~31;
   xr &= ~31;
-  r = VolBands{};
-  auto add = [&](int x0, int x1, int y0, int y1, int k0, int k1) {
+  for (int grp = 0; grp < kBandGroups; grp++) rg[grp] = VolBands{};
+  auto add = [&](int grp, int x0, int x1, int y0, int y1, int k0, int k1) {
     if (x1 <= x0 || y1 <= y0 || k1 <= k0) return;
+    VolBands &r = rg[grp];
     const int q = r.n++;
     r.x0[q] = x0; r.x1[q] = x1; r.y0[q] = y0; r.y1[q] = y1; r.k0[q] = k0; r.k1[q] = k1;
     r.ox[q] = x0 & ~31;
@@ -501,16 +533,40 @@ static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands &r) {
   };
   if (xr <= xl || yr <= yl || kr <= kl) {
     x = VolBox{3, 3, 3, 3, kv0, kv0, 0, 0};
-    add(3, g.ni + 3, 3, g.nj + 3, kv0, kv1);
+    add(0, 3, g.ni + 3, 3, g.nj + 3, kv0, kv1);
     return;
   }
   x = VolBox{xl, xr, yl, yr, kl, kr, (xr - xl + tw - 1) / tw, (yr - yl + 256 / tw - 1) / (256 / tw)};
-  add(3, xl, 3, g.nj + 3, kv0, kv1);
-  add(xr, g.ni + 3, 3, g.nj + 3, kv0, kv1);
-  add(xl, xr, 3, yl, kv0, kv1);
-  add(xl, xr, yr, g.nj + 3, kv0, kv1);
-  add(xl, xr, yl, yr, kv0, kl);
-  add(xl, xr, yl, yr, kr, kv1);
+  add(0, 3, xl, 3, g.nj + 3, kv0, kv1);
+  add(0, xr, g.ni + 3, 3, g.nj + 3, kv0, kv1);
+  add(1, xl, xr, 3, yl, kl, kr);
+  add(1, xl, xr, yr, g.nj + 3, kl, kr);
+  add(2, xl, xr, 3, yl, kv0, kl);
+  add(2, xl, xr, yr, g.nj + 3, kv0, kl);
+  add(2, xl, xr, 3, yl, kr, kv1);
+  add(2, xl, xr, yr, g.nj + 3, kr, kv1);
+  add(3, xl, xr, yl, yr, kv0, kl);
+  add(3, xl, xr, yl, yr, kr, kv1);
+}
+
+template <int ID, int JD, int KD, bool VC, int G>
+static void launch_band_group(const VolBands &r, bool batch, const VolArgs &a, const BatArgs &b, hipStream_t st) {
+  if (r.n == 0) return;
+  constexpr int DM = kBandDM[G];
+  if (batch)
+    hipLaunchKernelGGL((k_vol_bat_pml_band<ID, JD, KD, VC, DM>), dim3((unsigned)(r.cum[r.n] * b.nact)), dim3(256), 0,
+                       st, a, b, r);
+  else
+    hipLaunchKernelGGL((k_vol_pml_band<ID, JD, KD, VC, DM>), dim3((unsigned)r.cum[r.n]), dim3(256), 0, st, a, b, r);
+}
+
+template <int ID, int JD, int KD, bool VC>
+static void launch_bands(const VolBands (&rg)[kBandGroups], bool batch, const VolArgs &a, const BatArgs &b,
+                         hipStream_t st) {
+  launch_band_group<ID, JD, KD, VC, 0>(rg[0], batch, a, b, st);
+  launch_band_group<ID, JD, KD, VC, 1>(rg[1], batch, a, b, st);
+  launch_band_group<ID, JD, KD, VC, 2>(rg[2], batch, a, b, st);
+  launch_band_group<ID, JD, KD, VC, 3>(rg[3], batch, a, b, st);
 }
 
 /* ------------------------------------------------------------------------
@@ -807,7 +863,7 @@ template <int ID, int JD, int KD, bool VC>
 static void launch_box_split(int tw, bool sweeps, const VolArgs &a, const BatArgs &b, hipStream_t st,
                              hipStream_t st_pml) {
   VolBox x;
-  VolBands r;
+  VolBands r[kBandGroups];
   box_split(a, tw, x, r);
   if (b.nact > 1 && sweeps && x.ntx > 0) {  /* variant 32: per-scenario slab sweeps of the box (64 x 4 tiles) */
     const long nslab = ((long)x.nty + a.zchunk - 1) / a.zchunk;
@@ -823,9 +879,7 @@ static void launch_box_split(int tw, bool sweeps, const VolArgs &a, const BatArg
       else
         hipLaunchKernelGGL((k_vol_bat_acc_box<ID, JD, KD, 32, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x, items);
     }
-    if (r.n > 0)
-      hipLaunchKernelGGL((k_vol_bat_pml_band<ID, JD, KD, VC>), dim3((unsigned)(r.cum[r.n] * b.nact)), dim3(256), 0,
-                         st_pml, a, b, r);
+    launch_bands<ID, JD, KD, VC>(r, true, a, b, st_pml);
     return;
   }
   if (x.ntx > 0) {
@@ -834,8 +888,7 @@ static void launch_box_split(int tw, bool sweeps, const VolArgs &a, const BatArg
     if (tw == 64) hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 64, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
     else hipLaunchKernelGGL((k_vol_acc_box<ID, JD, KD, 32, VC>), dim3((unsigned)nb), dim3(256), 0, st, a, b, x);
   }
-  if (r.n > 0)
-    hipLaunchKernelGGL((k_vol_pml_band<ID, JD, KD, VC>), dim3((unsigned)r.cum[r.n]), dim3(256), 0, st_pml, a, b, r);
+  launch_bands<ID, JD, KD, VC>(r, false, a, b, st_pml);
 }
 
 template <int ID, int JD, int KD>
