@@ -115,15 +115,16 @@ struct VolBox {
   int x0, x1, y0, y1, k0, k1;
   int ntx, nty;  /* tiles of the box; ntx = 0: empty */
 };
+constexpr int kBandRegions = 6; /* regions per band launch, at most */
 struct VolBands {
   int n;  /* regions */
-  int x0[6], x1[6], y0[6], y1[6], k0[6], k1[6];
-  int ox[6];          /* x origin of a region's 32-wide tiles (32-aligned) */
-  int ntx[6], nty[6]; /* 32 x 8 tiles per plane */
-  int sy[6];          /* y tiles per slab: a region's blocks run x tile
-                         fastest, then the sy y tiles of a slab, then the
-                         plane, then the slab */
-  long cum[7];        /* block prefix sums over the regions (padded slabs) */
+  int x0[kBandRegions], x1[kBandRegions], y0[kBandRegions], y1[kBandRegions], k0[kBandRegions], k1[kBandRegions];
+  int ox[kBandRegions];                 /* x origin of a region's 32-wide tiles (32-aligned) */
+  int ntx[kBandRegions], nty[kBandRegions]; /* 32 x 8 tiles per plane */
+  int sy[kBandRegions];                 /* y tiles per slab: a region's blocks run x tile
+                                           fastest, then the sy y tiles of a slab, then the
+                                           plane, then the slab */
+  long cum[kBandRegions + 1];           /* block prefix sums over the regions (padded slabs) */
 };
 
 /* ---------------------------------------------------------------------- */

@@ -18,6 +18,8 @@
  * reference (sv_curv_col_el_iso_fault_gpu.cu:11-189).
  */
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include "cgfd_dev.h"
 #include "cgfd_kernels.h"
@@ -494,12 +496,14 @@ constexpr int kBandSlabBlocks = 192;
    in (the band kernel's DM): the x bands (any face), the y bands above the
    bottom and below the top z slab (y faces only), the y bands' corners in
    those slabs (y and z faces) and the z slabs inside the box's x and y
-   range (z faces only) */
+   range (z faces only); or, ungrouped, six slabs in one launch: the x
+   bands, the y bands over every plane and the z slabs inside the box's x
+   and y range */
 constexpr int kBandGroups = 4;
 constexpr int kBandDM[kBandGroups] = {7, 2, 6, 4};
 
 /* the interior box and the bands around it for one launch's planes */
-static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands (&rg)[kBandGroups]) {
+static void box_split(const VolArgs &a, int tw, bool grouped, VolBox &x, VolBands (&rg)[kBandGroups]) {
   const Grid &g = a.g;
   const int kv0 = a.k_lo + 3, kv1 = a.k_lo + 3 + a.nk_run;
   int xl = 3, xr = g.ni + 3, yl = 3, yr = g.nj + 3, kl = kv0, kr = kv1;
@@ -518,6 +522,10 @@ static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands (&rg)[kBandG
   auto add = [&](int grp, int x0, int x1, int y0, int y1, int k0, int k1) {
     if (x1 <= x0 || y1 <= y0 || k1 <= k0) return;
     VolBands &r = rg[grp];
+    if (r.n == kBandRegions) {  /* the splits below add at most six regions to a group */
+      std::fprintf(stderr, "box_split: more than %d band regions in group %d\n", kBandRegions, grp);
+      std::abort();
+    }
     const int q = r.n++;
     r.x0[q] = x0; r.x1[q] = x1; r.y0[q] = y0; r.y1[q] = y1; r.k0[q] = k0; r.k1[q] = k1;
     r.ox[q] = x0 & ~31;
@@ -539,6 +547,13 @@ static void box_split(const VolArgs &a, int tw, VolBox &x, VolBands (&rg)[kBandG
   x = VolBox{xl, xr, yl, yr, kl, kr, (xr - xl + tw - 1) / tw, (yr - yl + 256 / tw - 1) / (256 / tw)};
   add(0, 3, xl, 3, g.nj + 3, kv0, kv1);
   add(0, xr, g.ni + 3, 3, g.nj + 3, kv0, kv1);
+  if (!grouped) {
+    add(0, xl, xr, 3, yl, kv0, kv1);
+    add(0, xl, xr, yr, g.nj + 3, kv0, kv1);
+    add(0, xl, xr, yl, yr, kv0, kl);
+    add(0, xl, xr, yl, yr, kr, kv1);
+    return;
+  }
   add(1, xl, xr, 3, yl, kl, kr);
   add(1, xl, xr, yr, g.nj + 3, kl, kr);
   add(2, xl, xr, 3, yl, kv0, kl);
@@ -857,14 +872,14 @@ __global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const fl
 
 /* ---------------------------------------------------------------------- */
 /* host launchers */
-/* the box split (variants 23, 64-wide interior tiles, and 24, 32-wide): one
-   scenario or a batch */
+/* the box split (variants 23, 64-wide interior tiles, 24, 32-wide, and 25,
+   23 with every band in one launch): one scenario or a batch */
 template <int ID, int JD, int KD, bool VC>
-static void launch_box_split(int tw, bool sweeps, const VolArgs &a, const BatArgs &b, hipStream_t st,
+static void launch_box_split(int tw, bool sweeps, bool grouped, const VolArgs &a, const BatArgs &b, hipStream_t st,
                              hipStream_t st_pml) {
   VolBox x;
   VolBands r[kBandGroups];
-  box_split(a, tw, x, r);
+  box_split(a, tw, grouped, x, r);
   if (b.nact > 1 && sweeps && x.ntx > 0) {  /* variant 32: per-scenario slab sweeps of the box (64 x 4 tiles) */
     const long nslab = ((long)x.nty + a.zchunk - 1) / a.zchunk;
     const long per = (long)x.ntx * a.zchunk * (x.k1 - x.k0) * nslab;
@@ -902,10 +917,11 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     else hipLaunchKernelGGL((k_vol_top<ID, JD, KD, false>), tgrd, blk, 0, st, a, b);
     return;
   }
-  if (variant == 23 || variant == 24 || variant == 32) {
+  if (variant == 23 || variant == 24 || variant == 25 || variant == 32) {
     const int tw = variant == 24 ? 32 : 64;
-    if (a.vcol) launch_box_split<ID, JD, KD, true>(tw, variant == 32, a, b, st, st_pml);
-    else launch_box_split<ID, JD, KD, false>(tw, variant == 32, a, b, st, st_pml);
+    const bool grouped = variant != 25;
+    if (a.vcol) launch_box_split<ID, JD, KD, true>(tw, variant == 32, grouped, a, b, st, st_pml);
+    else launch_box_split<ID, JD, KD, false>(tw, variant == 32, grouped, a, b, st, st_pml);
     return;
   }
   if (b.nact > 1 && variant == 20) {  /* batches, split kernels */

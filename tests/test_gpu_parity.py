@@ -143,7 +143,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 20, 23, 24])
+@pytest.mark.parametrize("variant", [1, 20, 23, 24, 25])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation the autotuner chooses from."""
     from cgfd import configs, model
@@ -347,7 +347,7 @@ def test_slices_and_snapshots_match_the_wavefield(built_libs):
     st.close()
 
 
-@pytest.mark.parametrize("variant", [23, 24])
+@pytest.mark.parametrize("variant", [23, 24, 25])
 def test_box_split_ragged_grid_matches_oracle(built_libs, variant):
     """The box-split schedule where its interior box is not empty and no
     size is a multiple of a tile: 133 x 61 x 35 nodes with PML 5 and a free
