@@ -653,7 +653,7 @@ void fill_fault(cgfd_core *c, FaultArgs &f, int istage, int idir, int jdir, int 
 
 /* volume kernel variants (see cgfd_core_set_variant): 1 direct loads; 20
    an interior kernel in slab order and a lean kernel for the tiles touching
-   a CFS-PML slab; 23, 24, 25 and 32 the box split */
+   a CFS-PML slab; 23, 24, 32 and 33 the box split */
 void launch_vol(cgfd_core *c, const VolArgs &va, int idir, int jdir, int kdir) {
   launch_volume(c->variant, va, c->bat, idir, jdir, kdir, 0, c->st, c->st_pml());
 }
@@ -1552,7 +1552,7 @@ TuneKey tune_key(const cgfd_core *c) {
   return TuneKey{c->g.ni, c->g.nj, c->g.nk, c->nscen, c->nfault, c->isfree, pml};
 }
 
-bool valid_variant(int v) { return v == 1 || v == 20 || v == 23 || v == 24 || v == 25 || v == 32; }
+bool valid_variant(int v) { return v == 1 || v == 20 || v == 23 || v == 24 || v == 32 || v == 33; }
 
 /* the cache file: one "ni nj nk nscen nfault isfree pml variant zchunk" line
    per tuned shape; later lines win */
@@ -1624,7 +1624,8 @@ extern "C" int cgfd_core_set_vertical_columns(cgfd_core *c, int on) {
 extern "C" int cgfd_core_get_vertical_columns(const cgfd_core *c, int *grid, int *in_use) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (grid) *grid = c->vcol_grid;
-  if (in_use) *in_use = c->vcol_grid && c->vcol_on && (c->variant == 23 || c->variant == 24 || c->variant == 32);
+  if (in_use)
+    *in_use = c->vcol_grid && c->vcol_on && (c->variant == 23 || c->variant == 24 || c->variant == 32 || c->variant == 33);
   return CGFD_OK;
 }
 
@@ -1652,7 +1653,7 @@ extern "C" int cgfd_core_autotune(cgfd_core *c, int reps, int *variant_out, int 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   /* 32 differs from 23 on batches only */
-  const std::vector<int> variants = c->bat.nact > 1 ? std::vector<int>{23, 32, 25, 24, 20, 1} : std::vector<int>{23, 25, 24, 20, 1};
+  const std::vector<int> variants = c->bat.nact > 1 ? std::vector<int>{23, 32, 33, 24, 20, 1} : std::vector<int>{23, 33, 24, 20, 1};
   const int chunks[4] = {4, 8, 16, 32};
   float best = 1e30f;
   int bv = c->variant, bz = c->zchunk;

@@ -872,7 +872,7 @@ __global__ void k_pg(Grid g, BatArgs b, const float *__restrict__ wend, const fl
 
 /* ---------------------------------------------------------------------- */
 /* host launchers */
-/* the box split (variants 23, 64-wide interior tiles, 24, 32-wide, and 25,
+/* the box split (variants 23, 64-wide interior tiles, 24, 32-wide, and 33,
    23 with every band in one launch): one scenario or a batch */
 template <int ID, int JD, int KD, bool VC>
 static void launch_box_split(int tw, bool sweeps, bool grouped, const VolArgs &a, const BatArgs &b, hipStream_t st,
@@ -917,9 +917,9 @@ static void launch_dir(int variant, const VolArgs &a, const BatArgs &b, int top,
     else hipLaunchKernelGGL((k_vol_top<ID, JD, KD, false>), tgrd, blk, 0, st, a, b);
     return;
   }
-  if (variant == 23 || variant == 24 || variant == 25 || variant == 32) {
+  if (variant == 23 || variant == 24 || variant == 32 || variant == 33) {
     const int tw = variant == 24 ? 32 : 64;
-    const bool grouped = variant != 25;
+    const bool grouped = variant != 33;
     if (a.vcol) launch_box_split<ID, JD, KD, true>(tw, variant == 32, grouped, a, b, st, st_pml);
     else launch_box_split<ID, JD, KD, false>(tw, variant == 32, grouped, a, b, st, st_pml);
     return;

@@ -336,11 +336,11 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
    bounds on 32-column boundaries; 64 x 4 tiles for 23, 32 x 8 for 24; slab
    order) and the lean PML kernel over the up to six slabs around them in
    32 x 8 tiles, launched in up to four groups by the CFS-PML face
-   directions their points can lie in; 25 = 23 with every band in one
-   launch that handles all three directions; 32 = 23 on one scenario. Batches with more than one active
+   directions their points can lie in; 32 = 23 on one scenario; 33 = 23
+   with every band in one launch that handles all three directions. Batches with more than one active
    scenario run, for 20, the interior tiles with the scenarios back to back
    (XCD-contiguous) and the PML tiles in one slab-ordered sweep per
-   scenario; for 23, 24 and 25 the box split the same way (interior tiles with
+   scenario; for 23, 24 and 33 the box split the same way (interior tiles with
    the scenarios back to back, bands in one sweep per scenario); for 32 the
    box split with the interior box, too, in one slab-ordered sweep per
    scenario; for 1 one direct tile x scenario kernel. Returns CGFD_ERR_ARG
@@ -351,7 +351,7 @@ int cgfd_core_set_variant(cgfd_core *core, int variant);
    been resolved). */
 int cgfd_core_get_variant(const cgfd_core *core, int *variant, int *zchunk);
 
-/* y-tiles per slab of the slab-ordered kernels (20, 23, 24, 25, 32), 4..4096. */
+/* y-tiles per slab of the slab-ordered kernels (20, 23, 24, 32, 33), 4..4096. */
 int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 
 /* Vertical-column metric form, no counterpart in the reference (whose
@@ -362,7 +362,7 @@ int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
    rounding of its finite differences, about 1e-15 of xi_x and eta_y. At
    create the core checks on the device whether they stay below 2^-32 of
    xi_x and eta_y at every physical point. If so, the box-split schedules
-   (23, 24, 25, 32) and the adjoint's transposed gather read only the other five
+   (23, 24, 32, 33) and the adjoint's transposed gather read only the other five
    terms and omit the four products, which agrees with the nine-term form
    to the last bit at nearly every point and within an ulp elsewhere, with
    four fewer arrays per point; the free-surface and fault kernels take the
@@ -373,7 +373,7 @@ int cgfd_core_set_zchunk(cgfd_core *core, int zchunk);
 int cgfd_core_set_vertical_columns(cgfd_core *core, int on);
 int cgfd_core_get_vertical_columns(const cgfd_core *core, int *grid, int *in_use);
 
-/* Autotuner: times every candidate (variant, z-chunk) pair -- 23, 25, 24,
+/* Autotuner: times every candidate (variant, z-chunk) pair -- 23, 33, 24,
    20 and 1, and 32 on a batch, z-chunks 4..32 for the slab-ordered ones -- on
    `reps` repetitions
    of one step's four volume launches, keeps

@@ -29,11 +29,11 @@ def _plain_with_fields(sc, fields):
     return model.Setup(sc2)
 
 
-CASES = [(v, "default") for v in (1, 20, 23, 24, 25, 32)] + [(20, "first_retires"), (1, "first_retires"),
+CASES = [(v, "default") for v in (1, 20, 23, 24, 32, 33)] + [(20, "first_retires"), (1, "first_retires"),
                                                           (24, "first_retires"), (32, "first_retires"),
                                                           (23, "ragged_slab"), (20, "ragged_slab"),
                                                           (24, "ragged_slab"), (32, "ragged_slab"),
-                                                          (23, "wide"), (24, "wide"), (25, "wide"), (32, "wide")]
+                                                          (23, "wide"), (24, "wide"), (33, "wide"), (32, "wide")]
 
 
 @pytest.mark.parametrize("variant,case", CASES)

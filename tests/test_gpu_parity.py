@@ -143,7 +143,7 @@ def test_fp64_oracle_error_statement(built_libs):
     assert e_gpu <= TOL
 
 
-@pytest.mark.parametrize("variant", [1, 20, 23, 24, 25])
+@pytest.mark.parametrize("variant", [1, 20, 23, 24, 33])
 def test_volume_variants_match_oracle(built_libs, variant):
     """Every volume-stencil implementation the autotuner chooses from."""
     from cgfd import configs, model
@@ -347,7 +347,7 @@ def test_slices_and_snapshots_match_the_wavefield(built_libs):
     st.close()
 
 
-@pytest.mark.parametrize("variant", [23, 24, 25])
+@pytest.mark.parametrize("variant", [23, 24, 33])
 def test_box_split_ragged_grid_matches_oracle(built_libs, variant):
     """The box-split schedule where its interior box is not empty and no
     size is a multiple of a tile: 133 x 61 x 35 nodes with PML 5 and a free
@@ -372,7 +372,7 @@ def test_box_split_ragged_grid_matches_oracle(built_libs, variant):
     st.close()
 
 
-@pytest.mark.parametrize("variant", [23, 24])
+@pytest.mark.parametrize("variant", [23, 24, 33])
 def test_vertical_column_form_matches_the_nine_term_form(built_libs, variant):
     """On a grid of vertical columns under topography (xi_y, xi_z, eta_x and
     eta_z hold only the metric's rounding, about 1e-15 of xi_x and eta_y)
