@@ -66,7 +66,7 @@ struct cgfd_core {
   int nt_total = 0, isfree = 0, imethod = 0, nfault = 0, fix_q1 = 0, fix_sta = 0, nan_every = 0;
   int vcol_grid = 0;  /* xi_y, xi_z, eta_x, eta_z below 2^-32 of xi_x, eta_y everywhere */
   int vcol_on = 1;    /* use the vertical-column form where the grid has it */
-  int variant = 23;
+  int variant = 33;
   int zchunk = 4;
   int nvol = 0;
   DevBuf mem;
@@ -1582,8 +1582,9 @@ void record_tuned(const TuneKey &k, int v, int z) {
 }
 }  // namespace
 
-/* variant 0: the tuned choice for this shape, else the box split 23 with
-   8 y-tiles per slab (the fastest untuned choice on every shape measured) */
+/* variant 0: the tuned choice for this shape, else the box split with 8
+   y-tiles per slab: 33 (the bands in one launch) on one scenario, 32 (the
+   interior box, too, in per-scenario sweeps) on a batch */
 static void apply_auto_variant(cgfd_core *c) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   read_tune_file();
@@ -1592,7 +1593,7 @@ static void apply_auto_variant(cgfd_core *c) {
     c->variant = it->second.first;
     c->zchunk = it->second.second;
   } else {
-    c->variant = 23;
+    c->variant = c->nscen > 1 ? 32 : 33;
     c->zchunk = 8;
   }
 }
@@ -1600,7 +1601,7 @@ static void apply_auto_variant(cgfd_core *c) {
 extern "C" int cgfd_core_set_variant(cgfd_core *c, int variant) {
   if (!c) return cgfd_set_error(CGFD_ERR_ARG, "null core");
   if (variant != 0 && !valid_variant(variant))
-    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 20, 23, 24, 32)", variant);
+    return cgfd_set_error(CGFD_ERR_ARG, "unknown volume kernel variant %d (valid: 0, 1, 20, 23, 24, 32, 33)", variant);
   if (variant == 0) apply_auto_variant(c);
   else c->variant = variant;
   c->drop_graph();

@@ -326,7 +326,8 @@ int cgfd_core_run_timed(cgfd_core *core, int nsteps, float *ms_total,
 
 /* Kernel variant selection for the volume stencil. 0 = automatic: the
    variant and z-chunk cgfd_core_autotune chose for a problem of this shape
-   in this process (or in the cache file named by CGFD_TUNE_CACHE), else 23.
+   in this process (or in the cache file named by CGFD_TUNE_CACHE), else 33
+   (32 for a batch of more than one scenario).
    1 = one direct-load kernel (27 derivatives per point); 20 = an interior
    kernel that accumulates each field's gradient into the rhs (64 x 4 tiles
    in slab order: zchunk y-tiles per slab, then the plane) and a lean kernel
