@@ -312,11 +312,13 @@ __global__ void __launch_bounds__(128) k_trial(FaultArgs a, BatArgs b) {
 }
 
 /* Hooke's law on one column point (sv_curv_col_el_iso_fault_gpu.cu:1126-1154) */
-DEVI void hooke(const float *met, long sc, long p, float l, float m, const float *Dx, const float *Dy, const float *Dz,
-                float *h) {
+DEVI void hooke(const float *met, long sc, long p, bool vc, float l, float m, const float *Dx, const float *Dy,
+                const float *Dz, float *h) {
   float l2m = (float)(l + 2.0 * m);
-  float xx = met[sc + p], xy = met[2 * sc + p], xz = met[3 * sc + p];
-  float ex = met[4 * sc + p], ey = met[5 * sc + p], ez = met[6 * sc + p];
+  /* vc: vertical columns, xi_y, xi_z, eta_x and eta_z vanish (the volume
+     kernels' five-term form) and are not read */
+  float xx = met[sc + p], xy = vc ? 0.0f : met[2 * sc + p], xz = vc ? 0.0f : met[3 * sc + p];
+  float ex = vc ? 0.0f : met[4 * sc + p], ey = met[5 * sc + p], ez = vc ? 0.0f : met[6 * sc + p];
   float zx = met[7 * sc + p], zy = met[8 * sc + p], zz = met[9 * sc + p];
   h[0] = l2m * (xx * Dx[0] + ex * Dy[0] + zx * Dz[0]) + l * (xy * Dx[1] + ey * Dy[1] + zy * Dz[1] + xz * Dx[2] + ez * Dy[2] + zz * Dz[2]);
   h[1] = l2m * (xy * Dx[1] + ey * Dy[1] + zy * Dz[1]) + l * (xx * Dx[0] + ex * Dy[0] + zx * Dz[0] + xz * Dx[2] + ez * Dy[2] + zz * Dz[2]);
@@ -361,12 +363,11 @@ constexpr int kTvZ = 6 * kColNodes * (kColRows + 6);             /* 840 */
 constexpr int kTv = kTvX + kTvY + kTvZ;
 
 template <int IDIR>
-DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
+DEVI void k_fault_cols_body(const FaultArgs &a, int id, int j0, int k0) {
   __shared__ float tv[3][kTv];
   const FaultDev F = fdev(a, id);
   const long ss = a.g.sslice, sc = a.g.sicmp, ny = a.g.ny, sy = a.g.siy, sz = a.g.siz;
   const int i0 = F.i0;
-  const int j0 = blockIdx.x * kColNodes, k0 = blockIdx.y * kColRows;  /* first node of the block */
   const float *met = a.met;
   const float *W = a.w_cur;
   const float *Txx = W + TXX * sc, *Tyy = W + TYY * sc, *Tzz = W + TZZ * sc;
@@ -559,14 +560,15 @@ DEVI void k_fault_cols_body(const FaultArgs &a, int id) {
 #pragma unroll
     for (int c = 0; c < 3; c++) dz[c] = mac24(V[c] + p, sz, kdir);
   }
-  hooke(met, sc, p, a.lam[p], a.mu[p], dx, dy, dz, h + 3);
+  hooke(met, sc, p, a.vcol != 0, a.lam[p], a.mu[p], dx, dy, dz, h + 3);
   rk_cols(a, p, 0, 9, h);
 }
 
 template <int IDIR>
 __global__ void __launch_bounds__(64 * kColRows) k_fault_cols(FaultArgs a, BatArgs b) {
   const int id = (int)blockIdx.z % a.nfault;
-  k_fault_cols_body<IDIR>(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id);
+  k_fault_cols_body<IDIR>(fault_at(a, b, b.act[blockIdx.z / a.nfault]), id, (int)blockIdx.x * kColNodes,
+                          (int)blockIdx.y * kColRows);
 }
 
 /* ---------------------------------------------------------------------- */
