@@ -28,8 +28,9 @@ def built_libs():
 
 
 def pytest_runtest_logstart(nodeid, location):
-    """On the GPU box, note each test's start in gpurun_out/test_progress.log
-    (a run shows it is alive even while pytest captures the output)."""
+    """Where GRAFT_REPO_ROOT names the repository root of a remote run,
+    append each test's start to that run's progress log (a run shows it is
+    alive even while pytest captures the output)."""
     root = os.environ.get("GRAFT_REPO_ROOT")
     if not root:
         return

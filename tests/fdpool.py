@@ -21,7 +21,8 @@ def _init(threads: int, paths):
 
 
 def cpu_share() -> int:
-    """Cores this process may use (the GPU box gives one GPU's job 16)."""
+    """Cores this process may use: its affinity set, OMP_NUM_THREADS and at
+    most 16 (the host share of one GPU's job on an MI355X node)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:

@@ -20,9 +20,9 @@ def fault_phys(f, g=3):
 
 
 def progress(msg):
-    """A progress line: printed, and on the GPU box also appended to
-    gpurun_out/test_progress.log, so a long test shows it is alive even
-    when pytest captures its output."""
+    """A progress line: printed, and where GRAFT_REPO_ROOT is set also
+    appended to that run's progress log, so a long test shows it is alive
+    even when pytest captures its output."""
     import os
     print(msg, flush=True)
     root = os.environ.get("GRAFT_REPO_ROOT")
