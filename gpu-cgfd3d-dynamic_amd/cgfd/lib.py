@@ -167,6 +167,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
         "cgfd_batch_write_fault_state": (I32, [VP, I32, I32, VP, VP]),
         "cgfd_batch_read_pg": (I32, [VP, I32, VP]),
         "cgfd_batch_read_stations": (I32, [VP, I32, VP, C.POINTER(I32)]),
+        "cgfd_batch_read_fault_receivers": (I32, [VP, I32, VP, C.POINTER(I32)]),
         "cgfd_setup_create": (I32, [C.POINTER(SetupParams), C.POINTER(VP)]),
         "cgfd_setup_destroy": (I32, [VP]),
         "cgfd_setup_get_desc": (I32, [VP, C.POINTER(Desc)]),

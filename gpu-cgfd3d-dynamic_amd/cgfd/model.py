@@ -538,6 +538,21 @@ class Batch:
         flat = buf.reshape(-1)[: self._nst * nt.value * 9]
         return flat.reshape(self._nst, nt.value, 9).copy()
 
+    def set_fault_receivers(self, fjk_ext: np.ndarray, nt_max: int):
+        """Receivers at (fault id, ghost-inclusive j, k) nodes, recorded for
+        every active scenario (cgfd_core_set_fault_receivers)."""
+        a = np.ascontiguousarray(fjk_ext, np.int32).reshape(-1, 3)
+        L.check(self.lib, self.lib.cgfd_core_set_fault_receivers(self.handle, a.shape[0], L.ptr(a), nt_max))
+        self._nfr, self._nfr_nt = a.shape[0], nt_max
+
+    def read_fault_receivers(self, s: int) -> np.ndarray:
+        """Scenario s's receiver records [nrec][steps it recorded][9]."""
+        buf = np.zeros((self._nfr, self._nfr_nt, 9), np.float32)
+        nt = C.c_int()
+        L.check(self.lib, self.lib.cgfd_batch_read_fault_receivers(self.handle, s, L.ptr(buf), C.byref(nt)))
+        flat = buf.reshape(-1)[: self._nfr * nt.value * 9]
+        return flat.reshape(self._nfr, nt.value, 9).copy()
+
     def adjoint_forward(self, nt: int):
         """Arms every scenario's friction tape for the next nt steps."""
         L.check(self.lib, self.lib.cgfd_adj_forward(self.handle, int(nt)))

@@ -172,8 +172,8 @@ void launch_adj_inject(const long *term, const float *fac, int nterm, int nsta, 
                        long sicmp, float *P, hipStream_t st);
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *term, const float *fac,
                             int nterm, int n, float *buf, int nt_max, const int *it_dev, int it0, hipStream_t st);
-void launch_record_fault(const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n, float *buf,
-                         int nt_max, const int *it_dev, int it0, long ss, hipStream_t st);
+void launch_record_fault(const BatArgs &b, const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n,
+                         float *buf, int nt_max, const int *it_dev, int it0, long ss, hipStream_t st);
 /* output boxes (k_io.hip): start, count, stride per axis, ghost-inclusive */
 struct IoBox {
   int s[3], n[3], d[3];

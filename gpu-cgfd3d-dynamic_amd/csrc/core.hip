@@ -669,7 +669,7 @@ int do_step(cgfd_core *c, hipEvent_t *ev, int *nev) {
     launch_record_stations(c->bat, c->wlev(c->lv_pre), c->g.sicmp, c->st_term, c->st_fac, c->st_nterm, c->st_n,
                            c->st_buf, c->st_nt, c->it_dev, c->st_it0, c->st);
   if (c->fr_n > 0)
-    launch_record_fault(c->d_ftab, c->fr_idx, c->fr_fac, c->fr_nterm, c->fr_n, c->fr_buf, c->fr_nt, c->it_dev,
+    launch_record_fault(c->bat, c->d_ftab, c->fr_idx, c->fr_fac, c->fr_nterm, c->fr_n, c->fr_buf, c->fr_nt, c->it_dev,
                         c->fr_it0, c->g.sslice, c->st);
   for (int s = 0; s < 4; s++) {
     const int idir = (kPairs[ipair][0] + s) % 2;
@@ -1110,7 +1110,6 @@ extern "C" int cgfd_batch_adj_backward(cgfd_core *c, int s, const float *residua
    Ly1 = 1 - dj; djk = NULL: on the nodes themselves */
 static int set_fault_receivers(cgfd_core *c, int n, const int *fjk, const float *djk, int nt_max) {
   if (!c || n < 0 || (n > 0 && (!fjk || nt_max <= 0))) return cgfd_set_error(CGFD_ERR_ARG, "bad receiver arguments");
-  if (c->nscen > 1) return cgfd_set_error(CGFD_ERR_ARG, "fault receivers: single-scenario cores only");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
   const int nterm = djk ? 4 : 1;
@@ -1139,8 +1138,10 @@ static int set_fault_receivers(cgfd_core *c, int n, const int *fjk, const float 
     HIPCHK(hipMemcpy(c->fr_idx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice));
     if ((rc = c->grow(&c->fr_fac, &c->fr_fac_cap, fac.size()))) return rc;
     HIPCHK(hipMemcpy(c->fr_fac, fac.data(), fac.size() * 4, hipMemcpyHostToDevice));
-    if ((rc = c->grow(&c->fr_buf, &c->fr_buf_cap, (size_t)n * nt_max * 9))) return rc;
-    HIPCHK(hipMemset(c->fr_buf, 0, (size_t)n * nt_max * 9 * 4));
+    /* one [n][nt_max][9] block per scenario */
+    const size_t nbuf = (size_t)c->nscen * n * nt_max * 9;
+    if ((rc = c->grow(&c->fr_buf, &c->fr_buf_cap, nbuf))) return rc;
+    HIPCHK(hipMemset(c->fr_buf, 0, nbuf * 4));
   }
   c->fr_n = n;
   c->fr_nt = nt_max;
@@ -1160,20 +1161,27 @@ extern "C" int cgfd_core_set_fault_receivers_interp(cgfd_core *c, int n, const i
   return set_fault_receivers(c, n, fjk, djk, nt_max);
 }
 
-extern "C" int cgfd_core_read_fault_receivers(cgfd_core *c, float *dst, int *nt) {
-  if (!c || !dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
+static int check_scen(const cgfd_core *c, int s);
+
+extern "C" int cgfd_batch_read_fault_receivers(cgfd_core *c, int s, float *dst, int *nt) {
+  if (int rc = check_scen(c, s)) return rc;
+  if (!dst) return cgfd_set_error(CGFD_ERR_ARG, "null argument");
   HIPCHK(hipSetDevice(c->dev));
   HIPCHK(hipStreamSynchronize(c->st));
-  int rec = c->it_host - c->fr_it0;
+  int rec = scen_steps(c, s) - c->fr_it0;
   if (rec > c->fr_nt) rec = c->fr_nt;
   if (rec < 0) rec = 0;
   if (nt) *nt = rec;
   if (c->fr_n == 0) return CGFD_OK;
   std::vector<float> h((size_t)c->fr_n * c->fr_nt * 9);
-  HIPCHK(hipMemcpy(h.data(), c->fr_buf, h.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(h.data(), c->fr_buf + (size_t)s * c->fr_n * c->fr_nt * 9, h.size() * 4, hipMemcpyDeviceToHost));
   for (int q = 0; q < c->fr_n; q++)
     std::memcpy(dst + (size_t)q * rec * 9, &h[(size_t)q * c->fr_nt * 9], (size_t)rec * 9 * 4);
   return CGFD_OK;
+}
+
+extern "C" int cgfd_core_read_fault_receivers(cgfd_core *c, float *dst, int *nt) {
+  return cgfd_batch_read_fault_receivers(c, 0, dst, nt);
 }
 
 extern "C" int cgfd_core_set_overlap(cgfd_core *c, int on) {

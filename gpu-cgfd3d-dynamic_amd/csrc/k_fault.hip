@@ -912,9 +912,10 @@ void launch_nan_check(const float *p, long n, int *flag, hipStream_t st) {
 /* fault receivers: the nine outputs Tn, Ts1, Ts2, Vs, Vs1, Vs2, Slip, Slip1,
    Slip2 at each receiver, recorded at the step start as the sum of nterm
    terms (v * fy) * fz over (fault id, node) pairs (io_fault_recv_keep,
-   io_funcs.cu:2313-2351; one term with unit factors on a node) */
-__global__ void k_record_fault(const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n, float *buf,
-                               int nt_max, const int *it_dev, int it0, long ss) {
+   io_funcs.cu:2313-2351; one term with unit factors on a node), for every
+   active scenario of a batch into its own [n][nt_max][9] block of buf */
+__global__ void k_record_fault(BatArgs b, const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n,
+                               float *buf, int nt_max, const int *it_dev, int it0, long ss) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * 9) return;
@@ -923,14 +924,17 @@ __global__ void k_record_fault(const FaultDev *ftab, const int *rec, const float
   if (it < 0 || it >= nt_max) return;
   const int *tm = rec + 2L * r * nterm;
   const float *f = fac + 2L * r * nterm;
-  float acc = (ftab[tm[0]].out[v * ss + tm[1]] * f[0]) * f[1];
-  for (int u = 1; u < nterm; u++) acc = acc + (ftab[tm[2 * u]].out[v * ss + tm[2 * u + 1]] * f[2 * u]) * f[2 * u + 1];
-  buf[((long)r * nt_max + it) * 9 + v] = acc;
+  for (int q = 0; q < b.nact; q++) {
+    const long sc = b.act[q], so = sc * b.fpf + (long)v * ss;
+    float acc = (ftab[tm[0]].out[so + tm[1]] * f[0]) * f[1];
+    for (int u = 1; u < nterm; u++) acc = acc + (ftab[tm[2 * u]].out[so + tm[2 * u + 1]] * f[2 * u]) * f[2 * u + 1];
+    buf[sc * n * (long)nt_max * 9 + ((long)r * nt_max + it) * 9 + v] = acc;
+  }
 }
-void launch_record_fault(const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n, float *buf,
-                         int nt_max, const int *it_dev, int it0, long ss, hipStream_t st) {
+void launch_record_fault(const BatArgs &b, const FaultDev *ftab, const int *rec, const float *fac, int nterm, int n,
+                         float *buf, int nt_max, const int *it_dev, int it0, long ss, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_record_fault, dim3((n * 9 + 255) / 256), dim3(256), 0, st, ftab, rec, fac, nterm, n, buf,
+  hipLaunchKernelGGL(k_record_fault, dim3((n * 9 + 255) / 256), dim3(256), 0, st, b, ftab, rec, fac, nterm, n, buf,
                      nt_max, it_dev, it0, ss);
 }
 void launch_record_stations(const BatArgs &b, const float *w, long sicmp, const long *term, const float *fac,

@@ -308,8 +308,9 @@ int cgfd_core_read_snapshot_noalloc(cgfd_core *core, const int *box, int flags, 
    reference's call at drv_rk_curv_col.cu:217) into nt_max samples; reading
    gives host_dst[n][nt_recorded][9]. Receivers here sit on nodes (the
    reference's dj = dk = 0 case); cgfd_core_set_fault_receivers_interp
-   places them between nodes.
-   Single-scenario cores. */
+   places them between nodes. On a batch core every active scenario
+   records into a block of its own (cgfd_batch_read_fault_receivers);
+   cgfd_core_read_fault_receivers reads scenario 0's. */
 int cgfd_core_set_fault_receivers(cgfd_core *core, int n, const int *fjk, int nt_max);
 /* Fault receivers between nodes: increments djk[n][2] towards j+1, k+1
    (cgfd_io_read_fault_stations); each output is the bilinear sum of
@@ -421,6 +422,12 @@ int cgfd_batch_read_fault_state(cgfd_core *core, int s, int id, float *host_dst)
 int cgfd_batch_write_fault_state(cgfd_core *core, int s, int id, const float *host_src, const float *t1rec);
 int cgfd_batch_read_pg(cgfd_core *core, int s, float *host_dst);
 int cgfd_batch_read_stations(cgfd_core *core, int s, float *host_dst, int *nt);
+
+/* Fault receivers of scenario s of a batch (cgfd_core_set_fault_receivers
+   on the batch core records every active scenario): [nrec][*nt][9] as
+   cgfd_core_read_fault_receivers, *nt the steps scenario s recorded. No
+   reference counterpart (the reference runs one scenario). */
+int cgfd_batch_read_fault_receivers(cgfd_core *core, int s, float *host_dst, int *nt);
 
 /* The one-call drop-in: create + run nt_total steps + copy the final state
    into the optional outputs (NULL to skip) + destroy. */

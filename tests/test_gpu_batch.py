@@ -96,3 +96,39 @@ def test_batch_graph_replay_matches_eager(built_libs):
         for s in range(len(nts)):
             assert np.array_equal(bg.read_wavefield(s), be.read_wavefield(s))
             assert np.array_equal(bg.read_fault(s), be.read_fault(s))
+
+
+def test_batch_fault_receivers_match_plain_runs(built_libs):
+    """Fault receivers on a batch whose scenarios retire at different steps
+    (cgfd_core_set_fault_receivers on the batch core, read per scenario with
+    cgfd_batch_read_fault_receivers): each scenario holds the records of its
+    own steps only, equal to its plain run's receivers (rel L2 1e-6), and
+    the scenarios' records differ (their initial tractions do)."""
+    from cgfd import configs, model
+    sc = configs.small(imethod=2)
+    st = model.Setup(sc)
+    nts = [40, 25, 40, 33]
+    fields = [None] + [_scenario_fields(sc, 100 + s) for s in range(1, len(nts))]
+    j1, j2, k1, k2 = sc.faults[0].grid
+    fjk = np.array([[0, (j1 + j2) // 2 + 3, (k1 + k2) // 2 + 3], [0, j1 + 5, k2 + 2], [0, j2 + 2, k1 + 5]], np.int32)
+    bat = model.Batch(st, list(zip(fields, nts)))
+    bat.set_fault_receivers(fjk, 48)
+    bat.run(max(nts) + 5)
+    bat.sync()
+    recs = []
+    for s, nt in enumerate(nts):
+        rb = bat.read_fault_receivers(s)
+        assert rb.shape == (3, nt, 9)
+        st_s = st if fields[s] is None else _plain_with_fields(sc, fields[s])
+        core = model.Core(st_s)
+        core.set_fault_receivers(fjk, 48)
+        core.run(nt)
+        rp = core.read_fault_receivers()
+        core.close()
+        assert rp.shape == rb.shape and np.abs(rp).max() > 0
+        r = rel_l2(rb, rp)
+        assert r <= 1e-6, f"scenario {s}: batched vs plain fault receivers rel L2 {r:.3e}"
+        recs.append(rb)
+    assert np.abs(recs[0][:, :nts[1]] - recs[1]).max() > 0
+    bat.close()
+    st.close()
